@@ -1,0 +1,218 @@
+"""Benchmark: pair.slice action evaluations per second of the GPU-resident bisection chain.
+
+Workload (the configuration the headline metric is quoted on): Lennard-Jones
+particles of mass 4 amu, N=256, M=64 time slices, tau=0.0125 K^-1, level-4
+bisection moves, run as R independent replicas (PIMC++ clones) on one GPU.
+One step = `moves_per_step` bisection moves on every replica, in one kernel
+launch (one block per replica).  A pair.slice evaluation is one U(q,z,s^2) for
+one (moved particle, partner, link, OLD/NEW) -- an innermost iteration of
+ShortRangeClass::SingleAction.  `value` is the mixed-precision throughput;
+the FP64 path is measured the same way and reported beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import subprocess
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from pimcplusplus_amd import abi, native, workloads  # noqa: E402
+
+METRIC = "pair·slice action evals/s, mixed vs FP64 on GPU at ΔS err ≤1e-6"
+HBM_PEAK_GBS = 8000.0
+
+
+def bytes_per_launch(st, n: int) -> float:
+    """Algorithmic HBM bytes of the launches summarised by `st`: every executed
+    stage reads the K+1 slices of N+1 FP64 beads (partners, plus the moved
+    particle's OLD and NEW copies counted once more): 24 (K+1)(N+1) bytes.
+    sum_stages K = pair_evals / (2 (N-1))."""
+    sum_k = st.pair_evals / (2.0 * (n - 1))
+    return 24.0 * (n + 1) * (sum_k + st.stages)
+
+
+def load_system(lib, system):
+    ids = workloads.load(lib, system)
+    return ids
+
+
+def run_gpu(system, precision, steps, warmup, moves_per_step):
+    lib = native.open_context(0)
+    ids = load_system(lib, system)
+    L, seed = system.num_levels, system.seed
+    move = 0
+    for _ in range(warmup):
+        lib.bisection_run(ids, L, seed, move, moves_per_step, precision, records=False)
+        move += moves_per_step
+    lib.synchronize()
+    tot = abi.BisectionStats()
+    dev_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, st = lib.bisection_run(ids, L, seed, move, moves_per_step, precision, records=False)
+        move += moves_per_step
+        for f in ("moves", "accepted", "stages", "pair_evals", "k_evals"):
+            setattr(tot, f, getattr(tot, f) + getattr(st, f))
+        dev_ms.append(st.device_ms)
+    lib.synchronize()
+    wall = time.perf_counter() - t0
+    lib.close()
+    return tot, wall, float(np.sum(dev_ms)), dev_ms
+
+
+def delta_s_error(nrep: int = 16, nmoves: int = 40):
+    """Max |dS_mixed - dS_fp64| over stages attempted on identical paths (chains run
+    until their first differing decision)."""
+    system = workloads.config1(num_replicas=nrep)
+    out = {}
+    for prec in (abi.FP64, abi.MIXED):
+        lib = native.open_context(0)
+        ids = load_system(lib, system)
+        rec, _ = lib.bisection_run(ids, system.num_levels, system.seed, 0, nmoves, prec)
+        out[prec] = rec
+        lib.close()
+    a, b = out[abi.FP64], out[abi.MIXED]
+    err, n = 0.0, 0
+    for r in range(nrep):
+        for m in range(nmoves):
+            if a["num_stages"][r, m] != b["num_stages"][r, m] or a["accepted"][r, m] != b["accepted"][r, m]:
+                break
+            k = a["num_stages"][r, m]
+            da = a["new_action"][r, m, :k] - a["old_action"][r, m, :k]
+            db = b["new_action"][r, m, :k] - b["old_action"][r, m, :k]
+            err = max(err, float(np.max(np.abs(da - db))))
+            n += int(k)
+    return err, n
+
+
+def oracle_lib():
+    path = os.path.join(ROOT, "oracle", "_build", "liboracle_pimc.so")
+    if not os.path.exists(path):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    return ctypes.CDLL(path)
+
+
+def cpu_baseline(seconds: float = 8.0, threads: int | None = None):
+    """The oracle (the C++ restatement of the reference's CPU path) on host cores:
+    one replica per thread, the reference's clone-parallel model."""
+    lib = oracle_lib()
+    threads = threads or max(1, min(16, len(os.sched_getaffinity(0))))
+    system = workloads.config1(num_replicas=threads)
+    ctxs = []
+    for r in range(threads):
+        o = abi.PimcLib(lib)
+        sub = workloads.System(system.name, system.tables, system.desired_tau, [system.replicas[r]],
+                               system.num_levels, system.seed)
+        ids = workloads.load(o, sub)
+        ctxs.append((o, ids))
+
+    def chunk(o, ids, res, k):
+        evals, t0, move = 0, time.perf_counter(), 0
+        while time.perf_counter() - t0 < seconds:
+            _, st = o.bisection_run(ids, system.num_levels, system.seed, move, 20, records=False)
+            evals += st.pair_evals
+            move += 20
+        res[k] = (evals, time.perf_counter() - t0)
+
+    # one thread alone first (the reference's per-rank model), then all threads together
+    single = [None]
+    chunk(ctxs[0][0], ctxs[0][1], single, 0)
+    res = [None] * threads
+    ths = [threading.Thread(target=chunk, args=(o, ids, res, k)) for k, (o, ids) in enumerate(ctxs)]
+    t0 = time.perf_counter()
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    wall = time.perf_counter() - t0
+    total = sum(e for e, _ in res)
+    for o, _ in ctxs:
+        o.close()
+    return {"value": total / wall, "unit": "pair·slice evals/s", "cores": threads, "kind": "port",
+            "sample": f"config N=256 M=64 level-4, one replica per thread, {seconds:.0f} s per thread "
+                      f"({total} evaluations); single thread {single[0][0] / single[0][1]:.4g} evals/s",
+            "single_core_value": single[0][0] / single[0][1]}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--replicas", type=int, default=2048)
+    ap.add_argument("--moves-per-step", type=int, default=4)
+    ap.add_argument("--cpu-seconds", type=float, default=8.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(replicas would shard across GPUs with no communication; see the design notes)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    system = workloads.config1(num_replicas=args.replicas)
+    n = system.replicas[0].num_particles
+    res = {}
+    for name, prec in (("mixed", abi.MIXED), ("fp64", abi.FP64)):
+        tot, wall, dev_ms, per = run_gpu(system, prec, args.steps, args.warmup, args.moves_per_step)
+        res[name] = {
+            "evals_per_s": tot.pair_evals / wall,
+            "ms_per_step": 1e3 * wall / args.steps,
+            "kernel_ms_avg": dev_ms / args.steps,
+            "device_evals_per_s": tot.pair_evals / (dev_ms * 1e-3),
+            "bytes_per_launch": bytes_per_launch(tot, n) / args.steps,
+            "acceptance": tot.accepted / max(1, tot.moves),
+            "stages_per_move": tot.stages / max(1, tot.moves),
+            "pair_evals": tot.pair_evals,
+        }
+    err, nstages = delta_s_error()
+    mixed = res["mixed"]
+    achieved = mixed["bytes_per_launch"] / (mixed["kernel_ms_avg"] * 1e-3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": mixed["evals_per_s"],
+        "unit": "pair·slice evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 spline / f64 geometry+accumulation",
+        "data": "synthetic",
+        "config": {"workload": "LJ N=256 M=64 tau=0.0125 level-4 bisection, independent replicas",
+                   "replicas": args.replicas, "moves_per_step": args.moves_per_step,
+                   "parallelism": "replicas (one block per replica)"},
+        "fp64": {"value": res["fp64"]["evals_per_s"], "ms_per_step": res["fp64"]["ms_per_step"],
+                 "kernel_ms_avg": res["fp64"]["kernel_ms_avg"]},
+        "mixed_over_fp64": mixed["evals_per_s"] / res["fp64"]["evals_per_s"],
+        "delta_s_max_err": err,
+        "delta_s_stages_compared": nstages,
+        "acceptance": mixed["acceptance"],
+        "stages_per_move": mixed["stages_per_move"],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_bisection<NT=2,MIXED>",
+                     "note": "bytes = 24 (K+1)(N+1) per executed stage; the kernel's binding roof is "
+                             "VALU/LDS (spline evaluation), not HBM"},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,267 @@
+/*
+ * pimc_gpu.h -- C ABI of the MI355X action-evaluation library for PIMC++.
+ *
+ * This is the drop-in boundary for PIMC++'s action hot path: the short-range
+ * pair action, the kinetic action and the Ewald long-range (rho_k) action that
+ * the bisection stages evaluate once in OLDMODE and once in NEWMODE.  Every
+ * entry point takes plain pointers and sizes; nothing here mentions a GPU
+ * runtime type.  All calls return PIMC_OK or an error code; the message of the
+ * last error on the calling thread is in pimc_last_error().  The caller owns
+ * every host buffer.  A context is bound to one device and one stream and is
+ * not reentrant: calls on one context must be serialised by the caller.
+ *
+ * Reference interfaces replaced (paths relative to the PIMC++ source tree):
+ *   ActionBaseClass::SingleAction          PIMC++/src/Actions/ActionBase.h:43
+ *   ShortRangeClass::SingleAction          PIMC++/src/Actions/ShortRangeClass.cc:131-176
+ *   KineticClass::SingleAction             PIMC++/src/Actions/KineticClass.cc:33-90
+ *   LongRangeClass::SingleAction           PIMC++/src/Actions/LongRangeClass.cc:273-371
+ *   PairActionFitClass::U / DavidPAClass   Common/PairAction/PAFitBase.h:78,
+ *                                          Common/PairAction/DavidPAClass.cc:92-102,468-511
+ *   PathClass::DistDisp                    PIMC++/src/PathClass.h:696-746
+ *   PathClass::UpdateRho_ks                PIMC++/src/PathClass.cc:649-723
+ *   PathClass::AcceptCopy / RejectCopy     PIMC++/src/PathClass.cc:797-899
+ *   PathClass::MoveJoin                    PIMC++/src/PathClass.cc:727-793
+ *   BisectionBlockClass::MakeMove (NONE)   PIMC++/src/Moves/BisectionBlock.cc:380-451,
+ *                                          PIMC++/src/Moves/StageClass.cc:49-95,
+ *                                          PIMC++/src/Moves/BisectionStage.cc:128-289
+ *
+ * Conventions shared with the reference:
+ *   - a replica's path holds M+1 time slices [0, M]; bead (slice, ptcl) is a
+ *     3-vector of doubles; host buffers use the reference's layout
+ *     [slice][ptcl][xyz] (blitz Array<dVec,2>, row-major);
+ *   - particles of one species are contiguous and species are in order
+ *     (SpeciesClass FirstPtcl/LastPtcl);
+ *   - OLDMODE = 0, NEWMODE = 1 select the mirrored copy an action reads
+ *     (MirroredClass ModeType, PIMC++/src/MirroredClass.h:23-39).
+ */
+#ifndef PIMC_GPU_H
+#define PIMC_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PIMC_ABI_VERSION 1
+#define PIMC_MAX_ACTIVE 4   /* particles moved together (PermuteType TABLE moves up to 4) */
+#define PIMC_MAX_LEVELS 8   /* bisection levels */
+#define PIMC_MAX_SPECIES 4
+
+enum pimc_status {
+  PIMC_OK = 0,
+  PIMC_ERR_ARG = 1,         /* bad argument: the message names it */
+  PIMC_ERR_STATE = 2,       /* object missing or in the wrong state */
+  PIMC_ERR_DEVICE = 3,      /* the device runtime reported an error */
+  PIMC_ERR_NOMEM = 4,       /* host or device allocation failed */
+  PIMC_ERR_UNSUPPORTED = 5  /* valid request this build does not implement */
+};
+
+enum pimc_mode { PIMC_OLDMODE = 0, PIMC_NEWMODE = 1, PIMC_BOTHMODES = 2 };
+
+/* Arithmetic of the spline evaluation and the rho_k phase sums.
+ * PIMC_FP64: everything in FP64 (the correctness path).
+ * PIMC_MIXED: pair geometry and images in FP64, spline and polynomial in FP32,
+ * per-call sums in FP64 with Kahan compensation; rho_k totals, sum_k u_k|rho_k|^2
+ * and the Metropolis difference stay FP64. */
+enum pimc_precision { PIMC_FP64 = 0, PIMC_MIXED = 1 };
+
+enum pimc_grid_type { PIMC_GRID_LINEAR = 0, PIMC_GRID_LOG = 1 };
+
+typedef struct pimc_ctx pimc_ctx;
+
+/* ------------------------------------------------------------------ */
+/* Library and context                                                 */
+/* ------------------------------------------------------------------ */
+
+int pimc_abi_version(void);
+/* Message of the last failed call on this thread ("" if none). */
+const char* pimc_last_error(void);
+
+/* Binds device `device`, creates the context's stream. */
+int pimc_ctx_create(int32_t device, pimc_ctx** out);
+int pimc_ctx_destroy(pimc_ctx* ctx);
+/* Waits for all work queued on the context's stream. */
+int pimc_ctx_synchronize(pimc_ctx* ctx);
+
+/* ------------------------------------------------------------------ */
+/* Pair-action fits (PairActionFitClass)                               */
+/* ------------------------------------------------------------------ */
+
+/* A DavidPAClass fit: one MultiCubicSpline per tau level over an r grid, with
+ * column 0 the potential, column 1 u00 and columns 2.. the off-diagonal
+ * u_kj(q) in the order k(k+1)/2+j+1 (DavidPAClass.cc:440-456).  Splines are
+ * built as the reference's .dm reader builds them (DavidPAClass.cc:755-773):
+ * natural second derivative at the start, first derivative 0 at the end.
+ * tau_i = smallest_tau * 2^i; TauPos is the i with |tau_i - desired_tau| < 1e-4
+ * (DavidPAClass.cc:859-868).  The LOG grid is r_i = start * spacing^i with
+ * spacing = (end/start)^(1/(n-1)) (DavidPAClass.cc:714-719, Grid.h:537-547). */
+typedef struct pimc_david_fit_desc {
+  int32_t grid_type;       /* pimc_grid_type */
+  int32_t num_points;      /* r-grid points, >= 3 */
+  double grid_start;
+  double grid_end;
+  int32_t num_tau;         /* tau levels stored */
+  int32_t num_cols;        /* NumUKJ + 1 */
+  int32_t num_terms;       /* NumOffDiagonalTerms: needs num_cols >= (n+1)(n+2)/2 + 1 */
+  int32_t reserved;
+  double smallest_tau;
+  double desired_tau;
+  const double* y;         /* [num_tau][num_points][num_cols] */
+} pimc_david_fit_desc;
+
+/* Creates a fit; replaces ReadPAFit + DavidPAClass::Read for table data. */
+int pimc_fit_create_david(pimc_ctx* ctx, const pimc_david_fit_desc* desc, int32_t* fit_id);
+
+/* Batched PairActionFitClass::U(q, z, s2, level) for n points (DavidPAClass::U).
+ * level is the bisection level; the fit adds its TauPos. */
+int pimc_fit_eval_u(pimc_ctx* ctx, int32_t fit_id, int32_t level, int64_t n,
+                    const double* q, const double* z, const double* s2,
+                    int32_t precision, double* out);
+
+/* ------------------------------------------------------------------ */
+/* Replicas (PathClass + the actions' configuration)                   */
+/* ------------------------------------------------------------------ */
+
+typedef struct pimc_replica_desc {
+  int32_t num_particles;
+  int32_t num_time_slices;   /* M: the path holds M+1 slices (PathClass.cc:305,410) */
+  int32_t num_species;
+  int32_t num_images;        /* KineticClass NumImages */
+  double box[3];
+  int32_t periodic[3];       /* 1 = periodic in that dimension */
+  double tau;
+  const int32_t* species_count;  /* [num_species] */
+  const double* lambda;          /* [num_species] hbar^2/2m */
+  const int32_t* pair_fit;       /* [num_species*num_species], PairMatrix(s1,s2) */
+  double k_cutoff;               /* > 0: build the k set and keep rho_k (PathClass.cc:534-586) */
+  int32_t use_long_range;        /* ActionsClass UseLongRange */
+  int32_t use_background;        /* LongRangeClass UseBackground */
+  uint32_t rng_stream;           /* this replica's random stream (a clone's Local stream) */
+  int32_t reserved;
+} pimc_replica_desc;
+
+int pimc_replica_create(pimc_ctx* ctx, const pimc_replica_desc* desc, int32_t* replica_id);
+/* Number of k vectors of the replica's k set (0 if k_cutoff <= 0). */
+int pimc_replica_num_kvecs(pimc_ctx* ctx, int32_t replica, int32_t* nk);
+/* The k set: kvecs [nk][3] (Cartesian), kidx [nk][3] (integer index + MaxkIndex). Either may be NULL. */
+int pimc_replica_kvecs(pimc_ctx* ctx, int32_t replica, double* kvecs, int32_t* kidx);
+
+/* Long-range tables of species pair (sa, sb) on this replica's k set
+ * (PairActionFitClass Ulong_k, Ulong_r0, Ushort_k0, Ulong_k0, PAFitBase.h:38-46).
+ * ulong_k is [num_levels][nk]; the scalars are [num_levels]. */
+int pimc_replica_set_long_range(pimc_ctx* ctx, int32_t replica, int32_t sa, int32_t sb,
+                                int32_t num_levels, const double* ulong_k,
+                                const double* ulong_r0, const double* ushort_k0,
+                                const double* ulong_k0);
+
+/* Path data. mode: PIMC_OLDMODE, PIMC_NEWMODE or PIMC_BOTHMODES (set only).
+ * beads is [(M+1)][N][3]. */
+int pimc_path_set(pimc_ctx* ctx, int32_t replica, int32_t mode, const double* beads);
+int pimc_path_get(pimc_ctx* ctx, int32_t replica, int32_t mode, double* beads);
+/* Writes beads of one particle for slices [slice1, slice2] (Path.SetPos). xyz is [(slice2-slice1+1)][3]. */
+int pimc_path_set_beads(pimc_ctx* ctx, int32_t replica, int32_t mode, int32_t ptcl,
+                        int32_t slice1, int32_t slice2, const double* xyz);
+/* Permutation vector of a mirrored copy, [N]. */
+int pimc_permutation_get(pimc_ctx* ctx, int32_t replica, int32_t mode, int32_t* perm);
+
+/* PathClass::AcceptCopy / RejectCopy for the listed particles over slices [slice1, slice2]. */
+int pimc_accept_copy(pimc_ctx* ctx, int32_t replica, int32_t slice1, int32_t slice2,
+                     const int32_t* ptcls, int32_t nptcls);
+int pimc_reject_copy(pimc_ctx* ctx, int32_t replica, int32_t slice1, int32_t slice2,
+                     const int32_t* ptcls, int32_t nptcls);
+/* PathClass::MoveJoin(old_join, new_join). */
+int pimc_move_join(pimc_ctx* ctx, int32_t replica, int32_t old_join, int32_t new_join);
+
+/* PathClass::UpdateRho_ks(): recompute rho_k of every slice and species from
+ * the OLD copy, then NEW <- OLD (PathClass.cc:709-723). */
+int pimc_rho_k_update_all(pimc_ctx* ctx, int32_t replica);
+/* rho_k of a mirrored copy as [(M+1)][num_species][nk][re,im]. */
+int pimc_rho_k_get(pimc_ctx* ctx, int32_t replica, int32_t mode, double* rho);
+
+/* ------------------------------------------------------------------ */
+/* Actions (ActionBaseClass::SingleAction)                              */
+/* ------------------------------------------------------------------ */
+
+/* One SingleAction(slice1, slice2, activeParticles, level) call evaluated on
+ * copy `mode` of `replica`. */
+typedef struct pimc_action_req {
+  int32_t replica;
+  int32_t mode;              /* PIMC_OLDMODE or PIMC_NEWMODE */
+  int32_t slice1;
+  int32_t slice2;
+  int32_t level;
+  int32_t nptcls;            /* 1..PIMC_MAX_ACTIVE */
+  int32_t ptcls[PIMC_MAX_ACTIVE];
+} pimc_action_req;
+
+/* ShortRangeClass::SingleAction for each request; out[i] is request i's total. */
+int pimc_action_short_range(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t nreq,
+                            int32_t precision, double* out);
+/* KineticClass::SingleAction. */
+int pimc_action_kinetic(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t nreq, double* out);
+/* LongRangeClass::SingleAction: a NEWMODE request first updates rho_k of copy NEW
+ * over [slice1, slice2] step 2^level from OLD (PathClass::UpdateRho_ks).  Requests
+ * are processed in order. */
+int pimc_action_long_range(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t nreq,
+                           int32_t precision, double* out);
+
+/* PathClass::DistDisp(sliceA, sliceB, p1, p2, ...) for n index quadruples
+ * idx[n][4] = {sliceA, sliceB, p1, p2} on copy `mode`.  out[n][8] =
+ * {distA, distB, dispA[3], dispB[3]}; images[n][6] = {n[3], m[3]} as the
+ * reference computes them (n = -nearbyint(dispA/L), m = nearbyint((dispA-dispB)/L)). */
+int pimc_dist_disp(pimc_ctx* ctx, int32_t replica, int32_t mode, int64_t n,
+                   const int32_t* idx, double* out, int32_t* images);
+
+/* ------------------------------------------------------------------ */
+/* Bisection moves (BisectionBlockClass with PermuteType NONE)          */
+/* ------------------------------------------------------------------ */
+
+/* Stage actions: Kinetic + ShortRange at every level, + LongRange at the lowest
+ * level when the replica uses long range (BisectionBlock.cc:66-104 with
+ * HigherLevelActions = SamplingActions = ShortRange). */
+typedef struct pimc_bisection_desc {
+  int32_t num_levels;        /* NumLevels: the window is 2^num_levels slices */
+  int32_t lowest_level;      /* LowestLevel */
+  int32_t species;           /* species whose particles are moved */
+  int32_t reserved;
+  uint64_t seed;             /* random seed (the clone's Local stream key) */
+} pimc_bisection_desc;
+
+/* What one move did. Stage s is bisection level num_levels-1-s. */
+typedef struct pimc_move_record {
+  int32_t slice1;
+  int32_t ptcl;
+  int32_t accepted;          /* 1 if every stage accepted and the move was kept */
+  int32_t num_stages;        /* stages attempted (the first rejection ends the move) */
+  double old_action[PIMC_MAX_LEVELS];
+  double new_action[PIMC_MAX_LEVELS];
+  double log_sample_ratio[PIMC_MAX_LEVELS];
+  double log_u[PIMC_MAX_LEVELS];
+} pimc_move_record;
+
+/* Counters of a pimc_bisection_run call, summed over replicas and moves. */
+typedef struct pimc_bisection_stats {
+  int64_t moves;
+  int64_t accepted;
+  int64_t stages;
+  int64_t pair_evals;        /* pair.slice evaluations: one U(q,z,s2) per (moved, partner, link, mode) */
+  int64_t k_evals;           /* k.slice.species rho_k terms evaluated by the long-range action */
+  double device_ms;          /* time of the call's compute on the context's stream (device events) */
+} pimc_bisection_stats;
+
+/* Runs moves [first_move, first_move + num_moves) on every listed replica.
+ * Each replica's moves are a Metropolis chain in move order; replicas are
+ * independent.  records (may be NULL) is [nrep][num_moves].  The random
+ * numbers of move m of a replica are a pure function of (seed, rng_stream, m),
+ * so a chain is reproducible across calls and backends. */
+int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep,
+                       const pimc_bisection_desc* desc, int64_t first_move, int32_t num_moves,
+                       int32_t precision, pimc_move_record* records,
+                       pimc_bisection_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PIMC_GPU_H */

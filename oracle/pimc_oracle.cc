@@ -1,0 +1,1195 @@
+// pimc_oracle.cc -- TEST INFRASTRUCTURE ONLY (the parity oracle).
+//
+// A plain C++17 CPU restatement of PIMC++'s action hot path, in FP64, written
+// to follow the reference's arithmetic statement by statement.  It implements
+// every entry point of include/pimc_gpu.h so the tests can drive it and the
+// GPU library through one interface, plus the hooks of oracle/pimc_oracle.h.
+// It is compiled with -ffp-contract=off so no multiply-add is fused that the
+// reference source does not fuse.  Only tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg load it.
+//
+// Reference files followed (paths relative to the PIMC++ source tree):
+//   Common/Splines/Grid.h:81-84,527-547        grids and ReverseMap
+//   Common/Splines/CubicSpline.h:470-561       MultiCubicSpline evaluation
+//   Common/Splines/CubicSpline.cc:77-128       MultiCubicSpline::Update
+//   Common/PairAction/DavidPAClass.cc:92-102,468-511,755-773,859-868
+//   PIMC++/src/PathClass.h:696-746,831-865,897-921   DistDisp, Velocity, PutInBox
+//   PIMC++/src/PathClass.cc:281-292,534-586,616-723,727-899, 1389-1407
+//   PIMC++/src/Actions/ShortRangeClass.cc:131-176
+//   PIMC++/src/Actions/KineticClass.cc:33-90
+//   PIMC++/src/Actions/LongRangeClass.cc:273-371
+//   PIMC++/src/Moves/StageClass.cc:49-95,124-136, MultiStage.cc:42-110,
+//   PIMC++/src/Moves/BisectionStage.cc:128-289, BisectionBlock.cc:66-104,310-451,
+//   PIMC++/src/Moves/NoPermuteClass.cc:26-68, Common/Random/Random.h:45-46,156-180
+//
+// Deliberate, documented departures:
+//   * LogGrid::ReverseMap: the reference's floor(log(r/r0)/log(spacing)) is
+//     corrected by one step against the stored knots so that host and device
+//     logarithms that differ in the last bit cannot pick different intervals.
+//     It differs from the literal formula only within rounding distance of a
+//     knot, where both neighbouring cubic pieces give the same value.
+//   * The random stream: SPRNG2 is not available, so the clone's Local stream
+//     is Philox4x32-10 with the counter layout documented at rng_ctr() below.
+//     The polar Box-Muller method is kept, with its second variate used next,
+//     but pairs are indexed per (move, level) instead of a function-static cache.
+
+#include "../include/pimc_gpu.h"
+#include "pimc_oracle.h"
+
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace {
+
+typedef std::complex<double> cplx;
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+// ------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11)
+// ------------------------------------------------------------------------
+void philox(const uint32_t c_in[4], const uint32_t k_in[2], uint32_t out[4]) {
+  uint32_t c0 = c_in[0], c1 = c_in[1], c2 = c_in[2], c3 = c_in[3];
+  uint32_t k0 = k_in[0], k1 = k_in[1];
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    uint32_t n0 = hi1 ^ c1 ^ k0;
+    uint32_t n2 = hi0 ^ c3 ^ k1;
+    c0 = n0;
+    c1 = lo1;
+    c2 = n2;
+    c3 = lo0;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+double uniform53(uint32_t w0, uint32_t w1) {
+  uint64_t v = ((uint64_t)w1 << 32) | (uint64_t)w0;
+  return (double)(v >> 11) * 0x1.0p-53;
+}
+
+enum { TAG_SLICE = 1, TAG_PTCL = 2, TAG_GAUSS = 3, TAG_ACCEPT = 4 };
+
+// Counter layout: c0 = move, c1 = tag<<24 | level<<16 | attempt, c2 = index,
+// c3 = replica stream; key = seed (low word, high word).
+void rng_block(uint64_t seed, uint32_t stream, uint32_t move, int tag, int level,
+               uint32_t attempt, uint32_t index, uint32_t out[4]) {
+  uint32_t c[4] = {move, ((uint32_t)tag << 24) | (((uint32_t)level & 0xffu) << 16) | (attempt & 0xffffu),
+                   index, stream};
+  uint32_t k[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  philox(c, k, out);
+}
+
+double rng_uniform(uint64_t seed, uint32_t stream, uint32_t move, int tag, int level) {
+  uint32_t w[4];
+  rng_block(seed, stream, move, tag, level, 0, 0, w);
+  return uniform53(w[0], w[1]);
+}
+
+// Polar Box-Muller (Random.h:156-180) on the pair's own sub-stream.
+int gaussian_pair(uint64_t seed, uint32_t stream, uint32_t move, int level, uint32_t pair,
+                  double g[2]) {
+  for (uint32_t a = 0; a < 0x10000u; ++a) {
+    uint32_t w[4];
+    rng_block(seed, stream, move, TAG_GAUSS, level, a, pair, w);
+    double V1 = 2.0 * uniform53(w[0], w[1]) - 1.0;
+    double V2 = 2.0 * uniform53(w[2], w[3]) - 1.0;
+    double S = V1 * V1 + V2 * V2;
+    if (S < 1.0 && S > 0.0) {
+      double temp = sqrt((-2.0 * log(S)) / S);
+      g[0] = V1 * temp;
+      g[1] = V2 * temp;
+      return (int)a + 1;
+    }
+  }
+  g[0] = g[1] = 0.0;
+  return 0x10000;
+}
+
+// ------------------------------------------------------------------------
+// Grids (Common/Splines/Grid.h)
+// ------------------------------------------------------------------------
+struct Grid {
+  int type = PIMC_GRID_LINEAR;
+  int n = 0;
+  double start = 0, end = 0;
+  std::vector<double> x;
+  double delta = 0, deltainv = 0;  // LinearGrid
+  double r0 = 0, spacing = 0;      // LogGrid (Z = 1)
+
+  // LinearGrid::Init (Grid.h:87-96)
+  void init_linear(double s, double e, int np) {
+    type = PIMC_GRID_LINEAR;
+    start = s; end = e; n = np;
+    x.resize(n);
+    delta = (end - start) / (double)(n - 1);
+    deltainv = 1.0 / delta;
+    for (int i = 0; i < n; i++) x[i] = start + (double)i * delta;
+  }
+  // The .dm reader's LOG grid (DavidPAClass.cc:714-719) -> LogGrid::Init (Grid.h:537-547)
+  void init_log_dm(double s, double e, int np) {
+    type = PIMC_GRID_LOG;
+    double d = pow((e / s), 1.0 / (np - 1.0));
+    n = np; r0 = s; spacing = d;
+    start = r0;
+    end = r0 * pow(spacing, (double)n - 1);
+    x.resize(n);
+    for (int i = 0; i < n; i++) x[i] = r0 * pow(spacing, (double)i);
+  }
+  static int to_int(double t, int n) {  // (int) cast of a value clamped to [-2, n+1]
+    if (!(t >= -2.0)) t = -2.0;
+    if (t > (double)(n + 1)) t = (double)(n + 1);
+    return (int)t;
+  }
+  int reverse_map_literal(double r) const {
+    if (type == PIMC_GRID_LINEAR) return to_int(nearbyint((r - start) * deltainv - 0.5), n);
+    return to_int(floor(log(r / r0) / log(spacing)), n);
+  }
+  int reverse_map(double r) const {
+    int i = reverse_map_literal(r);
+    if (type == PIMC_GRID_LOG) {
+      if (i >= 1 && i <= n - 2 && r < x[i]) i -= 1;
+      else if (i >= -1 && i <= n - 3 && r >= x[i + 1]) i += 1;
+    }
+    return i;
+  }
+};
+
+// ------------------------------------------------------------------------
+// MultiCubicSpline (CubicSpline.h:470-561, CubicSpline.cc:77-128)
+// ------------------------------------------------------------------------
+struct MultiCubicSpline {
+  const Grid* grid = nullptr;
+  int ng = 0, ns = 0;
+  std::vector<double> y, d2y;  // [ng][ns]
+  std::vector<double> start_deriv, end_deriv;
+
+  void init(const Grid* g, const double* ys, int nsplines, const double* sd, const double* ed) {
+    grid = g; ng = g->n; ns = nsplines;
+    y.assign(ys, ys + (size_t)ng * ns);
+    d2y.assign((size_t)ng * ns, 0.0);
+    start_deriv.assign(sd, sd + ns);
+    end_deriv.assign(ed, ed + ns);
+    for (int i = 0; i < ns; i++) update(i);
+  }
+  double& Y(int j, int i) { return y[(size_t)j * ns + i]; }
+  double& D(int j, int i) { return d2y[(size_t)j * ns + i]; }
+
+  void update(int i) {
+    const std::vector<double>& x = grid->x;
+    int N = ng;
+    std::vector<double> U(N);
+    if (start_deriv[i] > 0.99e30) {
+      D(0, i) = 0.0;
+      U[0] = 0.0;
+    } else {
+      D(0, i) = -0.5;
+      U[0] = (3.0 / (x[1] - x[0])) * ((Y(1, i) - Y(0, i)) / (x[1] - x[0]) - start_deriv[i]);
+    }
+    D(N - 1, i) = 0.0;
+    for (int j = 1; j < N - 1; j++) {
+      double sig = (x[j] - x[j - 1]) / (x[j + 1] - x[j - 1]);
+      double p = sig * D(j - 1, i) + 2.0;
+      D(j, i) = (sig - 1.0) / p;
+      U[j] = (6.0 * ((Y(j + 1, i) - Y(j, i)) / (x[j + 1] - x[j]) -
+                     (Y(j, i) - Y(j - 1, i)) / (x[j] - x[j - 1])) /
+                  (x[j + 1] - x[j - 1]) -
+              sig * U[j - 1]) /
+             p;
+    }
+    double Un, Qn;
+    if (end_deriv[i] > 0.99e30) {
+      Un = 0.0;
+      Qn = 0.0;
+    } else {
+      Qn = 0.5;
+      Un = (3.0 / (x[N - 1] - x[N - 2])) * (end_deriv[i] - (Y(N - 1, i) - Y(N - 2, i)) / (x[N - 1] - x[N - 2]));
+    }
+    D(N - 1, i) = (Un - Qn * U[N - 2]) / (Qn * D(N - 2, i) + 1.0);
+    for (int k = N - 2; k >= 0; k--) D(k, i) = D(k, i) * D(k + 1, i) + U[k];
+  }
+
+  void locate(double xv, int& low, int& hi) const {
+    hi = grid->reverse_map(xv) + 1;
+    low = hi - 1;
+    if (low < 0) { low = 0; hi = 1; }
+    if (hi > (grid->n - 1)) { hi = grid->n - 1; low = hi - 1; }
+  }
+  double eval(int i, double xv) const {
+    int low, hi;
+    locate(xv, low, hi);
+    const std::vector<double>& X = grid->x;
+    double h = X[hi] - X[low];
+    double hinv = 1.0 / h;
+    double a = (X[hi] - xv) * hinv;
+    double b = (xv - X[low]) * hinv;
+    double sixinv = 0.1666666666666666666;
+    return (a * y[(size_t)low * ns + i] + b * y[(size_t)hi * ns + i] +
+            ((a * a * a - a) * d2y[(size_t)low * ns + i] + (b * b * b - b) * d2y[(size_t)hi * ns + i]) *
+                (h * h * sixinv));
+  }
+  void eval_all(double xv, double* out) const {
+    int low, hi;
+    locate(xv, low, hi);
+    const std::vector<double>& X = grid->x;
+    double h = X[hi] - X[low];
+    double hinv = 1.0 / h;
+    double a = (X[hi] - xv) * hinv;
+    double b = (xv - X[low]) * hinv;
+    double a3minusa = a * a * a - a;
+    double b3minusb = b * b * b - b;
+    double h2over6 = h * h * 0.1666666666666666666;
+    for (int i = 0; i < ns; i++)
+      out[i] = a * y[(size_t)low * ns + i] + b * y[(size_t)hi * ns + i] +
+               (a3minusa * d2y[(size_t)low * ns + i] + b3minusb * d2y[(size_t)hi * ns + i]) * h2over6;
+  }
+};
+
+// ------------------------------------------------------------------------
+// DavidPAClass (DavidPAClass.cc)
+// ------------------------------------------------------------------------
+struct DavidFit {
+  Grid grid;
+  std::vector<MultiCubicSpline> ukj;  // per tau level
+  int num_tau = 0, ncols = 0, num_terms = 0, taupos = 0;
+  double smallest_tau = 0;
+  mutable std::vector<double> temp;   // TempukjArray
+
+  // calcUsqzFast (DavidPAClass.cc:468-511)
+  double calc_usqz_fast(double s, double q, double z, int level) const {
+    level = level + taupos;
+    const MultiCubicSpline& sp = ukj[level];
+    double rmin = sp.grid->start;
+    double U = 0.0;
+    double r = q + 0.5 * z;
+    double rprime = q - 0.5 * z;
+    if (r > sp.grid->end) r = sp.grid->end;
+    if (rprime > sp.grid->end) rprime = sp.grid->end;
+    if ((rprime < rmin) || (r < rmin)) {
+      U = sp.eval(1, rmin);
+      return U;
+    }
+    U += 0.5 * (sp.eval(1, r) + sp.eval(1, rprime));
+    if (s > 0.0 && q < sp.grid->end) {
+      double zsquared = z * z;
+      double ssquared = s * s;
+      double ssquaredinverse = 1.0 / ssquared;
+      double Sto2k = ssquared;
+      sp.eval_all(q, temp.data());
+      for (int k = 1; k <= num_terms; k++) {
+        double Zto2j = 1;
+        double currS = Sto2k;
+        for (int j = 0; j <= k; j++) {
+          double Ucof = temp[k * (k + 1) / 2 + j + 1];
+          U += (Ucof)*Zto2j * currS;
+          Zto2j *= zsquared;
+          currS = currS * ssquaredinverse;
+        }
+        Sto2k = Sto2k * ssquared;
+      }
+    }
+    return U;
+  }
+  // DavidPAClass::U (DavidPAClass.cc:92-102)
+  double U(double q, double z, double s2, int level) const {
+    double s = sqrt(s2);
+    return calc_usqz_fast(s, q, z, level);
+  }
+};
+
+// ------------------------------------------------------------------------
+// Replica: PathClass state + action configuration
+// ------------------------------------------------------------------------
+struct LongRangeTables {
+  bool set = false;
+  int num_levels = 0;
+  std::vector<double> ulong_k;  // [levels][nk]
+  std::vector<double> ulong_r0, ushort_k0, ulong_k0;
+};
+
+struct Replica {
+  int N = 0, M = 0, nslices = 0, ns = 0, num_images = 0;
+  double box[3], boxinv[3], isperiodic[3];
+  double tau = 0;
+  std::vector<int> species_of, first, count;
+  std::vector<double> lambda;
+  std::vector<int> pair_fit;     // [ns*ns]
+  std::vector<double> path[2];   // [slice][ptcl][3]
+  std::vector<int> perm[2];
+  bool long_range = false, use_long_range = false, use_background = false;
+  double kcutoff = 0;
+  double kbox[3];
+  int maxk[3];
+  std::vector<double> kvecs;     // [nk][3]
+  std::vector<int> kidx;         // [nk][3]
+  int nk = 0;
+  std::vector<cplx> rho[2];      // [slice][species][k]
+  std::vector<LongRangeTables> lr;  // [ns*ns]
+  uint32_t stream = 0;
+  int join = 0;
+
+  double* P(int mode, int slice, int ptcl) { return &path[mode][((size_t)slice * N + ptcl) * 3]; }
+  cplx& R(int mode, int slice, int s, int k) { return rho[mode][((size_t)slice * ns + s) * nk + k]; }
+
+  // PathClass::PutInBox (PathClass.h:897-921)
+  void put_in_box(double v[3]) const {
+    for (int i = 0; i < 3; i++) {
+      double n = -floor(v[i] * boxinv[i] + 0.5);
+      v[i] += n * isperiodic[i] * box[i];
+    }
+  }
+  // PathClass::PutInBoxFast (PathClass.cc:1389-1407)
+  void put_in_box_fast(double v[3]) const {
+    for (int d = 0; d < 3; d++) {
+      double half = box[d] / 2;
+      if (v[d] > half) v[d] -= box[d];
+      if (v[d] < -half) v[d] += box[d];
+    }
+  }
+  // PathClass::Velocity (PathClass.h:831-865)
+  void velocity(int mode, int sA, int sB, int p, double vel[3]) {
+    const double* a = P(mode, sA, p);
+    const double* b = P(mode, sB, p);
+    for (int d = 0; d < 3; d++) vel[d] = b[d] - a[d];
+    put_in_box(vel);
+  }
+  // PathClass::DistDisp(sliceA, sliceB, ...) (PathClass.h:696-746)
+  void dist_disp(int mode, int sA, int sB, int p1, int p2, double& distA, double& distB,
+                 double dispA[3], double dispB[3], double nimg[3], double mimg[3]) {
+    const double* a1 = P(mode, sA, p1);
+    const double* a2 = P(mode, sA, p2);
+    const double* b1 = P(mode, sB, p1);
+    const double* b2 = P(mode, sB, p2);
+    for (int d = 0; d < 3; d++) {
+      dispA[d] = a2[d] - a1[d];
+      dispB[d] = b2[d] - b1[d];
+    }
+    double n[3], m[3];
+    for (int d = 0; d < 3; d++) n[d] = -nearbyint(dispA[d] * boxinv[d]);
+    for (int d = 0; d < 3; d++) dispA[d] += n[d] * isperiodic[d] * box[d];
+    for (int d = 0; d < 3; d++) m[d] = nearbyint((dispA[d] - dispB[d]) * boxinv[d]);
+    for (int d = 0; d < 3; d++) dispB[d] += m[d] * isperiodic[d] * box[d];
+    distA = sqrt(dispA[0] * dispA[0] + dispA[1] * dispA[1] + dispA[2] * dispA[2]);
+    distB = sqrt(dispB[0] * dispB[0] + dispB[1] * dispB[1] + dispB[2] * dispB[2]);
+    if (nimg) for (int d = 0; d < 3; d++) nimg[d] = n[d];
+    if (mimg) for (int d = 0; d < 3; d++) mimg[d] = m[d];
+  }
+
+  // PathClass::SetupkVecs3D + Include (PathClass.cc:281-292,534-586)
+  void setup_kvecs() {
+    for (int i = 0; i < 3; i++) kbox[i] = 2.0 * M_PI / box[i];
+    for (int i = 0; i < 3; i++) maxk[i] = (int)ceil(1.1 * kcutoff / kbox[i]);
+    kvecs.clear();
+    kidx.clear();
+    for (int ix = -maxk[0]; ix <= maxk[0]; ix++) {
+      double k0 = ix * kbox[0];
+      for (int iy = -maxk[1]; iy <= maxk[1]; iy++) {
+        double k1 = iy * kbox[1];
+        for (int iz = -maxk[2]; iz <= maxk[2]; iz++) {
+          double k2 = iz * kbox[2];
+          double kk = k0 * k0 + k1 * k1 + k2 * k2;
+          bool include = (k0 > 0.0) || ((k0 == 0.0) && (k1 > 0.0)) ||
+                         ((k0 == 0.0) && (k1 == 0.0) && (k2 > 0.0));
+          if ((kk < kcutoff * kcutoff) && include) {
+            kvecs.push_back(k0); kvecs.push_back(k1); kvecs.push_back(k2);
+            kidx.push_back(ix + maxk[0]); kidx.push_back(iy + maxk[1]); kidx.push_back(iz + maxk[2]);
+          }
+        }
+      }
+    }
+    nk = (int)(kvecs.size() / 3);
+  }
+
+  // The C_d(n) = e^{i n phi_d} tables of CalcRho_ks_Fast / UpdateRho_ks.
+  void phase_tables(const double* r, std::vector<cplx> C[3]) const {
+    for (int dim = 0; dim < 3; dim++) {
+      C[dim].assign(2 * maxk[dim] + 1, cplx(0.0, 0.0));
+      double phi = r[dim] * kbox[dim];
+      cplx tempC(cos(phi), sin(phi));
+      C[dim][maxk[dim]] = 1.0;
+      for (int n = 1; n <= maxk[dim]; n++) {
+        C[dim][maxk[dim] + n] = tempC * C[dim][maxk[dim] + n - 1];
+        C[dim][maxk[dim] - n] = std::conj(C[dim][maxk[dim] + n]);
+      }
+    }
+  }
+  // PathClass::CalcRho_ks_Fast (PathClass.cc:616-647)
+  void calc_rho_fast(int mode, int slice, int species) {
+    for (int ki = 0; ki < nk; ki++) R(mode, slice, species, ki) = 0.0;
+    std::vector<cplx> C[3];
+    for (int ptcl = first[species]; ptcl < first[species] + count[species]; ptcl++) {
+      phase_tables(P(mode, slice, ptcl), C);
+      for (int ki = 0; ki < nk; ki++) {
+        const int* kI = &kidx[3 * ki];
+        R(mode, slice, species, ki) += C[0][kI[0]] * C[1][kI[1]] * C[2][kI[2]];
+      }
+    }
+  }
+  // PathClass::UpdateRho_ks() (PathClass.cc:709-723)
+  void update_rho_all() {
+    for (int slice = 0; slice < nslices; slice++)
+      for (int s = 0; s < ns; s++) calc_rho_fast(PIMC_OLDMODE, slice, s);
+    rho[1] = rho[0];
+  }
+  // PathClass::UpdateRho_ks(slice1, slice2, changed, level) (PathClass.cc:649-707)
+  void update_rho(int slice1, int slice2, const int* ptcls, int nptcls, int level) {
+    int skip = 1 << level;
+    for (int slice = slice1; slice <= slice2; slice += skip)
+      for (int s = 0; s < ns; s++)
+        for (int ki = 0; ki < nk; ki++) R(1, slice, s, ki) = R(0, slice, s, ki);
+    std::vector<cplx> C[3];
+    for (int pi = 0; pi < nptcls; pi++) {
+      int ptcl = ptcls[pi];
+      int species = species_of[ptcl];
+      for (int slice = slice1; slice <= slice2; slice += skip) {
+        phase_tables(P(1, slice, ptcl), C);
+        for (int ki = 0; ki < nk; ki++) {
+          const int* kI = &kidx[3 * ki];
+          R(1, slice, species, ki) += C[0][kI[0]] * C[1][kI[1]] * C[2][kI[2]];
+        }
+        phase_tables(P(0, slice, ptcl), C);
+        for (int ki = 0; ki < nk; ki++) {
+          const int* kI = &kidx[3 * ki];
+          R(1, slice, species, ki) -= C[0][kI[0]] * C[1][kI[1]] * C[2][kI[2]];
+        }
+      }
+    }
+  }
+};
+
+struct Ctx {
+  std::vector<std::unique_ptr<DavidFit>> fits;
+  std::vector<std::unique_ptr<Replica>> reps;
+};
+
+Replica* get_rep(Ctx* c, int id) {
+  if (!c || id < 0 || id >= (int)c->reps.size()) return nullptr;
+  return c->reps[id].get();
+}
+
+// ------------------------------------------------------------------------
+// Actions
+// ------------------------------------------------------------------------
+
+// ShortRangeClass::SingleAction (ShortRangeClass.cc:131-176)
+double short_range_action(Ctx* c, Replica& R, int mode, int slice1, int slice2, const int* ptcls,
+                          int nptcls, int level, int64_t* evals) {
+  double TotalU = 0.0;
+  std::vector<char> DoPtcl(R.N, 1);
+  int skip = 1 << level;
+  for (int i1 = 0; i1 < nptcls; i1++) {
+    int ptcl1 = ptcls[i1];
+    DoPtcl[ptcl1] = 0;
+    int species1 = R.species_of[ptcl1];
+    for (int ptcl2 = 0; ptcl2 < R.N; ptcl2++) {
+      if (DoPtcl[ptcl2]) {
+        int species2 = R.species_of[ptcl2];
+        const DavidFit& PA = *c->fits[R.pair_fit[species1 * R.ns + species2]];
+        for (int slice = slice1; slice < slice2; slice += skip) {
+          double r[3], rp[3], rmag, rpmag;
+          R.dist_disp(mode, slice, slice + skip, ptcl1, ptcl2, rmag, rpmag, r, rp, nullptr, nullptr);
+          double d0 = r[0] - rp[0], d1 = r[1] - rp[1], d2 = r[2] - rp[2];
+          double s2 = d0 * d0 + d1 * d1 + d2 * d2;
+          double q = 0.5 * (rmag + rpmag);
+          double z = (rmag - rpmag);
+          double U = PA.U(q, z, s2, level);
+          TotalU += U;
+          if (evals) ++*evals;
+        }
+      }
+    }
+  }
+  return TotalU;
+}
+
+// KineticClass::SingleAction (KineticClass.cc:33-90)
+double kinetic_action(Replica& R, int mode, int slice1, int slice2, const int* ptcls, int nptcls,
+                      int level) {
+  double TotalK = 0.0;
+  int skip = 1 << level;
+  double levelTau = R.tau * (1 << level);
+  for (int pi = 0; pi < nptcls; pi++) {
+    int ptcl = ptcls[pi];
+    int species = R.species_of[ptcl];
+    double lambda = R.lambda[species];
+    if (lambda != 0) {
+      double FourLambdaTauInv = 1.0 / (4.0 * lambda * levelTau);
+      for (int slice = slice1; slice < slice2; slice += skip) {
+        double vel[3];
+        R.velocity(mode, slice, slice + skip, ptcl, vel);
+        double GaussProd = 1.0;
+        for (int dim = 0; dim < 3; dim++) {
+          double GaussSum = 0.0;
+          for (int image = -R.num_images; image <= R.num_images; image++) {
+            double dist = vel[dim] + (double)image * R.box[dim];
+            GaussSum += exp(-dist * dist * FourLambdaTauInv);
+          }
+          GaussProd *= GaussSum;
+        }
+        TotalK -= log(GaussProd);
+      }
+    }
+  }
+  return TotalK;
+}
+
+inline double mag2(const cplx& z) { return z.real() * z.real() + z.imag() * z.imag(); }
+
+// LongRangeClass::SingleAction (LongRangeClass.cc:273-371), without its debug output
+double long_range_action(Replica& R, int mode, int slice1, int slice2, const int* ptcls, int nptcls,
+                         int level, int64_t* kevals) {
+  if (R.nk == 0) return 0.0;
+  if (mode == PIMC_NEWMODE) R.update_rho(slice1, slice2, ptcls, nptcls, level);
+  int skip = 1 << level;
+  double homo = 0.0, hetero = 0.0, background = 0.0, k0Homo = 0.0, k0Hetero = 0.0;
+  for (int slice = slice1; slice <= slice2; slice += skip) {
+    double factor = ((slice == slice1) || (slice == slice2)) ? 0.5 : 1.0;
+    for (int species = 0; species < R.ns; species++) {
+      const LongRangeTables& pa = R.lr[species * R.ns + species];
+      if (pa.set) {
+        for (int ki = 0; ki < R.nk; ki++) {
+          double rhok2 = mag2(R.R(mode, slice, species, ki));
+          homo += factor * 0.5 * 2.0 * rhok2 * pa.ulong_k[(size_t)level * R.nk + ki];
+        }
+        if (kevals) *kevals += R.nk;
+        int N = R.count[species];
+        homo -= factor * 0.5 * N * pa.ulong_r0[level];
+        background -= factor * 0.5 * N * N * pa.ushort_k0[level];
+        k0Homo += factor * 0.5 * N * N * pa.ulong_k0[level];
+      }
+    }
+    for (int s1 = 0; s1 < R.ns; s1++)
+      for (int s2 = s1 + 1; s2 < R.ns; s2++) {
+        const LongRangeTables& pa = R.lr[s1 * R.ns + s2];
+        if (pa.set) {
+          for (int ki = 0; ki < R.nk; ki++) {
+            const cplx& a = R.R(mode, slice, s1, ki);
+            const cplx& b = R.R(mode, slice, s2, ki);
+            double rhorho = a.real() * b.real() + a.imag() * b.imag();
+            hetero += factor * 2.0 * rhorho * pa.ulong_k[(size_t)level * R.nk + ki];
+          }
+          if (kevals) *kevals += R.nk;
+          int N1 = R.count[s1], N2 = R.count[s2];
+          background -= factor * N1 * N2 * pa.ushort_k0[level];
+          k0Hetero += factor * N1 * N2 * pa.ulong_k0[level];
+        }
+      }
+  }
+  double U = homo + hetero;
+  if (R.use_background)
+    U += background;
+  else
+    U += (k0Homo + k0Hetero);
+  return U;
+}
+
+bool has_long_range(const Replica& R) {
+  if (!R.use_long_range || R.nk == 0) return false;
+  for (const auto& t : R.lr)
+    if (t.set) return true;
+  return false;
+}
+
+// PathClass::AcceptCopy / RejectCopy (PathClass.cc:797-899)
+void accept_copy(Replica& R, int slice1, int slice2, const int* ptcls, int nptcls) {
+  for (int pi = 0; pi < nptcls; pi++) {
+    int ptcl = ptcls[pi];
+    for (int slice = slice1; slice <= slice2; slice++) R.put_in_box_fast(R.P(1, slice, ptcl));
+    for (int slice = slice1; slice <= slice2; slice++)
+      memcpy(R.P(0, slice, ptcl), R.P(1, slice, ptcl), 3 * sizeof(double));
+    R.perm[0][ptcl] = R.perm[1][ptcl];
+  }
+  if (R.nk > 0)
+    for (int slice = slice1; slice <= slice2; slice++)
+      for (int s = 0; s < R.ns; s++)
+        for (int k = 0; k < R.nk; k++) R.R(0, slice, s, k) = R.R(1, slice, s, k);
+}
+void reject_copy(Replica& R, int slice1, int slice2, const int* ptcls, int nptcls) {
+  for (int pi = 0; pi < nptcls; pi++) {
+    int ptcl = ptcls[pi];
+    for (int slice = slice1; slice <= slice2; slice++)
+      memcpy(R.P(1, slice, ptcl), R.P(0, slice, ptcl), 3 * sizeof(double));
+    R.perm[1][ptcl] = R.perm[0][ptcl];
+  }
+  if (R.nk > 0)
+    for (int slice = slice1; slice <= slice2; slice++)
+      for (int s = 0; s < R.ns; s++)
+        for (int k = 0; k < R.nk; k++) R.R(1, slice, s, k) = R.R(0, slice, s, k);
+}
+// PathClass::MoveJoin (PathClass.cc:727-793), closed paths
+void move_join(Replica& R, int oldJoin, int newJoin) {
+  if (newJoin > oldJoin) {
+    for (int t = oldJoin + 1; t <= newJoin; t++)
+      for (int p = 0; p < R.N; p++) memcpy(R.P(0, t, p), R.P(1, t, R.perm[1][p]), 3 * sizeof(double));
+    for (int t = oldJoin + 1; t <= newJoin; t++)
+      for (int p = 0; p < R.N; p++) memcpy(R.P(1, t, p), R.P(0, t, p), 3 * sizeof(double));
+  } else if (oldJoin > newJoin) {
+    for (int t = newJoin + 1; t <= oldJoin; t++)
+      for (int p = 0; p < R.N; p++) memcpy(R.P(0, t, R.perm[1][p]), R.P(1, t, p), 3 * sizeof(double));
+    for (int t = newJoin + 1; t <= oldJoin; t++)
+      for (int p = 0; p < R.N; p++) memcpy(R.P(1, t, p), R.P(0, t, p), 3 * sizeof(double));
+  }
+}
+
+// ------------------------------------------------------------------------
+// Bisection move (BisectionBlockClass::MakeMove with NoPermuteStage)
+// ------------------------------------------------------------------------
+
+// BisectionStageClass::Sample (BisectionStage.cc:128-289), no correlated sampling.
+double bisection_sample(Replica& R, int slice1, int num_levels, int level, const int* ptcls,
+                        int nptcls, uint64_t seed, uint32_t move) {
+  int skip = 1 << (level + 1);
+  double levelTau = 0.5 * R.tau * skip;
+  int numImages = R.num_images;
+  int slice2 = slice1 + (1 << num_levels);
+  int nbeads = (slice2 - slice1) / skip;
+  double logSampleProb = 0.0;
+  double logOldSampleProb = 0.0;
+  double gpair[2];
+  int cached_pair = -1;
+  for (int pi = 0; pi < nptcls; pi++) {
+    int ptcl = ptcls[pi];
+    double lambda = R.lambda[R.species_of[ptcl]];
+    double sigma2 = (1.0 * lambda * levelTau);
+    double sigma = sqrt(sigma2);
+    double prefactorOfSampleProb = 0.0;
+    int b = 0;
+    for (int slice = slice1; slice < slice2; slice += skip, b++) {
+      double rOld[3], rdiffOld[3], rbarOld[3], DeltaOld[3];
+      memcpy(rOld, R.P(0, slice, ptcl), sizeof rOld);
+      R.velocity(0, slice, slice + skip, ptcl, rdiffOld);
+      for (int d = 0; d < 3; d++) rbarOld[d] = rOld[d] + 0.5 * rdiffOld[d];
+      const double* rppOld = R.P(0, slice + (skip >> 1), ptcl);
+      for (int d = 0; d < 3; d++) DeltaOld[d] = rppOld[d] - rbarOld[d];
+      R.put_in_box(DeltaOld);
+
+      double r[3], rdiff[3], rbar[3];
+      memcpy(r, R.P(1, slice, ptcl), sizeof r);
+      R.velocity(1, slice, slice + skip, ptcl, rdiff);
+      for (int d = 0; d < 3; d++) rbar[d] = r[d] + 0.5 * rdiff[d];
+
+      double Delta[3];
+      for (int d = 0; d < 3; d++) {
+        uint32_t gi = (uint32_t)(3 * (pi * nbeads + b) + d);
+        uint32_t pair = gi >> 1;
+        if ((int)pair != cached_pair) {
+          gaussian_pair(seed, R.stream, move, level, pair, gpair);
+          cached_pair = (int)pair;
+        }
+        Delta[d] = gpair[gi & 1] * sigma;  // LocalGaussian(sigma) = X1 * sigma
+      }
+      R.put_in_box(Delta);
+      double GaussProd = 1.0, GaussProdOld = 1.0;
+      for (int dim = 0; dim < 3; dim++) {
+        double GaussSum = 0.0, GaussSumOld = 0.0;
+        for (int image = -numImages; image <= numImages; image++) {
+          double dist = Delta[dim] + (double)image * R.box[dim];
+          double distOld = DeltaOld[dim] + (double)image * R.box[dim];
+          GaussSum += exp(-0.5 * dist * dist / sigma2);
+          GaussSumOld += exp(-0.5 * distOld * distOld / sigma2);
+        }
+        GaussProd *= GaussSum;
+        GaussProdOld *= GaussSumOld;
+      }
+      logOldSampleProb += prefactorOfSampleProb + log(GaussProdOld);
+      logSampleProb += prefactorOfSampleProb + log(GaussProd);
+      double* rpp = R.P(1, slice + (skip >> 1), ptcl);
+      for (int d = 0; d < 3; d++) rpp[d] = rbar[d] + Delta[d];
+    }
+  }
+  return -logSampleProb + logOldSampleProb;
+}
+
+// StageClass::StageAction with [Kinetic, ShortRange, (LongRange at the lowest level)].
+double stage_action(Ctx* c, Replica& R, int mode, int slice1, int slice2, const int* ptcls,
+                    int nptcls, int level, bool with_long_range, pimc_bisection_stats* st) {
+  double TotalAction = 0.0;
+  TotalAction += kinetic_action(R, mode, slice1, slice2, ptcls, nptcls, level);
+  int64_t evals = 0, kev = 0;
+  TotalAction += short_range_action(c, R, mode, slice1, slice2, ptcls, nptcls, level, &evals);
+  if (with_long_range) TotalAction += long_range_action(R, mode, slice1, slice2, ptcls, nptcls, level, &kev);
+  if (st) {
+    st->pair_evals += evals;
+    st->k_evals += kev;
+  }
+  return TotalAction;
+}
+
+void bisection_move(Ctx* c, Replica& R, const pimc_bisection_desc& bd, uint32_t move,
+                    pimc_move_record* rec, pimc_bisection_stats* st) {
+  const int L = bd.num_levels;
+  // BisectionBlockClass::ChooseTimeSlices (BisectionBlock.cc:363-374)
+  int sliceSep = 1 << L;
+  int numLeft = R.nslices - sliceSep;
+  int slice1 = (int)floor(rng_uniform(bd.seed, R.stream, move, TAG_SLICE, 0) * numLeft);
+  int slice2 = slice1 + sliceSep;
+  move_join(R, R.join, slice2);
+  R.join = slice2;
+  // NoPermuteStageClass::ChooseParticle (NoPermuteClass.cc:61-68)
+  int sp = bd.species;
+  int ptcl = (int)floor(rng_uniform(bd.seed, R.stream, move, TAG_PTCL, 0) * R.count[sp]) + R.first[sp];
+  int ptcls[1] = {ptcl};
+  bool lr = has_long_range(R);
+  if (rec) {
+    memset(rec, 0, sizeof *rec);
+    rec->slice1 = slice1;
+    rec->ptcl = ptcl;
+  }
+  // MultiStageClass::MakeMove (MultiStage.cc:42-86) + LocalStageClass::Attempt (StageClass.cc:49-95)
+  bool toAccept = true;
+  double prevActionChange = 0.0;
+  int stage = 0;
+  for (int level = L - 1; level >= bd.lowest_level && toAccept; level--, stage++) {
+    double logSampleRatio = bisection_sample(R, slice1, L, level, ptcls, 1, bd.seed, move);
+    bool with_lr = lr && (level == bd.lowest_level);
+    double oldAction = stage_action(c, R, PIMC_OLDMODE, slice1, slice2, ptcls, 1, level, with_lr, st);
+    double newAction = stage_action(c, R, PIMC_NEWMODE, slice1, slice2, ptcls, 1, level, with_lr, st);
+    double currActionChange = newAction - oldAction;
+    double logAcceptProb = logSampleRatio - currActionChange + prevActionChange;
+    double ran = rng_uniform(bd.seed, R.stream, move, TAG_ACCEPT, level);
+    double logu = log(ran);
+    toAccept = logAcceptProb >= logu;
+    prevActionChange = currActionChange;
+    if (st) st->stages++;
+    if (rec) {
+      rec->old_action[stage] = oldAction;
+      rec->new_action[stage] = newAction;
+      rec->log_sample_ratio[stage] = logSampleRatio;
+      rec->log_u[stage] = logu;
+      rec->num_stages = stage + 1;
+    }
+  }
+  if (toAccept)
+    accept_copy(R, slice1, slice2, ptcls, 1);
+  else
+    reject_copy(R, slice1, slice2, ptcls, 1);
+  if (rec) rec->accepted = toAccept ? 1 : 0;
+  if (st) {
+    st->moves++;
+    if (toAccept) st->accepted++;
+  }
+}
+
+int check_req(Ctx* c, const pimc_action_req& q, Replica** out) {
+  Replica* R = get_rep(c, q.replica);
+  if (!R) return fail(PIMC_ERR_ARG, "action request: bad replica id");
+  if (q.mode != PIMC_OLDMODE && q.mode != PIMC_NEWMODE) return fail(PIMC_ERR_ARG, "action request: bad mode");
+  if (q.level < 0 || q.level >= PIMC_MAX_LEVELS) return fail(PIMC_ERR_ARG, "action request: bad level");
+  if (q.slice1 < 0 || q.slice2 > R->M || q.slice1 > q.slice2) return fail(PIMC_ERR_ARG, "action request: bad slice range");
+  const int skip = 1 << q.level;
+  if (q.slice2 > q.slice1 && q.slice1 + ((q.slice2 - q.slice1 + skip - 1) / skip) * skip > R->M)
+    return fail(PIMC_ERR_ARG, "action request: slice range runs past the last slice at this level");
+  if (q.nptcls < 1 || q.nptcls > PIMC_MAX_ACTIVE) return fail(PIMC_ERR_ARG, "action request: bad particle count");
+  for (int i = 0; i < q.nptcls; i++)
+    if (q.ptcls[i] < 0 || q.ptcls[i] >= R->N) return fail(PIMC_ERR_ARG, "action request: bad particle index");
+  *out = R;
+  return PIMC_OK;
+}
+
+}  // namespace
+
+// ==========================================================================
+// C ABI
+// ==========================================================================
+struct pimc_ctx {
+  Ctx c;
+};
+
+extern "C" {
+
+int pimc_abi_version(void) { return PIMC_ABI_VERSION; }
+const char* pimc_last_error(void) { return g_err.c_str(); }
+
+int pimc_ctx_create(int32_t device, pimc_ctx** out) {
+  (void)device;
+  if (!out) return fail(PIMC_ERR_ARG, "ctx_create: out is NULL");
+  *out = new pimc_ctx();
+  return PIMC_OK;
+}
+int pimc_ctx_destroy(pimc_ctx* ctx) {
+  delete ctx;
+  return PIMC_OK;
+}
+int pimc_ctx_synchronize(pimc_ctx* ctx) { return ctx ? PIMC_OK : fail(PIMC_ERR_ARG, "NULL ctx"); }
+
+int pimc_fit_create_david(pimc_ctx* ctx, const pimc_david_fit_desc* d, int32_t* fit_id) {
+  if (!ctx || !d || !fit_id || !d->y) return fail(PIMC_ERR_ARG, "fit_create_david: NULL argument");
+  if (d->num_points < 3 || d->num_tau < 1 || d->num_cols < 2) return fail(PIMC_ERR_ARG, "fit_create_david: bad sizes");
+  int nt = d->num_terms;
+  if (nt < 0 || (nt > 0 && d->num_cols < nt * (nt + 1) / 2 + nt + 2))
+    return fail(PIMC_ERR_ARG, "fit_create_david: num_cols too small for num_terms");
+  if (!(d->grid_start < d->grid_end)) return fail(PIMC_ERR_ARG, "fit_create_david: bad grid range");
+  if (d->grid_type == PIMC_GRID_LOG && !(d->grid_start > 0)) return fail(PIMC_ERR_ARG, "fit_create_david: LOG grid needs start > 0");
+  auto f = std::make_unique<DavidFit>();
+  if (d->grid_type == PIMC_GRID_LINEAR)
+    f->grid.init_linear(d->grid_start, d->grid_end, d->num_points);
+  else if (d->grid_type == PIMC_GRID_LOG)
+    f->grid.init_log_dm(d->grid_start, d->grid_end, d->num_points);
+  else
+    return fail(PIMC_ERR_ARG, "fit_create_david: bad grid type");
+  f->num_tau = d->num_tau;
+  f->ncols = d->num_cols;
+  f->num_terms = nt;
+  f->smallest_tau = d->smallest_tau;
+  f->temp.resize(d->num_cols);
+  std::vector<double> sd(d->num_cols, 5.0e30), ed(d->num_cols, 0.0);
+  f->ukj.resize(d->num_tau);
+  for (int l = 0; l < d->num_tau; l++)
+    f->ukj[l].init(&f->grid, d->y + (size_t)l * d->num_points * d->num_cols, d->num_cols, sd.data(), ed.data());
+  // TauPos (DavidPAClass.cc:859-868)
+  f->taupos = -1;
+  double tau = d->smallest_tau;
+  for (int i = 0; i < d->num_tau; i++) {
+    if (fabs(tau - d->desired_tau) < 1e-4) f->taupos = i;
+    tau = tau * 2;
+  }
+  if (f->taupos < 0) return fail(PIMC_ERR_ARG, "fit_create_david: desired_tau matches no tau level");
+  ctx->c.fits.push_back(std::move(f));
+  *fit_id = (int32_t)ctx->c.fits.size() - 1;
+  return PIMC_OK;
+}
+
+int pimc_fit_eval_u(pimc_ctx* ctx, int32_t fit_id, int32_t level, int64_t n, const double* q,
+                    const double* z, const double* s2, int32_t precision, double* out) {
+  (void)precision;
+  if (!ctx || fit_id < 0 || fit_id >= (int)ctx->c.fits.size()) return fail(PIMC_ERR_ARG, "fit_eval_u: bad fit");
+  const DavidFit& f = *ctx->c.fits[fit_id];
+  if (level < 0 || level + f.taupos >= f.num_tau) return fail(PIMC_ERR_ARG, "fit_eval_u: level out of range");
+  for (int64_t i = 0; i < n; i++) out[i] = f.U(q[i], z[i], s2[i], level);
+  return PIMC_OK;
+}
+
+int pimc_replica_create(pimc_ctx* ctx, const pimc_replica_desc* d, int32_t* replica_id) {
+  if (!ctx || !d || !replica_id) return fail(PIMC_ERR_ARG, "replica_create: NULL argument");
+  if (d->num_particles < 1 || d->num_time_slices < 1 || d->num_species < 1 || d->num_species > PIMC_MAX_SPECIES)
+    return fail(PIMC_ERR_ARG, "replica_create: bad sizes");
+  auto R = std::make_unique<Replica>();
+  R->N = d->num_particles;
+  R->M = d->num_time_slices;
+  R->nslices = R->M + 1;
+  R->ns = d->num_species;
+  R->num_images = d->num_images;
+  R->tau = d->tau;
+  for (int i = 0; i < 3; i++) {
+    R->box[i] = d->box[i];
+    R->boxinv[i] = 1.0 / d->box[i];
+    R->isperiodic[i] = d->periodic[i] ? 1.0 : 0.0;
+  }
+  int tot = 0;
+  for (int s = 0; s < R->ns; s++) {
+    R->first.push_back(tot);
+    R->count.push_back(d->species_count[s]);
+    for (int i = 0; i < d->species_count[s]; i++) R->species_of.push_back(s);
+    tot += d->species_count[s];
+    R->lambda.push_back(d->lambda[s]);
+  }
+  if (tot != R->N) return fail(PIMC_ERR_ARG, "replica_create: species counts do not sum to num_particles");
+  R->pair_fit.assign(d->pair_fit, d->pair_fit + R->ns * R->ns);
+  for (int f : R->pair_fit)
+    if (f < 0 || f >= (int)ctx->c.fits.size()) return fail(PIMC_ERR_ARG, "replica_create: bad pair fit id");
+  size_t nb = (size_t)R->nslices * R->N * 3;
+  R->path[0].assign(nb, 0.0);
+  R->path[1].assign(nb, 0.0);
+  R->perm[0].resize(R->N);
+  R->perm[1].resize(R->N);
+  for (int p = 0; p < R->N; p++) R->perm[0][p] = R->perm[1][p] = p;
+  R->kcutoff = d->k_cutoff;
+  R->use_long_range = d->use_long_range != 0;
+  R->use_background = d->use_background != 0;
+  R->stream = d->rng_stream;
+  if (d->k_cutoff > 0) {
+    R->long_range = true;
+    R->setup_kvecs();
+    R->rho[0].assign((size_t)R->nslices * R->ns * R->nk, cplx(0, 0));
+    R->rho[1] = R->rho[0];
+  }
+  R->lr.resize(R->ns * R->ns);
+  ctx->c.reps.push_back(std::move(R));
+  *replica_id = (int32_t)ctx->c.reps.size() - 1;
+  return PIMC_OK;
+}
+
+int pimc_replica_num_kvecs(pimc_ctx* ctx, int32_t replica, int32_t* nk) {
+  Replica* R = get_rep(&ctx->c, replica);
+  if (!R || !nk) return fail(PIMC_ERR_ARG, "replica_num_kvecs: bad argument");
+  *nk = R->nk;
+  return PIMC_OK;
+}
+int pimc_replica_kvecs(pimc_ctx* ctx, int32_t replica, double* kvecs, int32_t* kidx) {
+  Replica* R = get_rep(&ctx->c, replica);
+  if (!R) return fail(PIMC_ERR_ARG, "replica_kvecs: bad replica");
+  if (kvecs) memcpy(kvecs, R->kvecs.data(), R->kvecs.size() * sizeof(double));
+  if (kidx) memcpy(kidx, R->kidx.data(), R->kidx.size() * sizeof(int32_t));
+  return PIMC_OK;
+}
+
+int pimc_replica_set_long_range(pimc_ctx* ctx, int32_t replica, int32_t sa, int32_t sb,
+                                int32_t num_levels, const double* ulong_k, const double* ulong_r0,
+                                const double* ushort_k0, const double* ulong_k0) {
+  Replica* R = get_rep(&ctx->c, replica);
+  if (!R || sa < 0 || sb < 0 || sa >= R->ns || sb >= R->ns || num_levels < 1 || !ulong_k || !ulong_r0 ||
+      !ushort_k0 || !ulong_k0)
+    return fail(PIMC_ERR_ARG, "replica_set_long_range: bad argument");
+  if (R->nk == 0) return fail(PIMC_ERR_STATE, "replica_set_long_range: replica has no k set");
+  LongRangeTables t;
+  t.set = true;
+  t.num_levels = num_levels;
+  t.ulong_k.assign(ulong_k, ulong_k + (size_t)num_levels * R->nk);
+  t.ulong_r0.assign(ulong_r0, ulong_r0 + num_levels);
+  t.ushort_k0.assign(ushort_k0, ushort_k0 + num_levels);
+  t.ulong_k0.assign(ulong_k0, ulong_k0 + num_levels);
+  R->lr[sa * R->ns + sb] = t;
+  R->lr[sb * R->ns + sa] = t;
+  return PIMC_OK;
+}
+
+int pimc_path_set(pimc_ctx* ctx, int32_t replica, int32_t mode, const double* beads) {
+  Replica* R = get_rep(&ctx->c, replica);
+  if (!R || !beads || mode < 0 || mode > 2) return fail(PIMC_ERR_ARG, "path_set: bad argument");
+  for (int m = 0; m < 2; m++)
+    if (mode == 2 || mode == m) memcpy(R->path[m].data(), beads, R->path[m].size() * sizeof(double));
+  return PIMC_OK;
+}
+int pimc_path_get(pimc_ctx* ctx, int32_t replica, int32_t mode, double* beads) {
+  Replica* R = get_rep(&ctx->c, replica);
+  if (!R || !beads || mode < 0 || mode > 1) return fail(PIMC_ERR_ARG, "path_get: bad argument");
+  memcpy(beads, R->path[mode].data(), R->path[mode].size() * sizeof(double));
+  return PIMC_OK;
+}
+int pimc_path_set_beads(pimc_ctx* ctx, int32_t replica, int32_t mode, int32_t ptcl, int32_t slice1,
+                        int32_t slice2, const double* xyz) {
+  Replica* R = get_rep(&ctx->c, replica);
+  if (!R || !xyz || mode < 0 || mode > 2 || ptcl < 0 || ptcl >= R->N || slice1 < 0 || slice2 > R->M || slice1 > slice2)
+    return fail(PIMC_ERR_ARG, "path_set_beads: bad argument");
+  for (int m = 0; m < 2; m++)
+    if (mode == 2 || mode == m)
+      for (int s = slice1; s <= slice2; s++) memcpy(R->P(m, s, ptcl), xyz + 3 * (s - slice1), 3 * sizeof(double));
+  return PIMC_OK;
+}
+int pimc_permutation_get(pimc_ctx* ctx, int32_t replica, int32_t mode, int32_t* perm) {
+  Replica* R = get_rep(&ctx->c, replica);
+  if (!R || !perm || mode < 0 || mode > 1) return fail(PIMC_ERR_ARG, "permutation_get: bad argument");
+  memcpy(perm, R->perm[mode].data(), R->N * sizeof(int32_t));
+  return PIMC_OK;
+}
+
+static int check_copy_args(Replica* R, int32_t slice1, int32_t slice2, const int32_t* ptcls, int32_t n) {
+  if (!R || !ptcls || n < 0 || slice1 < 0 || slice2 > R->M || slice1 > slice2)
+    return fail(PIMC_ERR_ARG, "accept/reject copy: bad argument");
+  for (int i = 0; i < n; i++)
+    if (ptcls[i] < 0 || ptcls[i] >= R->N) return fail(PIMC_ERR_ARG, "accept/reject copy: bad particle");
+  return PIMC_OK;
+}
+int pimc_accept_copy(pimc_ctx* ctx, int32_t replica, int32_t slice1, int32_t slice2, const int32_t* ptcls,
+                     int32_t nptcls) {
+  Replica* R = get_rep(&ctx->c, replica);
+  int rc = check_copy_args(R, slice1, slice2, ptcls, nptcls);
+  if (rc) return rc;
+  accept_copy(*R, slice1, slice2, ptcls, nptcls);
+  return PIMC_OK;
+}
+int pimc_reject_copy(pimc_ctx* ctx, int32_t replica, int32_t slice1, int32_t slice2, const int32_t* ptcls,
+                     int32_t nptcls) {
+  Replica* R = get_rep(&ctx->c, replica);
+  int rc = check_copy_args(R, slice1, slice2, ptcls, nptcls);
+  if (rc) return rc;
+  reject_copy(*R, slice1, slice2, ptcls, nptcls);
+  return PIMC_OK;
+}
+int pimc_move_join(pimc_ctx* ctx, int32_t replica, int32_t old_join, int32_t new_join) {
+  Replica* R = get_rep(&ctx->c, replica);
+  if (!R || old_join < 0 || new_join < 0 || old_join > R->M || new_join > R->M)
+    return fail(PIMC_ERR_ARG, "move_join: bad argument");
+  move_join(*R, old_join, new_join);
+  R->join = new_join;
+  return PIMC_OK;
+}
+
+int pimc_rho_k_update_all(pimc_ctx* ctx, int32_t replica) {
+  Replica* R = get_rep(&ctx->c, replica);
+  if (!R) return fail(PIMC_ERR_ARG, "rho_k_update_all: bad replica");
+  if (R->nk == 0) return fail(PIMC_ERR_STATE, "rho_k_update_all: replica has no k set");
+  R->update_rho_all();
+  return PIMC_OK;
+}
+int pimc_rho_k_get(pimc_ctx* ctx, int32_t replica, int32_t mode, double* rho) {
+  Replica* R = get_rep(&ctx->c, replica);
+  if (!R || !rho || mode < 0 || mode > 1) return fail(PIMC_ERR_ARG, "rho_k_get: bad argument");
+  memcpy(rho, R->rho[mode].data(), R->rho[mode].size() * sizeof(cplx));
+  return PIMC_OK;
+}
+
+int pimc_action_short_range(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t nreq, int32_t precision,
+                            double* out) {
+  (void)precision;
+  if (!ctx || (nreq > 0 && (!reqs || !out))) return fail(PIMC_ERR_ARG, "action_short_range: NULL argument");
+  for (int i = 0; i < nreq; i++) {
+    Replica* R;
+    int rc = check_req(&ctx->c, reqs[i], &R);
+    if (rc) return rc;
+    out[i] = short_range_action(&ctx->c, *R, reqs[i].mode, reqs[i].slice1, reqs[i].slice2, reqs[i].ptcls,
+                                reqs[i].nptcls, reqs[i].level, nullptr);
+  }
+  return PIMC_OK;
+}
+int pimc_action_kinetic(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t nreq, double* out) {
+  if (!ctx || (nreq > 0 && (!reqs || !out))) return fail(PIMC_ERR_ARG, "action_kinetic: NULL argument");
+  for (int i = 0; i < nreq; i++) {
+    Replica* R;
+    int rc = check_req(&ctx->c, reqs[i], &R);
+    if (rc) return rc;
+    out[i] = kinetic_action(*R, reqs[i].mode, reqs[i].slice1, reqs[i].slice2, reqs[i].ptcls, reqs[i].nptcls,
+                            reqs[i].level);
+  }
+  return PIMC_OK;
+}
+int pimc_action_long_range(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t nreq, int32_t precision,
+                           double* out) {
+  (void)precision;
+  if (!ctx || (nreq > 0 && (!reqs || !out))) return fail(PIMC_ERR_ARG, "action_long_range: NULL argument");
+  for (int i = 0; i < nreq; i++) {
+    Replica* R;
+    int rc = check_req(&ctx->c, reqs[i], &R);
+    if (rc) return rc;
+    if (R->nk == 0) return fail(PIMC_ERR_STATE, "action_long_range: replica has no k set");
+    out[i] = long_range_action(*R, reqs[i].mode, reqs[i].slice1, reqs[i].slice2, reqs[i].ptcls, reqs[i].nptcls,
+                               reqs[i].level, nullptr);
+  }
+  return PIMC_OK;
+}
+
+int pimc_dist_disp(pimc_ctx* ctx, int32_t replica, int32_t mode, int64_t n, const int32_t* idx, double* out,
+                   int32_t* images) {
+  Replica* R = get_rep(&ctx->c, replica);
+  if (!R || !idx || !out || mode < 0 || mode > 1) return fail(PIMC_ERR_ARG, "dist_disp: bad argument");
+  for (int64_t i = 0; i < n; i++) {
+    const int32_t* q = idx + 4 * i;
+    if (q[0] < 0 || q[0] > R->M || q[1] < 0 || q[1] > R->M || q[2] < 0 || q[2] >= R->N || q[3] < 0 || q[3] >= R->N)
+      return fail(PIMC_ERR_ARG, "dist_disp: index out of range");
+    double* o = out + 8 * i;
+    double nimg[3], mimg[3];
+    R->dist_disp(mode, q[0], q[1], q[2], q[3], o[0], o[1], o + 2, o + 5, nimg, mimg);
+    if (images)
+      for (int d = 0; d < 3; d++) {
+        images[6 * i + d] = (int32_t)nimg[d];
+        images[6 * i + 3 + d] = (int32_t)mimg[d];
+      }
+  }
+  return PIMC_OK;
+}
+
+int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, const pimc_bisection_desc* desc,
+                       int64_t first_move, int32_t num_moves, int32_t precision, pimc_move_record* records,
+                       pimc_bisection_stats* stats) {
+  (void)precision;
+  if (!ctx || !replicas || !desc || nrep < 0 || num_moves < 0 || first_move < 0 ||
+      first_move + num_moves > (int64_t)0xffffffffLL)
+    return fail(PIMC_ERR_ARG, "bisection_run: bad argument");
+  if (desc->num_levels < 1 || desc->num_levels > PIMC_MAX_LEVELS || desc->lowest_level < 0 ||
+      desc->lowest_level >= desc->num_levels)
+    return fail(PIMC_ERR_ARG, "bisection_run: bad levels");
+  for (int r = 0; r < nrep; r++) {
+    Replica* R = get_rep(&ctx->c, replicas[r]);
+    if (!R) return fail(PIMC_ERR_ARG, "bisection_run: bad replica");
+    if ((1 << desc->num_levels) >= R->nslices) return fail(PIMC_ERR_ARG, "bisection_run: too few slices for num_levels");
+    if (desc->species < 0 || desc->species >= R->ns) return fail(PIMC_ERR_ARG, "bisection_run: bad species");
+    for (int s = 0; s < R->ns; s++)
+      for (int t = 0; t < R->ns; t++) {
+        const DavidFit& f = *ctx->c.fits[R->pair_fit[s * R->ns + t]];
+        if (f.taupos + desc->num_levels > f.num_tau)
+          return fail(PIMC_ERR_ARG, "bisection_run: pair fit has too few tau levels");
+      }
+  }
+  pimc_bisection_stats st;
+  memset(&st, 0, sizeof st);
+  auto t0 = std::chrono::steady_clock::now();
+  for (int r = 0; r < nrep; r++) {
+    Replica& R = *get_rep(&ctx->c, replicas[r]);
+    for (int m = 0; m < num_moves; m++) {
+      pimc_move_record* rec = records ? records + (size_t)r * num_moves + m : nullptr;
+      bisection_move(&ctx->c, R, *desc, (uint32_t)(first_move + m), rec, &st);
+    }
+  }
+  st.device_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (stats) *stats = st;
+  return PIMC_OK;
+}
+
+// ---------------- oracle-only hooks ----------------
+void oracle_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out) { philox(ctr, key, out); }
+double oracle_uniform(uint32_t w0, uint32_t w1) { return uniform53(w0, w1); }
+int oracle_gaussian_pair(uint64_t seed, uint32_t stream, uint32_t move, int32_t level, uint32_t pair, double* g) {
+  return gaussian_pair(seed, stream, move, level, pair, g);
+}
+
+static int make_grid(int32_t type, double start, double end, int32_t npoints, Grid& g) {
+  if (npoints < 2 || !(start < end)) return fail(PIMC_ERR_ARG, "grid: bad arguments");
+  if (type == PIMC_GRID_LINEAR)
+    g.init_linear(start, end, npoints);
+  else if (type == PIMC_GRID_LOG && start > 0)
+    g.init_log_dm(start, end, npoints);
+  else
+    return fail(PIMC_ERR_ARG, "grid: bad type");
+  return PIMC_OK;
+}
+int oracle_reverse_map(int32_t type, double start, double end, int32_t npoints, int32_t literal, int64_t n,
+                       const double* r, int32_t* out) {
+  Grid g;
+  int rc = make_grid(type, start, end, npoints, g);
+  if (rc) return rc;
+  for (int64_t i = 0; i < n; i++) out[i] = literal ? g.reverse_map_literal(r[i]) : g.reverse_map(r[i]);
+  return PIMC_OK;
+}
+int oracle_grid_points(int32_t type, double start, double end, int32_t npoints, double* x) {
+  Grid g;
+  int rc = make_grid(type, start, end, npoints, g);
+  if (rc) return rc;
+  memcpy(x, g.x.data(), npoints * sizeof(double));
+  return PIMC_OK;
+}
+int oracle_multicubic(int32_t type, double start, double end, int32_t npoints, int32_t ncols, const double* y,
+                      const double* start_deriv, const double* end_deriv, double* d2y, int32_t col, int64_t n,
+                      const double* x, double* out) {
+  Grid g;
+  int rc = make_grid(type, start, end, npoints, g);
+  if (rc) return rc;
+  MultiCubicSpline s;
+  s.init(&g, y, ncols, start_deriv, end_deriv);
+  if (d2y) memcpy(d2y, s.d2y.data(), s.d2y.size() * sizeof(double));
+  for (int64_t i = 0; i < n; i++) {
+    if (col >= 0)
+      out[i] = s.eval(col, x[i]);
+    else
+      s.eval_all(x[i], out + i * ncols);
+  }
+  return PIMC_OK;
+}
+int oracle_bisection_sample(void* ctx, int32_t replica, int32_t ptcl, int32_t slice1, int32_t num_levels,
+                            int32_t level, uint64_t seed, uint32_t move, double* log_sample_ratio) {
+  pimc_ctx* c = (pimc_ctx*)ctx;
+  Replica* R = get_rep(&c->c, replica);
+  if (!R) return fail(PIMC_ERR_ARG, "bisection_sample: bad replica");
+  int p[1] = {ptcl};
+  *log_sample_ratio = bisection_sample(*R, slice1, num_levels, level, p, 1, seed, move);
+  return PIMC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,405 @@
+"""ctypes mirror of include/pimc_gpu.h and a thin numpy-facing wrapper.
+
+`PimcLib` wraps any shared library that exports the pimc_* C ABI.  The product
+loads the HIP library (`pimcplusplus_amd.native.load`); nothing here loads a
+library by itself.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+PIMC_ABI_VERSION = 1
+MAX_ACTIVE = 4
+MAX_LEVELS = 8
+MAX_SPECIES = 4
+
+OK, ERR_ARG, ERR_STATE, ERR_DEVICE, ERR_NOMEM, ERR_UNSUPPORTED = range(6)
+OLDMODE, NEWMODE, BOTHMODES = 0, 1, 2
+FP64, MIXED = 0, 1
+GRID_LINEAR, GRID_LOG = 0, 1
+
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+
+
+class DavidFitDesc(C.Structure):
+    _fields_ = [
+        ("grid_type", C.c_int32),
+        ("num_points", C.c_int32),
+        ("grid_start", C.c_double),
+        ("grid_end", C.c_double),
+        ("num_tau", C.c_int32),
+        ("num_cols", C.c_int32),
+        ("num_terms", C.c_int32),
+        ("reserved", C.c_int32),
+        ("smallest_tau", C.c_double),
+        ("desired_tau", C.c_double),
+        ("y", _f64p),
+    ]
+
+
+class ReplicaDesc(C.Structure):
+    _fields_ = [
+        ("num_particles", C.c_int32),
+        ("num_time_slices", C.c_int32),
+        ("num_species", C.c_int32),
+        ("num_images", C.c_int32),
+        ("box", C.c_double * 3),
+        ("periodic", C.c_int32 * 3),
+        ("tau", C.c_double),
+        ("species_count", _i32p),
+        ("lambda_", _f64p),
+        ("pair_fit", _i32p),
+        ("k_cutoff", C.c_double),
+        ("use_long_range", C.c_int32),
+        ("use_background", C.c_int32),
+        ("rng_stream", C.c_uint32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class ActionReq(C.Structure):
+    _fields_ = [
+        ("replica", C.c_int32),
+        ("mode", C.c_int32),
+        ("slice1", C.c_int32),
+        ("slice2", C.c_int32),
+        ("level", C.c_int32),
+        ("nptcls", C.c_int32),
+        ("ptcls", C.c_int32 * MAX_ACTIVE),
+    ]
+
+
+class BisectionDesc(C.Structure):
+    _fields_ = [
+        ("num_levels", C.c_int32),
+        ("lowest_level", C.c_int32),
+        ("species", C.c_int32),
+        ("reserved", C.c_int32),
+        ("seed", C.c_uint64),
+    ]
+
+
+class MoveRecord(C.Structure):
+    _fields_ = [
+        ("slice1", C.c_int32),
+        ("ptcl", C.c_int32),
+        ("accepted", C.c_int32),
+        ("num_stages", C.c_int32),
+        ("old_action", C.c_double * MAX_LEVELS),
+        ("new_action", C.c_double * MAX_LEVELS),
+        ("log_sample_ratio", C.c_double * MAX_LEVELS),
+        ("log_u", C.c_double * MAX_LEVELS),
+    ]
+
+
+class BisectionStats(C.Structure):
+    _fields_ = [
+        ("moves", C.c_int64),
+        ("accepted", C.c_int64),
+        ("stages", C.c_int64),
+        ("pair_evals", C.c_int64),
+        ("k_evals", C.c_int64),
+        ("device_ms", C.c_double),
+    ]
+
+
+# name -> (restype, argtypes); every pimc_* entry point of include/pimc_gpu.h
+_CTX = C.c_void_p
+SIGNATURES = {
+    "pimc_abi_version": (C.c_int, []),
+    "pimc_last_error": (C.c_char_p, []),
+    "pimc_ctx_create": (C.c_int, [C.c_int32, C.POINTER(C.c_void_p)]),
+    "pimc_ctx_destroy": (C.c_int, [_CTX]),
+    "pimc_ctx_synchronize": (C.c_int, [_CTX]),
+    "pimc_fit_create_david": (C.c_int, [_CTX, C.POINTER(DavidFitDesc), _i32p]),
+    "pimc_fit_eval_u": (C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int64, _f64p, _f64p, _f64p, C.c_int32, _f64p]),
+    "pimc_replica_create": (C.c_int, [_CTX, C.POINTER(ReplicaDesc), _i32p]),
+    "pimc_replica_num_kvecs": (C.c_int, [_CTX, C.c_int32, _i32p]),
+    "pimc_replica_kvecs": (C.c_int, [_CTX, C.c_int32, _f64p, _i32p]),
+    "pimc_replica_set_long_range": (
+        C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _f64p, _f64p, _f64p, _f64p]),
+    "pimc_path_set": (C.c_int, [_CTX, C.c_int32, C.c_int32, _f64p]),
+    "pimc_path_get": (C.c_int, [_CTX, C.c_int32, C.c_int32, _f64p]),
+    "pimc_path_set_beads": (C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _f64p]),
+    "pimc_permutation_get": (C.c_int, [_CTX, C.c_int32, C.c_int32, _i32p]),
+    "pimc_accept_copy": (C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32, _i32p, C.c_int32]),
+    "pimc_reject_copy": (C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32, _i32p, C.c_int32]),
+    "pimc_move_join": (C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32]),
+    "pimc_rho_k_update_all": (C.c_int, [_CTX, C.c_int32]),
+    "pimc_rho_k_get": (C.c_int, [_CTX, C.c_int32, C.c_int32, _f64p]),
+    "pimc_action_short_range": (C.c_int, [_CTX, C.POINTER(ActionReq), C.c_int32, C.c_int32, _f64p]),
+    "pimc_action_kinetic": (C.c_int, [_CTX, C.POINTER(ActionReq), C.c_int32, _f64p]),
+    "pimc_action_long_range": (C.c_int, [_CTX, C.POINTER(ActionReq), C.c_int32, C.c_int32, _f64p]),
+    "pimc_dist_disp": (C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int64, _i32p, _f64p, _i32p]),
+    "pimc_bisection_run": (
+        C.c_int,
+        [_CTX, _i32p, C.c_int32, C.POINTER(BisectionDesc), C.c_int64, C.c_int32, C.c_int32,
+         C.POINTER(MoveRecord), C.POINTER(BisectionStats)]),
+}
+
+
+class PimcError(RuntimeError):
+    def __init__(self, func: str, code: int, msg: str):
+        super().__init__(f"{func} failed with status {code}: {msg}")
+        self.code = code
+
+
+def bind(lib: C.CDLL) -> C.CDLL:
+    """Declares argument and return types of every pimc_* symbol on `lib`."""
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _p(a: np.ndarray, t):
+    return a.ctypes.data_as(t)
+
+
+@dataclass
+class ReplicaInfo:
+    num_particles: int
+    num_time_slices: int
+    num_species: int
+    nk: int
+
+
+class PimcLib:
+    """Numpy-facing wrapper of one library that exports the pimc_* ABI.
+
+    One instance owns one context.  Methods raise PimcError on a non-zero status.
+    """
+
+    def __init__(self, lib: C.CDLL, device: int = 0):
+        self.lib = bind(lib)
+        if self.lib.pimc_abi_version() != PIMC_ABI_VERSION:
+            raise RuntimeError("pimc ABI version mismatch")
+        ctx = C.c_void_p()
+        self._check("pimc_ctx_create", self.lib.pimc_ctx_create(device, C.byref(ctx)))
+        self.ctx = ctx
+        self.replicas: dict[int, ReplicaInfo] = {}
+
+    # -- plumbing --------------------------------------------------------
+    def _check(self, name: str, rc: int):
+        if rc != OK:
+            msg = self.lib.pimc_last_error()
+            raise PimcError(name, rc, msg.decode() if msg else "")
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.pimc_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        self._check("pimc_ctx_synchronize", self.lib.pimc_ctx_synchronize(self.ctx))
+
+    # -- fits --------------------------------------------------------------
+    def fit_create_david(self, grid_type: int, grid_start: float, grid_end: float, y: np.ndarray,
+                         num_terms: int, smallest_tau: float, desired_tau: float) -> int:
+        """y is [num_tau][num_points][num_cols] (column 0 potential, 1 u00, 2.. u_kj)."""
+        y = _f64(y)
+        if y.ndim != 3:
+            raise ValueError("y must be [num_tau][num_points][num_cols]")
+        d = DavidFitDesc(grid_type=grid_type, num_points=y.shape[1], grid_start=grid_start,
+                         grid_end=grid_end, num_tau=y.shape[0], num_cols=y.shape[2],
+                         num_terms=num_terms, reserved=0, smallest_tau=smallest_tau,
+                         desired_tau=desired_tau, y=_p(y, _f64p))
+        fid = C.c_int32()
+        self._check("pimc_fit_create_david", self.lib.pimc_fit_create_david(self.ctx, C.byref(d), C.byref(fid)))
+        return fid.value
+
+    def fit_eval_u(self, fit: int, level: int, q, z, s2, precision: int = FP64) -> np.ndarray:
+        q, z, s2 = _f64(q), _f64(z), _f64(s2)
+        n = q.size
+        if z.size != n or s2.size != n:
+            raise ValueError("q, z, s2 sizes differ")
+        out = np.empty(n, dtype=np.float64)
+        self._check("pimc_fit_eval_u", self.lib.pimc_fit_eval_u(
+            self.ctx, fit, level, n, _p(q, _f64p), _p(z, _f64p), _p(s2, _f64p), precision, _p(out, _f64p)))
+        return out
+
+    # -- replicas ----------------------------------------------------------
+    def replica_create(self, num_particles: int, num_time_slices: int, box, tau: float,
+                       species_count, lambdas, pair_fit, num_images: int = 0,
+                       periodic=(1, 1, 1), k_cutoff: float = 0.0, use_long_range: bool = False,
+                       use_background: bool = False, rng_stream: int = 0) -> int:
+        sc = _i32(species_count)
+        lam = _f64(lambdas)
+        pf = _i32(pair_fit).reshape(-1)
+        ns = sc.size
+        if lam.size != ns or pf.size != ns * ns:
+            raise ValueError("species arrays have inconsistent sizes")
+        d = ReplicaDesc()
+        d.num_particles = num_particles
+        d.num_time_slices = num_time_slices
+        d.num_species = ns
+        d.num_images = num_images
+        for i in range(3):
+            d.box[i] = float(box[i])
+            d.periodic[i] = int(periodic[i])
+        d.tau = tau
+        d.species_count = _p(sc, _i32p)
+        d.lambda_ = _p(lam, _f64p)
+        d.pair_fit = _p(pf, _i32p)
+        d.k_cutoff = k_cutoff
+        d.use_long_range = int(use_long_range)
+        d.use_background = int(use_background)
+        d.rng_stream = rng_stream
+        rid = C.c_int32()
+        self._check("pimc_replica_create", self.lib.pimc_replica_create(self.ctx, C.byref(d), C.byref(rid)))
+        nk = C.c_int32()
+        self._check("pimc_replica_num_kvecs", self.lib.pimc_replica_num_kvecs(self.ctx, rid.value, C.byref(nk)))
+        self.replicas[rid.value] = ReplicaInfo(num_particles, num_time_slices, ns, nk.value)
+        return rid.value
+
+    def kvecs(self, rep: int):
+        info = self.replicas[rep]
+        kv = np.empty((info.nk, 3), dtype=np.float64)
+        ki = np.empty((info.nk, 3), dtype=np.int32)
+        self._check("pimc_replica_kvecs", self.lib.pimc_replica_kvecs(self.ctx, rep, _p(kv, _f64p), _p(ki, _i32p)))
+        return kv, ki
+
+    def set_long_range(self, rep: int, sa: int, sb: int, ulong_k, ulong_r0, ushort_k0, ulong_k0):
+        uk = _f64(ulong_k)
+        r0, sk0, lk0 = _f64(ulong_r0), _f64(ushort_k0), _f64(ulong_k0)
+        nl = r0.size
+        if uk.shape != (nl, self.replicas[rep].nk) or sk0.size != nl or lk0.size != nl:
+            raise ValueError("long-range table shapes")
+        self._check("pimc_replica_set_long_range", self.lib.pimc_replica_set_long_range(
+            self.ctx, rep, sa, sb, nl, _p(uk, _f64p), _p(r0, _f64p), _p(sk0, _f64p), _p(lk0, _f64p)))
+
+    def _path_shape(self, rep: int):
+        info = self.replicas[rep]
+        return (info.num_time_slices + 1, info.num_particles, 3)
+
+    def path_set(self, rep: int, beads, mode: int = BOTHMODES):
+        b = _f64(beads)
+        if b.shape != self._path_shape(rep):
+            raise ValueError(f"beads must have shape {self._path_shape(rep)}")
+        self._check("pimc_path_set", self.lib.pimc_path_set(self.ctx, rep, mode, _p(b, _f64p)))
+
+    def path_get(self, rep: int, mode: int = OLDMODE) -> np.ndarray:
+        out = np.empty(self._path_shape(rep), dtype=np.float64)
+        self._check("pimc_path_get", self.lib.pimc_path_get(self.ctx, rep, mode, _p(out, _f64p)))
+        return out
+
+    def path_set_beads(self, rep: int, mode: int, ptcl: int, slice1: int, xyz):
+        x = _f64(xyz).reshape(-1, 3)
+        slice2 = slice1 + x.shape[0] - 1
+        self._check("pimc_path_set_beads", self.lib.pimc_path_set_beads(
+            self.ctx, rep, mode, ptcl, slice1, slice2, _p(x, _f64p)))
+
+    def permutation(self, rep: int, mode: int = OLDMODE) -> np.ndarray:
+        out = np.empty(self.replicas[rep].num_particles, dtype=np.int32)
+        self._check("pimc_permutation_get", self.lib.pimc_permutation_get(self.ctx, rep, mode, _p(out, _i32p)))
+        return out
+
+    def accept_copy(self, rep: int, slice1: int, slice2: int, ptcls):
+        p = _i32(ptcls)
+        self._check("pimc_accept_copy", self.lib.pimc_accept_copy(self.ctx, rep, slice1, slice2, _p(p, _i32p), p.size))
+
+    def reject_copy(self, rep: int, slice1: int, slice2: int, ptcls):
+        p = _i32(ptcls)
+        self._check("pimc_reject_copy", self.lib.pimc_reject_copy(self.ctx, rep, slice1, slice2, _p(p, _i32p), p.size))
+
+    def move_join(self, rep: int, old_join: int, new_join: int):
+        self._check("pimc_move_join", self.lib.pimc_move_join(self.ctx, rep, old_join, new_join))
+
+    def rho_k_update_all(self, rep: int):
+        self._check("pimc_rho_k_update_all", self.lib.pimc_rho_k_update_all(self.ctx, rep))
+
+    def rho_k(self, rep: int, mode: int = OLDMODE) -> np.ndarray:
+        info = self.replicas[rep]
+        out = np.empty((info.num_time_slices + 1, info.num_species, info.nk, 2), dtype=np.float64)
+        self._check("pimc_rho_k_get", self.lib.pimc_rho_k_get(self.ctx, rep, mode, _p(out, _f64p)))
+        return out[..., 0] + 1j * out[..., 1]
+
+    # -- actions -----------------------------------------------------------
+    @staticmethod
+    def make_reqs(reqs) -> C.Array:
+        """reqs: iterable of (replica, mode, slice1, slice2, level, ptcls)."""
+        reqs = list(reqs)
+        arr = (ActionReq * max(1, len(reqs)))()
+        for i, (rep, mode, s1, s2, level, ptcls) in enumerate(reqs):
+            ptcls = list(ptcls)
+            if not 1 <= len(ptcls) <= MAX_ACTIVE:
+                raise ValueError("1..4 active particles per request")
+            arr[i].replica, arr[i].mode, arr[i].slice1, arr[i].slice2 = rep, mode, s1, s2
+            arr[i].level, arr[i].nptcls = level, len(ptcls)
+            for j, p in enumerate(ptcls):
+                arr[i].ptcls[j] = p
+        return arr, len(reqs)
+
+    def action_short_range(self, reqs, precision: int = FP64) -> np.ndarray:
+        arr, n = self.make_reqs(reqs)
+        out = np.zeros(max(1, n), dtype=np.float64)
+        self._check("pimc_action_short_range", self.lib.pimc_action_short_range(
+            self.ctx, arr, n, precision, _p(out, _f64p)))
+        return out[:n]
+
+    def action_kinetic(self, reqs) -> np.ndarray:
+        arr, n = self.make_reqs(reqs)
+        out = np.zeros(max(1, n), dtype=np.float64)
+        self._check("pimc_action_kinetic", self.lib.pimc_action_kinetic(self.ctx, arr, n, _p(out, _f64p)))
+        return out[:n]
+
+    def action_long_range(self, reqs, precision: int = FP64) -> np.ndarray:
+        arr, n = self.make_reqs(reqs)
+        out = np.zeros(max(1, n), dtype=np.float64)
+        self._check("pimc_action_long_range", self.lib.pimc_action_long_range(
+            self.ctx, arr, n, precision, _p(out, _f64p)))
+        return out[:n]
+
+    def dist_disp(self, rep: int, mode: int, idx):
+        idx = _i32(idx).reshape(-1, 4)
+        n = idx.shape[0]
+        out = np.empty((n, 8), dtype=np.float64)
+        img = np.empty((n, 6), dtype=np.int32)
+        self._check("pimc_dist_disp", self.lib.pimc_dist_disp(
+            self.ctx, rep, mode, n, _p(idx, _i32p), _p(out, _f64p), _p(img, _i32p)))
+        return out, img
+
+    # -- bisection ---------------------------------------------------------
+    def bisection_run(self, replicas, num_levels: int, seed: int, first_move: int, num_moves: int,
+                      precision: int = FP64, species: int = 0, lowest_level: int = 0,
+                      records: bool = True):
+        reps = _i32(replicas)
+        d = BisectionDesc(num_levels=num_levels, lowest_level=lowest_level, species=species,
+                          reserved=0, seed=seed)
+        st = BisectionStats()
+        rec = (MoveRecord * (reps.size * num_moves))() if records and num_moves > 0 else None
+        self._check("pimc_bisection_run", self.lib.pimc_bisection_run(
+            self.ctx, _p(reps, _i32p), reps.size, C.byref(d), first_move, num_moves, precision,
+            rec, C.byref(st)))
+        return (records_to_numpy(rec, reps.size, num_moves) if rec is not None else None), st
+
+
+def records_to_numpy(rec, nrep: int, nmoves: int) -> dict:
+    """Converts a MoveRecord array to a dict of numpy arrays shaped [nrep, nmoves, ...]."""
+    raw = np.ctypeslib.as_array(C.cast(rec, C.POINTER(C.c_uint8)), shape=(C.sizeof(rec),)).copy()
+    dt = np.dtype([("slice1", "<i4"), ("ptcl", "<i4"), ("accepted", "<i4"), ("num_stages", "<i4"),
+                   ("old_action", "<f8", (MAX_LEVELS,)), ("new_action", "<f8", (MAX_LEVELS,)),
+                   ("log_sample_ratio", "<f8", (MAX_LEVELS,)), ("log_u", "<f8", (MAX_LEVELS,))])
+    assert dt.itemsize == C.sizeof(MoveRecord)
+    arr = raw.view(dt).reshape(nrep, nmoves)
+    return {name: np.array(arr[name]) for name in dt.names}

@@ -1,0 +1,754 @@
+// pimc_abi.hip -- host implementation of the C ABI in include/pimc_gpu.h.
+//
+// Owns device memory, builds the spline tables the kernels read, keeps the
+// device copies of the fit and replica descriptors, and launches the kernels
+// of pimc_actions.hip and pimc_bisection.hip on the context's stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "../../include/pimc_gpu.h"
+#include "pimc_device.hpp"
+#include "pimc_kernels.hpp"
+#include "pimc_launch.hpp"
+
+using namespace pimc;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(PIMC_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Device buffer owned by the context.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int ensure(size_t n) {
+    if (n <= bytes) return PIMC_OK;
+    if (p) {
+      (void)hipFree(p);
+      p = nullptr;
+      bytes = 0;
+    }
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) return fail(PIMC_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    bytes = n;
+    return PIMC_OK;
+  }
+  template <typename T>
+  T* as() const {
+    return (T*)p;
+  }
+};
+
+// --------------------------------------------------------------------------
+// Host spline setup: grids and MultiCubicSpline::Update (CubicSpline.cc:77-128)
+// --------------------------------------------------------------------------
+struct HostGrid {
+  int type = 0, n = 0;
+  double start = 0, end = 0, delta = 0, deltainv = 0, r0 = 0, spacing = 0;
+  std::vector<double> x;
+};
+
+void make_linear(HostGrid& g, double s, double e, int n) {
+  g.type = PIMC_GRID_LINEAR;
+  g.start = s;
+  g.end = e;
+  g.n = n;
+  g.delta = (e - s) / (double)(n - 1);
+  g.deltainv = 1.0 / g.delta;
+  g.x.resize(n);
+  for (int i = 0; i < n; i++) g.x[i] = s + (double)i * g.delta;
+}
+// The .dm reader's LOG grid: spacing = (end/start)^(1/(n-1)), x_i = start * spacing^i.
+void make_log(HostGrid& g, double s, double e, int n) {
+  g.type = PIMC_GRID_LOG;
+  g.spacing = pow((e / s), 1.0 / (n - 1.0));
+  g.r0 = s;
+  g.n = n;
+  g.start = s;
+  g.end = s * pow(g.spacing, (double)n - 1);
+  g.x.resize(n);
+  for (int i = 0; i < n; i++) g.x[i] = s * pow(g.spacing, (double)i);
+}
+
+// Second derivatives of one column (Numerical Recipes tridiagonal, the
+// reference's MultiCubicSpline::Update); deriv > 0.99e30 means natural.
+void spline_d2(const std::vector<double>& x, const double* y, int stride, double start_deriv, double end_deriv,
+               std::vector<double>& d2) {
+  const int N = (int)x.size();
+  d2.assign(N, 0.0);
+  std::vector<double> U(N, 0.0);
+  auto Y = [&](int j) { return y[(size_t)j * stride]; };
+  if (start_deriv > 0.99e30) {
+    d2[0] = 0.0;
+    U[0] = 0.0;
+  } else {
+    d2[0] = -0.5;
+    U[0] = (3.0 / (x[1] - x[0])) * ((Y(1) - Y(0)) / (x[1] - x[0]) - start_deriv);
+  }
+  d2[N - 1] = 0.0;
+  for (int j = 1; j < N - 1; j++) {
+    double sig = (x[j] - x[j - 1]) / (x[j + 1] - x[j - 1]);
+    double p = sig * d2[j - 1] + 2.0;
+    d2[j] = (sig - 1.0) / p;
+    U[j] = (6.0 * ((Y(j + 1) - Y(j)) / (x[j + 1] - x[j]) - (Y(j) - Y(j - 1)) / (x[j] - x[j - 1])) /
+                (x[j + 1] - x[j - 1]) -
+            sig * U[j - 1]) /
+           p;
+  }
+  double Un, Qn;
+  if (end_deriv > 0.99e30) {
+    Un = 0.0;
+    Qn = 0.0;
+  } else {
+    Qn = 0.5;
+    Un = (3.0 / (x[N - 1] - x[N - 2])) * (end_deriv - (Y(N - 1) - Y(N - 2)) / (x[N - 1] - x[N - 2]));
+  }
+  d2[N - 1] = (Un - Qn * U[N - 2]) / (Qn * d2[N - 2] + 1.0);
+  for (int k = N - 2; k >= 0; k--) d2[k] = d2[k] * d2[k + 1] + U[k];
+}
+
+struct HostFit {
+  FitDev dev{};            // pointers refer to `mem`
+  DevBuf mem;
+  int num_terms = 0;
+  size_t staged64 = 0, staged32 = 0;
+};
+
+struct HostReplica {
+  ReplicaDev dev{};
+  DevBuf path0, path1, perm0, perm1;
+  std::vector<int> species_of;
+  std::vector<double> kvecs;
+  std::vector<int> kidx;
+  int M = 0;
+  int join = 0;
+};
+
+}  // namespace
+
+struct pimc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<std::unique_ptr<HostFit>> fits;
+  std::vector<std::unique_ptr<HostReplica>> reps;
+  DevBuf d_fits, d_reps;
+  bool fits_dirty = true, reps_dirty = true;
+  DevBuf d_reqs, d_partials, d_out, d_in0, d_in1, d_in2, d_idx, d_img, d_records, d_stats, d_repids;
+
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;   // brackets the compute of timed calls
+  // Host staging of the request upload; kept alive until the call that uses it synchronizes.
+  std::vector<ReqDev> h_reqs;
+
+  int sync_descriptors() {
+    std::vector<FitDev> hf;
+    std::vector<ReplicaDev> hr;
+    if (fits_dirty && !fits.empty()) {
+      for (auto& f : fits) hf.push_back(f->dev);
+      int rc = d_fits.ensure(hf.size() * sizeof(FitDev));
+      if (rc) return rc;
+      HIP_TRY(hipMemcpyAsync(d_fits.p, hf.data(), hf.size() * sizeof(FitDev), hipMemcpyHostToDevice, stream));
+      fits_dirty = false;
+    }
+    if (reps_dirty && !reps.empty()) {
+      for (auto& r : reps) hr.push_back(r->dev);
+      int rc = d_reps.ensure(hr.size() * sizeof(ReplicaDev));
+      if (rc) return rc;
+      HIP_TRY(hipMemcpyAsync(d_reps.p, hr.data(), hr.size() * sizeof(ReplicaDev), hipMemcpyHostToDevice, stream));
+      reps_dirty = false;
+    }
+    HIP_TRY(hipStreamSynchronize(stream));
+    return PIMC_OK;
+  }
+  HostReplica* rep(int id) { return (id >= 0 && id < (int)reps.size()) ? reps[id].get() : nullptr; }
+};
+
+namespace {
+
+
+int check_req(pimc_ctx* ctx, const pimc_action_req& q) {
+  HostReplica* H = ctx->rep(q.replica);
+  if (!H) return fail(PIMC_ERR_ARG, "action request: bad replica id");
+  if (q.mode != PIMC_OLDMODE && q.mode != PIMC_NEWMODE) return fail(PIMC_ERR_ARG, "action request: bad mode");
+  if (q.level < 0 || q.level >= PIMC_MAX_LEVELS) return fail(PIMC_ERR_ARG, "action request: bad level");
+  if (q.slice1 < 0 || q.slice2 > H->M || q.slice1 > q.slice2)
+    return fail(PIMC_ERR_ARG, "action request: bad slice range");
+  // the link loop reads slice + 2^level for every slice < slice2
+  const int skip = 1 << q.level;
+  if (q.slice2 > q.slice1 && q.slice1 + ((q.slice2 - q.slice1 + skip - 1) / skip) * skip > H->M)
+    return fail(PIMC_ERR_ARG, "action request: slice range runs past the last slice at this level");
+  if (q.nptcls < 1 || q.nptcls > PIMC_MAX_ACTIVE) return fail(PIMC_ERR_ARG, "action request: bad particle count");
+  for (int i = 0; i < q.nptcls; i++)
+    if (q.ptcls[i] < 0 || q.ptcls[i] >= H->dev.core.N) return fail(PIMC_ERR_ARG, "action request: bad particle index");
+  return PIMC_OK;
+}
+
+int upload_reqs(pimc_ctx* ctx, const pimc_action_req* reqs, int nreq) {
+  std::vector<ReqDev>& h = ctx->h_reqs;
+  h.assign(nreq, ReqDev{});
+  for (int i = 0; i < nreq; i++) {
+    int rc = check_req(ctx, reqs[i]);
+    if (rc) return rc;
+    h[i].replica = reqs[i].replica;
+    h[i].mode = reqs[i].mode;
+    h[i].slice1 = reqs[i].slice1;
+    h[i].slice2 = reqs[i].slice2;
+    h[i].level = reqs[i].level;
+    h[i].nptcls = reqs[i].nptcls;
+    for (int k = 0; k < PIMC_MAX_ACTIVE; k++) h[i].ptcls[k] = k < reqs[i].nptcls ? reqs[i].ptcls[k] : -1;
+  }
+  int rc = ctx->d_reqs.ensure(h.size() * sizeof(ReqDev));
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(ctx->d_reqs.p, h.data(), h.size() * sizeof(ReqDev), hipMemcpyHostToDevice, ctx->stream));
+  return PIMC_OK;
+}
+
+// Host [slice][ptcl][3] <-> device [slice][3][N]
+void to_device_layout(const double* src, double* dst, int nslices, int N) {
+  for (int s = 0; s < nslices; s++)
+    for (int p = 0; p < N; p++)
+      for (int d = 0; d < 3; d++) dst[((size_t)s * 3 + d) * N + p] = src[((size_t)s * N + p) * 3 + d];
+}
+void to_host_layout(const double* src, double* dst, int nslices, int N) {
+  for (int s = 0; s < nslices; s++)
+    for (int p = 0; p < N; p++)
+      for (int d = 0; d < 3; d++) dst[((size_t)s * N + p) * 3 + d] = src[((size_t)s * 3 + d) * N + p];
+}
+
+}  // namespace
+
+extern "C" {
+
+int pimc_abi_version(void) { return PIMC_ABI_VERSION; }
+const char* pimc_last_error(void) { return g_err.c_str(); }
+
+int pimc_ctx_create(int32_t device, pimc_ctx** out) {
+  if (!out) return fail(PIMC_ERR_ARG, "ctx_create: out is NULL");
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0)
+    return fail(PIMC_ERR_DEVICE, std::string("ctx_create: no GPU device (") + hipGetErrorString(e) + ")");
+  if (device < 0 || device >= count) return fail(PIMC_ERR_ARG, "ctx_create: bad device index");
+  HIP_TRY(hipSetDevice(device));
+  auto c = std::make_unique<pimc_ctx>();
+  c->device = device;
+  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  *out = c.release();
+  return PIMC_OK;
+}
+
+int pimc_ctx_destroy(pimc_ctx* ctx) {
+  if (!ctx) return PIMC_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  hipStream_t s = ctx->stream;
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  delete ctx;
+  if (s) (void)hipStreamDestroy(s);
+  return PIMC_OK;
+}
+
+int pimc_ctx_synchronize(pimc_ctx* ctx) {
+  if (!ctx) return fail(PIMC_ERR_ARG, "NULL ctx");
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return PIMC_OK;
+}
+
+int pimc_fit_create_david(pimc_ctx* ctx, const pimc_david_fit_desc* d, int32_t* fit_id) {
+  if (!ctx || !d || !fit_id || !d->y) return fail(PIMC_ERR_ARG, "fit_create_david: NULL argument");
+  if (d->num_points < 3 || d->num_tau < 1 || d->num_cols < 2)
+    return fail(PIMC_ERR_ARG, "fit_create_david: bad sizes");
+  const int nt = d->num_terms;
+  if (nt < 0 || nt > 3) return fail(PIMC_ERR_UNSUPPORTED, "fit_create_david: num_terms must be 0..3");
+  const int noff = nt * (nt + 3) / 2;
+  if (d->num_cols < 2 + noff) return fail(PIMC_ERR_ARG, "fit_create_david: num_cols too small for num_terms");
+  if (!(d->grid_start < d->grid_end)) return fail(PIMC_ERR_ARG, "fit_create_david: bad grid range");
+  HostGrid g;
+  if (d->grid_type == PIMC_GRID_LINEAR)
+    make_linear(g, d->grid_start, d->grid_end, d->num_points);
+  else if (d->grid_type == PIMC_GRID_LOG && d->grid_start > 0)
+    make_log(g, d->grid_start, d->grid_end, d->num_points);
+  else
+    return fail(PIMC_ERR_ARG, "fit_create_david: bad grid type");
+  int taupos = -1;
+  double tau = d->smallest_tau;
+  for (int i = 0; i < d->num_tau; i++) {
+    if (fabs(tau - d->desired_tau) < 1e-4) taupos = i;
+    tau = tau * 2;
+  }
+  if (taupos < 0) return fail(PIMC_ERR_ARG, "fit_create_david: desired_tau matches no tau level");
+
+  const int ng = g.n, nint = ng - 1, ncol = 1 + noff, ntau = d->num_tau, nc = d->num_cols;
+  std::vector<d2v> geo(nint);
+  for (int i = 0; i < nint; i++) {
+    double h = g.x[i + 1] - g.x[i];
+    geo[i] = d2v{g.x[i], 1.0 / h};
+  }
+  std::vector<d4v> c64((size_t)ntau * nint * ncol);
+  std::vector<f4v> c32(c64.size());
+  std::vector<unsigned char> big((size_t)ntau * nint, 0);
+  std::vector<double> d2;
+  for (int lev = 0; lev < ntau; lev++) {
+    const double* y = d->y + (size_t)lev * ng * nc;
+    for (int c = 0; c < ncol; c++) {
+      const int col = 1 + c;  // table column: u00, then the off-diagonal columns
+      // DavidPAClass builds its splines with natural start and zero end slope (DavidPAClass.cc:760-763)
+      spline_d2(g.x, y + col, nc, 5.0e30, 0.0, d2);
+      for (int i = 0; i < nint; i++) {
+        double h = g.x[i + 1] - g.x[i];
+        double h2over6 = h * h * 0.1666666666666666666;
+        double ylo = y[(size_t)i * nc + col], yhi = y[(size_t)(i + 1) * nc + col];
+        double Dlo = d2[i] * h2over6, Dhi = d2[i + 1] * h2over6;
+        // a*ylo + b*yhi + (a^3-a)Dlo + (b^3-b)Dhi with a = 1-b, as a cubic in b
+        d4v cf = d4v{ylo, yhi - ylo - 2.0 * Dlo - Dhi, 3.0 * Dlo, Dhi - Dlo};
+        size_t at = ((size_t)lev * nint + i) * ncol + c;
+        c64[at] = cf;
+        c32[at] = f4v{(float)cf.x, (float)cf.y, (float)cf.z, (float)cf.w};
+        if (fabs(cf.x) + fabs(cf.y) + fabs(cf.z) + fabs(cf.w) > kMixedLimit) big[(size_t)lev * nint + i] = 1;
+      }
+    }
+  }
+  auto f = std::make_unique<HostFit>();
+  size_t bx = (size_t)ng * 8, bgeo = (size_t)nint * 16, b64 = c64.size() * 32, b32 = c32.size() * 16;
+  size_t off_geo = 0, off_x = off_geo + bgeo, off64 = (off_x + bx + 255) & ~(size_t)255,
+         off32 = off64 + b64, offbig = off32 + b32, total = offbig + big.size();
+  int rc = f->mem.ensure(total);
+  if (rc) return rc;
+  unsigned char* base = f->mem.as<unsigned char>();
+  HIP_TRY(hipMemcpy(base + off_geo, geo.data(), bgeo, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(base + off_x, g.x.data(), bx, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(base + off64, c64.data(), b64, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(base + off32, c32.data(), b32, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(base + offbig, big.data(), big.size(), hipMemcpyHostToDevice));
+  FitDev& F = f->dev;
+  F.grid_type = g.type;
+  F.ng = ng;
+  F.nint = nint;
+  F.num_tau = ntau;
+  F.ncol = ncol;
+  F.noff = noff;
+  F.num_terms = nt;
+  F.taupos = taupos;
+  F.start = g.start;
+  F.end = g.end;
+  F.deltainv = g.deltainv;
+  F.r0 = g.r0;
+  F.spacing = g.spacing;
+  F.logspacing = g.type == PIMC_GRID_LOG ? log(g.spacing) : 0.0;
+  F.x = (const double*)(base + off_x);
+  F.geo = (const d2v*)(base + off_geo);
+  F.c64 = (const d4v*)(base + off64);
+  F.c32 = (const f4v*)(base + off32);
+  F.big = (const unsigned char*)(base + offbig);
+  f->num_terms = nt;
+  f->staged64 = staged_bytes(nint, ncol, ng, false);
+  f->staged32 = staged_bytes(nint, ncol, ng, true);
+  ctx->fits.push_back(std::move(f));
+  ctx->fits_dirty = true;
+  *fit_id = (int32_t)ctx->fits.size() - 1;
+  return PIMC_OK;
+}
+
+int pimc_fit_eval_u(pimc_ctx* ctx, int32_t fit_id, int32_t level, int64_t n, const double* q, const double* z,
+                    const double* s2, int32_t precision, double* out) {
+  if (!ctx || fit_id < 0 || fit_id >= (int)ctx->fits.size()) return fail(PIMC_ERR_ARG, "fit_eval_u: bad fit");
+  if (n < 0 || (n > 0 && (!q || !z || !s2 || !out))) return fail(PIMC_ERR_ARG, "fit_eval_u: bad arrays");
+  if (precision != PIMC_FP64 && precision != PIMC_MIXED) return fail(PIMC_ERR_ARG, "fit_eval_u: bad precision");
+  const FitDev& F = ctx->fits[fit_id]->dev;
+  if (level < 0 || level + F.taupos >= F.num_tau) return fail(PIMC_ERR_ARG, "fit_eval_u: level out of range");
+  if (n == 0) return PIMC_OK;
+  HIP_TRY(hipSetDevice(ctx->device));
+  int rc = ctx->sync_descriptors();
+  if (rc) return rc;
+  size_t b = (size_t)n * sizeof(double);
+  if ((rc = ctx->d_in0.ensure(b)) || (rc = ctx->d_in1.ensure(b)) || (rc = ctx->d_in2.ensure(b)) ||
+      (rc = ctx->d_out.ensure(b)))
+    return rc;
+  HIP_TRY(hipMemcpyAsync(ctx->d_in0.p, q, b, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(ctx->d_in1.p, z, b, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(ctx->d_in2.p, s2, b, hipMemcpyHostToDevice, ctx->stream));
+  LaunchFitEval a{ctx->d_fits.as<FitDev>(), fit_id, level, n, ctx->d_in0.as<double>(), ctx->d_in1.as<double>(),
+                  ctx->d_in2.as<double>(), ctx->d_out.as<double>()};
+  launch_fit_eval(F.num_terms, precision == PIMC_MIXED, a, ctx->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, ctx->d_out.p, b, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return PIMC_OK;
+}
+
+int pimc_replica_create(pimc_ctx* ctx, const pimc_replica_desc* d, int32_t* replica_id) {
+  if (!ctx || !d || !replica_id) return fail(PIMC_ERR_ARG, "replica_create: NULL argument");
+  if (d->num_particles < 1 || d->num_time_slices < 1 || d->num_species < 1 || d->num_species > PIMC_MAX_SPECIES)
+    return fail(PIMC_ERR_ARG, "replica_create: bad sizes");
+  if (!d->species_count || !d->lambda || !d->pair_fit) return fail(PIMC_ERR_ARG, "replica_create: NULL arrays");
+  if (d->num_images < 0) return fail(PIMC_ERR_ARG, "replica_create: bad num_images");
+  auto H = std::make_unique<HostReplica>();
+  RepCore& R = H->dev.core;
+  RepSpecies& S = H->dev.sp;
+  R.N = d->num_particles;
+  R.nslices = d->num_time_slices + 1;
+  R.ns = d->num_species;
+  R.num_images = d->num_images;
+  R.tau = d->tau;
+  for (int i = 0; i < 3; i++) {
+    if (!(d->box[i] > 0)) return fail(PIMC_ERR_ARG, "replica_create: box must be positive");
+    R.box[i] = d->box[i];
+    R.boxinv[i] = 1.0 / d->box[i];
+    R.isper[i] = d->periodic[i] ? 1.0 : 0.0;
+  }
+  int tot = 0;
+  for (int s = 0; s < R.ns; s++) {
+    if (d->species_count[s] < 0) return fail(PIMC_ERR_ARG, "replica_create: negative species count");
+    S.first[s] = tot;
+    S.count[s] = d->species_count[s];
+    S.lambda[s] = d->lambda[s];
+    tot += d->species_count[s];
+  }
+  if (tot != R.N) return fail(PIMC_ERR_ARG, "replica_create: species counts do not sum to num_particles");
+  for (int i = 0; i < R.ns * R.ns; i++) {
+    if (d->pair_fit[i] < 0 || d->pair_fit[i] >= (int)ctx->fits.size())
+      return fail(PIMC_ERR_ARG, "replica_create: bad pair fit id");
+    S.fit[i] = d->pair_fit[i];
+  }
+  R.stream = d->rng_stream;
+  H->M = d->num_time_slices;
+  if (d->k_cutoff > 0 || d->use_long_range)
+    return fail(PIMC_ERR_UNSUPPORTED, "replica_create: long range is not available in this build");
+  R.nk = 0;
+  HIP_TRY(hipSetDevice(ctx->device));
+  size_t nb = (size_t)R.nslices * 3 * R.N * sizeof(double);
+  int rc;
+  if ((rc = H->path0.ensure(nb)) || (rc = H->path1.ensure(nb)) || (rc = H->perm0.ensure(R.N * sizeof(int))) ||
+      (rc = H->perm1.ensure(R.N * sizeof(int))))
+    return rc;
+  HIP_TRY(hipMemset(H->path0.p, 0, nb));
+  HIP_TRY(hipMemset(H->path1.p, 0, nb));
+  std::vector<int> id(R.N);
+  for (int p = 0; p < R.N; p++) id[p] = p;
+  HIP_TRY(hipMemcpy(H->perm0.p, id.data(), R.N * sizeof(int), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(H->perm1.p, id.data(), R.N * sizeof(int), hipMemcpyHostToDevice));
+  R.path0 = H->path0.as<double>();
+  R.path1 = H->path1.as<double>();
+  R.perm0 = H->perm0.as<int>();
+  R.perm1 = H->perm1.as<int>();
+  ctx->reps.push_back(std::move(H));
+  ctx->reps_dirty = true;
+  *replica_id = (int32_t)ctx->reps.size() - 1;
+  return PIMC_OK;
+}
+
+int pimc_replica_num_kvecs(pimc_ctx* ctx, int32_t replica, int32_t* nk) {
+  HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
+  if (!H || !nk) return fail(PIMC_ERR_ARG, "replica_num_kvecs: bad argument");
+  *nk = H->dev.core.nk;
+  return PIMC_OK;
+}
+int pimc_replica_kvecs(pimc_ctx* ctx, int32_t replica, double* kvecs, int32_t* kidx) {
+  HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
+  if (!H) return fail(PIMC_ERR_ARG, "replica_kvecs: bad replica");
+  if (kvecs && !H->kvecs.empty()) memcpy(kvecs, H->kvecs.data(), H->kvecs.size() * sizeof(double));
+  if (kidx && !H->kidx.empty()) memcpy(kidx, H->kidx.data(), H->kidx.size() * sizeof(int32_t));
+  return PIMC_OK;
+}
+int pimc_replica_set_long_range(pimc_ctx* ctx, int32_t replica, int32_t, int32_t, int32_t, const double*,
+                                const double*, const double*, const double*) {
+  HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
+  if (!H) return fail(PIMC_ERR_ARG, "replica_set_long_range: bad replica");
+  return fail(PIMC_ERR_STATE, "replica_set_long_range: replica has no k set");
+}
+
+int pimc_path_set(pimc_ctx* ctx, int32_t replica, int32_t mode, const double* beads) {
+  HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
+  if (!H || !beads || mode < 0 || mode > 2) return fail(PIMC_ERR_ARG, "path_set: bad argument");
+  const RepCore& R = H->dev.core;
+  std::vector<double> tmp((size_t)R.nslices * 3 * R.N);
+  to_device_layout(beads, tmp.data(), R.nslices, R.N);
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  size_t nb = tmp.size() * sizeof(double);
+  if (mode != 1) HIP_TRY(hipMemcpy(H->path0.p, tmp.data(), nb, hipMemcpyHostToDevice));
+  if (mode != 0) HIP_TRY(hipMemcpy(H->path1.p, tmp.data(), nb, hipMemcpyHostToDevice));
+  return PIMC_OK;
+}
+int pimc_path_get(pimc_ctx* ctx, int32_t replica, int32_t mode, double* beads) {
+  HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
+  if (!H || !beads || mode < 0 || mode > 1) return fail(PIMC_ERR_ARG, "path_get: bad argument");
+  const RepCore& R = H->dev.core;
+  std::vector<double> tmp((size_t)R.nslices * 3 * R.N);
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(hipMemcpy(tmp.data(), mode ? H->path1.p : H->path0.p, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+  to_host_layout(tmp.data(), beads, R.nslices, R.N);
+  return PIMC_OK;
+}
+int pimc_path_set_beads(pimc_ctx* ctx, int32_t replica, int32_t mode, int32_t ptcl, int32_t slice1, int32_t slice2,
+                        const double* xyz) {
+  HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
+  if (!H || !xyz || mode < 0 || mode > 2 || ptcl < 0 || ptcl >= H->dev.core.N || slice1 < 0 || slice2 > H->M ||
+      slice1 > slice2)
+    return fail(PIMC_ERR_ARG, "path_set_beads: bad argument");
+  const RepCore& R = H->dev.core;
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (int m = 0; m < 2; m++) {
+    if (!(mode == 2 || mode == m)) continue;
+    double* base = (double*)(m ? H->path1.p : H->path0.p);
+    for (int s = slice1; s <= slice2; s++)
+      for (int d = 0; d < 3; d++)
+        HIP_TRY(hipMemcpy(base + ((size_t)s * 3 + d) * R.N + ptcl, xyz + 3 * (s - slice1) + d, sizeof(double),
+                          hipMemcpyHostToDevice));
+  }
+  return PIMC_OK;
+}
+int pimc_permutation_get(pimc_ctx* ctx, int32_t replica, int32_t mode, int32_t* perm) {
+  HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
+  if (!H || !perm || mode < 0 || mode > 1) return fail(PIMC_ERR_ARG, "permutation_get: bad argument");
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(hipMemcpy(perm, mode ? H->perm1.p : H->perm0.p, H->dev.core.N * sizeof(int), hipMemcpyDeviceToHost));
+  return PIMC_OK;
+}
+
+static int accept_reject(pimc_ctx* ctx, int32_t replica, int32_t slice1, int32_t slice2, const int32_t* ptcls,
+                         int32_t n, bool accept) {
+  HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
+  if (!H || (n > 0 && !ptcls) || n < 0 || n > PIMC_MAX_ACTIVE || slice1 < 0 || slice2 > H->M || slice1 > slice2)
+    return fail(PIMC_ERR_ARG, "accept/reject copy: bad argument");
+  for (int i = 0; i < n; i++)
+    if (ptcls[i] < 0 || ptcls[i] >= H->dev.core.N) return fail(PIMC_ERR_ARG, "accept/reject copy: bad particle");
+  HIP_TRY(hipSetDevice(ctx->device));
+  int rc = ctx->sync_descriptors();
+  if (rc) return rc;
+  launch_accept_reject(ctx->d_reps.as<ReplicaDev>(), replica, slice1, slice2, ptcls, n, accept, ctx->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return PIMC_OK;
+}
+int pimc_accept_copy(pimc_ctx* ctx, int32_t replica, int32_t slice1, int32_t slice2, const int32_t* ptcls,
+                     int32_t nptcls) {
+  return accept_reject(ctx, replica, slice1, slice2, ptcls, nptcls, true);
+}
+int pimc_reject_copy(pimc_ctx* ctx, int32_t replica, int32_t slice1, int32_t slice2, const int32_t* ptcls,
+                     int32_t nptcls) {
+  return accept_reject(ctx, replica, slice1, slice2, ptcls, nptcls, false);
+}
+int pimc_move_join(pimc_ctx* ctx, int32_t replica, int32_t old_join, int32_t new_join) {
+  HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
+  if (!H || old_join < 0 || new_join < 0 || old_join > H->M || new_join > H->M)
+    return fail(PIMC_ERR_ARG, "move_join: bad argument");
+  HIP_TRY(hipSetDevice(ctx->device));
+  int rc = ctx->sync_descriptors();
+  if (rc) return rc;
+  launch_move_join(ctx->d_reps.as<ReplicaDev>(), replica, H->dev.core.N, old_join, new_join, ctx->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  H->join = new_join;
+  return PIMC_OK;
+}
+
+int pimc_rho_k_update_all(pimc_ctx* ctx, int32_t replica) {
+  HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
+  if (!H) return fail(PIMC_ERR_ARG, "rho_k_update_all: bad replica");
+  return fail(PIMC_ERR_STATE, "rho_k_update_all: replica has no k set");
+}
+int pimc_rho_k_get(pimc_ctx* ctx, int32_t replica, int32_t mode, double* rho) {
+  HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
+  if (!H || !rho || mode < 0 || mode > 1) return fail(PIMC_ERR_ARG, "rho_k_get: bad argument");
+  return fail(PIMC_ERR_STATE, "rho_k_get: replica has no k set");
+}
+
+int pimc_action_short_range(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t nreq, int32_t precision,
+                            double* out) {
+  if (!ctx || nreq < 0 || (nreq > 0 && (!reqs || !out))) return fail(PIMC_ERR_ARG, "action_short_range: bad argument");
+  if (precision != PIMC_FP64 && precision != PIMC_MIXED) return fail(PIMC_ERR_ARG, "action_short_range: bad precision");
+  if (nreq == 0) return PIMC_OK;
+  HIP_TRY(hipSetDevice(ctx->device));
+  int rc = ctx->sync_descriptors();
+  if (rc) return rc;
+  rc = upload_reqs(ctx, reqs, nreq);
+  if (rc) return rc;
+  int maxN = 0, nt = -1;
+  for (int i = 0; i < nreq; i++) {
+    const ReplicaDev& R = ctx->reps[reqs[i].replica]->dev;
+    maxN = std::max(maxN, R.core.N);
+    for (int f = 0; f < R.core.ns * R.core.ns; f++) {
+      const HostFit& F = *ctx->fits[R.sp.fit[f]];
+      if (nt < 0) nt = F.num_terms;
+      if (F.num_terms != nt)
+        return fail(PIMC_ERR_UNSUPPORTED, "action_short_range: fits with different num_terms in one call");
+      if (reqs[i].level + F.dev.taupos >= F.dev.num_tau)
+        return fail(PIMC_ERR_ARG, "action_short_range: level beyond the fit's tau levels");
+    }
+  }
+  const int ntiles = (maxN + kBlock - 1) / kBlock;
+  if ((rc = ctx->d_partials.ensure((size_t)nreq * ntiles * sizeof(double))) ||
+      (rc = ctx->d_out.ensure((size_t)nreq * sizeof(double))))
+    return rc;
+  LaunchShortRange a{ctx->d_reps.as<ReplicaDev>(), ctx->d_fits.as<FitDev>(), ctx->d_reqs.as<ReqDev>(), nreq, ntiles,
+                     ctx->d_partials.as<double>(), ctx->d_out.as<double>()};
+  launch_short_range(nt, precision == PIMC_MIXED, a, ctx->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, ctx->d_out.p, (size_t)nreq * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return PIMC_OK;
+}
+
+int pimc_action_kinetic(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t nreq, double* out) {
+  if (!ctx || nreq < 0 || (nreq > 0 && (!reqs || !out))) return fail(PIMC_ERR_ARG, "action_kinetic: bad argument");
+  if (nreq == 0) return PIMC_OK;
+  HIP_TRY(hipSetDevice(ctx->device));
+  int rc = ctx->sync_descriptors();
+  if (rc) return rc;
+  rc = upload_reqs(ctx, reqs, nreq);
+  if (rc) return rc;
+  if ((rc = ctx->d_out.ensure((size_t)nreq * sizeof(double)))) return rc;
+  launch_kinetic(ctx->d_reps.as<ReplicaDev>(), ctx->d_reqs.as<ReqDev>(), nreq, ctx->d_out.as<double>(), ctx->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, ctx->d_out.p, (size_t)nreq * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return PIMC_OK;
+}
+
+int pimc_action_long_range(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t nreq, int32_t precision,
+                           double* out) {
+  if (!ctx || nreq < 0 || (nreq > 0 && (!reqs || !out))) return fail(PIMC_ERR_ARG, "action_long_range: bad argument");
+  for (int i = 0; i < nreq; i++) {
+    int rc = check_req(ctx, reqs[i]);
+    if (rc) return rc;
+    return fail(PIMC_ERR_STATE, "action_long_range: replica has no k set");
+  }
+  return PIMC_OK;
+}
+
+int pimc_dist_disp(pimc_ctx* ctx, int32_t replica, int32_t mode, int64_t n, const int32_t* idx, double* out,
+                   int32_t* images) {
+  HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
+  if (!H || !idx || !out || mode < 0 || mode > 1 || n < 0) return fail(PIMC_ERR_ARG, "dist_disp: bad argument");
+  for (int64_t i = 0; i < n; i++) {
+    const int32_t* q = idx + 4 * i;
+    if (q[0] < 0 || q[0] > H->M || q[1] < 0 || q[1] > H->M || q[2] < 0 || q[2] >= H->dev.core.N || q[3] < 0 ||
+        q[3] >= H->dev.core.N)
+      return fail(PIMC_ERR_ARG, "dist_disp: index out of range");
+  }
+  if (n == 0) return PIMC_OK;
+  HIP_TRY(hipSetDevice(ctx->device));
+  int rc = ctx->sync_descriptors();
+  if (rc) return rc;
+  if ((rc = ctx->d_idx.ensure((size_t)n * 4 * sizeof(int))) || (rc = ctx->d_out.ensure((size_t)n * 8 * sizeof(double))) ||
+      (rc = ctx->d_img.ensure((size_t)n * 6 * sizeof(int))))
+    return rc;
+  HIP_TRY(hipMemcpyAsync(ctx->d_idx.p, idx, (size_t)n * 4 * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  launch_dist_disp(ctx->d_reps.as<ReplicaDev>(), replica, mode, n, ctx->d_idx.as<int>(), ctx->d_out.as<double>(),
+                   ctx->d_img.as<int>(), ctx->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, ctx->d_out.p, (size_t)n * 8 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  if (images)
+    HIP_TRY(hipMemcpyAsync(images, ctx->d_img.p, (size_t)n * 6 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return PIMC_OK;
+}
+
+int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, const pimc_bisection_desc* desc,
+                       int64_t first_move, int32_t num_moves, int32_t precision, pimc_move_record* records,
+                       pimc_bisection_stats* stats) {
+  if (!ctx || !desc || nrep < 0 || (nrep > 0 && !replicas) || num_moves < 0 || first_move < 0 ||
+      first_move + num_moves > (int64_t)0xffffffffLL)
+    return fail(PIMC_ERR_ARG, "bisection_run: bad argument");
+  if (precision != PIMC_FP64 && precision != PIMC_MIXED) return fail(PIMC_ERR_ARG, "bisection_run: bad precision");
+  if (desc->num_levels < 1 || desc->num_levels > PIMC_MAX_LEVELS || desc->lowest_level < 0 ||
+      desc->lowest_level >= desc->num_levels)
+    return fail(PIMC_ERR_ARG, "bisection_run: bad levels");
+  std::set<int> seen;
+  int nt = -1;
+  size_t table_bytes = 0;
+  for (int r = 0; r < nrep; r++) {
+    HostReplica* H = ctx->rep(replicas[r]);
+    if (!H) return fail(PIMC_ERR_ARG, "bisection_run: bad replica");
+    if (!seen.insert(replicas[r]).second) return fail(PIMC_ERR_ARG, "bisection_run: replica listed twice");
+    const RepCore& R = H->dev.core;
+    if ((1 << desc->num_levels) >= R.nslices) return fail(PIMC_ERR_ARG, "bisection_run: too few slices for num_levels");
+    if (desc->species < 0 || desc->species >= R.ns || H->dev.sp.count[desc->species] < 1)
+      return fail(PIMC_ERR_ARG, "bisection_run: bad species");
+    for (int f = 0; f < R.ns * R.ns; f++) {
+      const HostFit& F = *ctx->fits[H->dev.sp.fit[f]];
+      if (nt < 0) nt = F.num_terms;
+      if (F.num_terms != nt) return fail(PIMC_ERR_UNSUPPORTED, "bisection_run: fits with different num_terms");
+      if (F.dev.taupos + desc->num_levels > F.dev.num_tau)
+        return fail(PIMC_ERR_ARG, "bisection_run: pair fit has too few tau levels");
+      table_bytes = std::max(table_bytes, precision == PIMC_MIXED ? F.staged32 : F.staged64);
+    }
+  }
+  if (stats) memset(stats, 0, sizeof *stats);
+  if (nrep == 0 || num_moves == 0) return PIMC_OK;
+  HIP_TRY(hipSetDevice(ctx->device));
+  int rc = ctx->sync_descriptors();
+  if (rc) return rc;
+  if ((rc = ctx->d_repids.ensure((size_t)nrep * sizeof(int))) || (rc = ctx->d_stats.ensure((size_t)nrep * 5 * sizeof(int64_t))))
+    return rc;
+  if (records && (rc = ctx->d_records.ensure((size_t)nrep * num_moves * sizeof(pimc_move_record)))) return rc;
+  HIP_TRY(hipMemcpyAsync(ctx->d_repids.p, replicas, (size_t)nrep * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  LaunchBisection a{};
+  a.reps = ctx->d_reps.as<ReplicaDev>();
+  a.fits = ctx->d_fits.as<FitDev>();
+  a.rep_ids = ctx->d_repids.as<int>();
+  a.nrep = nrep;
+  a.num_levels = desc->num_levels;
+  a.lowest_level = desc->lowest_level;
+  a.species = desc->species;
+  a.seed = desc->seed;
+  a.first_move = (uint32_t)first_move;
+  a.num_moves = num_moves;
+  a.records = records ? ctx->d_records.as<pimc_move_record>() : nullptr;
+  a.stats = ctx->d_stats.as<int64_t>();
+  a.lds_table_bytes = table_bytes;
+  if (!ctx->ev0) {
+    HIP_TRY(hipEventCreate(&ctx->ev0));
+    HIP_TRY(hipEventCreate(&ctx->ev1));
+  }
+  HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+  hipError_t e = launch_bisection(nt, precision == PIMC_MIXED, a, ctx->stream);
+  if (e != hipSuccess) return fail(PIMC_ERR_DEVICE, std::string("bisection launch: ") + hipGetErrorString(e));
+  HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+  if (records)
+    HIP_TRY(hipMemcpyAsync(records, ctx->d_records.p, (size_t)nrep * num_moves * sizeof(pimc_move_record),
+                           hipMemcpyDeviceToHost, ctx->stream));
+  std::vector<int64_t> st((size_t)nrep * 5);
+  HIP_TRY(hipMemcpyAsync(st.data(), ctx->d_stats.p, st.size() * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (stats) {
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    stats->device_ms = ms;
+    for (int r = 0; r < nrep; r++) {
+      stats->moves += st[5 * r + 0];
+      stats->accepted += st[5 * r + 1];
+      stats->stages += st[5 * r + 2];
+      stats->pair_evals += st[5 * r + 3];
+      stats->k_evals += st[5 * r + 4];
+    }
+  }
+  return PIMC_OK;
+}
+
+}  // extern "C"

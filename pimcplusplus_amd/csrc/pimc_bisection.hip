@@ -1,0 +1,284 @@
+// pimc_bisection.hip -- the GPU-resident bisection chain.
+//
+// One block (4 waves) owns one replica and runs `num_moves` bisection moves
+// (BisectionBlockClass::MakeMove with NoPermuteStage, StepsPerBlock = 1) in
+// order, so the Metropolis chain never leaves the device:
+//   ChooseTimeSlices, ChooseParticle            BisectionBlock.cc:363-374, NoPermuteClass.cc:61-68
+//   per level L-1 .. LowestLevel:
+//     BisectionStageClass::Sample                BisectionStage.cc:128-289
+//     StageAction(OLD), StageAction(NEW)         StageClass.cc:124-136 (Kinetic + ShortRange)
+//     Metropolis test                            StageClass.cc:69-72
+//   AcceptCopy / RejectCopy                      PathClass.cc:797-899
+// The moved particle's 2^L+1 beads (OLD and NEW) live in LDS; the level's
+// spline table is staged in LDS once per stage; partner beads stream from HBM
+// slice-major (one row per dimension, coalesced across the block's threads).
+// The path copies OLD and NEW are equal outside a move, so partners are read
+// from OLD and accepted beads are written to both.
+#include "pimc_kernels.hpp"
+#include "pimc_launch.hpp"
+
+namespace pimc {
+
+template <int NT, bool MIXED>
+__global__ void __launch_bounds__(kBlock) k_bisection(const LaunchBisection a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  PIMC_LDS unsigned char* smem = (PIMC_LDS unsigned char*)smem_raw;
+  const ReplicaDev* RD = a.reps + a.rep_ids[blockIdx.x];
+  const RepCore R = load_uniform(&RD->core);
+  const RepSpecies& S = RD->sp;
+  const int L = a.num_levels;
+  const int W = 1 << L;
+  const int sp = a.species;
+  const int tid = threadIdx.x;
+
+  PIMC_LDS unsigned char* tbl = smem;
+  PIMC_LDS double* mold = (PIMC_LDS double*)(smem + a.lds_table_bytes);   // [W+1][3]
+  PIMC_LDS double* mnew = mold + 3 * (W + 1);                              // [W+1][3]
+  PIMC_LDS double* lnew = mnew + 3 * (W + 1);                              // [W/2]
+  PIMC_LDS double* lold = lnew + W / 2;                                    // [W/2]
+  PIMC_LDS double* kold = lold + W / 2;                                    // [W]
+  PIMC_LDS double* knew = kold + W;                                        // [W]
+  PIMC_LDS double* scratch = knew + W;                                     // [kWaves]
+  PIMC_LDS int* ctl = (PIMC_LDS int*)(scratch + kWaves);                  // slice1, ptcl, accept
+
+  int64_t st_moves = 0, st_acc = 0, st_stages = 0, st_evals = 0;
+  const int numLeft = R.nslices - W;
+  const double lambda = S.lambda[sp];
+
+  for (int m = 0; m < a.num_moves; m++) {
+    const uint32_t move = a.first_move + (uint32_t)m;
+    if (tid == 0) {
+      ctl[0] = (int)floor(rng_uniform(a.seed, R.stream, move, kTagSlice, 0) * numLeft);
+      ctl[1] = (int)floor(rng_uniform(a.seed, R.stream, move, kTagPtcl, 0) * S.count[sp]) + S.first[sp];
+    }
+    __syncthreads();
+    const int slice1 = ctl[0];
+    const int ptcl = ctl[1];
+    for (int t = tid; t <= W; t += blockDim.x) {
+      double v[3];
+      load_bead(R, 0, slice1 + t, ptcl, v);
+#pragma unroll
+      for (int d = 0; d < 3; d++) mold[3 * t + d] = mnew[3 * t + d] = v[d];
+    }
+    __syncthreads();
+
+    double prev = 0.0;   // thread 0 only
+    bool accepted = true;
+    int stage = 0;
+    for (int level = L - 1; level >= a.lowest_level; level--, stage++) {
+      const int skip = 1 << (level + 1);
+      const int half = skip >> 1;
+      const int nb = W / skip;
+      const int K = W >> level;
+      const int step = 1 << level;
+
+      // ---- Sample: new midpoints of this level ----
+      const double levelTau = 0.5 * R.tau * skip;
+      const double sigma2 = (1.0 * lambda * levelTau);
+      const double sigma = sqrt(sigma2);
+      for (int b = tid; b < nb; b += blockDim.x) {
+        const int s = b * skip;
+        // gaussians 3b .. 3b+2 of this (move, level) come from pairs (3b)>>1 and (3b)>>1 + 1
+        double g0, g1, g2, g3;
+        const uint32_t p0 = (uint32_t)(3 * b) >> 1;
+        gaussian_pair(a.seed, R.stream, move, level, p0, g0, g1);
+        gaussian_pair(a.seed, R.stream, move, level, p0 + 1, g2, g3);
+        const bool odd = (3 * b) & 1;
+        const double gd[3] = {odd ? g1 : g0, odd ? g2 : g1, odd ? g3 : g2};
+        double GaussProd = 1.0, GaussProdOld = 1.0;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          const double bx = R.box[d], bi = R.boxinv[d], ip = R.isper[d];
+          double rdiffOld = put_in_box_1(mold[3 * (s + skip) + d] - mold[3 * s + d], bx, bi, ip);
+          double rbarOld = mold[3 * s + d] + 0.5 * rdiffOld;
+          double DeltaOld = put_in_box_1(mold[3 * (s + half) + d] - rbarOld, bx, bi, ip);
+          double rdiff = put_in_box_1(mnew[3 * (s + skip) + d] - mnew[3 * s + d], bx, bi, ip);
+          double rbar = mnew[3 * s + d] + 0.5 * rdiff;
+          double Delta = put_in_box_1(gd[d] * sigma, bx, bi, ip);
+          double GaussSum = 0.0, GaussSumOld = 0.0;
+          for (int image = -R.num_images; image <= R.num_images; image++) {
+            double dist = Delta + (double)image * bx;
+            double distOld = DeltaOld + (double)image * bx;
+            GaussSum += exp(-0.5 * dist * dist / sigma2);
+            GaussSumOld += exp(-0.5 * distOld * distOld / sigma2);
+          }
+          GaussProd *= GaussSum;
+          GaussProdOld *= GaussSumOld;
+          mnew[3 * (s + half) + d] = rbar + Delta;
+        }
+        lnew[b] = log(GaussProd);
+        lold[b] = log(GaussProdOld);
+      }
+      __syncthreads();
+
+      // ---- Kinetic action of every link, OLD and NEW ----
+      const double FourLambdaTauInv = 1.0 / (4.0 * lambda * (R.tau * (1 << level)));
+      for (int k = tid; k < K; k += blockDim.x) {
+        kold[k] = kinetic_link(R, mold + 3 * k * step, mold + 3 * (k + 1) * step, FourLambdaTauInv);
+        knew[k] = kinetic_link(R, mnew + 3 * k * step, mnew + 3 * (k + 1) * step, FourLambdaTauInv);
+      }
+
+      // ---- Short-range pair action, OLD and NEW (ShortRangeClass.cc:145-171) ----
+      Kahan acc_old, acc_new;
+      for (int s2 = 0; s2 < R.ns; s2++) {
+        const FitDev F = load_uniform(&a.fits[S.fit[sp * R.ns + s2]]);
+        __syncthreads();
+        auto T = [&] {
+          if constexpr (MIXED)
+            return stage_level32(F, level + F.taupos, tbl);
+          else
+            return stage_level64(F, level + F.taupos, tbl);
+        }();
+        __syncthreads();
+        const int j0 = S.first[s2], j1 = S.first[s2] + S.count[s2];
+        const size_t row = (size_t)3 * R.N * step;     // distance between consecutive link slices
+        for (int j = j0 + tid; j < j1; j += blockDim.x) {
+          if (j == ptcl) continue;
+          // partner beads walk the links in order; the bead one link ahead is
+          // loaded while the current link is evaluated
+          const double* pj = R.path0 + (size_t)slice1 * 3 * R.N + j;
+          double pa[3] = {pj[0], pj[R.N], pj[2 * R.N]};
+          pj += row;
+          double pb[3] = {pj[0], pj[R.N], pj[2 * R.N]};
+          for (int k = 0; k < K; k++) {
+            double pn[3] = {0.0, 0.0, 0.0};
+            if (k + 1 < K) {
+              pj += row;
+              pn[0] = pj[0];
+              pn[1] = pj[R.N];
+              pn[2] = pj[2 * R.N];
+            }
+            double moa[3], mob[3], mna[3], mnb[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+              moa[d] = mold[3 * k * step + d];
+              mob[d] = mold[3 * (k + 1) * step + d];
+              mna[d] = mnew[3 * k * step + d];
+              mnb[d] = mnew[3 * (k + 1) * step + d];
+            }
+            const double uo = pair_link_u<NT>(R, F, T, moa, mob, pa, pb);
+            const double un = pair_link_u<NT>(R, F, T, mna, mnb, pa, pb);
+            if (MIXED) {
+              acc_old.add(uo);
+              acc_new.add(un);
+            } else {
+              acc_old.s += uo;
+              acc_new.s += un;
+            }
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+              pa[d] = pb[d];
+              pb[d] = pn[d];
+            }
+          }
+        }
+      }
+      const double sr_old = block_sum(acc_old.value(), scratch);
+      const double sr_new = block_sum(acc_new.value(), scratch);
+
+      // ---- Metropolis test (StageClass.cc:69-72) ----
+      if (tid == 0) {
+        double logSampleProb = 0.0, logOldSampleProb = 0.0;
+        for (int b = 0; b < nb; b++) {
+          logOldSampleProb += 0.0 + lold[b];
+          logSampleProb += 0.0 + lnew[b];
+        }
+        const double logSampleRatio = -logSampleProb + logOldSampleProb;
+        double kin_o = 0.0, kin_n = 0.0;
+        for (int k = 0; k < K; k++) {
+          kin_o += kold[k];
+          kin_n += knew[k];
+        }
+        double oldAction = 0.0, newAction = 0.0;
+        oldAction += kin_o;
+        oldAction += sr_old;
+        newAction += kin_n;
+        newAction += sr_new;
+        const double currActionChange = newAction - oldAction;
+        const double logAcceptProb = logSampleRatio - currActionChange + prev;
+        const double logu = log(rng_uniform(a.seed, R.stream, move, kTagAccept, level));
+        const int ok = logAcceptProb >= logu;
+        prev = currActionChange;
+        ctl[2] = ok;
+        st_stages++;
+        st_evals += 2 * (int64_t)(R.N - 1) * K;
+        if (a.records) {
+          pimc_move_record* rec = a.records + (size_t)blockIdx.x * a.num_moves + m;
+          rec->old_action[stage] = oldAction;
+          rec->new_action[stage] = newAction;
+          rec->log_sample_ratio[stage] = logSampleRatio;
+          rec->log_u[stage] = logu;
+        }
+      }
+      __syncthreads();
+      if (!ctl[2]) {
+        accepted = false;
+        stage++;
+        break;
+      }
+    }
+
+    // ---- AcceptCopy: PutInBoxFast on the NEW beads, then OLD <- NEW ----
+    if (accepted) {
+      for (int t = tid; t <= W; t += blockDim.x) {
+        double v[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) v[d] = put_in_box_fast_1(mnew[3 * t + d], R.box[d]);
+        store_bead(R, 0, slice1 + t, ptcl, v);
+        store_bead(R, 1, slice1 + t, ptcl, v);
+      }
+    }
+    if (tid == 0) {
+      st_moves++;
+      st_acc += accepted ? 1 : 0;
+      if (a.records) {
+        pimc_move_record* rec = a.records + (size_t)blockIdx.x * a.num_moves + m;
+        rec->slice1 = slice1;
+        rec->ptcl = ptcl;
+        rec->accepted = accepted ? 1 : 0;
+        rec->num_stages = stage;
+        for (int s = stage; s < PIMC_MAX_LEVELS; s++) {
+          rec->old_action[s] = 0.0;
+          rec->new_action[s] = 0.0;
+          rec->log_sample_ratio[s] = 0.0;
+          rec->log_u[s] = 0.0;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    int64_t* st = a.stats + (size_t)blockIdx.x * 5;
+    st[0] = st_moves;
+    st[1] = st_acc;
+    st[2] = st_stages;
+    st[3] = st_evals;
+    st[4] = 0;
+  }
+}
+
+static size_t bisection_lds_bytes(const LaunchBisection& a) {
+  const int W = 1 << a.num_levels;
+  return a.lds_table_bytes + sizeof(double) * (6 * (W + 1) + W + 2 * W + kWaves) + 4 * sizeof(int);
+}
+
+template <bool MIXED>
+static hipError_t launch_bis_t(int nt, const LaunchBisection& a, hipStream_t s) {
+  const size_t lds = bisection_lds_bytes(a);
+  dim3 grid(a.nrep);
+  switch (nt) {
+    case 0: k_bisection<0, MIXED><<<grid, kBlock, lds, s>>>(a); break;
+    case 1: k_bisection<1, MIXED><<<grid, kBlock, lds, s>>>(a); break;
+    case 2: k_bisection<2, MIXED><<<grid, kBlock, lds, s>>>(a); break;
+    default: k_bisection<3, MIXED><<<grid, kBlock, lds, s>>>(a); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_bisection(int num_terms, bool mixed, const LaunchBisection& a, hipStream_t s) {
+  if (a.nrep <= 0 || a.num_moves <= 0) return hipSuccess;
+  if (bisection_lds_bytes(a) > 64 * 1024) return hipErrorInvalidValue;
+  return mixed ? launch_bis_t<true>(num_terms, a, s) : launch_bis_t<false>(num_terms, a, s);
+}
+
+}  // namespace pimc

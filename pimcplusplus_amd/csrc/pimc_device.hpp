@@ -1,0 +1,363 @@
+// pimc_device.hpp -- device-side building blocks of the action kernels (gfx950).
+//
+// Pair geometry, the spline-fitted pair action, the kinetic link action and the
+// counter-based random stream.  Geometry that decides an integer (periodic
+// image, grid interval) is written with floating-point contraction off, so the
+// integers are exactly the ones PIMC++'s source computes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pimc {
+
+constexpr int kMaxSpecies = 4;
+constexpr int kMaxLevels = 8;
+constexpr int kMaxActive = 4;
+
+// Plain vector types for table data (they load from LDS as single ds_read_b128s).
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+#define PIMC_LDS __attribute__((address_space(3)))
+
+// Pointer to const T, in LDS (InLds) or in global memory.
+template <bool InLds>
+struct Ptr {
+  template <typename T>
+  using c = const T*;
+};
+template <>
+struct Ptr<true> {
+  template <typename T>
+  using c = const PIMC_LDS T*;
+};
+
+// --------------------------------------------------------------------------
+// Device descriptors
+// --------------------------------------------------------------------------
+
+// One DavidPAClass fit.  Per tau level and grid interval i the spline columns
+// used by U are stored as cubic polynomials in the local coordinate
+// b = (x - X_i) * hinv_i:  value = c0 + b(c1 + b(c2 + b c3)).  Column 1 (u00)
+// comes first, then the off-diagonal columns 2..num_cols-1.
+struct FitDev {
+  int grid_type;      // 0 linear, 1 log
+  int ng;             // grid points
+  int nint;           // intervals = ng - 1
+  int num_tau;
+  int ncol;           // stored columns: 1 + noff
+  int noff;           // off-diagonal columns used by num_terms
+  int num_terms;
+  int taupos;
+  double start, end;
+  double deltainv;    // linear grid
+  double r0, spacing; // log grid
+  double logspacing;  // log(spacing), computed on the host
+  const double* x;        // [ng] grid points
+  const d2v* geo;         // [nint] {X_i, 1/(X_{i+1}-X_i)}
+  const d4v* c64;         // [num_tau][nint][ncol]
+  const f4v* c32;         // [num_tau][nint][ncol]
+  // [num_tau][nint]: 1 where an interval's coefficients are too large for FP32
+  // (sum over |c0..c3| of a column above kMixedLimit); the mixed path evaluates
+  // those intervals from c64.
+  const unsigned char* big;
+};
+
+// Largest |c0|+|c1|+|c2|+|c3| of a column the mixed path evaluates in FP32.
+constexpr double kMixedLimit = 1.0;
+
+// Scalars of a replica the inner loops use; kernels hold a block-uniform copy
+// (scalar registers).  Arrays are only indexed with compile-time constants.
+struct RepCore {
+  int N;              // particles
+  int nslices;        // M + 1
+  int ns;             // species
+  int num_images;
+  double box[3], boxinv[3], isper[3];
+  double tau;
+  uint32_t stream;
+  int nk;
+  double* path0;      // OLD copy, [slice][3][N] (slice-major, one row per dimension)
+  double* path1;      // NEW copy
+  int* perm0;         // [N]
+  int* perm1;
+  __device__ __host__ inline double* path(int mode) const { return mode ? path1 : path0; }
+  __device__ __host__ inline int* perm(int mode) const { return mode ? perm1 : perm0; }
+};
+
+// Per-species data, read through the descriptor pointer.
+struct RepSpecies {
+  int first[kMaxSpecies], count[kMaxSpecies];
+  double lambda[kMaxSpecies];
+  int fit[kMaxSpecies * kMaxSpecies];
+};
+
+struct ReplicaDev {
+  RepCore core;
+  RepSpecies sp;
+};
+
+// --------------------------------------------------------------------------
+// Philox4x32-10 and the random-stream layout
+//   key = seed; counter = {move, tag<<24 | level<<16 | attempt, index, stream}
+// --------------------------------------------------------------------------
+enum { kTagSlice = 1, kTagPtcl = 2, kTagGauss = 3, kTagAccept = 4 };
+
+__host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = (uint32_t)p1;
+    c[2] = n2;
+    c[3] = (uint32_t)p0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+__host__ __device__ inline double uniform53(uint32_t w0, uint32_t w1) {
+  uint64_t v = ((uint64_t)w1 << 32) | (uint64_t)w0;
+  return (double)(v >> 11) * 0x1.0p-53;
+}
+
+__host__ __device__ inline void rng_block(uint64_t seed, uint32_t stream, uint32_t move, int tag, int level,
+                                          uint32_t attempt, uint32_t index, uint32_t out[4]) {
+  out[0] = move;
+  out[1] = ((uint32_t)tag << 24) | (((uint32_t)level & 0xffu) << 16) | (attempt & 0xffffu);
+  out[2] = index;
+  out[3] = stream;
+  philox4x32_10(out, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+__host__ __device__ inline double rng_uniform(uint64_t seed, uint32_t stream, uint32_t move, int tag,
+                                              int level) {
+  uint32_t w[4];
+  rng_block(seed, stream, move, tag, level, 0, 0, w);
+  return uniform53(w[0], w[1]);
+}
+
+// Polar Box-Muller pair (Common/Random/Random.h:156-180) on its own sub-stream.
+__device__ inline void gaussian_pair(uint64_t seed, uint32_t stream, uint32_t move, int level, uint32_t pair,
+                                     double& g0, double& g1) {
+  for (uint32_t a = 0; a < 0x10000u; ++a) {
+    uint32_t w[4];
+    rng_block(seed, stream, move, kTagGauss, level, a, pair, w);
+    double V1 = 2.0 * uniform53(w[0], w[1]) - 1.0;
+    double V2 = 2.0 * uniform53(w[2], w[3]) - 1.0;
+    double S = V1 * V1 + V2 * V2;
+    if (S < 1.0 && S > 0.0) {
+      double temp = sqrt((-2.0 * log(S)) / S);
+      g0 = V1 * temp;
+      g1 = V2 * temp;
+      return;
+    }
+  }
+  g0 = g1 = 0.0;
+}
+
+// --------------------------------------------------------------------------
+// Periodic geometry (PIMC++/src/PathClass.h:696-746, 897-921; PathClass.cc:1389-1407)
+// --------------------------------------------------------------------------
+
+// PutInBox: n = -floor(v/L + 1/2); v += n L  (ties go up).
+__device__ inline double put_in_box_1(double v, double box, double boxinv, double isper) {
+#pragma clang fp contract(off)
+  double n = -floor(v * boxinv + 0.5);
+  return v + n * isper * box;
+}
+
+// PutInBoxFast.
+__device__ inline double put_in_box_fast_1(double v, double box) {
+#pragma clang fp contract(off)
+  double half = box / 2;
+  if (v > half) v -= box;
+  if (v < -half) v += box;
+  return v;
+}
+
+// DistDisp(sliceA, sliceB, p1, p2): dispA = r2(A) - r1(A) moved to its nearest
+// image n = -nearbyint(dispA/L); dispB = r2(B) - r1(B) moved by
+// m = nearbyint((dispA - dispB)/L), the image consistent with dispA.
+__device__ inline void dist_disp_1(double a, double b, double box, double boxinv, double isper, double& da,
+                                   double& db, double& n, double& m) {
+#pragma clang fp contract(off)
+  n = -rint(a * boxinv);
+  da = a + n * isper * box;
+  m = rint((da - b) * boxinv);
+  db = b + m * isper * box;
+}
+
+// --------------------------------------------------------------------------
+// Spline interval lookup (Common/Splines/Grid.h:81-84, 527-530; CubicSpline.h:490-501)
+// --------------------------------------------------------------------------
+__device__ inline int clamp_to_int(double t, int n) {
+  t = fmax(t, -2.0);
+  t = fmin(t, (double)(n + 1));
+  return (int)t;
+}
+
+// Returns `low`, the interval index the reference's MultiCubicSpline uses.
+// LINEAR: ReverseMap = (int)nearbyint((x-Start)*deltainv - 0.5).  LOG: the
+// reference's floor(log(x/r0)/log(spacing)), corrected by one step against the
+// stored knots so that a last-bit difference in log() cannot change it.
+template <typename XPtr>
+__device__ inline int spline_low(const FitDev& F, XPtr xs, double x) {
+#pragma clang fp contract(off)
+  int i;
+  if (F.grid_type == 0) {
+    i = clamp_to_int(rint((x - F.start) * F.deltainv - 0.5), F.ng);
+  } else {
+    i = clamp_to_int(floor(log(x / F.r0) / F.logspacing), F.ng);
+    if (i >= 1 && i <= F.ng - 2 && x < xs[i])
+      i -= 1;
+    else if (i >= -1 && i <= F.ng - 3 && x >= xs[i + 1])
+      i += 1;
+  }
+  int hi = i + 1;
+  if (hi < 1) hi = 1;
+  if (hi > F.ng - 1) hi = F.ng - 1;
+  return hi - 1;
+}
+
+// --------------------------------------------------------------------------
+// Pair action U(q, z, s^2) of a DavidPAClass fit at one tau level
+// (Common/PairAction/DavidPAClass.cc:92-102, 468-511), tables in LDS.
+// --------------------------------------------------------------------------
+template <bool InLds>
+struct LevelTable64 {
+  typename Ptr<InLds>::template c<d2v> geo;   // [nint]
+  typename Ptr<InLds>::template c<d4v> c;     // [nint][ncol]
+  typename Ptr<InLds>::template c<double> x;  // [ng] (log grid correction)
+};
+template <bool InLds>
+struct LevelTable32 {
+  typename Ptr<InLds>::template c<d2v> geo;
+  typename Ptr<InLds>::template c<f4v> c;            // FP32 coefficients
+  typename Ptr<InLds>::template c<double> x;
+  typename Ptr<InLds>::template c<unsigned char> big; // [nint] FP64 fallback flags
+  const d4v* c64;                                     // FP64 coefficients of this level (global memory)
+};
+
+__device__ inline double horner64(const d4v c, double b) { return c.x + b * (c.y + b * (c.z + b * c.w)); }
+__device__ inline float horner32(const f4v c, float b) { return c.x + b * (c.y + b * (c.z + b * c.w)); }
+
+template <int NT, bool InLds>
+__device__ inline double pair_u64(const FitDev& F, const LevelTable64<InLds>& T, double q, double z, double s2) {
+  const int ncol = F.ncol;
+  double rmin = F.start;
+  double r = q + 0.5 * z;
+  double rp = q - 0.5 * z;
+  if (r > F.end) r = F.end;
+  if (rp > F.end) rp = F.end;
+  if ((rp < rmin) || (r < rmin)) return T.c[0].x;  // u00(rmin): interval 0 at b = 0
+  int lr = spline_low(F, T.x, r);
+  int lp = spline_low(F, T.x, rp);
+  d2v gr = T.geo[lr], gp = T.geo[lp];
+  double br = (r - gr.x) * gr.y;
+  double bp = (rp - gp.x) * gp.y;
+  double U = 0.5 * (horner64(T.c[lr * ncol], br) + horner64(T.c[lp * ncol], bp));
+  if constexpr (NT > 0) {
+    if (!(s2 > 0.0 && q < F.end)) return U;
+    int lq = spline_low(F, T.x, q);
+    d2v gq = T.geo[lq];
+    double bq = (q - gq.x) * gq.y;
+    const int cq = lq * ncol + 1;
+    // sum_{k=1..NT} sum_{j=0..k} u_kj(q) z^{2j} s^{2(k-j)}, table column k(k+1)/2 + j + 1
+    double zp[NT + 1], sp[NT + 1];
+    zp[0] = 1.0;
+    sp[0] = 1.0;
+#pragma unroll
+    for (int t = 1; t <= NT; t++) {
+      zp[t] = zp[t - 1] * (z * z);
+      sp[t] = sp[t - 1] * s2;
+    }
+    double off = 0.0;
+    int col = 0;
+#pragma unroll
+    for (int k = 1; k <= NT; k++)
+#pragma unroll
+      for (int j = 0; j <= k; j++, col++) off += horner64(T.c[cq + col], bq) * (zp[j] * sp[k - j]);
+    U += off;
+  }
+  return U;
+}
+
+// sqrt in FP64 from an FP32 reciprocal-square-root seed and one Goldschmidt
+// step (relative error ~1e-15); used by the mixed path for pair distances.
+__device__ inline double sqrt_seeded(double a) {
+  if (!(a > 1e-30) || !(a < 1e30)) return sqrt(a);
+  double y = (double)__builtin_amdgcn_rsqf((float)a);
+  double g = a * y;
+  double h = 0.5 * y;
+  double d = fma(-g, h, 0.5);
+  return fma(g, d, g);
+}
+
+// Columns [c0col, c0col + NC) of interval i at local coordinate b: FP32 Horner
+// from the FP32 table, or FP64 from the global FP64 table where flagged.
+template <int NC, bool InLds>
+__device__ inline void eval_cols_mixed(const LevelTable32<InLds>& T, int ncol, int i, int c0col, double b,
+                                       double out[NC]) {
+  if (T.big[i]) {
+    const d4v* c = T.c64 + (size_t)i * ncol + c0col;
+#pragma unroll
+    for (int k = 0; k < NC; k++) out[k] = horner64(c[k], b);
+  } else {
+    const float bf = (float)b;
+    const int at = i * ncol + c0col;
+#pragma unroll
+    for (int k = 0; k < NC; k++) out[k] = (double)horner32(T.c[at + k], bf);
+  }
+}
+
+// Mixed-precision U: FP64 geometry, interval lookup and powers; the spline
+// polynomials in FP32 except on flagged intervals; the sum in FP64.
+template <int NT, bool InLds>
+__device__ inline double pair_u_mixed(const FitDev& F, const LevelTable32<InLds>& T, double q, double z, double s2) {
+  const int ncol = F.ncol;
+  double rmin = F.start;
+  double r = q + 0.5 * z;
+  double rp = q - 0.5 * z;
+  if (r > F.end) r = F.end;
+  if (rp > F.end) rp = F.end;
+  if ((rp < rmin) || (r < rmin)) return T.c64[0].x;  // u00(rmin)
+  int lr = spline_low(F, T.x, r);
+  int lp = spline_low(F, T.x, rp);
+  d2v gr = T.geo[lr], gp = T.geo[lp];
+  double ur[1], up[1];
+  eval_cols_mixed<1>(T, ncol, lr, 0, (r - gr.x) * gr.y, ur);
+  eval_cols_mixed<1>(T, ncol, lp, 0, (rp - gp.x) * gp.y, up);
+  double U = 0.5 * (ur[0] + up[0]);
+  if constexpr (NT > 0) {
+    if (!(s2 > 0.0 && q < F.end)) return U;
+    constexpr int NOFF = NT * (NT + 3) / 2;
+    int lq = spline_low(F, T.x, q);
+    d2v gq = T.geo[lq];
+    double cv[NOFF];
+    eval_cols_mixed<NOFF>(T, ncol, lq, 1, (q - gq.x) * gq.y, cv);
+    double zp[NT + 1], sp[NT + 1];
+    zp[0] = 1.0;
+    sp[0] = 1.0;
+#pragma unroll
+    for (int t = 1; t <= NT; t++) {
+      zp[t] = zp[t - 1] * (z * z);
+      sp[t] = sp[t - 1] * s2;
+    }
+    double off = 0.0;
+    int col = 0;
+#pragma unroll
+    for (int k = 1; k <= NT; k++)
+#pragma unroll
+      for (int j = 0; j <= k; j++, col++) off += cv[col] * (zp[j] * sp[k - j]);
+    U += off;
+  }
+  return U;
+}
+
+}  // namespace pimc

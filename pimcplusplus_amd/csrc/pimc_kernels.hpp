@@ -1,0 +1,217 @@
+// pimc_kernels.hpp -- kernel building blocks of the action library (gfx950, wave64).
+#pragma once
+
+#include "pimc_device.hpp"
+#include "../../include/pimc_gpu.h"
+
+namespace pimc {
+
+constexpr int kBlock = 256;          // threads per block: 4 waves
+constexpr int kWaves = kBlock / 64;
+
+// Request of the SingleAction entry points, as uploaded to the device.
+struct ReqDev {
+  int replica, mode, slice1, slice2, level, nptcls;
+  int ptcls[kMaxActive];
+};
+
+// A block-uniform copy of *p: every 32-bit word goes through readfirstlane, so
+// the copy lives in scalar registers instead of one copy per lane.
+template <typename T>
+__device__ inline T load_uniform(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "word-sized struct");
+  T out;
+  const int* src = (const int*)p;
+  int* dst = (int*)&out;
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); i++) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+  return out;
+}
+
+// --------------------------------------------------------------------------
+// Reductions
+// --------------------------------------------------------------------------
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum of v over the block, in a fixed order; the result is valid in thread 0.
+__device__ inline double block_sum(double v, PIMC_LDS double* scratch) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) scratch[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += scratch[w];
+  __syncthreads();
+  return t;
+}
+
+// Kahan-compensated FP64 accumulation of the mixed path's terms.
+struct Kahan {
+  double s = 0.0, c = 0.0;
+  __device__ inline void add(double x) {
+    double y = x - c;
+    double t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  __device__ inline double value() const { return s; }
+};
+
+// --------------------------------------------------------------------------
+// Table views: one tau level of a fit, in global memory or staged in LDS
+// --------------------------------------------------------------------------
+__device__ inline LevelTable64<false> level_table64(const FitDev& F, int lev) {
+  LevelTable64<false> t;
+  t.geo = F.geo;
+  t.c = F.c64 + (size_t)lev * F.nint * F.ncol;
+  t.x = F.x;
+  return t;
+}
+__device__ inline LevelTable32<false> level_table32(const FitDev& F, int lev) {
+  LevelTable32<false> t;
+  t.geo = F.geo;
+  t.c = F.c32 + (size_t)lev * F.nint * F.ncol;
+  t.x = F.x;
+  t.big = F.big + (size_t)lev * F.nint;
+  t.c64 = F.c64 + (size_t)lev * F.nint * F.ncol;
+  return t;
+}
+
+template <int NT, bool InLds>
+__device__ inline double pair_u(const FitDev& F, const LevelTable64<InLds>& T, double q, double z, double s2) {
+  return pair_u64<NT>(F, T, q, z, s2);
+}
+template <int NT, bool InLds>
+__device__ inline double pair_u(const FitDev& F, const LevelTable32<InLds>& T, double q, double z, double s2) {
+  return pair_u_mixed<NT>(F, T, q, z, s2);
+}
+
+template <typename Table>
+struct IsMixed {
+  static constexpr bool value = false;
+};
+template <bool InLds>
+struct IsMixed<LevelTable32<InLds>> {
+  static constexpr bool value = true;
+};
+
+// Bytes of LDS one staged level of fit F takes: geo + coefficients + grid (+ flags).
+__host__ __device__ inline size_t staged_bytes(int nint, int ncol, int ng, bool mixed) {
+  size_t b = (size_t)nint * 16 + (size_t)nint * ncol * (mixed ? 16 : 32) + (size_t)ng * 8 + (mixed ? nint : 0);
+  return (b + 15) & ~(size_t)15;
+}
+
+// Block-wide copy of n elements from global memory to LDS, four loads in flight per thread.
+template <typename T>
+__device__ inline void block_copy(PIMC_LDS T* dst, const T* __restrict__ src, int n) {
+  const int bs = blockDim.x;
+  for (int base = threadIdx.x; base < n; base += 4 * bs) {
+    T v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (base + u * bs < n) v[u] = src[base + u * bs];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (base + u * bs < n) dst[base + u * bs] = v[u];
+  }
+}
+
+// Cooperative copy of level `lev` of F into LDS at `base` (FP64 view).
+__device__ inline LevelTable64<true> stage_level64(const FitDev& F, int lev, PIMC_LDS unsigned char* base) {
+  PIMC_LDS d2v* geo = (PIMC_LDS d2v*)base;
+  PIMC_LDS d4v* c = (PIMC_LDS d4v*)(base + (size_t)F.nint * 16);
+  const int nc = F.nint * F.ncol;
+  PIMC_LDS double* xs = (PIMC_LDS double*)((PIMC_LDS unsigned char*)c + (size_t)nc * 32);
+  block_copy(geo, F.geo, F.nint);
+  block_copy(c, F.c64 + (size_t)lev * nc, nc);
+  block_copy(xs, F.x, F.ng);
+  LevelTable64<true> t;
+  t.geo = geo;
+  t.c = c;
+  t.x = xs;
+  return t;
+}
+
+// Cooperative copy of level `lev` of F into LDS at `base` (mixed view).
+__device__ inline LevelTable32<true> stage_level32(const FitDev& F, int lev, PIMC_LDS unsigned char* base) {
+  PIMC_LDS d2v* geo = (PIMC_LDS d2v*)base;
+  PIMC_LDS f4v* c = (PIMC_LDS f4v*)(base + (size_t)F.nint * 16);
+  const int nc = F.nint * F.ncol;
+  PIMC_LDS double* xs = (PIMC_LDS double*)((PIMC_LDS unsigned char*)c + (size_t)nc * 16);
+  PIMC_LDS unsigned char* big = (PIMC_LDS unsigned char*)(xs + F.ng);
+  block_copy(geo, F.geo, F.nint);
+  block_copy(c, F.c32 + (size_t)lev * nc, nc);
+  block_copy(xs, F.x, F.ng);
+  for (int i = threadIdx.x; i < F.nint; i += blockDim.x) big[i] = F.big[(size_t)lev * F.nint + i];
+  LevelTable32<true> t;
+  t.geo = geo;
+  t.c = c;
+  t.x = xs;
+  t.big = big;
+  t.c64 = F.c64 + (size_t)lev * nc;
+  return t;
+}
+
+// --------------------------------------------------------------------------
+// One pair-link term: DistDisp + U for moved bead positions (m_a, m_b) and
+// partner positions (p_a, p_b), returning U (ShortRangeClass.cc:153-167).
+// --------------------------------------------------------------------------
+template <int NT, typename Table>
+__device__ inline double pair_link_u(const RepCore& R, const FitDev& F, const Table& T, const double ma[3],
+                                     const double mb[3], const double pa[3], const double pb[3]) {
+  constexpr bool kMixed = IsMixed<Table>::value;
+  double da[3], db[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    double n, m;
+    dist_disp_1(pa[d] - ma[d], pb[d] - mb[d], R.box[d], R.boxinv[d], R.isper[d], da[d], db[d], n, m);
+  }
+  double a2 = da[0] * da[0] + da[1] * da[1] + da[2] * da[2];
+  double b2 = db[0] * db[0] + db[1] * db[1] + db[2] * db[2];
+  double ra = kMixed ? sqrt_seeded(a2) : sqrt(a2);
+  double rb = kMixed ? sqrt_seeded(b2) : sqrt(b2);
+  double e0 = da[0] - db[0], e1 = da[1] - db[1], e2 = da[2] - db[2];
+  double s2 = e0 * e0 + e1 * e1 + e2 * e2;
+  double q = 0.5 * (ra + rb);
+  double z = ra - rb;
+  return pair_u<NT>(F, T, q, z, s2);
+}
+
+__device__ inline void load_bead(const RepCore& R, int mode, int slice, int p, double out[3]) {
+  const double* base = R.path(mode) + (size_t)slice * 3 * R.N;
+  out[0] = base[p];
+  out[1] = base[R.N + p];
+  out[2] = base[2 * R.N + p];
+}
+__device__ inline void store_bead(const RepCore& R, int mode, int slice, int p, const double v[3]) {
+  double* base = R.path(mode) + (size_t)slice * 3 * R.N;
+  base[p] = v[0];
+  base[R.N + p] = v[1];
+  base[2 * R.N + p] = v[2];
+}
+
+// --------------------------------------------------------------------------
+// Kinetic link action (KineticClass.cc:33-90): -log prod_d sum_images exp(-(v_d + iL)^2/(4 lambda tau))
+// --------------------------------------------------------------------------
+template <typename P>
+__device__ inline double kinetic_link(const RepCore& R, P a, P b, double FourLambdaTauInv) {
+  double GaussProd = 1.0;
+#pragma unroll
+  for (int dim = 0; dim < 3; dim++) {
+    double vel = put_in_box_1(b[dim] - a[dim], R.box[dim], R.boxinv[dim], R.isper[dim]);
+    double GaussSum = 0.0;
+    for (int image = -R.num_images; image <= R.num_images; image++) {
+      double dist = vel + (double)image * R.box[dim];
+      GaussSum += exp(-dist * dist * FourLambdaTauInv);
+    }
+    GaussProd *= GaussSum;
+  }
+  return -log(GaussProd);
+}
+
+}  // namespace pimc

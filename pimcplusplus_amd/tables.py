@@ -1,0 +1,223 @@
+"""Pair-action tables in the layout of PIMC++'s DavidPAClass.
+
+Two sources:
+
+* `read_dm` parses a squarer ``.dm`` file as DavidPAClass::ReadDavidSquarerFile
+  does (Common/PairAction/DavidPAClass.cc:639-912): the potential block, then
+  the rank-3 ``fit= n`` tables; the table whose fit order NMax is 2 becomes the
+  spline data, with column 0 the potential and the last grid row zeroed.
+* `lj_david_table` builds the same layout for a Lennard-Jones pair.  PIMC++
+  ships no Lennard-Jones pair action, so this is this package's own generator:
+  the diagonal is the primitive end-point action u00(r) = tau V(r) and the
+  off-diagonal u_kj(q) are the fourth-order expansion, in the DavidPAClass
+  basis z^{2j} s^{2(k-j)}, of the straight-line (semiclassical) action
+  tau * int_0^1 V(|r + t(r' - r)|) dt minus the end-point average.
+"""
+
+from __future__ import annotations
+
+import re
+from dataclasses import dataclass
+
+import numpy as np
+
+GRID_LINEAR, GRID_LOG = 0, 1
+
+
+@dataclass
+class DavidTable:
+    grid_type: int
+    grid_start: float
+    grid_end: float
+    y: np.ndarray            # [num_tau][num_points][num_cols]
+    num_terms: int
+    smallest_tau: float
+    dy: np.ndarray | None = None   # beta-derivative table, same layout (dukj), if known
+
+    @property
+    def num_tau(self) -> int:
+        return self.y.shape[0]
+
+    @property
+    def num_points(self) -> int:
+        return self.y.shape[1]
+
+    def grid_points(self) -> np.ndarray:
+        n = self.num_points
+        if self.grid_type == GRID_LINEAR:
+            delta = (self.grid_end - self.grid_start) / float(n - 1)
+            return self.grid_start + np.arange(n, dtype=np.float64) * delta
+        spacing = (self.grid_end / self.grid_start) ** (1.0 / (n - 1.0))
+        return self.grid_start * spacing ** np.arange(n, dtype=np.float64)
+
+
+# --------------------------------------------------------------------------
+# .dm squarer files
+# --------------------------------------------------------------------------
+_NUM = re.compile(r"[-+]?(?:\d+\.?\d*|\.\d+)(?:[EeDd][-+]?\d+)?")
+
+
+def _floats(line: str) -> list[float]:
+    return [float(t.replace("D", "E").replace("d", "e")) for t in _NUM.findall(line)]
+
+
+def read_dm(path: str) -> DavidTable:
+    """Reads a squarer .dm file into the DavidPAClass table layout.
+
+    Follows DavidPAClass::ReadDavidSquarerFile (DavidPAClass.cc:639-912): the
+    number of fits is the fourth number on the SQUARER line; the potential is
+    the first RANK 2 block; each RANK 3 block carries a GRID 1 (r grid), a
+    GRID 3 LOG tau grid and a 'BEGIN density matrix table <NMax> <derv>' header
+    followed by the table in Fortran order (r fastest, then term, then tau).
+    The NMax == 2 tables (U and its beta derivative) become the spline data.
+    """
+    with open(path) as f:
+        lines = f.read().splitlines()
+    i = 0
+
+    def skip_to(key: str) -> str:
+        nonlocal i
+        while i < len(lines) and key not in lines[i]:
+            i += 1
+        if i >= len(lines):
+            raise ValueError(f"{path}: no '{key}' found")
+        line = lines[i]
+        i += 1
+        return line
+
+    sq = skip_to("SQUARER").split()
+    num_fits = int(float(sq[4]))
+    rank_line = skip_to("RANK")
+    npot = int(rank_line.split()[2])
+    skip_to("potential")
+    potential: list[float] = []
+    while len(potential) < npot:
+        potential += _floats(lines[i])
+        i += 1
+    potential_arr = np.array(potential[:npot])
+
+    def read_tables(expect_derv: int):
+        nonlocal i
+        found = None
+        counter = 0
+        while counter <= num_fits:
+            words = skip_to("RANK").split()
+            if int(words[1]) != 3:
+                continue
+            ngrid, nukj, ntau = int(words[2]), int(words[3]), int(words[4])
+            g = skip_to("GRID 1").split()
+            gtype, gstart, gend = g[2], float(g[3]), float(g[4])
+            t = skip_to("GRID   3").split()
+            if t[2] != "LOG":
+                raise ValueError(f"{path}: tau grid is not LOG")
+            tmin, tmax = float(t[3]), float(t[4])
+            begin = skip_to("BEGIN density matrix table")
+            ints = [int(v) for v in re.findall(r"=\s*(\d+)", begin)]
+            nmax, derv = ints[0], ints[1]
+            if derv != expect_derv:
+                raise ValueError(f"{path}: table derv {derv}, expected {expect_derv}")
+            vals: list[float] = []
+            need = ngrid * nukj * ntau
+            while len(vals) < need:
+                vals += _floats(lines[i])
+                i += 1
+            tab = np.array(vals[:need]).reshape(ntau, nukj, ngrid)  # Fortran order
+            if nmax == 2:
+                found = (gtype, gstart, gend, ngrid, nukj, ntau, tmin, tmax, tab)
+            counter += 1
+        if found is None:
+            raise ValueError(f"{path}: no NMax == 2 table")
+        return found
+
+    gtype, gstart, gend, ngrid, nukj, ntau, tmin, tmax, u = read_tables(1)
+    _, _, _, _, _, _, _, _, du = read_tables(2)
+
+    def assemble(tab):
+        y = np.zeros((ntau, ngrid, nukj + 1))
+        y[:, :, 0] = potential_arr[None, :]
+        y[:, :, 1:] = np.transpose(tab, (0, 2, 1))
+        y[:, ngrid - 1, :] = 0.0        # DavidPAClass.cc:759,871
+        return y
+
+    grid_type = {"LINEAR": GRID_LINEAR, "LOG": GRID_LOG}[gtype]
+    # fit order 2 -> NumOffDiagonalTerms 2 uses columns up to 2*3/2+2+1 = 6
+    return DavidTable(grid_type=grid_type, grid_start=gstart, grid_end=gend, y=assemble(u),
+                      num_terms=2, smallest_tau=tmin, dy=assemble(du))
+
+
+# --------------------------------------------------------------------------
+# Lennard-Jones generator
+# --------------------------------------------------------------------------
+def lj_potential(r, eps: float, sigma: float):
+    sr6 = (sigma / np.asarray(r, dtype=np.float64)) ** 6
+    return 4.0 * eps * (sr6 * sr6 - sr6)
+
+
+def num_cols_for_terms(num_terms: int) -> int:
+    return num_terms * (num_terms + 1) // 2 + num_terms + 2
+
+
+def _lj_f_derivs(x, eps: float, sigma: float):
+    """f(x) = V(sqrt(x)) for Lennard-Jones and its first four x-derivatives."""
+    s6, s12 = sigma ** 6, sigma ** 12
+    c = 4.0 * eps
+    f1 = c * (-6.0 * s12 * x ** -7 + 3.0 * s6 * x ** -4)
+    f2 = c * (42.0 * s12 * x ** -8 - 12.0 * s6 * x ** -5)
+    f3 = c * (-336.0 * s12 * x ** -9 + 60.0 * s6 * x ** -6)
+    f4 = c * (3024.0 * s12 * x ** -10 - 360.0 * s6 * x ** -7)
+    return f1, f2, f3, f4
+
+
+def straight_line_ukj(q, tau: float, f1, f2, f3, f4):
+    """Off-diagonal coefficients of tau*int_0^1 V(|r + t(r'-r)|)dt - tau(V(r)+V(r'))/2.
+
+    With r = q + z/2, r' = q - z/2, s = |r - r'| and f(x) = V(sqrt x), the squared
+    distance along the chord is q^2 + z^2/4 + (1-2t) q z - t(1-t) s^2; expanding both
+    the chord integral and the end-point average to fourth order in (z, s) gives
+    u10 s^2 + u11 z^2 + u20 s^4 + u21 z^2 s^2 + u22 z^4 with the coefficients below
+    (derivatives of f taken at x = q^2).
+    """
+    q2 = q * q
+    u10 = -tau * f1 / 6.0
+    u11 = -tau * f2 * q2 / 3.0
+    u20 = tau * f2 / 60.0
+    u21 = -tau * f2 / 24.0 - tau * f3 * q2 / 60.0
+    u22 = -tau * f3 * q2 / 12.0 - tau * f4 * q2 * q2 / 30.0
+    return np.stack([u10, u11, u20, u21, u22], axis=-1)
+
+
+def lj_david_table(eps: float, sigma: float, r_min: float, r_max: float, num_points: int,
+                   smallest_tau: float, num_tau: int, num_terms: int = 2,
+                   damp: tuple = (0.80, 0.90)) -> DavidTable:
+    """A Lennard-Jones pair action table on LinearGrid(r_min, r_max, num_points).
+
+    Column 0 is V, column 1 u00 = tau V, columns 2..6 the straight-line expansion
+    coefficients (straight_line_ukj), switched off smoothly inside the repulsive core
+    (q between damp[0]*sigma and damp[1]*sigma) where the expansion does not converge.
+    num_terms is 0 (primitive action) or 2.
+    """
+    if num_terms not in (0, 2):
+        raise ValueError("num_terms must be 0 or 2")
+    delta = (r_max - r_min) / float(num_points - 1)
+    r = r_min + np.arange(num_points) * delta
+    vr = lj_potential(r, eps, sigma)
+    ncols = num_cols_for_terms(num_terms)
+    y = np.zeros((num_tau, num_points, ncols))
+    dy = np.zeros_like(y)
+    a, b = damp[0] * sigma, damp[1] * sigma
+    t = np.clip((r - a) / (b - a), 0.0, 1.0)
+    switch = t * t * (3.0 - 2.0 * t)
+    f1, f2, f3, f4 = _lj_f_derivs(r * r, eps, sigma)
+    for lev in range(num_tau):
+        tau = smallest_tau * 2.0 ** lev
+        y[lev, :, 0] = vr
+        y[lev, :, 1] = tau * vr
+        if num_terms == 2:
+            y[lev, :, 2:7] = switch[:, None] * straight_line_ukj(r, tau, f1, f2, f3, f4)
+        # the straight-line action is linear in tau: dU/dtau = U/tau; column 0 stays V
+        dy[lev, :, 0] = vr
+        dy[lev, :, 1:] = y[lev, :, 1:] / tau
+    y[:, num_points - 1, :] = 0.0    # the .dm reader's convention (DavidPAClass.cc:759)
+    dy[:, num_points - 1, :] = 0.0
+    return DavidTable(grid_type=GRID_LINEAR, grid_start=float(r_min), grid_end=float(r_max), y=y,
+                      num_terms=num_terms, smallest_tau=smallest_tau, dy=dy)

@@ -1,0 +1,145 @@
+"""Synthetic PIMC systems of the benchmark configurations.
+
+Each builder returns plain data (tables, replica parameters, initial beads) so
+the same system can be loaded into any library exporting the pimc_* ABI.
+
+Units: Kelvin and Angstrom for the Lennard-Jones systems.  lambda = hbar^2/2m
+for a 4 amu particle is 24.2542/4 K A^2.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .tables import DavidTable, lj_david_table
+
+LAMBDA_AMU = 24.2542          # hbar^2 / (2 * 1 amu * k_B) in K A^2
+LJ_EPS = 10.22                # K
+LJ_SIGMA = 2.556              # A
+LJ_DENSITY = 0.02186          # A^-3
+
+
+@dataclass
+class ReplicaSpec:
+    num_particles: int
+    num_time_slices: int
+    box: tuple
+    tau: float
+    species_count: list
+    lambdas: list
+    fit_keys: list             # [ns*ns] keys into System.tables
+    beads: np.ndarray          # [(M+1)][N][3]
+    rng_stream: int = 0
+    num_images: int = 0
+    k_cutoff: float = 0.0
+    use_long_range: bool = False
+    use_background: bool = False
+
+
+@dataclass
+class System:
+    name: str
+    tables: dict               # key -> DavidTable
+    desired_tau: dict          # key -> tau the fit is selected at
+    replicas: list             # [ReplicaSpec]
+    num_levels: int
+    seed: int
+    species: int = 0
+    meta: dict = field(default_factory=dict)
+
+
+def lj_box(n: int, density: float = LJ_DENSITY) -> float:
+    return (n / density) ** (1.0 / 3.0)
+
+
+def random_start(n: int, box: float, nslices: int, rng: np.random.Generator,
+                 jitter: float = 0.1) -> np.ndarray:
+    """A classical start, identical on every slice: n sites drawn at random from the
+    smallest fcc lattice (4k^3 sites) that holds them, each jittered uniformly by
+    +-jitter, in [-L/2, L/2)^3.  A uniform random start at this density places pairs
+    deep in the repulsive core; the lattice is the low-energy equivalent of the
+    reference's RANDOMFIXED initial paths (PIMC++/src/InitPaths.cc:967-999)."""
+    k = 1
+    while 4 * k ** 3 < n:
+        k += 1
+    a = box / k
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    cells = np.stack(np.meshgrid(np.arange(k), np.arange(k), np.arange(k), indexing="ij"), -1).reshape(-1, 3)
+    sites = ((cells[:, None, :] + basis[None, :, :]).reshape(-1, 3) + 0.25) * a - 0.5 * box
+    pick = rng.permutation(sites.shape[0])[:n]
+    pos = sites[pick] + rng.uniform(-jitter, jitter, size=(n, 3))
+    pos -= box * np.floor(pos / box + 0.5)
+    return np.repeat(pos[None, :, :], nslices + 1, axis=0).copy()
+
+
+def lj_table_for(box: float, tau: float, num_levels: int, num_points: int = 200,
+                 r_min: float = 0.6 * LJ_SIGMA) -> DavidTable:
+    return lj_david_table(LJ_EPS, LJ_SIGMA, r_min, box / 2.0, num_points, tau, num_levels)
+
+
+def lj_system(name: str, n: int, m: int, tau: float, num_levels: int, seed: int,
+              num_replicas: int = 1) -> System:
+    """num_replicas independent copies of one LJ system (different starts and random streams)."""
+    box = lj_box(n)
+    table = lj_table_for(box, tau, num_levels)
+    rng = np.random.default_rng(seed)
+    lam = LAMBDA_AMU / 4.0
+    reps = []
+    for r in range(num_replicas):
+        reps.append(ReplicaSpec(num_particles=n, num_time_slices=m, box=(box, box, box), tau=tau,
+                                species_count=[n], lambdas=[lam], fit_keys=["lj"],
+                                beads=random_start(n, box, m, rng), rng_stream=r))
+    return System(name=name, tables={"lj": table}, desired_tau={"lj": tau}, replicas=reps,
+                  num_levels=num_levels, seed=seed)
+
+
+def config0(num_replicas: int = 1) -> System:
+    """64 LJ particles, M=32, tau=0.025, level-3 bisection, seed 1000."""
+    return lj_system("lj-n64-m32-l3", 64, 32, 0.025, 3, 1000, num_replicas)
+
+
+def config1(num_replicas: int = 1) -> System:
+    """256 LJ particles, M=64, tau=0.0125, level-4 bisection, seed 1001."""
+    return lj_system("lj-n256-m64-l4", 256, 64, 0.0125, 4, 1001, num_replicas)
+
+
+def config3(num_replicas: int = 128, seed: int = 1003) -> System:
+    """Uneven LJ replicas: N log-uniform in [32, 512], M uniform in {16, 32, 64, 128},
+    tau=0.025, level-3 bisection.  Each replica has its own box (density fixed) and table."""
+    rng = np.random.default_rng(seed)
+    tau, num_levels = 0.025, 3
+    lam = LAMBDA_AMU / 4.0
+    tables, dtau, reps = {}, {}, []
+    for r in range(num_replicas):
+        n = int(round(np.exp(rng.uniform(np.log(32), np.log(512)))))
+        m = int(rng.choice([16, 32, 64, 128]))
+        box = lj_box(n)
+        key = f"lj{n}"
+        if key not in tables:
+            tables[key] = lj_table_for(box, tau, num_levels)
+            dtau[key] = tau
+        reps.append(ReplicaSpec(num_particles=n, num_time_slices=m, box=(box, box, box), tau=tau,
+                                species_count=[n], lambdas=[lam], fit_keys=[key],
+                                beads=random_start(n, box, m, rng), rng_stream=r))
+    return System(name="lj-uneven", tables=tables, desired_tau=dtau, replicas=reps,
+                  num_levels=num_levels, seed=seed)
+
+
+def load(lib, system: System):
+    """Creates the system's fits and replicas in `lib` (a PimcLib).  Returns replica ids."""
+    fit_ids = {}
+    for key, t in system.tables.items():
+        fit_ids[key] = lib.fit_create_david(t.grid_type, t.grid_start, t.grid_end, t.y, t.num_terms,
+                                            t.smallest_tau, system.desired_tau[key])
+    ids = []
+    for spec in system.replicas:
+        rid = lib.replica_create(spec.num_particles, spec.num_time_slices, spec.box, spec.tau,
+                                 spec.species_count, spec.lambdas,
+                                 [fit_ids[k] for k in spec.fit_keys], num_images=spec.num_images,
+                                 k_cutoff=spec.k_cutoff, use_long_range=spec.use_long_range,
+                                 use_background=spec.use_background, rng_stream=spec.rng_stream)
+        lib.path_set(rid, spec.beads)
+        ids.append(rid)
+    return ids

@@ -1,0 +1,96 @@
+"""Test helpers: the oracle library (built on demand) and common fixtures."""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "_build", "liboracle_pimc.so")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+REFERENCE = "/root/reference"
+
+_oracle = None
+
+
+def oracle_cdll() -> ctypes.CDLL:
+    """The parity oracle (oracle/_build/liboracle_pimc.so), built with make if absent."""
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_LIB):
+            subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+        lib = ctypes.CDLL(ORACLE_LIB)
+        i32, u32, u64, f64 = ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double
+        P = ctypes.POINTER
+        lib.oracle_philox4x32_10.argtypes = [P(u32), P(u32), P(u32)]
+        lib.oracle_philox4x32_10.restype = None
+        lib.oracle_uniform.argtypes = [u32, u32]
+        lib.oracle_uniform.restype = f64
+        lib.oracle_gaussian_pair.argtypes = [u64, u32, u32, i32, u32, P(f64)]
+        lib.oracle_gaussian_pair.restype = ctypes.c_int
+        lib.oracle_reverse_map.argtypes = [i32, f64, f64, i32, i32, ctypes.c_int64, P(f64), P(i32)]
+        lib.oracle_reverse_map.restype = ctypes.c_int
+        lib.oracle_grid_points.argtypes = [i32, f64, f64, i32, P(f64)]
+        lib.oracle_grid_points.restype = ctypes.c_int
+        lib.oracle_multicubic.argtypes = [i32, f64, f64, i32, i32, P(f64), P(f64), P(f64), P(f64), i32,
+                                          ctypes.c_int64, P(f64), P(f64)]
+        lib.oracle_multicubic.restype = ctypes.c_int
+        lib.oracle_bisection_sample.argtypes = [ctypes.c_void_p, i32, i32, i32, i32, i32, u64, u32, P(f64)]
+        lib.oracle_bisection_sample.restype = ctypes.c_int
+        _oracle = lib
+    return _oracle
+
+
+def f64p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def i32p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def grid_points(gtype, start, end, n):
+    x = np.empty(n)
+    assert oracle_cdll().oracle_grid_points(gtype, start, end, n, f64p(x)) == 0
+    return x
+
+
+def reverse_map(gtype, start, end, n, r, literal=False):
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    out = np.empty(r.size, dtype=np.int32)
+    assert oracle_cdll().oracle_reverse_map(gtype, start, end, n, int(literal), r.size, f64p(r), i32p(out)) == 0
+    return out
+
+
+def multicubic(gtype, start, end, y, start_deriv, end_deriv, x, col=-1):
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    n, nc = y.shape
+    sd = np.ascontiguousarray(start_deriv, dtype=np.float64)
+    ed = np.ascontiguousarray(end_deriv, dtype=np.float64)
+    d2 = np.empty_like(y)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty(x.size if col >= 0 else x.size * nc)
+    assert oracle_cdll().oracle_multicubic(gtype, start, end, n, nc, f64p(y), f64p(sd), f64p(ed), f64p(d2), col,
+                                           x.size, f64p(x), f64p(out)) == 0
+    return d2, (out if col >= 0 else out.reshape(x.size, nc))
+
+
+def small_random_table(rng, num_points=40, num_tau=3, num_terms=2, grid_type=0, start=0.5, end=4.0):
+    """A smooth random DavidPA-layout table for parity tests."""
+    from pimcplusplus_amd.tables import num_cols_for_terms
+    nc = num_cols_for_terms(num_terms)
+    if grid_type == 0:
+        x = np.linspace(start, end, num_points)
+    else:
+        x = start * (end / start) ** (np.arange(num_points) / (num_points - 1))
+    y = np.zeros((num_tau, num_points, nc))
+    for lev in range(num_tau):
+        for c in range(nc):
+            a = rng.normal(size=4)
+            y[lev, :, c] = (a[0] * np.exp(-x * (0.5 + abs(a[1]))) + 0.1 * a[2] * np.sin(2 * x + a[3])) / (1 + c)
+    y[:, -1, :] = 0.0
+    return y

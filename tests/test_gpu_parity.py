@@ -1,0 +1,177 @@
+"""GPU parity: the HIP library against the oracle on identical inputs.
+
+Tolerances (FP64 path): U and actions to 1e-12 relative (absolute floor scaled by
+the sum of |terms|), images and bookkeeping bit-exact, identical Metropolis
+decisions.  Mixed path: |Delta S| error <= 1e-6 per stage."""
+
+import numpy as np
+import pytest
+
+from pimcplusplus_amd import abi, workloads
+from tests.helpers import GOLDEN, small_random_table
+from tests.test_oracle_geometry_actions import make_system
+
+pytestmark = pytest.mark.gpu
+
+FP64, MIXED = abi.FP64, abi.MIXED
+
+
+def load_both(oracle, gpu, system):
+    return workloads.load(oracle, system), workloads.load(gpu, system)
+
+
+@pytest.mark.parametrize("gtype", [0, 1])
+def test_fit_eval_u_random_tables(oracle, gpu, gtype):
+    rng = np.random.default_rng(100 + gtype)
+    y = small_random_table(rng, num_points=80, num_tau=3, grid_type=gtype, start=0.3, end=5.0)
+    fo = oracle.fit_create_david(gtype, 0.3, 5.0, y, 2, 0.1, 0.1)
+    fg = gpu.fit_create_david(gtype, 0.3, 5.0, y, 2, 0.1, 0.1)
+    n = 20000
+    q = rng.uniform(0.2, 5.5, n)
+    s = rng.uniform(0, 1.5, n) * np.minimum(1, q)
+    z = rng.uniform(-1, 1, n) * s
+    s[:50] = 0.0
+    for lev in range(3):
+        want = oracle.fit_eval_u(fo, lev, q, z, s * s)
+        got = gpu.fit_eval_u(fg, lev, q, z, s * s, FP64)
+        scale = 1.0 + np.abs(want)
+        assert np.max(np.abs(got - want) / scale) < 1e-13
+        mixed = gpu.fit_eval_u(fg, lev, q, z, s * s, MIXED)
+        assert np.max(np.abs(mixed - want) / scale) < 2e-6
+
+
+def test_fit_eval_u_testcoul(oracle, gpu):
+    d = np.load(f"{GOLDEN}/testcoul_fit2.npz")
+    args = (int(d["grid_type"]), float(d["grid_start"]), float(d["grid_end"]), d["y"], 2,
+            float(d["smallest_tau"]), 0.125)
+    fo, fg = oracle.fit_create_david(*args), gpu.fit_create_david(*args)
+    rng = np.random.default_rng(101)
+    q = np.exp(rng.uniform(np.log(0.002), np.log(11.0), 20000))
+    s = rng.uniform(0, 1, q.size) * np.minimum(2 * q, 2.0)
+    z = rng.uniform(-1, 1, q.size) * s
+    for lev in range(4):
+        want = oracle.fit_eval_u(fo, lev, q, z, s * s)
+        got = gpu.fit_eval_u(fg, lev, q, z, s * s, FP64)
+        assert np.max(np.abs(got - want) / (1 + np.abs(want))) < 1e-13
+
+
+def test_dist_disp_images_bit_exact(oracle, gpu):
+    rng = np.random.default_rng(102)
+    _, ro, beads = make_system(oracle, rng, n=9, m=6, box=2.0, spread=9.0)
+    rng = np.random.default_rng(102)
+    _, rg, beads_g = make_system(gpu, rng, n=9, m=6, box=2.0, spread=9.0)
+    assert np.array_equal(beads, beads_g)
+    idx = rng.integers(0, [7, 7, 9, 9], size=(5000, 4))
+    oo, io = oracle.dist_disp(ro, 0, idx)
+    og, ig = gpu.dist_disp(rg, 0, idx)
+    assert np.array_equal(io, ig)
+    assert np.array_equal(oo[:, 2:], og[:, 2:])      # displacements: same operations, same rounding
+    assert np.max(np.abs(oo[:, :2] - og[:, :2]) / oo[:, :2].clip(1e-300)) < 1e-15
+
+
+@pytest.mark.parametrize("nspecies", [1, 2])
+def test_short_range_and_kinetic_requests(oracle, gpu, nspecies):
+    species = [40] if nspecies == 1 else [25, 15]
+    reqs_all = []
+    rng = np.random.default_rng(103)
+    _, ro, beads = make_system(oracle, rng, n=40, m=16, box=4.0, species=species)
+    rng = np.random.default_rng(103)
+    _, rg, _ = make_system(gpu, rng, n=40, m=16, box=4.0, species=species)
+    # NEW copy differs from OLD on a few particles
+    new = beads.copy()
+    new[3:12, [2, 7, 30]] += rng.normal(scale=0.3, size=(9, 3, 3))
+    oracle.path_set(ro, new, abi.NEWMODE)
+    gpu.path_set(rg, new, abi.NEWMODE)
+    for _ in range(60):
+        level = int(rng.integers(0, 3))
+        span = 8 << (level > 1)
+        s1 = int(rng.integers(0, 17 - span))
+        k = int(rng.integers(1, 5))
+        ptcls = [int(p) for p in rng.choice(40, size=k, replace=False)]
+        reqs_all.append((0, int(rng.integers(0, 2)), s1, s1 + span, level, ptcls))
+    ro_reqs = [(ro,) + r[1:] for r in reqs_all]
+    rg_reqs = [(rg,) + r[1:] for r in reqs_all]
+    want = oracle.action_short_range(ro_reqs)
+    got = gpu.action_short_range(rg_reqs, FP64)
+    assert np.max(np.abs(got - want) / (1 + np.abs(want))) < 1e-12
+    mixed = gpu.action_short_range(rg_reqs, MIXED)
+    assert np.max(np.abs(mixed - want)) < 1e-5
+    kw = oracle.action_kinetic(ro_reqs)
+    kg = gpu.action_kinetic(rg_reqs)
+    assert np.max(np.abs(kg - kw) / (1 + np.abs(kw))) < 1e-13
+
+
+def test_accept_reject_join_bit_exact(oracle, gpu):
+    rng = np.random.default_rng(104)
+    _, ro, beads = make_system(oracle, rng, n=6, m=10, box=3.0)
+    rng = np.random.default_rng(104)
+    _, rg, _ = make_system(gpu, rng, n=6, m=10, box=3.0)
+    new = beads.copy()
+    new[2:9, [1, 4]] += 1.7
+    for lib, r in ((oracle, ro), (gpu, rg)):
+        lib.path_set(r, new, abi.NEWMODE)
+        lib.accept_copy(r, 2, 8, [1])
+        lib.reject_copy(r, 2, 8, [4])
+        lib.move_join(r, 0, 7)
+        lib.move_join(r, 7, 3)
+    for mode in (0, 1):
+        assert np.array_equal(oracle.path_get(ro, mode), gpu.path_get(rg, mode))
+        assert np.array_equal(oracle.permutation(ro, mode), gpu.permutation(rg, mode))
+
+
+def compare_chains(ro_rec, rg_rec, rel=1e-12):
+    assert np.array_equal(ro_rec["slice1"], rg_rec["slice1"])
+    assert np.array_equal(ro_rec["ptcl"], rg_rec["ptcl"])
+    assert np.array_equal(ro_rec["num_stages"], rg_rec["num_stages"])
+    assert np.array_equal(ro_rec["accepted"], rg_rec["accepted"])
+    for key in ("old_action", "new_action", "log_sample_ratio"):
+        a, b = ro_rec[key], rg_rec[key]
+        assert np.max(np.abs(a - b) / (1 + np.abs(a))) < rel, key
+    ds_o = ro_rec["new_action"] - ro_rec["old_action"]
+    ds_g = rg_rec["new_action"] - rg_rec["old_action"]
+    scale = 1 + np.abs(ro_rec["old_action"]) + np.abs(ro_rec["new_action"])
+    assert np.max(np.abs(ds_o - ds_g) / scale) < rel
+    assert np.array_equal(ro_rec["log_u"], rg_rec["log_u"]) or \
+        np.max(np.abs(ro_rec["log_u"] - rg_rec["log_u"])) < 1e-15
+
+
+def test_bisection_fp64_matches_oracle_config0(oracle, gpu):
+    sysm = workloads.config0(num_replicas=4)
+    io, ig = load_both(oracle, gpu, sysm)
+    ro, so = oracle.bisection_run(io, sysm.num_levels, sysm.seed, 0, 500)
+    rg, sg = gpu.bisection_run(ig, sysm.num_levels, sysm.seed, 0, 500, FP64)
+    compare_chains(ro, rg)
+    assert (so.moves, so.accepted, so.stages, so.pair_evals) == (sg.moves, sg.accepted, sg.stages, sg.pair_evals)
+    for a, b in zip(io, ig):
+        assert np.max(np.abs(oracle.path_get(a) - gpu.path_get(b, 0))) < 1e-12
+        assert np.array_equal(gpu.path_get(b, 0), gpu.path_get(b, 1))
+
+
+def test_bisection_fp64_matches_oracle_config1(oracle, gpu):
+    sysm = workloads.config1(num_replicas=2)
+    io, ig = load_both(oracle, gpu, sysm)
+    ro, _ = oracle.bisection_run(io, sysm.num_levels, sysm.seed, 0, 300)
+    rg, _ = gpu.bisection_run(ig, sysm.num_levels, sysm.seed, 0, 300, FP64)
+    compare_chains(ro, rg)
+
+
+def test_bisection_mixed_delta_s_within_1e6(oracle, gpu):
+    sysm = workloads.config1(num_replicas=2)
+    io, ig = load_both(oracle, gpu, sysm)
+    ro, _ = oracle.bisection_run(io, sysm.num_levels, sysm.seed, 0, 300)
+    rg, _ = gpu.bisection_run(ig, sysm.num_levels, sysm.seed, 0, 300, MIXED)
+    same = np.ones(2, dtype=bool)
+    compared = 0
+    for m in range(300):
+        for r in range(2):
+            if not same[r]:
+                continue
+            ns = ro["num_stages"][r, m]
+            if ns != rg["num_stages"][r, m] or ro["accepted"][r, m] != rg["accepted"][r, m]:
+                same[r] = False      # a decision differed (near tie): the chains part here
+                continue
+            dso = ro["new_action"][r, m, :ns] - ro["old_action"][r, m, :ns]
+            dsg = rg["new_action"][r, m, :ns] - rg["old_action"][r, m, :ns]
+            assert np.max(np.abs(dso - dsg)) <= 1e-6
+            compared += ns
+    assert compared > 300

@@ -1,0 +1,74 @@
+"""Pins the oracle's bisection chain (BisectionStage/StageClass/MultiStage).
+
+For free particles (zero pair action, no images) the bisection samples the
+kinetic action exactly, so every stage's log acceptance is zero up to rounding
+and every move is accepted: this ties Sample, the kinetic action and the
+Metropolis bookkeeping (prevActionChange) together."""
+
+import numpy as np
+
+from pimcplusplus_amd import workloads
+from tests.helpers import oracle_cdll
+
+
+def free_system(lib, n=5, m=32, box=30.0):
+    y = np.zeros((4, 10, 7))
+    fid = lib.fit_create_david(0, 0.1, box / 2, y, 2, 0.025, 0.025)
+    rid = lib.replica_create(n, m, (box,) * 3, 0.025, [n], [6.06], [fid], rng_stream=3)
+    rng = np.random.default_rng(1)
+    beads = np.repeat(rng.uniform(-3, 3, (1, n, 3)), m + 1, axis=0)
+    lib.path_set(rid, beads)
+    return rid
+
+
+def test_free_particles_always_accept(oracle):
+    rid = free_system(oracle)
+    rec, st = oracle.bisection_run([rid], 3, seed=99, first_move=0, num_moves=300)
+    assert st.accepted == 300
+    ns = rec["num_stages"][0]
+    assert np.all(ns == 3)
+    for s in range(3):
+        old, new = rec["old_action"][0, :, s], rec["new_action"][0, :, s]
+        lsr = rec["log_sample_ratio"][0, :, s]
+        prev = 0 if s == 0 else (rec["new_action"][0, :, s - 1] - rec["old_action"][0, :, s - 1])
+        loga = lsr - (new - old) + prev
+        assert np.max(np.abs(loga)) < 1e-10 * (1 + np.max(np.abs(new)))
+
+
+def test_chain_is_reproducible_and_continues(oracle_lib):
+    from pimcplusplus_amd import abi
+    sysm = workloads.config0()
+    a = abi.PimcLib(oracle_lib)
+    ids = workloads.load(a, sysm)
+    r1, _ = a.bisection_run(ids, 3, sysm.seed, 0, 40)
+    r2, _ = a.bisection_run(ids, 3, sysm.seed, 40, 40)
+    b = abi.PimcLib(oracle_lib)
+    ids_b = workloads.load(b, sysm)
+    rb, st = b.bisection_run(ids_b, 3, sysm.seed, 0, 80)
+    for key in ("slice1", "ptcl", "accepted", "num_stages", "old_action", "new_action"):
+        assert np.array_equal(np.concatenate([r1[key], r2[key]], axis=1), rb[key])
+    assert np.array_equal(a.path_get(ids[0]), b.path_get(ids_b[0]))
+    assert st.moves == 80 and 0 < st.accepted < 80
+
+
+def test_pair_eval_count(oracle):
+    sysm = workloads.config0()
+    ids = workloads.load(oracle, sysm)
+    rec, st = oracle.bisection_run(ids, 3, sysm.seed, 0, 50)
+    # 2 modes x (N-1) partners x K links; K = 2^(L-l) for the stages attempted
+    per_stage = [2 * 63 * (8 >> l) for l in (2, 1, 0)]
+    want = sum(sum(per_stage[:k]) for k in rec["num_stages"][0])
+    assert st.pair_evals == want
+
+
+def test_bisection_sample_moves_only_midpoints(oracle):
+    lib = oracle_cdll()
+    import ctypes
+    rid = free_system(oracle, n=3, m=16)
+    before = oracle.path_get(rid, 1)
+    lsr = ctypes.c_double()
+    assert lib.oracle_bisection_sample(oracle.ctx, rid, 1, 4, 3, 1, 5, 0, ctypes.byref(lsr)) == 0
+    after = oracle.path_get(rid, 1)
+    changed = np.where(np.any(after != before, axis=2))
+    assert set(changed[1]) == {1}
+    assert set(changed[0]) == {6, 10}     # slice1 + 2 + 4b for level 1 in a window of 8
