@@ -138,12 +138,44 @@ struct HostFit {
 struct HostReplica {
   ReplicaDev dev{};
   DevBuf path0, path1, perm0, perm1;
-  std::vector<int> species_of;
-  std::vector<double> kvecs;
-  std::vector<int> kidx;
+  DevBuf rho0, rho1, kidx_dev, ulong_k, lr_const;
+  std::vector<double> kvecs;     // [nk][3]
+  std::vector<int> kidx;         // [nk][3] index + maxk
+  bool use_long_range = false;
   int M = 0;
   int join = 0;
 };
+
+// PathClass::SetupkVecs3D with Include (PathClass.cc:281-292, 534-586): integer
+// triples inside |k| < kc in the half space, in ix, iy, iz loop order.
+void setup_kvecs(HostReplica& H, double kcut) {
+  RepCore& R = H.dev.core;
+  int maxk[3];
+  for (int i = 0; i < 3; i++) {
+    R.kbox[i] = 2.0 * M_PI / R.box[i];
+    maxk[i] = (int)ceil(1.1 * kcut / R.kbox[i]);
+  }
+  R.maxk0 = maxk[0];
+  R.maxk1 = maxk[1];
+  R.maxk2 = maxk[2];
+  H.kvecs.clear();
+  H.kidx.clear();
+  for (int ix = -maxk[0]; ix <= maxk[0]; ix++) {
+    double k0 = ix * R.kbox[0];
+    for (int iy = -maxk[1]; iy <= maxk[1]; iy++) {
+      double k1 = iy * R.kbox[1];
+      for (int iz = -maxk[2]; iz <= maxk[2]; iz++) {
+        double k2 = iz * R.kbox[2];
+        bool include = (k0 > 0.0) || ((k0 == 0.0) && (k1 > 0.0)) || ((k0 == 0.0) && (k1 == 0.0) && (k2 > 0.0));
+        if ((k0 * k0 + k1 * k1 + k2 * k2 < kcut * kcut) && include) {
+          H.kvecs.insert(H.kvecs.end(), {k0, k1, k2});
+          H.kidx.insert(H.kidx.end(), {ix + maxk[0], iy + maxk[1], iz + maxk[2]});
+        }
+      }
+    }
+  }
+  R.nk = (int)(H.kvecs.size() / 3);
+}
 
 }  // namespace
 
@@ -433,9 +465,17 @@ int pimc_replica_create(pimc_ctx* ctx, const pimc_replica_desc* d, int32_t* repl
   }
   R.stream = d->rng_stream;
   H->M = d->num_time_slices;
-  if (d->k_cutoff > 0 || d->use_long_range)
-    return fail(PIMC_ERR_UNSUPPORTED, "replica_create: long range is not available in this build");
   R.nk = 0;
+  H->use_long_range = d->use_long_range != 0;
+  R.use_background = d->use_background != 0;
+  R.lr_on = 0;
+  R.lr_mask = 0;
+  R.lr_levels = 0;
+  if (d->k_cutoff > 0) {
+    setup_kvecs(*H, d->k_cutoff);
+    if (R.maxk0 > 64 || R.maxk1 > 64 || R.maxk2 > 64)
+      return fail(PIMC_ERR_UNSUPPORTED, "replica_create: k_cutoff needs more than 64 k indices per dimension");
+  }
   HIP_TRY(hipSetDevice(ctx->device));
   size_t nb = (size_t)R.nslices * 3 * R.N * sizeof(double);
   int rc;
@@ -452,6 +492,20 @@ int pimc_replica_create(pimc_ctx* ctx, const pimc_replica_desc* d, int32_t* repl
   R.path1 = H->path1.as<double>();
   R.perm0 = H->perm0.as<int>();
   R.perm1 = H->perm1.as<int>();
+  if (R.nk > 0) {
+    size_t nr = (size_t)R.nslices * R.ns * R.nk * sizeof(d2v);
+    if ((rc = H->rho0.ensure(nr)) || (rc = H->rho1.ensure(nr)) || (rc = H->kidx_dev.ensure((size_t)R.nk * 4 * sizeof(int))))
+      return rc;
+    HIP_TRY(hipMemset(H->rho0.p, 0, nr));
+    HIP_TRY(hipMemset(H->rho1.p, 0, nr));
+    std::vector<int> k4((size_t)R.nk * 4, 0);
+    for (int k = 0; k < R.nk; k++)
+      for (int d = 0; d < 3; d++) k4[4 * k + d] = H->kidx[3 * k + d];
+    HIP_TRY(hipMemcpy(H->kidx_dev.p, k4.data(), k4.size() * sizeof(int), hipMemcpyHostToDevice));
+    R.rho0 = H->rho0.as<d2v>();
+    R.rho1 = H->rho1.as<d2v>();
+    R.kidx = H->kidx_dev.as<int>();
+  }
   ctx->reps.push_back(std::move(H));
   ctx->reps_dirty = true;
   *replica_id = (int32_t)ctx->reps.size() - 1;
@@ -471,11 +525,45 @@ int pimc_replica_kvecs(pimc_ctx* ctx, int32_t replica, double* kvecs, int32_t* k
   if (kidx && !H->kidx.empty()) memcpy(kidx, H->kidx.data(), H->kidx.size() * sizeof(int32_t));
   return PIMC_OK;
 }
-int pimc_replica_set_long_range(pimc_ctx* ctx, int32_t replica, int32_t, int32_t, int32_t, const double*,
-                                const double*, const double*, const double*) {
+int pimc_replica_set_long_range(pimc_ctx* ctx, int32_t replica, int32_t sa, int32_t sb, int32_t num_levels,
+                                const double* ulong_k, const double* ulong_r0, const double* ushort_k0,
+                                const double* ulong_k0) {
   HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
-  if (!H) return fail(PIMC_ERR_ARG, "replica_set_long_range: bad replica");
-  return fail(PIMC_ERR_STATE, "replica_set_long_range: replica has no k set");
+  if (!H || num_levels < 1 || num_levels > PIMC_MAX_LEVELS || !ulong_k || !ulong_r0 || !ushort_k0 || !ulong_k0)
+    return fail(PIMC_ERR_ARG, "replica_set_long_range: bad argument");
+  RepCore& R = H->dev.core;
+  if (sa < 0 || sb < 0 || sa >= R.ns || sb >= R.ns) return fail(PIMC_ERR_ARG, "replica_set_long_range: bad species");
+  if (R.nk == 0) return fail(PIMC_ERR_STATE, "replica_set_long_range: replica has no k set");
+  if (R.lr_levels != 0 && R.lr_levels != num_levels)
+    return fail(PIMC_ERR_ARG, "replica_set_long_range: every pair needs the same number of levels");
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  const int npair = R.ns * R.ns;
+  if (R.lr_levels == 0) {
+    int rc;
+    size_t bk = (size_t)npair * num_levels * R.nk * sizeof(double), bc = (size_t)npair * 3 * num_levels * sizeof(double);
+    if ((rc = H->ulong_k.ensure(bk)) || (rc = H->lr_const.ensure(bc))) return rc;
+    HIP_TRY(hipMemset(H->ulong_k.p, 0, bk));
+    HIP_TRY(hipMemset(H->lr_const.p, 0, bc));
+    R.lr_levels = num_levels;
+    R.ulong_k = H->ulong_k.as<double>();
+    R.lr_const = H->lr_const.as<double>();
+  }
+  for (int pr : {sa * R.ns + sb, sb * R.ns + sa}) {
+    HIP_TRY(hipMemcpy(H->ulong_k.as<double>() + (size_t)pr * num_levels * R.nk, ulong_k,
+                      (size_t)num_levels * R.nk * sizeof(double), hipMemcpyHostToDevice));
+    double* c = H->lr_const.as<double>() + (size_t)pr * 3 * num_levels;
+    HIP_TRY(hipMemcpy(c, ulong_r0, num_levels * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c + num_levels, ushort_k0, num_levels * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c + 2 * num_levels, ulong_k0, num_levels * sizeof(double), hipMemcpyHostToDevice));
+    R.lr_mask |= 1u << pr;
+  }
+  R.lr_on = (H->use_long_range && R.lr_mask) ? 1 : 0;
+  R.lr_terms = 0;
+  for (int a = 0; a < R.ns; a++)
+    for (int b = a; b < R.ns; b++) R.lr_terms += (R.lr_mask >> (a * R.ns + b)) & 1u;
+  ctx->reps_dirty = true;
+  return PIMC_OK;
 }
 
 int pimc_path_set(pimc_ctx* ctx, int32_t replica, int32_t mode, const double* beads) {
@@ -542,6 +630,14 @@ static int accept_reject(pimc_ctx* ctx, int32_t replica, int32_t slice1, int32_t
   if (rc) return rc;
   launch_accept_reject(ctx->d_reps.as<ReplicaDev>(), replica, slice1, slice2, ptcls, n, accept, ctx->stream);
   HIP_TRY(hipGetLastError());
+  const RepCore& R = H->dev.core;
+  if (R.nk > 0) {
+    // Rho_k[OLD] <- Rho_k[NEW] (accept) or the reverse over the slice range, all species
+    const size_t off = (size_t)slice1 * R.ns * R.nk, cnt = (size_t)(slice2 - slice1 + 1) * R.ns * R.nk;
+    d2v* dst = accept ? H->rho0.as<d2v>() : H->rho1.as<d2v>();
+    const d2v* src = accept ? H->rho1.as<d2v>() : H->rho0.as<d2v>();
+    HIP_TRY(hipMemcpyAsync(dst + off, src + off, cnt * sizeof(d2v), hipMemcpyDeviceToDevice, ctx->stream));
+  }
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   return PIMC_OK;
 }
@@ -570,12 +666,27 @@ int pimc_move_join(pimc_ctx* ctx, int32_t replica, int32_t old_join, int32_t new
 int pimc_rho_k_update_all(pimc_ctx* ctx, int32_t replica) {
   HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
   if (!H) return fail(PIMC_ERR_ARG, "rho_k_update_all: bad replica");
-  return fail(PIMC_ERR_STATE, "rho_k_update_all: replica has no k set");
+  const RepCore& R = H->dev.core;
+  if (R.nk == 0) return fail(PIMC_ERR_STATE, "rho_k_update_all: replica has no k set");
+  HIP_TRY(hipSetDevice(ctx->device));
+  int rc = ctx->sync_descriptors();
+  if (rc) return rc;
+  launch_rho_all(ctx->d_reps.as<ReplicaDev>(), replica, R.nslices, R.ns, R.nk,
+                 2 * (R.maxk0 + R.maxk1 + R.maxk2) + 3, ctx->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return PIMC_OK;
 }
 int pimc_rho_k_get(pimc_ctx* ctx, int32_t replica, int32_t mode, double* rho) {
   HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
   if (!H || !rho || mode < 0 || mode > 1) return fail(PIMC_ERR_ARG, "rho_k_get: bad argument");
-  return fail(PIMC_ERR_STATE, "rho_k_get: replica has no k set");
+  const RepCore& R = H->dev.core;
+  if (R.nk == 0) return fail(PIMC_ERR_STATE, "rho_k_get: replica has no k set");
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(hipMemcpy(rho, mode ? H->rho1.p : H->rho0.p, (size_t)R.nslices * R.ns * R.nk * sizeof(d2v),
+                    hipMemcpyDeviceToHost));
+  return PIMC_OK;
 }
 
 int pimc_action_short_range(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t nreq, int32_t precision,
@@ -633,11 +744,29 @@ int pimc_action_kinetic(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t nreq
 int pimc_action_long_range(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t nreq, int32_t precision,
                            double* out) {
   if (!ctx || nreq < 0 || (nreq > 0 && (!reqs || !out))) return fail(PIMC_ERR_ARG, "action_long_range: bad argument");
+  if (precision != PIMC_FP64 && precision != PIMC_MIXED) return fail(PIMC_ERR_ARG, "action_long_range: bad precision");
+  if (nreq == 0) return PIMC_OK;
+  std::vector<int> strides(nreq);
   for (int i = 0; i < nreq; i++) {
     int rc = check_req(ctx, reqs[i]);
     if (rc) return rc;
-    return fail(PIMC_ERR_STATE, "action_long_range: replica has no k set");
+    const RepCore& R = ctx->reps[reqs[i].replica]->dev.core;
+    if (R.nk == 0) return fail(PIMC_ERR_STATE, "action_long_range: replica has no k set");
+    if (R.lr_mask && reqs[i].level >= R.lr_levels)
+      return fail(PIMC_ERR_ARG, "action_long_range: level beyond the long-range tables");
+    strides[i] = 2 * (R.maxk0 + R.maxk1 + R.maxk2) + 3;
   }
+  HIP_TRY(hipSetDevice(ctx->device));
+  int rc = ctx->sync_descriptors();
+  if (rc) return rc;
+  rc = upload_reqs(ctx, reqs, nreq);
+  if (rc) return rc;
+  if ((rc = ctx->d_out.ensure((size_t)nreq * sizeof(double)))) return rc;
+  launch_long_range(precision == PIMC_MIXED, ctx->d_reps.as<ReplicaDev>(), ctx->d_reqs.as<ReqDev>(), nreq,
+                    strides.data(), ctx->d_out.as<double>(), ctx->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, ctx->d_out.p, (size_t)nreq * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
   return PIMC_OK;
 }
 
@@ -681,13 +810,17 @@ int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, con
     return fail(PIMC_ERR_ARG, "bisection_run: bad levels");
   std::set<int> seen;
   int nt = -1;
-  size_t table_bytes = 0;
+  size_t table_bytes = 0, lr_bytes = 0;
   for (int r = 0; r < nrep; r++) {
     HostReplica* H = ctx->rep(replicas[r]);
     if (!H) return fail(PIMC_ERR_ARG, "bisection_run: bad replica");
     if (!seen.insert(replicas[r]).second) return fail(PIMC_ERR_ARG, "bisection_run: replica listed twice");
     const RepCore& R = H->dev.core;
     if ((1 << desc->num_levels) >= R.nslices) return fail(PIMC_ERR_ARG, "bisection_run: too few slices for num_levels");
+    if (R.lr_on) {
+      if (desc->lowest_level >= R.lr_levels) return fail(PIMC_ERR_ARG, "bisection_run: long-range tables lack the lowest level");
+      lr_bytes = std::max(lr_bytes, (size_t)2 * (2 * (R.maxk0 + R.maxk1 + R.maxk2) + 3) * sizeof(d2v));
+    }
     if (desc->species < 0 || desc->species >= R.ns || H->dev.sp.count[desc->species] < 1)
       return fail(PIMC_ERR_ARG, "bisection_run: bad species");
     for (int f = 0; f < R.ns * R.ns; f++) {
@@ -722,6 +855,7 @@ int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, con
   a.records = records ? ctx->d_records.as<pimc_move_record>() : nullptr;
   a.stats = ctx->d_stats.as<int64_t>();
   a.lds_table_bytes = table_bytes;
+  a.lds_lr_bytes = lr_bytes;
   if (!ctx->ev0) {
     HIP_TRY(hipEventCreate(&ctx->ev0));
     HIP_TRY(hipEventCreate(&ctx->ev1));

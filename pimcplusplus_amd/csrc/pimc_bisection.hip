@@ -40,8 +40,10 @@ __global__ void __launch_bounds__(kBlock) k_bisection(const LaunchBisection a) {
   PIMC_LDS double* knew = kold + W;                                        // [W]
   PIMC_LDS double* scratch = knew + W;                                     // [kWaves]
   PIMC_LDS int* ctl = (PIMC_LDS int*)(scratch + kWaves);                  // slice1, ptcl, accept
+  PIMC_LDS d2v* lrC = (PIMC_LDS d2v*)(ctl + 4);                           // [2][kstride]: NEW, OLD rows
 
-  int64_t st_moves = 0, st_acc = 0, st_stages = 0, st_evals = 0;
+  int64_t st_moves = 0, st_acc = 0, st_stages = 0, st_evals = 0, st_kevals = 0;
+  typedef typename std::conditional<MIXED, float, double>::type LrT;   // phase-sum arithmetic
   const int numLeft = R.nslices - W;
   const double lambda = S.lambda[sp];
 
@@ -123,12 +125,21 @@ __global__ void __launch_bounds__(kBlock) k_bisection(const LaunchBisection a) {
       for (int s2 = 0; s2 < R.ns; s2++) {
         const FitDev F = load_uniform(&a.fits[S.fit[sp * R.ns + s2]]);
         __syncthreads();
+#ifdef PIMC_VARIANT_GLOBAL_TABLES
+        auto T = [&] {
+          if constexpr (MIXED)
+            return level_table32(F, level + F.taupos);
+          else
+            return level_table64(F, level + F.taupos);
+        }();
+#else
         auto T = [&] {
           if constexpr (MIXED)
             return stage_level32(F, level + F.taupos, tbl);
           else
             return stage_level64(F, level + F.taupos, tbl);
         }();
+#endif
         __syncthreads();
         const int j0 = S.first[s2], j1 = S.first[s2] + S.count[s2];
         const size_t row = (size_t)3 * R.N * step;     // distance between consecutive link slices
@@ -176,6 +187,69 @@ __global__ void __launch_bounds__(kBlock) k_bisection(const LaunchBisection a) {
       const double sr_old = block_sum(acc_old.value(), scratch);
       const double sr_new = block_sum(acc_new.value(), scratch);
 
+      // ---- Long-range action at the lowest level (LongRangeClass.cc:273-371) ----
+      // rho_k NEW = OLD + e^{ik r_new} - e^{ik r_old} over the window, computed on the
+      // fly; copy OLD is read, nothing is written until the move is accepted.
+      double lr_kold = 0.0, lr_knew = 0.0;
+      const bool with_lr = R.lr_on && level == a.lowest_level;
+      if (with_lr) {
+        const int stride = kstride(R), o1 = koff_of(R, 1), o2 = koff_of(R, 2);
+        double part_o = 0.0, part_n = 0.0;
+        for (int t = 0; t <= W; t += step) {
+          __syncthreads();
+          if (tid < 6) {
+            const int d = tid % 3;
+            PIMC_LDS double* src = (tid < 3 ? mnew : mold) + 3 * t;
+            const double x[3] = {src[0], src[1], src[2]};
+            phase_rows_lane<LrT>(R, x, d, lrC + (tid < 3 ? 0 : stride));
+          }
+          __syncthreads();
+          const double factor = (t == 0 || t == W) ? 0.5 : 1.0;
+          const int slice = slice1 + t;
+          const d2v* rho = R.rho0 + (size_t)slice * R.ns * R.nk;
+          for (int k = tid; k < R.nk; k += blockDim.x) {
+            const int ki[3] = {R.kidx[4 * k], R.kidx[4 * k + 1], R.kidx[4 * k + 2]};
+            LrT nr, ni, orr, oi;
+            kterm<LrT>(lrC, o1, o2, ki, nr, ni);
+            kterm<LrT>(lrC + stride, o1, o2, ki, orr, oi);
+            d2v ro[kMaxSpecies], rn[kMaxSpecies];
+#pragma unroll
+            for (int s = 0; s < kMaxSpecies; s++) {
+              if (s < R.ns) {
+                ro[s] = rho[(size_t)s * R.nk + k];
+                rn[s] = ro[s];
+                if (s == sp) {
+                  rn[s].x += (double)nr;
+                  rn[s].y += (double)ni;
+                  rn[s].x -= (double)orr;
+                  rn[s].y -= (double)oi;
+                }
+              }
+            }
+#pragma unroll
+            for (int s = 0; s < kMaxSpecies; s++) {
+              if (s < R.ns && ((R.lr_mask >> (s * R.ns + s)) & 1u)) {
+                const double u = R.ulong_k[((size_t)(s * R.ns + s) * R.lr_levels + level) * R.nk + k];
+                part_o += factor * 0.5 * 2.0 * (ro[s].x * ro[s].x + ro[s].y * ro[s].y) * u;
+                part_n += factor * 0.5 * 2.0 * (rn[s].x * rn[s].x + rn[s].y * rn[s].y) * u;
+              }
+            }
+#pragma unroll
+            for (int s1 = 0; s1 < kMaxSpecies; s1++)
+#pragma unroll
+              for (int s2 = s1 + 1; s2 < kMaxSpecies; s2++) {
+                if (s2 < R.ns && ((R.lr_mask >> (s1 * R.ns + s2)) & 1u)) {
+                  const double u = R.ulong_k[((size_t)(s1 * R.ns + s2) * R.lr_levels + level) * R.nk + k];
+                  part_o += factor * 2.0 * (ro[s1].x * ro[s2].x + ro[s1].y * ro[s2].y) * u;
+                  part_n += factor * 2.0 * (rn[s1].x * rn[s2].x + rn[s1].y * rn[s2].y) * u;
+                }
+              }
+          }
+        }
+        lr_kold = block_sum(part_o, scratch);
+        lr_knew = block_sum(part_n, scratch);
+      }
+
       // ---- Metropolis test (StageClass.cc:69-72) ----
       if (tid == 0) {
         double logSampleProb = 0.0, logOldSampleProb = 0.0;
@@ -194,6 +268,16 @@ __global__ void __launch_bounds__(kBlock) k_bisection(const LaunchBisection a) {
         oldAction += sr_old;
         newAction += kin_n;
         newAction += sr_new;
+        if (with_lr) {
+          // the k-independent terms are the same on both copies
+          double homo = 0.0, background = 0.0, k0 = 0.0;
+          for (int t = 0; t <= W; t += step)
+            lr_const_terms(R, S, level, (t == 0 || t == W) ? 0.5 : 1.0, homo, background, k0);
+          const double cst = R.use_background ? background : k0;
+          oldAction += (lr_kold + homo) + cst;
+          newAction += (lr_knew + homo) + cst;
+          st_kevals += 2 * (int64_t)(W / step + 1) * R.nk * R.lr_terms;
+        }
         const double currActionChange = newAction - oldAction;
         const double logAcceptProb = logSampleRatio - currActionChange + prev;
         const double logu = log(rng_uniform(a.seed, R.stream, move, kTagAccept, level));
@@ -218,7 +302,35 @@ __global__ void __launch_bounds__(kBlock) k_bisection(const LaunchBisection a) {
       }
     }
 
-    // ---- AcceptCopy: PutInBoxFast on the NEW beads, then OLD <- NEW ----
+    // ---- AcceptCopy: rho_k OLD, NEW <- the updated rho_k; PutInBoxFast on the NEW beads, OLD <- NEW ----
+    if (accepted && R.lr_on) {
+      const int step = 1 << a.lowest_level;
+      const int stride = kstride(R), o1 = koff_of(R, 1), o2 = koff_of(R, 2);
+      for (int t = 0; t <= W; t += step) {
+        __syncthreads();
+        if (tid < 6) {
+          const int d = tid % 3;
+          PIMC_LDS double* src = (tid < 3 ? mnew : mold) + 3 * t;
+          const double x[3] = {src[0], src[1], src[2]};
+          phase_rows_lane<LrT>(R, x, d, lrC + (tid < 3 ? 0 : stride));
+        }
+        __syncthreads();
+        const size_t base = ((size_t)(slice1 + t) * R.ns + sp) * R.nk;
+        for (int k = tid; k < R.nk; k += blockDim.x) {
+          const int ki[3] = {R.kidx[4 * k], R.kidx[4 * k + 1], R.kidx[4 * k + 2]};
+          LrT nr, ni, orr, oi;
+          kterm<LrT>(lrC, o1, o2, ki, nr, ni);
+          kterm<LrT>(lrC + stride, o1, o2, ki, orr, oi);
+          d2v v = R.rho0[base + k];
+          v.x += (double)nr;
+          v.y += (double)ni;
+          v.x -= (double)orr;
+          v.y -= (double)oi;
+          R.rho0[base + k] = v;
+          R.rho1[base + k] = v;
+        }
+      }
+    }
     if (accepted) {
       for (int t = tid; t <= W; t += blockDim.x) {
         double v[3];
@@ -253,13 +365,13 @@ __global__ void __launch_bounds__(kBlock) k_bisection(const LaunchBisection a) {
     st[1] = st_acc;
     st[2] = st_stages;
     st[3] = st_evals;
-    st[4] = 0;
+    st[4] = st_kevals;
   }
 }
 
 static size_t bisection_lds_bytes(const LaunchBisection& a) {
   const int W = 1 << a.num_levels;
-  return a.lds_table_bytes + sizeof(double) * (6 * (W + 1) + W + 2 * W + kWaves) + 4 * sizeof(int);
+  return a.lds_table_bytes + sizeof(double) * (6 * (W + 1) + W + 2 * W + kWaves) + 4 * sizeof(int) + a.lds_lr_bytes;
 }
 
 template <bool MIXED>

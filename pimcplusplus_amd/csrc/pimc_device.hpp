@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace pimc {
 
 constexpr int kMaxSpecies = 4;
@@ -83,8 +85,22 @@ struct RepCore {
   double* path1;      // NEW copy
   int* perm0;         // [N]
   int* perm1;
+  // k space (PathClass::SetupkVecs3D): C_d(n) tables run over n in [-maxk_d, maxk_d]
+  int maxk0, maxk1, maxk2;
+  int lr_on;          // long-range action evaluated (UseLongRange and tables present)
+  double kbox[3];     // 2 pi / box
+  const int* kidx;    // [nk][4]: index + maxk per dimension, pad
+  d2v* rho0;          // OLD rho_k, [nslices][ns][nk] (re, im)
+  d2v* rho1;          // NEW rho_k
+  int lr_levels;      // levels stored in the long-range tables
+  int use_background;
+  const double* ulong_k;   // [ns*ns][lr_levels][nk] (zero rows where a pair has no table)
+  const double* lr_const;  // [ns*ns][3][lr_levels]: Ulong_r0, Ushort_k0, Ulong_k0
+  unsigned lr_mask;        // bit sa*ns+sb set where pair (sa, sb) has tables
+  int lr_terms;            // homologous species + heterologous pairs with tables
   __device__ __host__ inline double* path(int mode) const { return mode ? path1 : path0; }
   __device__ __host__ inline int* perm(int mode) const { return mode ? perm1 : perm0; }
+  __device__ __host__ inline d2v* rho(int mode) const { return mode ? rho1 : rho0; }
 };
 
 // Per-species data, read through the descriptor pointer.

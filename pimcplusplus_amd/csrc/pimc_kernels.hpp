@@ -214,4 +214,76 @@ __device__ inline double kinetic_link(const RepCore& R, P a, P b, double FourLam
   return -log(GaussProd);
 }
 
+// --------------------------------------------------------------------------
+// Structure factors (PathClass.cc:616-707): C_d(n) = e^{i n phi_d} for
+// n in [-maxk_d, maxk_d] by the reference's recursion, and the k term
+// C_0(kx) C_1(ky) C_2(kz).  T = double for FP64, float for the mixed phase sums.
+// --------------------------------------------------------------------------
+__device__ inline int kstride(const RepCore& R) { return 2 * (R.maxk0 + R.maxk1 + R.maxk2) + 3; }
+__device__ inline int kmax_of(const RepCore& R, int d) { return d == 0 ? R.maxk0 : (d == 1 ? R.maxk1 : R.maxk2); }
+__device__ inline int koff_of(const RepCore& R, int d) {
+  return d == 0 ? 0 : (d == 1 ? 2 * R.maxk0 + 1 : 2 * (R.maxk0 + R.maxk1) + 2);
+}
+
+template <typename T>
+__device__ inline void phase_row(double phi, int maxk, PIMC_LDS d2v* C) {
+  T c, s;
+  if constexpr (sizeof(T) == 8) {
+    c = cos(phi);
+    s = sin(phi);
+  } else {
+    const float pf = (float)phi;
+    c = cosf(pf);
+    s = sinf(pf);
+  }
+  T re = 1, im = 0;
+  C[maxk] = d2v{1.0, 0.0};
+  for (int n = 1; n <= maxk; n++) {
+    T nre = c * re - s * im;   // tempC * C(n-1)
+    T nim = c * im + s * re;
+    re = nre;
+    im = nim;
+    C[maxk + n] = d2v{(double)re, (double)im};
+    C[maxk - n] = d2v{(double)re, -(double)im};
+  }
+}
+
+// The three rows of one position r (dimension d at koff_of(d)); lanes 0..2 of the caller.
+template <typename T>
+__device__ inline void phase_rows_lane(const RepCore& R, const double r[3], int d, PIMC_LDS d2v* C) {
+  phase_row<T>(r[d] * R.kbox[d], kmax_of(R, d), C + koff_of(R, d));
+}
+
+template <typename T>
+__device__ inline void kterm(const PIMC_LDS d2v* C, int o1, int o2, const int* ki, T& re, T& im) {
+  const d2v a = C[ki[0]], b = C[o1 + ki[1]], c = C[o2 + ki[2]];
+  const T ar = (T)a.x, ai = (T)a.y, br = (T)b.x, bi = (T)b.y, cr = (T)c.x, ci = (T)c.y;
+  const T abr = ar * br - ai * bi, abi = ar * bi + ai * br;   // C0 * C1
+  re = abr * cr - abi * ci;                                   // (C0 * C1) * C2
+  im = abr * ci + abi * cr;
+}
+
+// The k-independent terms (Madelung, background, k=0) of one slice, added to
+// homo / background / k0 as the reference orders them.
+__device__ inline void lr_const_terms(const RepCore& R, const RepSpecies& S, int level, double factor, double& homo,
+                                      double& background, double& k0) {
+  const int L = R.lr_levels;
+  for (int s = 0; s < R.ns; s++) {
+    const int pr = s * R.ns + s;
+    if (!((R.lr_mask >> pr) & 1u)) continue;  // the reference adds these for every species; tables are zero
+    const int N = S.count[s];
+    homo -= factor * 0.5 * N * R.lr_const[(size_t)(pr * 3 + 0) * L + level];
+    background -= factor * 0.5 * N * N * R.lr_const[(size_t)(pr * 3 + 1) * L + level];
+    k0 += factor * 0.5 * N * N * R.lr_const[(size_t)(pr * 3 + 2) * L + level];
+  }
+  for (int s1 = 0; s1 < R.ns; s1++)
+    for (int s2 = s1 + 1; s2 < R.ns; s2++) {
+      const int pr = s1 * R.ns + s2;
+      if (!((R.lr_mask >> pr) & 1u)) continue;
+      const int N1 = S.count[s1], N2 = S.count[s2];
+      background -= factor * N1 * N2 * R.lr_const[(size_t)(pr * 3 + 1) * L + level];
+      k0 += factor * N1 * N2 * R.lr_const[(size_t)(pr * 3 + 2) * L + level];
+    }
+}
+
 }  // namespace pimc

@@ -37,6 +37,9 @@ void launch_dist_disp(const ReplicaDev* reps, int rep, int mode, int64_t n, cons
 void launch_accept_reject(const ReplicaDev* reps, int rep, int slice1, int slice2, const int* ptcls, int n,
                           bool accept, hipStream_t s);
 void launch_move_join(const ReplicaDev* reps, int rep, int N, int old_join, int new_join, hipStream_t s);
+void launch_rho_all(const ReplicaDev* reps, int rep, int nslices, int ns, int nk, int stride, hipStream_t s);
+void launch_long_range(bool mixed, const ReplicaDev* reps, const ReqDev* reqs, int nreq, const int* strides,
+                       double* out, hipStream_t s);
 
 // The GPU-resident bisection chain: one block per replica, num_moves moves each.
 struct LaunchBisection {
@@ -51,6 +54,7 @@ struct LaunchBisection {
   pimc_move_record* records;  // [nrep][num_moves] or nullptr
   int64_t* stats;             // [nrep][5]
   size_t lds_table_bytes;     // max staged level size over the fits used
+  size_t lds_lr_bytes;        // phase tables of the long-range stage term (0 without long range)
 };
 // Returns hipSuccess or the launch error.
 hipError_t launch_bisection(int num_terms, bool mixed, const LaunchBisection& a, hipStream_t s);

@@ -1,0 +1,166 @@
+// pimc_longrange.hip -- rho_k kernels and LongRangeClass::SingleAction.
+//
+//   k_rho_all   PathClass::UpdateRho_ks()        PathClass.cc:709-723 (+ CalcRho_ks_Fast :616-647)
+//   k_lr_req    LongRangeClass::SingleAction     LongRangeClass.cc:273-371, with
+//               PathClass::UpdateRho_ks(s1, s2, changed, level) PathClass.cc:649-707 in NEWMODE
+// rho_k is [slice][species][k] complex double; a thread owns one k and adds
+// the particles in the reference's order, so every rho_k sum runs in the same
+// order as the CPU path.
+#include "pimc_kernels.hpp"
+#include "pimc_launch.hpp"
+
+namespace pimc {
+
+constexpr int kRhoChunk = 16;   // particles whose phase tables are staged together
+
+__global__ void __launch_bounds__(kBlock) k_rho_all(const ReplicaDev* __restrict__ reps, int rep) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  PIMC_LDS d2v* C = (PIMC_LDS d2v*)smem_raw;
+  const ReplicaDev* RD = reps + rep;
+  const RepCore R = load_uniform(&RD->core);
+  const int slice = blockIdx.x, s = blockIdx.y;
+  const int k = blockIdx.z * blockDim.x + threadIdx.x;
+  const int first = RD->sp.first[s], end = first + RD->sp.count[s];
+  const int stride = kstride(R);
+  const int o1 = koff_of(R, 1), o2 = koff_of(R, 2);
+  int ki[3] = {0, 0, 0};
+  if (k < R.nk) {
+    ki[0] = R.kidx[4 * k];
+    ki[1] = R.kidx[4 * k + 1];
+    ki[2] = R.kidx[4 * k + 2];
+  }
+  double re = 0.0, im = 0.0;
+  for (int p0 = first; p0 < end; p0 += kRhoChunk) {
+    const int np = min(kRhoChunk, end - p0);
+    __syncthreads();
+    for (int t = threadIdx.x; t < np * 3; t += blockDim.x) {
+      const int p = p0 + t / 3, d = t % 3;
+      const double x = R.path0[((size_t)slice * 3 + d) * R.N + p];
+      phase_row<double>(x * R.kbox[d], kmax_of(R, d), C + (size_t)(t / 3) * stride + koff_of(R, d));
+    }
+    __syncthreads();
+    if (k < R.nk)
+      for (int pp = 0; pp < np; pp++) {
+        double tr, ti;
+        kterm<double>(C + (size_t)pp * stride, o1, o2, ki, tr, ti);
+        re += tr;
+        im += ti;
+      }
+  }
+  if (k < R.nk) {
+    const size_t at = ((size_t)slice * R.ns + s) * R.nk + k;
+    R.rho0[at] = d2v{re, im};
+    R.rho1[at] = d2v{re, im};
+  }
+}
+
+void launch_rho_all(const ReplicaDev* reps, int rep, int nslices, int ns, int nk, int stride, hipStream_t s) {
+  dim3 grid(nslices, ns, (nk + kBlock - 1) / kBlock);
+  k_rho_all<<<grid, kBlock, (size_t)kRhoChunk * stride * 16, s>>>(reps, rep);
+}
+
+// Sum over k of this thread's k-dependent terms of LongRangeClass::SingleAction on copy `mode`
+// at one slice (homologous |rho|^2 u_k and heterologous 2 Re(rho_a rho_b*) u_k).
+__device__ inline double lr_k_terms(const RepCore& R, int mode, int slice, int level, double factor) {
+  double part = 0.0;
+  const d2v* rho = R.rho(mode) + (size_t)slice * R.ns * R.nk;
+  for (int k = threadIdx.x; k < R.nk; k += blockDim.x) {
+    for (int s = 0; s < R.ns; s++) {
+      if (!((R.lr_mask >> (s * R.ns + s)) & 1u)) continue;
+      const d2v r = rho[(size_t)s * R.nk + k];
+      const double rhok2 = r.x * r.x + r.y * r.y;
+      part += factor * 0.5 * 2.0 * rhok2 * R.ulong_k[((size_t)(s * R.ns + s) * R.lr_levels + level) * R.nk + k];
+    }
+    for (int s1 = 0; s1 < R.ns; s1++)
+      for (int s2 = s1 + 1; s2 < R.ns; s2++) {
+        if (!((R.lr_mask >> (s1 * R.ns + s2)) & 1u)) continue;
+        const d2v a = rho[(size_t)s1 * R.nk + k], b = rho[(size_t)s2 * R.nk + k];
+        const double rhorho = a.x * b.x + a.y * b.y;
+        part += factor * 2.0 * rhorho * R.ulong_k[((size_t)(s1 * R.ns + s2) * R.lr_levels + level) * R.nk + k];
+      }
+  }
+  return part;
+}
+
+template <bool MIXED>
+__global__ void __launch_bounds__(kBlock) k_lr_req(const ReplicaDev* __restrict__ reps, const ReqDev* __restrict__ reqs,
+                                                   int r, double* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  PIMC_LDS d2v* Cn = (PIMC_LDS d2v*)smem_raw;
+  const ReqDev Q = reqs[r];
+  const ReplicaDev* RD = reps + Q.replica;
+  const RepCore R = load_uniform(&RD->core);
+  const RepSpecies& S = RD->sp;
+  const int stride = kstride(R);
+  PIMC_LDS d2v* Co = Cn + stride;
+  PIMC_LDS double* scratch = (PIMC_LDS double*)(Co + stride);
+  const int o1 = koff_of(R, 1), o2 = koff_of(R, 2);
+  const int skip = 1 << Q.level;
+  typedef typename std::conditional<MIXED, float, double>::type PT;
+
+  if (Q.mode == 1) {
+    // NEW <- OLD over the range, then add the moved particles' new phases and remove the old ones
+    for (int slice = Q.slice1; slice <= Q.slice2; slice += skip)
+      for (int i = threadIdx.x; i < R.ns * R.nk; i += blockDim.x)
+        R.rho1[(size_t)slice * R.ns * R.nk + i] = R.rho0[(size_t)slice * R.ns * R.nk + i];
+    for (int pi = 0; pi < Q.nptcls; pi++) {
+      const int p = Q.ptcls[pi];
+      int sp = 0;
+      while (sp + 1 < R.ns && p >= S.first[sp + 1]) sp++;
+      for (int slice = Q.slice1; slice <= Q.slice2; slice += skip) {
+        __syncthreads();
+        if (threadIdx.x < 6) {
+          const int mode = threadIdx.x < 3 ? 1 : 0;
+          const int d = threadIdx.x % 3;
+          double x[3];
+          load_bead(R, mode, slice, p, x);
+          phase_rows_lane<PT>(R, x, d, mode ? Cn : Co);
+        }
+        __syncthreads();
+        d2v* rho = R.rho1 + ((size_t)slice * R.ns + sp) * R.nk;
+        for (int k = threadIdx.x; k < R.nk; k += blockDim.x) {
+          const int ki[3] = {R.kidx[4 * k], R.kidx[4 * k + 1], R.kidx[4 * k + 2]};
+          PT nr, ni, orr, oi;
+          kterm<PT>(Cn, o1, o2, ki, nr, ni);
+          kterm<PT>(Co, o1, o2, ki, orr, oi);
+          d2v v = rho[k];
+          v.x += (double)nr;
+          v.y += (double)ni;
+          v.x -= (double)orr;
+          v.y -= (double)oi;
+          rho[k] = v;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  double part = 0.0;
+  for (int slice = Q.slice1; slice <= Q.slice2; slice += skip) {
+    const double factor = (slice == Q.slice1 || slice == Q.slice2) ? 0.5 : 1.0;
+    part += lr_k_terms(R, Q.mode, slice, Q.level, factor);
+  }
+  const double ksum = block_sum(part, scratch);
+  if (threadIdx.x == 0) {
+    double homo = 0.0, background = 0.0, k0 = 0.0;
+    for (int slice = Q.slice1; slice <= Q.slice2; slice += skip) {
+      const double factor = (slice == Q.slice1 || slice == Q.slice2) ? 0.5 : 1.0;
+      lr_const_terms(R, S, Q.level, factor, homo, background, k0);
+    }
+    double U = ksum + homo;
+    U += R.use_background ? background : k0;
+    out[r] = U;
+  }
+}
+
+void launch_long_range(bool mixed, const ReplicaDev* reps, const ReqDev* reqs, int nreq, const int* strides,
+                       double* out, hipStream_t s) {
+  for (int r = 0; r < nreq; r++) {
+    const size_t lds = (size_t)2 * strides[r] * 16 + kWaves * 8;
+    if (mixed)
+      k_lr_req<true><<<1, kBlock, lds, s>>>(reps, reqs, r, out);
+    else
+      k_lr_req<false><<<1, kBlock, lds, s>>>(reps, reqs, r, out);
+  }
+}
+
+}  // namespace pimc

@@ -1,0 +1,41 @@
+"""Long-range (k-space) tables of a Coulomb pair in the classic Ewald split.
+
+The pair potential Z1 Z2 / r splits into erfc(a r)/r (short) and erf(a r)/r
+(long).  In LongRangeClass::SingleAction's conventions (LongRangeClass.cc:313-367,
+half-space k set, the homologous sum weighted 0.5 * 2.0) the tables per level
+with tau_l = tau 2^l are
+
+    Ulong_k   = tau_l Z1Z2 4 pi exp(-k^2 / 4 a^2) / (Omega k^2)
+    Ulong_r0  = tau_l Z1Z2 2 a / sqrt(pi)          (the long part at r = 0)
+    Ushort_k0 = tau_l Z1Z2 pi / (a^2 Omega)         (k = 0 of the short part)
+    Ulong_k0  = 0                                   (divergent; neutral systems cancel it)
+
+These are the closed-form tables of the plain Ewald breakup; PIMC++ fits an
+optimized breakup instead (OptimizedBreakup.cc), which is a setup-time step.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+def ewald_tables(kvecs: np.ndarray, box, z1z2: float, alpha: float, tau: float, num_levels: int):
+    """Returns (ulong_k [levels][nk], ulong_r0 [levels], ushort_k0 [levels], ulong_k0 [levels])."""
+    omega = float(np.prod(box))
+    k2 = np.einsum("ij,ij->i", kvecs, kvecs)
+    vk = z1z2 * 4.0 * math.pi * np.exp(-k2 / (4.0 * alpha * alpha)) / (omega * k2)
+    taus = tau * 2.0 ** np.arange(num_levels)
+    ulong_k = taus[:, None] * vk[None, :]
+    ulong_r0 = taus * z1z2 * 2.0 * alpha / math.sqrt(math.pi)
+    ushort_k0 = taus * z1z2 * math.pi / (alpha * alpha * omega)
+    ulong_k0 = np.zeros(num_levels)
+    return ulong_k, ulong_r0, ushort_k0, ulong_k0
+
+
+def kset_size(n: int) -> int:
+    """Half-space k vectors with |k| < n 2 pi / L in a cubic box: (#{m in Z^3 : 0 < |m|^2 < n^2}) / 2."""
+    r = np.arange(-n, n + 1)
+    m2 = r[:, None, None] ** 2 + r[None, :, None] ** 2 + r[None, None, :] ** 2
+    return int(((m2 > 0) & (m2 < n * n)).sum() // 2)

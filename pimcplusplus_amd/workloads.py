@@ -13,6 +13,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+from .ewald import ewald_tables
 from .tables import DavidTable, lj_david_table
 
 LAMBDA_AMU = 24.2542          # hbar^2 / (2 * 1 amu * k_B) in K A^2
@@ -36,6 +37,8 @@ class ReplicaSpec:
     k_cutoff: float = 0.0
     use_long_range: bool = False
     use_background: bool = False
+    # (sa, sb) -> (z1z2, alpha): closed-form Ewald long-range tables on the replica's k set
+    ewald: dict = field(default_factory=dict)
 
 
 @dataclass
@@ -141,5 +144,36 @@ def load(lib, system: System):
                                  k_cutoff=spec.k_cutoff, use_long_range=spec.use_long_range,
                                  use_background=spec.use_background, rng_stream=spec.rng_stream)
         lib.path_set(rid, spec.beads)
+        if spec.ewald:
+            kv, _ = lib.kvecs(rid)
+            for (sa, sb), (z1z2, alpha) in spec.ewald.items():
+                lib.set_long_range(rid, sa, sb, *ewald_tables(kv, spec.box, z1z2, alpha, spec.tau,
+                                                              system.num_levels))
+            lib.rho_k_update_all(rid)
         ids.append(rid)
     return ids
+
+
+def charged_lj_system(n_plus: int = 24, n_minus: int = 24, m: int = 32, tau: float = 0.025,
+                      num_levels: int = 3, kc_units: float = 4.0, seed: int = 1100,
+                      num_replicas: int = 1, charge_scale: float = 2.0) -> System:
+    """Two LJ species carrying charges +-q with an Ewald long-range action (test system for the
+    k-space path): short range is the LJ pair action, long range the Ewald k sum of q^2/r in
+    K*A units (q^2 = charge_scale K A)."""
+    n = n_plus + n_minus
+    box = lj_box(n)
+    table = lj_table_for(box, tau, num_levels)
+    rng = np.random.default_rng(seed)
+    lam = LAMBDA_AMU / 4.0
+    kc = kc_units * 2.0 * np.pi / box
+    alpha = 5.0 / box
+    reps = []
+    for r in range(num_replicas):
+        reps.append(ReplicaSpec(num_particles=n, num_time_slices=m, box=(box, box, box), tau=tau,
+                                species_count=[n_plus, n_minus], lambdas=[lam, lam],
+                                fit_keys=["lj"] * 4, beads=random_start(n, box, m, rng), rng_stream=r,
+                                k_cutoff=kc, use_long_range=True,
+                                ewald={(0, 0): (charge_scale, alpha), (1, 1): (charge_scale, alpha),
+                                       (0, 1): (-charge_scale, alpha)}))
+    return System(name="charged-lj", tables={"lj": table}, desired_tau={"lj": tau}, replicas=reps,
+                  num_levels=num_levels, seed=seed)

@@ -1,0 +1,79 @@
+"""GPU parity of the k-space path: rho_k, LongRange SingleAction, and the
+bisection chain with the long-range stage term."""
+
+import numpy as np
+import pytest
+
+from pimcplusplus_amd import abi, workloads
+from tests.test_gpu_parity import compare_chains
+
+pytestmark = pytest.mark.gpu
+
+
+def both(oracle, gpu, system):
+    return workloads.load(oracle, system), workloads.load(gpu, system)
+
+
+def test_rho_k_update_all(oracle, gpu):
+    sysm = workloads.charged_lj_system(n_plus=20, n_minus=12, m=16, kc_units=4.0)
+    io, ig = both(oracle, gpu, sysm)
+    for mode in (0, 1):
+        a, b = oracle.rho_k(io[0], mode), gpu.rho_k(ig[0], mode)
+        assert np.max(np.abs(a - b)) < 1e-12 * 20
+
+
+def test_long_range_requests_and_copies(oracle, gpu):
+    sysm = workloads.charged_lj_system(n_plus=16, n_minus=16, m=16, kc_units=4.0)
+    io, ig = both(oracle, gpu, sysm)
+    beads = oracle.path_get(io[0])
+    rng = np.random.default_rng(300)
+    new = beads.copy()
+    new[4:13, [5, 20]] += rng.normal(scale=0.3, size=(9, 2, 3))
+    for lib, r in ((oracle, io[0]), (gpu, ig[0])):
+        lib.path_set(r, new, abi.NEWMODE)
+    reqs = [(0, 0, 4, 12, 0, [5, 20]), (0, 1, 4, 12, 0, [5, 20]), (0, 0, 4, 12, 1, [5]), (0, 1, 4, 12, 2, [20])]
+    # the mixed path first (a NEWMODE request rewrites rho_k NEW over its range from OLD)
+    mixed = gpu.action_long_range([(ig[0],) + reqs[1][1:]], abi.MIXED)
+    want = np.concatenate([oracle.action_long_range([(io[0],) + q[1:]]) for q in reqs])
+    got = np.concatenate([gpu.action_long_range([(ig[0],) + q[1:]], abi.FP64) for q in reqs])
+    assert np.max(np.abs(got - want) / (1 + np.abs(want))) < 1e-12
+    assert np.max(np.abs(oracle.rho_k(io[0], 1) - gpu.rho_k(ig[0], 1))) < 1e-11
+    assert abs(mixed[0] - want[1]) < 1e-6 * (1 + abs(want[1]))
+    for lib, r in ((oracle, io[0]), (gpu, ig[0])):
+        lib.accept_copy(r, 4, 12, [5])
+        lib.reject_copy(r, 4, 12, [20])
+    for mode in (0, 1):
+        assert np.max(np.abs(oracle.rho_k(io[0], mode) - gpu.rho_k(ig[0], mode))) < 1e-11
+        assert np.array_equal(oracle.path_get(io[0], mode), gpu.path_get(ig[0], mode))
+
+
+def test_bisection_with_long_range_fp64(oracle, gpu):
+    sysm = workloads.charged_lj_system(n_plus=24, n_minus=24, m=32, kc_units=4.0, num_replicas=3)
+    io, ig = both(oracle, gpu, sysm)
+    ro, so = oracle.bisection_run(io, sysm.num_levels, sysm.seed, 0, 400)
+    rg, sg = gpu.bisection_run(ig, sysm.num_levels, sysm.seed, 0, 400, abi.FP64)
+    compare_chains(ro, rg)
+    assert so.accepted == sg.accepted > 0
+    assert so.k_evals == sg.k_evals > 0
+    for a, b in zip(io, ig):
+        assert np.max(np.abs(oracle.rho_k(a) - gpu.rho_k(b))) < 1e-10
+        assert np.max(np.abs(oracle.path_get(a) - gpu.path_get(b))) < 1e-12
+
+
+def test_bisection_with_long_range_mixed(oracle, gpu):
+    sysm = workloads.charged_lj_system(n_plus=24, n_minus=24, m=32, kc_units=4.0, num_replicas=3)
+    io, ig = both(oracle, gpu, sysm)
+    ro, _ = oracle.bisection_run(io, sysm.num_levels, sysm.seed, 0, 200)
+    rg, _ = gpu.bisection_run(ig, sysm.num_levels, sysm.seed, 0, 200, abi.MIXED)
+    worst, n = 0.0, 0
+    for r in range(3):
+        for m in range(200):
+            ns = ro["num_stages"][r, m]
+            if ns != rg["num_stages"][r, m] or ro["accepted"][r, m] != rg["accepted"][r, m]:
+                break
+            d = (ro["new_action"][r, m, :ns] - ro["old_action"][r, m, :ns]) - \
+                (rg["new_action"][r, m, :ns] - rg["old_action"][r, m, :ns])
+            worst = max(worst, float(np.max(np.abs(d))))
+            n += ns
+    assert n > 100
+    assert worst <= 1e-6
