@@ -119,6 +119,31 @@ int pimc_fit_eval_u(pimc_ctx* ctx, int32_t fit_id, int32_t level, int64_t n,
                     const double* q, const double* z, const double* s2,
                     int32_t precision, double* out);
 
+/* A PAcoulombBCFitClass fit (Common/PairAction/PAcoulombBCFit.cc:159-176,
+ * 242-311): per level (beta_l = smallest_beta 2^l) a BicubicSpline of U over
+ * (q, t = s/2q), z-independent; beyond the q grid U = beta_l Z1Z2 / q.  The
+ * bicubic's node derivatives are built as BicubicSpline::Init does
+ * (Common/Splines/BicubicSpline.cc:20-154).  The fit is long-ranged: when the
+ * replica uses long range, ShortRange subtracts (Ulong(r) + Ulong(r'))/2
+ * (ShortRangeClass.cc:165-166) with the replica's Ulong tables. */
+typedef struct pimc_coulomb_bc_fit_desc {
+  int32_t q_grid_type;     /* pimc_grid_type */
+  int32_t q_points;
+  double q_start;
+  double q_end;
+  int32_t t_grid_type;     /* pimc_grid_type */
+  int32_t t_points;
+  double t_start;
+  double t_end;
+  int32_t num_betas;       /* levels stored */
+  int32_t reserved;
+  double smallest_beta;    /* the level-0 tau */
+  double z1z2;             /* Coulomb potential Z1 Z2 / r beyond the grid */
+  const double* umat;      /* [num_betas][q_points][t_points] */
+} pimc_coulomb_bc_fit_desc;
+
+int pimc_fit_create_coulomb_bc(pimc_ctx* ctx, const pimc_coulomb_bc_fit_desc* desc, int32_t* fit_id);
+
 /* ------------------------------------------------------------------ */
 /* Replicas (PathClass + the actions' configuration)                   */
 /* ------------------------------------------------------------------ */
@@ -154,6 +179,14 @@ int pimc_replica_set_long_range(pimc_ctx* ctx, int32_t replica, int32_t sa, int3
                                 int32_t num_levels, const double* ulong_k,
                                 const double* ulong_r0, const double* ushort_k0,
                                 const double* ulong_k0);
+
+/* Ulong(level)(r) of species pair (sa, sb) (PairActionFitClass::Ulong, PAFitBase.h:38): a
+ * QuinticSpline per level on LinearGrid(grid_start, grid_end, num_points), given as the
+ * polynomial coefficients the reference evaluates (QuinticSpline.h:204-218):
+ * coefs [num_levels][num_points][6] = {Y, B, C, D, E, F} of each knot,
+ * S(x) = ((((F P + E) P + D) P + C) P + B) P + Y with P = x - X_I. */
+int pimc_replica_set_ulong(pimc_ctx* ctx, int32_t replica, int32_t sa, int32_t sb, int32_t num_levels,
+                           double grid_start, double grid_end, int32_t num_points, const double* coefs);
 
 /* Path data. mode: PIMC_OLDMODE, PIMC_NEWMODE or PIMC_BOTHMODES (set only).
  * beads is [(M+1)][N][3]. */

@@ -264,9 +264,194 @@ struct MultiCubicSpline {
 };
 
 // ------------------------------------------------------------------------
+// PairActionFitClass (PAFitBase.h:29-138): the part of the interface the path uses
+// ------------------------------------------------------------------------
+struct PairFit {
+  virtual ~PairFit() {}
+  virtual double U(double q, double z, double s2, int level) const = 0;
+  virtual bool IsLongRange() const = 0;
+  // bisection levels the fit covers
+  virtual int num_levels() const = 0;
+};
+
+// ------------------------------------------------------------------------
+// BicubicSpline (BicubicSpline.h:24-30, 266-300, 356-413; BicubicSpline.cc:20-154)
+// ------------------------------------------------------------------------
+struct BCpoint {
+  double z, dzdx, dzdy, d2zdxdy;
+};
+
+inline double p1(double t) { return ((t - 1.0) * (t - 1.0) * (1.0 + 2.0 * t)); }
+inline double p2(double t) { return (t * t * (3.0 - 2.0 * t)); }
+inline double q1(double t) { return (t * (t - 1.0) * (t - 1.0)); }
+inline double q2(double t) { return (t * t * (t - 1.0)); }
+
+struct BicubicSpline {
+  const Grid* Xgrid = nullptr;
+  const Grid* Ygrid = nullptr;
+  int Nx = 0, Ny = 0;
+  std::vector<BCpoint> Fv;
+  BCpoint& F(int i, int j) { return Fv[(size_t)i * Ny + j]; }
+  const BCpoint& F(int i, int j) const { return Fv[(size_t)i * Ny + j]; }
+
+  void init(const Grid* xg, const Grid* yg, const double* f) {
+    Xgrid = xg;
+    Ygrid = yg;
+    Nx = xg->n;
+    Ny = yg->n;
+    Fv.assign((size_t)Nx * Ny, BCpoint{0, 0, 0, 0});
+    for (int i = 0; i < Nx; i++)
+      for (int j = 0; j < Ny; j++) F(i, j).z = f[(size_t)i * Ny + j];
+    for (int i = 0; i < Ny; i++) XUpdate(i);
+    for (int i = 0; i < Nx; i++) YUpdate(i);
+    BiUpdate();
+  }
+  void XUpdate(int iy) {
+    const std::vector<double>& x = Xgrid->x;
+    std::vector<double> mu(Nx);
+    F(0, iy).dzdx = 1.5 * (F(1, iy).z - F(0, iy).z) / (x[1] - x[0]);
+    F(Nx - 1, iy).dzdx = 1.5 * (F(Nx - 1, iy).z - F(Nx - 2, iy).z) / (x[Nx - 1] - x[Nx - 2]);
+    mu[0] = 0.5;
+    for (int j = 1; j < (Nx - 1); j++) {
+      double lambda = 0.5 * (x[j + 1] - x[j]) / (x[j + 1] - x[j - 1]);
+      mu[j] = 0.5 - lambda;
+      F(j, iy).dzdx = 3.0 * (lambda * (F(j, iy).z - F(j - 1, iy).z) / (x[j] - x[j - 1]) +
+                             mu[j] * (F(j + 1, iy).z - F(j, iy).z) / (x[j + 1] - x[j]));
+      double cj = 1.0 - lambda * mu[j - 1];
+      F(j, iy).dzdx -= lambda * F(j - 1, iy).dzdx;
+      mu[j] /= cj;
+      F(j, iy).dzdx /= cj;
+    }
+    int j = Nx - 1;
+    double lambda = 0.5;
+    mu[j] = 0.5 - lambda;
+    F(j, iy).dzdx = 3.0 * (lambda * (F(j, iy).z - F(j - 1, iy).z) / (x[j] - x[j - 1]));
+    double cj = 1.0 - lambda * mu[j - 1];
+    F(j, iy).dzdx -= lambda * F(j - 1, iy).dzdx;
+    mu[j] /= cj;
+    F(j, iy).dzdx /= cj;
+    for (j = Nx - 2; j >= 0; j--) F(j, iy).dzdx -= mu[j] * F(j + 1, iy).dzdx;
+  }
+  void YUpdate(int ix) {
+    const std::vector<double>& y = Ygrid->x;
+    std::vector<double> mu(Ny);
+    F(ix, 0).dzdy = 1.5 * (F(ix, 1).z - F(ix, 0).z) / (y[1] - y[0]);
+    F(ix, Ny - 1).dzdy = 1.5 * (F(ix, Ny - 1).z - F(ix, Ny - 2).z) / (y[Ny - 1] - y[Ny - 2]);
+    mu[0] = 0.5;
+    for (int j = 1; j < (Ny - 1); j++) {
+      double lambda = 0.5 * (y[j + 1] - y[j]) / (y[j + 1] - y[j - 1]);
+      mu[j] = 0.5 - lambda;
+      F(ix, j).dzdy = 3.0 * (lambda * (F(ix, j).z - F(ix, j - 1).z) / (y[j] - y[j - 1]) +
+                             mu[j] * (F(ix, j + 1).z - F(ix, j).z) / (y[j + 1] - y[j]));
+      double cj = 1.0 - lambda * mu[j - 1];
+      F(ix, j).dzdy -= lambda * F(ix, j - 1).dzdy;
+      mu[j] /= cj;
+      F(ix, j).dzdy /= cj;
+    }
+    int j = Ny - 1;
+    double lambda = 0.5;
+    mu[j] = 0.5 - lambda;
+    F(ix, j).dzdy = 3.0 * (lambda * (F(ix, j).z - F(ix, j - 1).z) / (y[j] - y[j - 1]));
+    double cj = 1.0 - lambda * mu[j - 1];
+    F(ix, j).dzdy -= lambda * F(ix, j - 1).dzdy;
+    mu[j] /= cj;
+    F(ix, j).dzdy /= cj;
+    for (j = Ny - 2; j >= 0; j--) F(ix, j).dzdy -= mu[j] * F(ix, j + 1).dzdy;
+  }
+  void BiUpdate() {
+    const std::vector<double>& x = Xgrid->x;
+    std::vector<double> mu(Nx);
+    for (int iy = 0; iy < Ny; iy++) {
+      F(0, iy).d2zdxdy = 1.5 * (F(1, iy).dzdy - F(0, iy).dzdy) / (x[1] - x[0]);
+      F(Nx - 1, iy).d2zdxdy = 1.5 * (F(Nx - 1, iy).dzdy - F(Nx - 2, iy).dzdy) / (x[Nx - 1] - x[Nx - 2]);
+      mu[0] = 0.5;
+      for (int j = 1; j < (Nx - 1); j++) {
+        double lambda = 0.5 * (x[j + 1] - x[j]) / (x[j + 1] - x[j - 1]);
+        mu[j] = 0.5 - lambda;
+        F(j, iy).d2zdxdy = 3.0 * (lambda * (F(j, iy).dzdy - F(j - 1, iy).dzdy) / (x[j] - x[j - 1]) +
+                                  mu[j] * (F(j + 1, iy).dzdy - F(j, iy).dzdy) / (x[j + 1] - x[j]));
+        double cj = 1.0 - lambda * mu[j - 1];
+        F(j, iy).d2zdxdy -= lambda * F(j - 1, iy).d2zdxdy;
+        mu[j] /= cj;
+        F(j, iy).d2zdxdy /= cj;
+      }
+      int j = Nx - 1;
+      double lambda = 0.5;
+      mu[j] = 0.5 - lambda;
+      F(j, iy).d2zdxdy = 3.0 * (lambda * (F(j, iy).dzdy - F(j - 1, iy).dzdy) / (x[j] - x[j - 1]));
+      double cj = 1.0 - lambda * mu[j - 1];
+      F(j, iy).d2zdxdy -= lambda * F(j - 1, iy).d2zdxdy;
+      mu[j] /= cj;
+      F(j, iy).d2zdxdy /= cj;
+      for (j = Nx - 2; j >= 0; j--) F(j, iy).d2zdxdy -= mu[j] * F(j + 1, iy).d2zdxdy;
+    }
+  }
+  double operator()(double x, double y) const {
+    int ix = Xgrid->reverse_map(x);
+    int iy = Ygrid->reverse_map(y);
+    ix = std::max(0, ix);
+    ix = std::min(ix, Nx - 2);
+    iy = std::max(0, iy);
+    iy = std::min(iy, Ny - 2);
+    double h = Xgrid->x[ix + 1] - Xgrid->x[ix];
+    double k = Ygrid->x[iy + 1] - Ygrid->x[iy];
+    double u = (x - Xgrid->x[ix]) / h;
+    double v = (y - Ygrid->x[iy]) / k;
+    double a0 = p1(u), a1 = p2(u), a2 = h * q1(u), a3 = h * q2(u);
+    double b0 = p1(v), b1 = p2(v), b2 = k * q1(v), b3 = k * q2(v);
+    const BCpoint &A = F(ix, iy), &B = F(ix, iy + 1), &Cc = F(ix + 1, iy), &D = F(ix + 1, iy + 1);
+    double val = a0 * (A.z * b0 + B.z * b1 + A.dzdy * b2 + B.dzdy * b3) +
+                 a1 * (Cc.z * b0 + D.z * b1 + Cc.dzdy * b2 + D.dzdy * b3) +
+                 a2 * (A.dzdx * b0 + B.dzdx * b1 + A.d2zdxdy * b2 + B.d2zdxdy * b3) +
+                 a3 * (Cc.dzdx * b0 + D.dzdx * b1 + Cc.d2zdxdy * b2 + D.d2zdxdy * b3);
+    return val;
+  }
+};
+
+// ------------------------------------------------------------------------
+// PAcoulombBCFitClass::U (PAcoulombBCFit.cc:159-176)
+// ------------------------------------------------------------------------
+struct CoulombBCFit : PairFit {
+  Grid qgrid, tgrid;
+  std::vector<BicubicSpline> Usplines;
+  double SmallestBeta = 0, Z1Z2 = 0;
+  double U(double q, double z, double s2, int level) const override {
+    if (q <= (qgrid.end * 1.0000001)) {
+      if (q == 0) return (Usplines[level](0.0, 0.0));
+      double t = sqrt(s2) / (2.0 * q);
+      return (Usplines[level](q, t));
+    }
+    double beta = SmallestBeta;
+    for (int i = 0; i < level; i++) beta *= 2.0;
+    return (beta * (Z1Z2 / q));   // CoulombPot::V
+  }
+  bool IsLongRange() const override { return true; }
+  int num_levels() const override { return (int)Usplines.size(); }
+};
+
+// ------------------------------------------------------------------------
+// QuinticSpline evaluation (QuinticSpline.h:172-218) on a LinearGrid
+// ------------------------------------------------------------------------
+struct QuinticTable {
+  bool set = false;
+  Grid grid;
+  int levels = 0;
+  std::vector<double> c;   // [levels][n][6] = Y, B, C, D, E, F
+  double eval(int level, double x) const {
+    int J = grid.reverse_map(x) + 1;
+    int I = J - 1;
+    if (I < 0) { I = 0; J = 1; }
+    if (J > (grid.n - 1)) { J = grid.n - 1; I = J - 1; }
+    double P = (x - grid.x[I]);
+    const double* k = &c[((size_t)level * grid.n + I) * 6];
+    return ((((k[5] * P + k[4]) * P + k[3]) * P + k[2]) * P + k[1]) * P + k[0];
+  }
+};
+
+// ------------------------------------------------------------------------
 // DavidPAClass (DavidPAClass.cc)
 // ------------------------------------------------------------------------
-struct DavidFit {
+struct DavidFit : PairFit {
   Grid grid;
   std::vector<MultiCubicSpline> ukj;  // per tau level
   int num_tau = 0, ncols = 0, num_terms = 0, taupos = 0;
@@ -309,10 +494,12 @@ struct DavidFit {
     return U;
   }
   // DavidPAClass::U (DavidPAClass.cc:92-102)
-  double U(double q, double z, double s2, int level) const {
+  double U(double q, double z, double s2, int level) const override {
     double s = sqrt(s2);
     return calc_usqz_fast(s, q, z, level);
   }
+  bool IsLongRange() const override { return false; }   // DavidPAClass.cc:127-130
+  int num_levels() const override { return num_tau - taupos; }
 };
 
 // ------------------------------------------------------------------------
@@ -343,6 +530,7 @@ struct Replica {
   int nk = 0;
   std::vector<cplx> rho[2];      // [slice][species][k]
   std::vector<LongRangeTables> lr;  // [ns*ns]
+  std::vector<QuinticTable> ulong;  // [ns*ns] PairActionFitClass::Ulong of the pair
   uint32_t stream = 0;
   int join = 0;
 
@@ -475,8 +663,19 @@ struct Replica {
   }
 };
 
+int make_grid_checked(int32_t type, double start, double end, int32_t npoints, Grid& g) {
+  if (npoints < 2 || !(start < end)) return fail(PIMC_ERR_ARG, "grid: bad range or size");
+  if (type == PIMC_GRID_LINEAR)
+    g.init_linear(start, end, npoints);
+  else if (type == PIMC_GRID_LOG && start > 0)
+    g.init_log_dm(start, end, npoints);
+  else
+    return fail(PIMC_ERR_ARG, "grid: bad type");
+  return PIMC_OK;
+}
+
 struct Ctx {
-  std::vector<std::unique_ptr<DavidFit>> fits;
+  std::vector<std::unique_ptr<PairFit>> fits;
   std::vector<std::unique_ptr<Replica>> reps;
 };
 
@@ -502,7 +701,9 @@ double short_range_action(Ctx* c, Replica& R, int mode, int slice1, int slice2, 
     for (int ptcl2 = 0; ptcl2 < R.N; ptcl2++) {
       if (DoPtcl[ptcl2]) {
         int species2 = R.species_of[ptcl2];
-        const DavidFit& PA = *c->fits[R.pair_fit[species1 * R.ns + species2]];
+        const PairFit& PA = *c->fits[R.pair_fit[species1 * R.ns + species2]];
+        const QuinticTable& Ulong = R.ulong[species1 * R.ns + species2];
+        const bool subtract = PA.IsLongRange() && R.use_long_range;
         for (int slice = slice1; slice < slice2; slice += skip) {
           double r[3], rp[3], rmag, rpmag;
           R.dist_disp(mode, slice, slice + skip, ptcl1, ptcl2, rmag, rpmag, r, rp, nullptr, nullptr);
@@ -511,6 +712,7 @@ double short_range_action(Ctx* c, Replica& R, int mode, int slice1, int slice2, 
           double q = 0.5 * (rmag + rpmag);
           double z = (rmag - rpmag);
           double U = PA.U(q, z, s2, level);
+          if (subtract) U -= 0.5 * (Ulong.eval(level, rmag) + Ulong.eval(level, rpmag));
           TotalU += U;
           if (evals) ++*evals;
         }
@@ -867,12 +1069,47 @@ int pimc_fit_create_david(pimc_ctx* ctx, const pimc_david_fit_desc* d, int32_t* 
   return PIMC_OK;
 }
 
+int pimc_fit_create_coulomb_bc(pimc_ctx* ctx, const pimc_coulomb_bc_fit_desc* d, int32_t* fit_id) {
+  if (!ctx || !d || !fit_id || !d->umat) return fail(PIMC_ERR_ARG, "fit_create_coulomb_bc: NULL argument");
+  if (d->q_points < 3 || d->t_points < 3 || d->num_betas < 1 || d->num_betas > PIMC_MAX_LEVELS)
+    return fail(PIMC_ERR_ARG, "fit_create_coulomb_bc: bad sizes");
+  auto f = std::make_unique<CoulombBCFit>();
+  int rc = make_grid_checked(d->q_grid_type, d->q_start, d->q_end, d->q_points, f->qgrid);
+  if (rc) return rc;
+  rc = make_grid_checked(d->t_grid_type, d->t_start, d->t_end, d->t_points, f->tgrid);
+  if (rc) return rc;
+  f->SmallestBeta = d->smallest_beta;
+  f->Z1Z2 = d->z1z2;
+  f->Usplines.resize(d->num_betas);
+  for (int b = 0; b < d->num_betas; b++)
+    f->Usplines[b].init(&f->qgrid, &f->tgrid, d->umat + (size_t)b * d->q_points * d->t_points);
+  ctx->c.fits.push_back(std::move(f));
+  *fit_id = (int32_t)ctx->c.fits.size() - 1;
+  return PIMC_OK;
+}
+
+int pimc_replica_set_ulong(pimc_ctx* ctx, int32_t replica, int32_t sa, int32_t sb, int32_t num_levels,
+                           double grid_start, double grid_end, int32_t num_points, const double* coefs) {
+  Replica* R = get_rep(&ctx->c, replica);
+  if (!R || sa < 0 || sb < 0 || sa >= R->ns || sb >= R->ns || num_levels < 1 || num_points < 2 || !coefs ||
+      !(grid_start < grid_end))
+    return fail(PIMC_ERR_ARG, "replica_set_ulong: bad argument");
+  QuinticTable t;
+  t.set = true;
+  t.grid.init_linear(grid_start, grid_end, num_points);
+  t.levels = num_levels;
+  t.c.assign(coefs, coefs + (size_t)num_levels * num_points * 6);
+  R->ulong[sa * R->ns + sb] = t;
+  R->ulong[sb * R->ns + sa] = t;
+  return PIMC_OK;
+}
+
 int pimc_fit_eval_u(pimc_ctx* ctx, int32_t fit_id, int32_t level, int64_t n, const double* q,
                     const double* z, const double* s2, int32_t precision, double* out) {
   (void)precision;
   if (!ctx || fit_id < 0 || fit_id >= (int)ctx->c.fits.size()) return fail(PIMC_ERR_ARG, "fit_eval_u: bad fit");
-  const DavidFit& f = *ctx->c.fits[fit_id];
-  if (level < 0 || level + f.taupos >= f.num_tau) return fail(PIMC_ERR_ARG, "fit_eval_u: level out of range");
+  const PairFit& f = *ctx->c.fits[fit_id];
+  if (level < 0 || level >= f.num_levels()) return fail(PIMC_ERR_ARG, "fit_eval_u: level out of range");
   for (int64_t i = 0; i < n; i++) out[i] = f.U(q[i], z[i], s2[i], level);
   return PIMC_OK;
 }
@@ -922,6 +1159,7 @@ int pimc_replica_create(pimc_ctx* ctx, const pimc_replica_desc* d, int32_t* repl
     R->rho[1] = R->rho[0];
   }
   R->lr.resize(R->ns * R->ns);
+  R->ulong.resize(R->ns * R->ns);
   ctx->c.reps.push_back(std::move(R));
   *replica_id = (int32_t)ctx->c.reps.size() - 1;
   return PIMC_OK;
@@ -1045,6 +1283,12 @@ int pimc_action_short_range(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t 
     Replica* R;
     int rc = check_req(&ctx->c, reqs[i], &R);
     if (rc) return rc;
+    for (int f = 0; f < R->ns * R->ns; f++) {
+      const PairFit& PA = *ctx->c.fits[R->pair_fit[f]];
+      if (reqs[i].level >= PA.num_levels()) return fail(PIMC_ERR_ARG, "action_short_range: level beyond the fit's tau levels");
+      if (PA.IsLongRange() && R->use_long_range && (!R->ulong[f].set || reqs[i].level >= R->ulong[f].levels))
+        return fail(PIMC_ERR_STATE, "action_short_range: long-range pair fit without Ulong tables");
+    }
     out[i] = short_range_action(&ctx->c, *R, reqs[i].mode, reqs[i].slice1, reqs[i].slice2, reqs[i].ptcls,
                                 reqs[i].nptcls, reqs[i].level, nullptr);
   }
@@ -1113,9 +1357,12 @@ int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, con
     if (desc->species < 0 || desc->species >= R->ns) return fail(PIMC_ERR_ARG, "bisection_run: bad species");
     for (int s = 0; s < R->ns; s++)
       for (int t = 0; t < R->ns; t++) {
-        const DavidFit& f = *ctx->c.fits[R->pair_fit[s * R->ns + t]];
-        if (f.taupos + desc->num_levels > f.num_tau)
+        const PairFit& f = *ctx->c.fits[R->pair_fit[s * R->ns + t]];
+        if (desc->num_levels > f.num_levels())
           return fail(PIMC_ERR_ARG, "bisection_run: pair fit has too few tau levels");
+        const QuinticTable& ul = R->ulong[s * R->ns + t];
+        if (f.IsLongRange() && R->use_long_range && (!ul.set || desc->num_levels > ul.levels))
+          return fail(PIMC_ERR_STATE, "bisection_run: long-range pair fit without Ulong tables");
       }
   }
   pimc_bisection_stats st;

@@ -42,6 +42,24 @@ class DavidFitDesc(C.Structure):
     ]
 
 
+class CoulombBCFitDesc(C.Structure):
+    _fields_ = [
+        ("q_grid_type", C.c_int32),
+        ("q_points", C.c_int32),
+        ("q_start", C.c_double),
+        ("q_end", C.c_double),
+        ("t_grid_type", C.c_int32),
+        ("t_points", C.c_int32),
+        ("t_start", C.c_double),
+        ("t_end", C.c_double),
+        ("num_betas", C.c_int32),
+        ("reserved", C.c_int32),
+        ("smallest_beta", C.c_double),
+        ("z1z2", C.c_double),
+        ("umat", _f64p),
+    ]
+
+
 class ReplicaDesc(C.Structure):
     _fields_ = [
         ("num_particles", C.c_int32),
@@ -118,7 +136,10 @@ SIGNATURES = {
     "pimc_ctx_synchronize": (C.c_int, [_CTX]),
     "pimc_fit_create_david": (C.c_int, [_CTX, C.POINTER(DavidFitDesc), _i32p]),
     "pimc_fit_eval_u": (C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int64, _f64p, _f64p, _f64p, C.c_int32, _f64p]),
+    "pimc_fit_create_coulomb_bc": (C.c_int, [_CTX, C.POINTER(CoulombBCFitDesc), _i32p]),
     "pimc_replica_create": (C.c_int, [_CTX, C.POINTER(ReplicaDesc), _i32p]),
+    "pimc_replica_set_ulong": (
+        C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32, _f64p]),
     "pimc_replica_num_kvecs": (C.c_int, [_CTX, C.c_int32, _i32p]),
     "pimc_replica_kvecs": (C.c_int, [_CTX, C.c_int32, _f64p, _i32p]),
     "pimc_replica_set_long_range": (
@@ -227,6 +248,29 @@ class PimcLib:
         fid = C.c_int32()
         self._check("pimc_fit_create_david", self.lib.pimc_fit_create_david(self.ctx, C.byref(d), C.byref(fid)))
         return fid.value
+
+    def fit_create_coulomb_bc(self, q_grid: tuple, t_grid: tuple, umat: np.ndarray, smallest_beta: float,
+                              z1z2: float) -> int:
+        """q_grid, t_grid = (grid_type, start, end); umat is [num_betas][nq][nt]."""
+        u = _f64(umat)
+        if u.ndim != 3:
+            raise ValueError("umat must be [num_betas][nq][nt]")
+        d = CoulombBCFitDesc(q_grid_type=q_grid[0], q_points=u.shape[1], q_start=q_grid[1], q_end=q_grid[2],
+                             t_grid_type=t_grid[0], t_points=u.shape[2], t_start=t_grid[1], t_end=t_grid[2],
+                             num_betas=u.shape[0], reserved=0, smallest_beta=smallest_beta, z1z2=z1z2,
+                             umat=_p(u, _f64p))
+        fid = C.c_int32()
+        self._check("pimc_fit_create_coulomb_bc",
+                    self.lib.pimc_fit_create_coulomb_bc(self.ctx, C.byref(d), C.byref(fid)))
+        return fid.value
+
+    def set_ulong(self, rep: int, sa: int, sb: int, grid_start: float, grid_end: float, coefs: np.ndarray):
+        """coefs [num_levels][num_points][6] = {Y, B, C, D, E, F} per knot."""
+        c = _f64(coefs)
+        if c.ndim != 3 or c.shape[2] != 6:
+            raise ValueError("coefs must be [num_levels][num_points][6]")
+        self._check("pimc_replica_set_ulong", self.lib.pimc_replica_set_ulong(
+            self.ctx, rep, sa, sb, c.shape[0], grid_start, grid_end, c.shape[1], _p(c, _f64p)))
 
     def fit_eval_u(self, fit: int, level: int, q, z, s2, precision: int = FP64) -> np.ndarray:
         q, z, s2 = _f64(q), _f64(z), _f64(s2)

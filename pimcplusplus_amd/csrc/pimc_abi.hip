@@ -131,14 +131,57 @@ void spline_d2(const std::vector<double>& x, const double* y, int stride, double
 struct HostFit {
   FitDev dev{};            // pointers refer to `mem`
   DevBuf mem;
-  int num_terms = 0;
+  int num_terms = 0;       // -1 for fits without the DavidPA expansion
   size_t staged64 = 0, staged32 = 0;
+  int num_levels() const { return dev.num_tau - dev.taupos; }
 };
+
+// Slopes of one line of a bicubic table: the first-derivative tridiagonal solve of
+// BicubicSpline::XUpdate / YUpdate / BiUpdate (BicubicSpline.cc:20-154), natural ends.
+// v[i * stride] are the values along the line, d[i * stride] receives the slopes.
+void bc_slopes(const std::vector<double>& x, const double* v, double* d, size_t stride) {
+  const int N = (int)x.size();
+  std::vector<double> mu(N);
+  auto V = [&](int i) { return v[(size_t)i * stride]; };
+  auto Dd = [&](int i) -> double& { return d[(size_t)i * stride]; };
+  Dd(0) = 1.5 * (V(1) - V(0)) / (x[1] - x[0]);
+  Dd(N - 1) = 1.5 * (V(N - 1) - V(N - 2)) / (x[N - 1] - x[N - 2]);
+  mu[0] = 0.5;
+  for (int j = 1; j < (N - 1); j++) {
+    double lambda = 0.5 * (x[j + 1] - x[j]) / (x[j + 1] - x[j - 1]);
+    mu[j] = 0.5 - lambda;
+    Dd(j) = 3.0 * (lambda * (V(j) - V(j - 1)) / (x[j] - x[j - 1]) + mu[j] * (V(j + 1) - V(j)) / (x[j + 1] - x[j]));
+    double cj = 1.0 - lambda * mu[j - 1];
+    Dd(j) -= lambda * Dd(j - 1);
+    mu[j] /= cj;
+    Dd(j) /= cj;
+  }
+  int j = N - 1;
+  double lambda = 0.5;
+  mu[j] = 0.5 - lambda;
+  Dd(j) = 3.0 * (lambda * (V(j) - V(j - 1)) / (x[j] - x[j - 1]));
+  double cj = 1.0 - lambda * mu[j - 1];
+  Dd(j) -= lambda * Dd(j - 1);
+  mu[j] /= cj;
+  Dd(j) /= cj;
+  for (j = N - 2; j >= 0; j--) Dd(j) -= mu[j] * Dd(j + 1);
+}
+
+int make_host_grid(int type, double start, double end, int n, HostGrid& g) {
+  if (n < 3 || !(start < end)) return fail(PIMC_ERR_ARG, "grid: bad range or size");
+  if (type == PIMC_GRID_LINEAR)
+    make_linear(g, start, end, n);
+  else if (type == PIMC_GRID_LOG && start > 0)
+    make_log(g, start, end, n);
+  else
+    return fail(PIMC_ERR_ARG, "grid: bad type");
+  return PIMC_OK;
+}
 
 struct HostReplica {
   ReplicaDev dev{};
   DevBuf path0, path1, perm0, perm1;
-  DevBuf rho0, rho1, kidx_dev, ulong_k, lr_const;
+  DevBuf rho0, rho1, kidx_dev, ulong_k, lr_const, ulong;
   std::vector<double> kvecs;     // [nk][3]
   std::vector<int> kidx;         // [nk][3] index + maxk
   bool use_long_range = false;
@@ -402,6 +445,94 @@ int pimc_fit_create_david(pimc_ctx* ctx, const pimc_david_fit_desc* d, int32_t* 
   return PIMC_OK;
 }
 
+int pimc_fit_create_coulomb_bc(pimc_ctx* ctx, const pimc_coulomb_bc_fit_desc* d, int32_t* fit_id) {
+  if (!ctx || !d || !fit_id || !d->umat) return fail(PIMC_ERR_ARG, "fit_create_coulomb_bc: NULL argument");
+  if (d->num_betas < 1 || d->num_betas > PIMC_MAX_LEVELS) return fail(PIMC_ERR_ARG, "fit_create_coulomb_bc: bad num_betas");
+  HostGrid qg, tg;
+  int rc = make_host_grid(d->q_grid_type, d->q_start, d->q_end, d->q_points, qg);
+  if (rc) return rc;
+  rc = make_host_grid(d->t_grid_type, d->t_start, d->t_end, d->t_points, tg);
+  if (rc) return rc;
+  const int nq = qg.n, nt = tg.n, nb = d->num_betas;
+  // BCpoint {z, dzdx, dzdy, d2zdxdy} per node and level (BicubicSpline::Init)
+  std::vector<double> z((size_t)nq * nt), dx(z.size()), dy(z.size()), dxy(z.size());
+  std::vector<d4v> bc((size_t)nb * nq * nt);
+  for (int b = 0; b < nb; b++) {
+    const double* u = d->umat + (size_t)b * nq * nt;
+    std::copy(u, u + z.size(), z.begin());
+    for (int iy = 0; iy < nt; iy++) bc_slopes(qg.x, z.data() + iy, dx.data() + iy, nt);   // XUpdate
+    for (int ix = 0; ix < nq; ix++) bc_slopes(tg.x, z.data() + (size_t)ix * nt, dy.data() + (size_t)ix * nt, 1);
+    for (int iy = 0; iy < nt; iy++) bc_slopes(qg.x, dy.data() + iy, dxy.data() + iy, nt);  // BiUpdate
+    for (size_t i = 0; i < z.size(); i++) bc[(size_t)b * z.size() + i] = d4v{z[i], dx[i], dy[i], dxy[i]};
+  }
+  auto f = std::make_unique<HostFit>();
+  size_t bq = (size_t)nq * 8, bt = (size_t)nt * 8, bb = bc.size() * sizeof(d4v);
+  size_t off_q = 0, off_t = (bq + 255) & ~(size_t)255, off_b = (off_t + bt + 255) & ~(size_t)255, total = off_b + bb;
+  if ((rc = f->mem.ensure(total))) return rc;
+  unsigned char* base = f->mem.as<unsigned char>();
+  HIP_TRY(hipMemcpy(base + off_q, qg.x.data(), bq, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(base + off_t, tg.x.data(), bt, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(base + off_b, bc.data(), bb, hipMemcpyHostToDevice));
+  FitDev& F = f->dev;
+  F.kind = 1;
+  F.grid_type = qg.type;
+  F.ng = nq;
+  F.nint = nq - 1;
+  F.num_tau = nb;
+  F.taupos = 0;
+  F.start = qg.start;
+  F.end = qg.end;
+  F.deltainv = qg.deltainv;
+  F.r0 = qg.r0;
+  F.spacing = qg.spacing;
+  F.logspacing = qg.type == PIMC_GRID_LOG ? log(qg.spacing) : 0.0;
+  F.x = (const double*)(base + off_q);
+  F.nt = nt;
+  F.tgrid = GridDev{tg.type, nt, tg.start, tg.end, tg.deltainv, tg.r0,
+                    tg.type == PIMC_GRID_LOG ? log(tg.spacing) : 0.0, (const double*)(base + off_t)};
+  F.bc = (const d4v*)(base + off_b);
+  F.smallest_beta = d->smallest_beta;
+  F.z1z2 = d->z1z2;
+  f->num_terms = -1;
+  ctx->fits.push_back(std::move(f));
+  ctx->fits_dirty = true;
+  *fit_id = (int32_t)ctx->fits.size() - 1;
+  return PIMC_OK;
+}
+
+int pimc_replica_set_ulong(pimc_ctx* ctx, int32_t replica, int32_t sa, int32_t sb, int32_t num_levels,
+                           double grid_start, double grid_end, int32_t num_points, const double* coefs) {
+  HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
+  if (!H || num_levels < 1 || num_levels > PIMC_MAX_LEVELS || num_points < 2 || !coefs || !(grid_start < grid_end))
+    return fail(PIMC_ERR_ARG, "replica_set_ulong: bad argument");
+  RepCore& R = H->dev.core;
+  if (sa < 0 || sb < 0 || sa >= R.ns || sb >= R.ns) return fail(PIMC_ERR_ARG, "replica_set_ulong: bad species");
+  const double delta = (grid_end - grid_start) / (double)(num_points - 1);
+  if (R.ulong_n != 0 && (R.ulong_n != num_points || R.ulong_levels != num_levels || R.ulong_start != grid_start ||
+                         R.ulong_delta != delta))
+    return fail(PIMC_ERR_ARG, "replica_set_ulong: every pair of a replica needs the same grid and levels");
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  const size_t per_pair = (size_t)num_levels * num_points * 6;
+  if (R.ulong_n == 0) {
+    int rc = H->ulong.ensure((size_t)R.ns * R.ns * per_pair * sizeof(double));
+    if (rc) return rc;
+    HIP_TRY(hipMemset(H->ulong.p, 0, (size_t)R.ns * R.ns * per_pair * sizeof(double)));
+    R.ulong_n = num_points;
+    R.ulong_levels = num_levels;
+    R.ulong_start = grid_start;
+    R.ulong_delta = delta;
+    R.ulong_deltainv = 1.0 / delta;
+    R.ulong = H->ulong.as<double>();
+  }
+  for (int pr : {sa * R.ns + sb, sb * R.ns + sa}) {
+    HIP_TRY(hipMemcpy(H->ulong.as<double>() + pr * per_pair, coefs, per_pair * sizeof(double), hipMemcpyHostToDevice));
+    R.ulong_mask |= 1u << pr;
+  }
+  ctx->reps_dirty = true;
+  return PIMC_OK;
+}
+
 int pimc_fit_eval_u(pimc_ctx* ctx, int32_t fit_id, int32_t level, int64_t n, const double* q, const double* z,
                     const double* s2, int32_t precision, double* out) {
   if (!ctx || fit_id < 0 || fit_id >= (int)ctx->fits.size()) return fail(PIMC_ERR_ARG, "fit_eval_u: bad fit");
@@ -410,6 +541,7 @@ int pimc_fit_eval_u(pimc_ctx* ctx, int32_t fit_id, int32_t level, int64_t n, con
   const FitDev& F = ctx->fits[fit_id]->dev;
   if (level < 0 || level + F.taupos >= F.num_tau) return fail(PIMC_ERR_ARG, "fit_eval_u: level out of range");
   if (n == 0) return PIMC_OK;
+  const int nt_launch = std::max(0, ctx->fits[fit_id]->num_terms);
   HIP_TRY(hipSetDevice(ctx->device));
   int rc = ctx->sync_descriptors();
   if (rc) return rc;
@@ -422,7 +554,7 @@ int pimc_fit_eval_u(pimc_ctx* ctx, int32_t fit_id, int32_t level, int64_t n, con
   HIP_TRY(hipMemcpyAsync(ctx->d_in2.p, s2, b, hipMemcpyHostToDevice, ctx->stream));
   LaunchFitEval a{ctx->d_fits.as<FitDev>(), fit_id, level, n, ctx->d_in0.as<double>(), ctx->d_in1.as<double>(),
                   ctx->d_in2.as<double>(), ctx->d_out.as<double>()};
-  launch_fit_eval(F.num_terms, precision == PIMC_MIXED, a, ctx->stream);
+  launch_fit_eval(nt_launch, precision == PIMC_MIXED, a, ctx->stream);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(out, ctx->d_out.p, b, hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -467,6 +599,7 @@ int pimc_replica_create(pimc_ctx* ctx, const pimc_replica_desc* d, int32_t* repl
   H->M = d->num_time_slices;
   R.nk = 0;
   H->use_long_range = d->use_long_range != 0;
+  R.use_long_range = H->use_long_range ? 1 : 0;
   R.use_background = d->use_background != 0;
   R.lr_on = 0;
   R.lr_mask = 0;
@@ -705,13 +838,19 @@ int pimc_action_short_range(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t 
     maxN = std::max(maxN, R.core.N);
     for (int f = 0; f < R.core.ns * R.core.ns; f++) {
       const HostFit& F = *ctx->fits[R.sp.fit[f]];
-      if (nt < 0) nt = F.num_terms;
-      if (F.num_terms != nt)
-        return fail(PIMC_ERR_UNSUPPORTED, "action_short_range: fits with different num_terms in one call");
-      if (reqs[i].level + F.dev.taupos >= F.dev.num_tau)
+      if (F.num_terms >= 0) {
+        if (nt < 0) nt = F.num_terms;
+        if (F.num_terms != nt)
+          return fail(PIMC_ERR_UNSUPPORTED, "action_short_range: fits with different num_terms in one call");
+      }
+      if (reqs[i].level >= F.num_levels())
         return fail(PIMC_ERR_ARG, "action_short_range: level beyond the fit's tau levels");
+      if (F.dev.kind == 1 && R.core.use_long_range &&
+          (!((R.core.ulong_mask >> f) & 1u) || reqs[i].level >= R.core.ulong_levels))
+        return fail(PIMC_ERR_STATE, "action_short_range: long-range pair fit without Ulong tables");
     }
   }
+  if (nt < 0) nt = 0;
   const int ntiles = (maxN + kBlock - 1) / kBlock;
   if ((rc = ctx->d_partials.ensure((size_t)nreq * ntiles * sizeof(double))) ||
       (rc = ctx->d_out.ensure((size_t)nreq * sizeof(double))))
@@ -825,13 +964,17 @@ int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, con
       return fail(PIMC_ERR_ARG, "bisection_run: bad species");
     for (int f = 0; f < R.ns * R.ns; f++) {
       const HostFit& F = *ctx->fits[H->dev.sp.fit[f]];
-      if (nt < 0) nt = F.num_terms;
-      if (F.num_terms != nt) return fail(PIMC_ERR_UNSUPPORTED, "bisection_run: fits with different num_terms");
-      if (F.dev.taupos + desc->num_levels > F.dev.num_tau)
-        return fail(PIMC_ERR_ARG, "bisection_run: pair fit has too few tau levels");
-      table_bytes = std::max(table_bytes, precision == PIMC_MIXED ? F.staged32 : F.staged64);
+      if (F.num_terms >= 0) {
+        if (nt < 0) nt = F.num_terms;
+        if (F.num_terms != nt) return fail(PIMC_ERR_UNSUPPORTED, "bisection_run: fits with different num_terms");
+        table_bytes = std::max(table_bytes, precision == PIMC_MIXED ? F.staged32 : F.staged64);
+      }
+      if (desc->num_levels > F.num_levels()) return fail(PIMC_ERR_ARG, "bisection_run: pair fit has too few tau levels");
+      if (F.dev.kind == 1 && R.use_long_range && (!((R.ulong_mask >> f) & 1u) || desc->num_levels > R.ulong_levels))
+        return fail(PIMC_ERR_STATE, "bisection_run: long-range pair fit without Ulong tables");
     }
   }
+  if (nt < 0) nt = 0;
   if (stats) memset(stats, 0, sizeof *stats);
   if (nrep == 0 || num_moves == 0) return PIMC_OK;
   HIP_TRY(hipSetDevice(ctx->device));

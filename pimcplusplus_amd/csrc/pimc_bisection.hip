@@ -19,6 +19,54 @@
 
 namespace pimc {
 
+// ShortRange partner loop of one stage (ShortRangeClass.cc:149-169) for partners
+// [j0, j1): the partner's beads walk the links in order, the bead one link ahead
+// loaded while the current link is evaluated.
+template <int NT, bool KAHAN, typename Table>
+__device__ inline void sr_partners(const RepCore& R, const FitDev& F, const Table& T, const double* ul, int j0,
+                                   int j1, int ptcl, int slice1, int K, int step, const PIMC_LDS double* mold,
+                                   const PIMC_LDS double* mnew, Kahan& acc_old, Kahan& acc_new) {
+  const size_t row = (size_t)3 * R.N * step;     // distance between consecutive link slices
+  for (int j = j0 + (int)threadIdx.x; j < j1; j += blockDim.x) {
+    if (j == ptcl) continue;
+    const double* pj = R.path0 + (size_t)slice1 * 3 * R.N + j;
+    double pa[3] = {pj[0], pj[R.N], pj[2 * R.N]};
+    pj += row;
+    double pb[3] = {pj[0], pj[R.N], pj[2 * R.N]};
+    for (int k = 0; k < K; k++) {
+      double pn[3] = {0.0, 0.0, 0.0};
+      if (k + 1 < K) {
+        pj += row;
+        pn[0] = pj[0];
+        pn[1] = pj[R.N];
+        pn[2] = pj[2 * R.N];
+      }
+      double moa[3], mob[3], mna[3], mnb[3];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        moa[d] = mold[3 * k * step + d];
+        mob[d] = mold[3 * (k + 1) * step + d];
+        mna[d] = mnew[3 * k * step + d];
+        mnb[d] = mnew[3 * (k + 1) * step + d];
+      }
+      const double uo = pair_link_u<NT>(R, F, T, moa, mob, pa, pb, ul);
+      const double un = pair_link_u<NT>(R, F, T, mna, mnb, pa, pb, ul);
+      if (KAHAN) {
+        acc_old.add(uo);
+        acc_new.add(un);
+      } else {
+        acc_old.s += uo;
+        acc_new.s += un;
+      }
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        pa[d] = pb[d];
+        pb[d] = pn[d];
+      }
+    }
+  }
+}
+
 template <int NT, bool MIXED>
 __global__ void __launch_bounds__(kBlock) k_bisection(const LaunchBisection a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -124,65 +172,22 @@ __global__ void __launch_bounds__(kBlock) k_bisection(const LaunchBisection a) {
       Kahan acc_old, acc_new;
       for (int s2 = 0; s2 < R.ns; s2++) {
         const FitDev F = load_uniform(&a.fits[S.fit[sp * R.ns + s2]]);
+        const double* ul = ulong_for(R, F, sp * R.ns + s2, level);
+        const int j0 = S.first[s2], j1 = S.first[s2] + S.count[s2];
+        if (F.kind == 1) {
+          const BcView T{qgrid_of(F), level};
+          sr_partners<NT, false>(R, F, T, ul, j0, j1, ptcl, slice1, K, step, mold, mnew, acc_old, acc_new);
+          continue;
+        }
         __syncthreads();
-#ifdef PIMC_VARIANT_GLOBAL_TABLES
-        auto T = [&] {
-          if constexpr (MIXED)
-            return level_table32(F, level + F.taupos);
-          else
-            return level_table64(F, level + F.taupos);
-        }();
-#else
         auto T = [&] {
           if constexpr (MIXED)
             return stage_level32(F, level + F.taupos, tbl);
           else
             return stage_level64(F, level + F.taupos, tbl);
         }();
-#endif
         __syncthreads();
-        const int j0 = S.first[s2], j1 = S.first[s2] + S.count[s2];
-        const size_t row = (size_t)3 * R.N * step;     // distance between consecutive link slices
-        for (int j = j0 + tid; j < j1; j += blockDim.x) {
-          if (j == ptcl) continue;
-          // partner beads walk the links in order; the bead one link ahead is
-          // loaded while the current link is evaluated
-          const double* pj = R.path0 + (size_t)slice1 * 3 * R.N + j;
-          double pa[3] = {pj[0], pj[R.N], pj[2 * R.N]};
-          pj += row;
-          double pb[3] = {pj[0], pj[R.N], pj[2 * R.N]};
-          for (int k = 0; k < K; k++) {
-            double pn[3] = {0.0, 0.0, 0.0};
-            if (k + 1 < K) {
-              pj += row;
-              pn[0] = pj[0];
-              pn[1] = pj[R.N];
-              pn[2] = pj[2 * R.N];
-            }
-            double moa[3], mob[3], mna[3], mnb[3];
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-              moa[d] = mold[3 * k * step + d];
-              mob[d] = mold[3 * (k + 1) * step + d];
-              mna[d] = mnew[3 * k * step + d];
-              mnb[d] = mnew[3 * (k + 1) * step + d];
-            }
-            const double uo = pair_link_u<NT>(R, F, T, moa, mob, pa, pb);
-            const double un = pair_link_u<NT>(R, F, T, mna, mnb, pa, pb);
-            if (MIXED) {
-              acc_old.add(uo);
-              acc_new.add(un);
-            } else {
-              acc_old.s += uo;
-              acc_new.s += un;
-            }
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-              pa[d] = pb[d];
-              pb[d] = pn[d];
-            }
-          }
-        }
+        sr_partners<NT, MIXED>(R, F, T, ul, j0, j1, ptcl, slice1, K, step, mold, mnew, acc_old, acc_new);
       }
       const double sr_old = block_sum(acc_old.value(), scratch);
       const double sr_new = block_sum(acc_new.value(), scratch);

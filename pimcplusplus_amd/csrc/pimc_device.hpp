@@ -40,7 +40,17 @@ struct Ptr<true> {
 // Device descriptors
 // --------------------------------------------------------------------------
 
-// One DavidPAClass fit.  Per tau level and grid interval i the spline columns
+// A 1-D grid as the spline lookups see it (LinearGrid / the .dm reader's LogGrid).
+struct GridDev {
+  int type;           // 0 linear, 1 log
+  int n;
+  double start, end;
+  double deltainv;    // linear
+  double r0, logspacing;
+  const double* x;    // [n] points
+};
+
+// One DavidPAClass fit (kind 0) or PAcoulombBCFitClass fit (kind 1).  Per tau level and grid interval i the spline columns
 // used by U are stored as cubic polynomials in the local coordinate
 // b = (x - X_i) * hinv_i:  value = c0 + b(c1 + b(c2 + b c3)).  Column 1 (u00)
 // comes first, then the off-diagonal columns 2..num_cols-1.
@@ -65,6 +75,13 @@ struct FitDev {
   // (sum over |c0..c3| of a column above kMixedLimit); the mixed path evaluates
   // those intervals from c64.
   const unsigned char* big;
+  // kind 1: bicubic U(q, t) per level; the q grid is the fields above
+  int kind;
+  int nt;
+  GridDev tgrid;
+  const d4v* bc;          // [num_tau][ng][nt] BCpoint {z, dzdx, dzdy, d2zdxdy}
+  double smallest_beta;
+  double z1z2;
 };
 
 // Largest |c0|+|c1|+|c2|+|c3| of a column the mixed path evaluates in FP32.
@@ -98,6 +115,12 @@ struct RepCore {
   const double* lr_const;  // [ns*ns][3][lr_levels]: Ulong_r0, Ushort_k0, Ulong_k0
   unsigned lr_mask;        // bit sa*ns+sb set where pair (sa, sb) has tables
   int lr_terms;            // homologous species + heterologous pairs with tables
+  // Ulong(level)(r) quintic splines (one LinearGrid for every pair) for long-ranged fits
+  int use_long_range;      // ActionsClass UseLongRange
+  int ulong_n, ulong_levels;
+  unsigned ulong_mask;     // bit sa*ns+sb set where the pair has Ulong tables
+  double ulong_start, ulong_delta, ulong_deltainv;
+  const double* ulong;     // [ns*ns][ulong_levels][ulong_n][6]
   __device__ __host__ inline double* path(int mode) const { return mode ? path1 : path0; }
   __device__ __host__ inline int* perm(int mode) const { return mode ? perm1 : perm0; }
   __device__ __host__ inline d2v* rho(int mode) const { return mode ? rho1 : rho0; }
@@ -239,6 +262,85 @@ __device__ inline int spline_low(const FitDev& F, XPtr xs, double x) {
   if (hi < 1) hi = 1;
   if (hi > F.ng - 1) hi = F.ng - 1;
   return hi - 1;
+}
+
+// The same lookup on a GridDev (BicubicSpline: ix = ReverseMap clamped to [0, n-2]).
+__device__ inline int grid_low(const GridDev& G, double x) {
+#pragma clang fp contract(off)
+  int i;
+  if (G.type == 0) {
+    i = clamp_to_int(rint((x - G.start) * G.deltainv - 0.5), G.n);
+  } else {
+    i = clamp_to_int(floor(log(x / G.r0) / G.logspacing), G.n);
+    if (i >= 1 && i <= G.n - 2 && x < G.x[i])
+      i -= 1;
+    else if (i >= -1 && i <= G.n - 3 && x >= G.x[i + 1])
+      i += 1;
+  }
+  if (i < 0) i = 0;
+  if (i > G.n - 2) i = G.n - 2;
+  return i;
+}
+
+// --------------------------------------------------------------------------
+// PAcoulombBCFitClass::U (PAcoulombBCFit.cc:159-176) with BicubicSpline
+// evaluation (BicubicSpline.h:266-300, 356-413); the table is read from global memory.
+// --------------------------------------------------------------------------
+__device__ inline double bicubic(const FitDev& F, const GridDev& Q, const d4v* __restrict__ tab, double x,
+                                 double y) {
+  const int ix = grid_low(Q, x);
+  const int iy = grid_low(F.tgrid, y);
+  const double h = Q.x[ix + 1] - Q.x[ix];
+  const double k = F.tgrid.x[iy + 1] - F.tgrid.x[iy];
+  const double u = (x - Q.x[ix]) / h;
+  const double v = (y - F.tgrid.x[iy]) / k;
+  const double a0 = (u - 1.0) * (u - 1.0) * (1.0 + 2.0 * u), a1 = u * u * (3.0 - 2.0 * u);
+  const double a2 = h * (u * (u - 1.0) * (u - 1.0)), a3 = h * (u * u * (u - 1.0));
+  const double b0 = (v - 1.0) * (v - 1.0) * (1.0 + 2.0 * v), b1 = v * v * (3.0 - 2.0 * v);
+  const double b2 = k * (v * (v - 1.0) * (v - 1.0)), b3 = k * (v * v * (v - 1.0));
+  const d4v A = tab[(size_t)ix * F.nt + iy], B = tab[(size_t)ix * F.nt + iy + 1];
+  const d4v C = tab[(size_t)(ix + 1) * F.nt + iy], D = tab[(size_t)(ix + 1) * F.nt + iy + 1];
+  // BCpoint = {z, dzdx, dzdy, d2zdxdy}
+  return a0 * (A.x * b0 + B.x * b1 + A.z * b2 + B.z * b3) + a1 * (C.x * b0 + D.x * b1 + C.z * b2 + D.z * b3) +
+         a2 * (A.y * b0 + B.y * b1 + A.w * b2 + B.w * b3) + a3 * (C.y * b0 + D.y * b1 + C.w * b2 + D.w * b3);
+}
+
+__device__ inline double pair_u_bc(const FitDev& F, const GridDev& Q, int level, double q, double s2) {
+  const d4v* tab = F.bc + (size_t)level * F.ng * F.nt;
+  if (q <= (F.end * 1.0000001)) {
+    if (q == 0) return bicubic(F, Q, tab, 0.0, 0.0);
+    const double t = sqrt(s2) / (2.0 * q);
+    return bicubic(F, Q, tab, q, t);
+  }
+  double beta = F.smallest_beta;
+  for (int i = 0; i < level; i++) beta *= 2.0;
+  return beta * (F.z1z2 / q);
+}
+
+// QuinticSpline::operator() (QuinticSpline.h:172-218) of the replica's Ulong table.
+__device__ inline double ulong_eval(const RepCore& R, const double* __restrict__ tab, double x) {
+  int J;
+  {
+#pragma clang fp contract(off)
+    J = clamp_to_int(rint((x - R.ulong_start) * R.ulong_deltainv - 0.5), R.ulong_n) + 1;
+  }
+  int I = J - 1;
+  if (I < 0) {
+    I = 0;
+    J = 1;
+  }
+  if (J > (R.ulong_n - 1)) {
+    J = R.ulong_n - 1;
+    I = J - 1;
+  }
+  double X;
+  {
+#pragma clang fp contract(off)
+    X = R.ulong_start + (double)I * R.ulong_delta;
+  }
+  const double P = (x - X);
+  const double* k = tab + (size_t)I * 6;
+  return ((((k[5] * P + k[4]) * P + k[3]) * P + k[2]) * P + k[1]) * P + k[0];
 }
 
 // --------------------------------------------------------------------------

@@ -91,6 +91,28 @@ __device__ inline double pair_u(const FitDev& F, const LevelTable32<InLds>& T, d
   return pair_u_mixed<NT>(F, T, q, z, s2);
 }
 
+// A level of a PAcoulombBCFitClass fit (table in global memory).
+struct BcView {
+  GridDev q;
+  int level;
+};
+__device__ inline GridDev qgrid_of(const FitDev& F) {
+  GridDev g;
+  g.type = F.grid_type;
+  g.n = F.ng;
+  g.start = F.start;
+  g.end = F.end;
+  g.deltainv = F.deltainv;
+  g.r0 = F.r0;
+  g.logspacing = F.logspacing;
+  g.x = F.x;
+  return g;
+}
+template <int NT>
+__device__ inline double pair_u(const FitDev& F, const BcView& T, double q, double z, double s2) {
+  return pair_u_bc(F, T.q, T.level, q, s2);
+}
+
 template <typename Table>
 struct IsMixed {
   static constexpr bool value = false;
@@ -163,7 +185,8 @@ __device__ inline LevelTable32<true> stage_level32(const FitDev& F, int lev, PIM
 // --------------------------------------------------------------------------
 template <int NT, typename Table>
 __device__ inline double pair_link_u(const RepCore& R, const FitDev& F, const Table& T, const double ma[3],
-                                     const double mb[3], const double pa[3], const double pb[3]) {
+                                     const double mb[3], const double pa[3], const double pb[3],
+                                     const double* __restrict__ ulong_tab = nullptr) {
   constexpr bool kMixed = IsMixed<Table>::value;
   double da[3], db[3];
 #pragma unroll
@@ -179,7 +202,16 @@ __device__ inline double pair_link_u(const RepCore& R, const FitDev& F, const Ta
   double s2 = e0 * e0 + e1 * e1 + e2 * e2;
   double q = 0.5 * (ra + rb);
   double z = ra - rb;
-  return pair_u<NT>(F, T, q, z, s2);
+  double U = pair_u<NT>(F, T, q, z, s2);
+  // long-ranged fits: subtract the part the k-space action carries (ShortRangeClass.cc:165-166)
+  if (ulong_tab) U -= 0.5 * (ulong_eval(R, ulong_tab, ra) + ulong_eval(R, ulong_tab, rb));
+  return U;
+}
+
+// Ulong table of pair `pair` at `level` when ShortRange must subtract it, else nullptr.
+__device__ inline const double* ulong_for(const RepCore& R, const FitDev& F, int pair, int level) {
+  if (F.kind != 1 || !R.use_long_range) return nullptr;
+  return R.ulong + ((size_t)pair * R.ulong_levels + level) * R.ulong_n * 6;
 }
 
 __device__ inline void load_bead(const RepCore& R, int mode, int slice, int p, double out[3]) {

@@ -39,3 +39,21 @@ def kset_size(n: int) -> int:
     r = np.arange(-n, n + 1)
     m2 = r[:, None, None] ** 2 + r[None, :, None] ** 2 + r[None, None, :] ** 2
     return int(((m2 > 0) & (m2 < n * n)).sum() // 2)
+
+
+def ulong_quintic(z1z2: float, alpha: float, tau: float, num_levels: int, r_end: float, n: int = 1000):
+    """Ulong(level)(r) = tau_l Z1Z2 erf(alpha r)/r as the reference's QuinticSpline data on
+    LinearGrid(0, r_end, n): coefficients [levels][n][6] = {Y, B, C, D, E, F}, the Taylor
+    coefficients f^(m)(x_i)/m! of the natural quintic interpolant (third and fourth
+    derivatives zero at both ends) at every knot."""
+    from scipy.interpolate import make_interp_spline
+    from scipy.special import erf
+    x = 0.0 + np.arange(n) * ((r_end - 0.0) / float(n - 1))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        f = np.where(x > 0, erf(alpha * x) / np.where(x > 0, x, 1.0), 2.0 * alpha / math.sqrt(math.pi))
+    sp = make_interp_spline(x, f, k=5, bc_type=([(3, 0.0), (4, 0.0)], [(3, 0.0), (4, 0.0)]))
+    base = np.stack([sp(x, m) / math.factorial(m) for m in range(6)], axis=-1)   # [n][6]
+    out = np.empty((num_levels, n, 6))
+    for lev in range(num_levels):
+        out[lev] = tau * 2 ** lev * z1z2 * base
+    return 0.0, float(x[-1]), out

@@ -148,6 +148,14 @@ def read_dm(path: str) -> DavidTable:
 # --------------------------------------------------------------------------
 # Lennard-Jones generator
 # --------------------------------------------------------------------------
+def load_npz(path: str) -> DavidTable:
+    """A DavidTable saved with numpy.savez (fields y, dy, grid_type, grid_start, grid_end,
+    smallest_tau, num_terms)."""
+    d = np.load(path, allow_pickle=False)
+    return DavidTable(int(d["grid_type"]), float(d["grid_start"]), float(d["grid_end"]), d["y"],
+                      int(d["num_terms"]), float(d["smallest_tau"]), d["dy"])
+
+
 def lj_potential(r, eps: float, sigma: float):
     sr6 = (sigma / np.asarray(r, dtype=np.float64)) ** 6
     return 4.0 * eps * (sr6 * sr6 - sr6)
@@ -221,3 +229,83 @@ def lj_david_table(eps: float, sigma: float, r_min: float, r_max: float, num_poi
     dy[:, num_points - 1, :] = 0.0
     return DavidTable(grid_type=GRID_LINEAR, grid_start=float(r_min), grid_end=float(r_max), y=y,
                       num_terms=num_terms, smallest_tau=smallest_tau, dy=dy)
+
+
+# --------------------------------------------------------------------------
+# Coulomb tables in the PAcoulombBCFitClass layout: U over (q, t = s / 2q)
+# --------------------------------------------------------------------------
+@dataclass
+class CoulombBCTable:
+    q_grid: tuple              # (grid_type, start, end)
+    t_grid: tuple
+    umat: np.ndarray           # [num_betas][nq][nt]
+    smallest_beta: float
+    z1z2: float
+
+    def grid_points(self, which: str) -> np.ndarray:
+        gtype, start, end = self.q_grid if which == "q" else self.t_grid
+        n = self.umat.shape[1] if which == "q" else self.umat.shape[2]
+        return _grid(gtype, start, end, n)
+
+
+def _grid(gtype: int, start: float, end: float, n: int) -> np.ndarray:
+    if gtype == GRID_LINEAR:
+        delta = (end - start) / float(n - 1)
+        return start + np.arange(n, dtype=np.float64) * delta
+    spacing = (end / start) ** (1.0 / (n - 1.0))
+    return start * spacing ** np.arange(n, dtype=np.float64)
+
+
+def david_u(table: DavidTable, level_index: int, q, z, s2):
+    """DavidPAClass::U of a DavidTable level (natural start, zero end slope), evaluated
+    with scipy splines; used to convert squarer tables."""
+    from scipy.interpolate import CubicSpline
+    x = table.grid_points()
+    y = table.y[level_index]
+    sp = [CubicSpline(x, y[:, c], bc_type=((2, 0.0), (1, 0.0))) for c in range(y.shape[1])]
+    q, z, s2 = (np.asarray(v, dtype=np.float64) for v in (q, z, s2))
+    rmin, end = x[0], x[-1]
+    r = np.minimum(q + 0.5 * z, end)
+    rp = np.minimum(q - 0.5 * z, end)
+    out = 0.5 * (sp[1](np.maximum(r, rmin)) + sp[1](np.maximum(rp, rmin)))
+    low = (r < rmin) | (rp < rmin)
+    out = np.where(low, sp[1](rmin), out)
+    use = (~low) & (s2 > 0) & (q < end)
+    z2 = z * z
+    for k in range(1, table.num_terms + 1):
+        for j in range(0, k + 1):
+            c = k * (k + 1) // 2 + j + 1
+            out = out + np.where(use, sp[c](np.minimum(q, end)) * z2 ** j * s2 ** (k - j), 0.0)
+    return out
+
+
+def coulomb_bc_from_david(table: DavidTable, tau: float, num_betas: int, z1z2: float,
+                          q_grid=(GRID_LOG, 0.0025, 9.5), nq: int = 200,
+                          t_grid=(GRID_LINEAR, 0.0, 1.0), nt: int = 20) -> CoulombBCTable:
+    """A z-independent (q, t) table from a squarer DavidPA table: U(q, t) = U_David(q, 0, (2 q t)^2)
+    at tau_l = tau 2^l, l < num_betas."""
+    q = _grid(*q_grid, nq)
+    t = _grid(*t_grid, nt)
+    taus = table.smallest_tau * 2.0 ** np.arange(table.num_tau)
+    umat = np.zeros((num_betas, nq, nt))
+    Q, T = np.meshgrid(q, t, indexing="ij")
+    for b in range(num_betas):
+        lev = int(np.argmin(np.abs(taus - tau * 2 ** b)))
+        if abs(taus[lev] - tau * 2 ** b) > 1e-4:
+            raise ValueError(f"table has no tau level {tau * 2 ** b}")
+        umat[b] = david_u(table, lev, Q.ravel(), np.zeros(Q.size), (2 * Q * T).ravel() ** 2).reshape(nq, nt)
+    return CoulombBCTable(q_grid, t_grid, umat, tau, z1z2)
+
+
+def coulomb_chord_table(z1z2: float, tau: float, num_betas: int, q_grid=(GRID_LOG, 0.01, 10.0), nq: int = 200,
+                        t_grid=(GRID_LINEAR, 0.0, 0.95), nt: int = 20) -> CoulombBCTable:
+    """Straight-line (semiclassical) action of Z1Z2/r in the (q, t) layout: with r = r' = q and
+    s = 2 q t the chord distance is q sqrt(1 - 4 u(1-u) t^2), whose inverse averages to
+    atanh(t) / (q t); U = tau_l Z1Z2 atanh(t) / (q t)."""
+    q = _grid(*q_grid, nq)
+    t = _grid(*t_grid, nt)
+    g = np.where(t > 0, np.arctanh(np.minimum(t, 0.999999)) / np.where(t > 0, t, 1.0), 1.0)
+    umat = np.zeros((num_betas, nq, nt))
+    for b in range(num_betas):
+        umat[b] = tau * 2 ** b * z1z2 * g[None, :] / q[:, None]
+    return CoulombBCTable(q_grid, t_grid, umat, tau, z1z2)

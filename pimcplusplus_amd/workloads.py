@@ -13,8 +13,11 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .ewald import ewald_tables
-from .tables import DavidTable, lj_david_table
+import math
+
+from .ewald import ewald_tables, ulong_quintic
+from .tables import (GRID_LINEAR, GRID_LOG, CoulombBCTable, DavidTable, coulomb_bc_from_david,
+                     coulomb_chord_table, lj_david_table)
 
 LAMBDA_AMU = 24.2542          # hbar^2 / (2 * 1 amu * k_B) in K A^2
 LJ_EPS = 10.22                # K
@@ -39,13 +42,16 @@ class ReplicaSpec:
     use_background: bool = False
     # (sa, sb) -> (z1z2, alpha): closed-form Ewald long-range tables on the replica's k set
     ewald: dict = field(default_factory=dict)
+    # (sa, sb) -> (z1z2, alpha): Ulong(r) = tau_l z1z2 erf(alpha r)/r subtracted from long-ranged
+    # pair fits, a quintic on LinearGrid(0, 0.75 |box|, 1000)
+    ulong: dict = field(default_factory=dict)
 
 
 @dataclass
 class System:
     name: str
-    tables: dict               # key -> DavidTable
-    desired_tau: dict          # key -> tau the fit is selected at
+    tables: dict               # key -> DavidTable or CoulombBCTable
+    desired_tau: dict          # key -> tau the fit is selected at (DavidTable keys)
     replicas: list             # [ReplicaSpec]
     num_levels: int
     seed: int
@@ -134,8 +140,11 @@ def load(lib, system: System):
     """Creates the system's fits and replicas in `lib` (a PimcLib).  Returns replica ids."""
     fit_ids = {}
     for key, t in system.tables.items():
-        fit_ids[key] = lib.fit_create_david(t.grid_type, t.grid_start, t.grid_end, t.y, t.num_terms,
-                                            t.smallest_tau, system.desired_tau[key])
+        if isinstance(t, CoulombBCTable):
+            fit_ids[key] = lib.fit_create_coulomb_bc(t.q_grid, t.t_grid, t.umat, t.smallest_beta, t.z1z2)
+        else:
+            fit_ids[key] = lib.fit_create_david(t.grid_type, t.grid_start, t.grid_end, t.y, t.num_terms,
+                                                t.smallest_tau, system.desired_tau[key])
     ids = []
     for spec in system.replicas:
         rid = lib.replica_create(spec.num_particles, spec.num_time_slices, spec.box, spec.tau,
@@ -144,6 +153,11 @@ def load(lib, system: System):
                                  k_cutoff=spec.k_cutoff, use_long_range=spec.use_long_range,
                                  use_background=spec.use_background, rng_stream=spec.rng_stream)
         lib.path_set(rid, spec.beads)
+        if spec.ulong:
+            r_end = 0.75 * math.sqrt(sum(b * b for b in spec.box))
+            for (sa, sb), (z1z2, alpha) in spec.ulong.items():
+                start, end, coefs = ulong_quintic(z1z2, alpha, spec.tau, system.num_levels, r_end)
+                lib.set_ulong(rid, sa, sb, start, end, coefs)
         if spec.ewald:
             kv, _ = lib.kvecs(rid)
             for (sa, sb), (z1z2, alpha) in spec.ewald.items():
@@ -177,3 +191,92 @@ def charged_lj_system(n_plus: int = 24, n_minus: int = 24, m: int = 32, tau: flo
                                        (0, 1): (-charge_scale, alpha)}))
     return System(name="charged-lj", tables={"lj": table}, desired_tau={"lj": tau}, replicas=reps,
                   num_levels=num_levels, seed=seed)
+
+
+# --------------------------------------------------------------------------
+# Coulomb systems (Hartree atomic units)
+# --------------------------------------------------------------------------
+LAMBDA_E = 0.5                 # hbar^2 / 2 m_e
+PROTON_MASS = 1836.15267
+
+
+def coulomb_box(n: int, rs: float) -> float:
+    return (4.0 * math.pi * n / 3.0) ** (1.0 / 3.0) * rs
+
+
+def kcut(units: float, box: float) -> float:
+    """k cutoff units * 2 pi / L, placed just inside the shell |n|^2 = units^2 so the
+    strict |k|^2 < kc^2 test of the k set is not decided by rounding."""
+    return (units - 1e-9) * 2.0 * math.pi / box
+
+
+def _coulomb_pairs(z: list, alpha: float) -> dict:
+    ns = len(z)
+    return {(a, b): (z[a] * z[b], alpha) for a in range(ns) for b in range(a, ns)}
+
+
+def hydrogen_system(dm_table: DavidTable, num_replicas: int = 1, n_pairs: int = 32, rs: float = 1.75,
+                    m: int = 128, tau: float = 0.125, num_levels: int = 4, kc_units: float = 5.0,
+                    seed: int = 1002) -> System:
+    """Hydrogen plasma: n_pairs electrons (species 0, moved) and n_pairs protons (species 1).
+    e-p: the squarer table `dm_table` (testcoul.dm, Z1Z2 = -1) resampled on the
+    PAcoulombBCFitClass (q, t) layout; e-e and p-p: straight-line Coulomb tables.  Every pair
+    is long-ranged: Ulong = tau_l Z1Z2 erf(alpha r)/r is subtracted in the short-range action
+    and the Ewald k-space action is added at the lowest bisection level (alpha = kc / 6)."""
+    n = 2 * n_pairs
+    box = coulomb_box(n_pairs, rs)
+    kc = kcut(kc_units, box)
+    alpha = kc / 6.0
+    q_end = 0.75 * math.sqrt(3.0) * box
+    tabs = {
+        "ee": coulomb_chord_table(1.0, tau, num_levels, q_grid=(GRID_LOG, 0.005, q_end)),
+        "pp": coulomb_chord_table(1.0, tau, num_levels, q_grid=(GRID_LOG, 0.005, q_end)),
+        "ep": coulomb_bc_from_david(dm_table, tau, num_levels, -1.0,
+                                    q_grid=(GRID_LOG, 0.0025, min(q_end, dm_table.grid_end * 0.95))),
+    }
+    rng = np.random.default_rng(seed)
+    reps = []
+    for r in range(num_replicas):
+        beads = random_start(n, box, m, rng, jitter=0.05 * box)
+        reps.append(ReplicaSpec(num_particles=n, num_time_slices=m, box=(box, box, box), tau=tau,
+                                species_count=[n_pairs, n_pairs],
+                                lambdas=[LAMBDA_E, LAMBDA_E / PROTON_MASS],
+                                fit_keys=["ee", "ep", "ep", "pp"], beads=beads, rng_stream=r,
+                                k_cutoff=kc, use_long_range=True,
+                                ewald=_coulomb_pairs([-1.0, 1.0], alpha),
+                                ulong=_coulomb_pairs([-1.0, 1.0], alpha)))
+    return System(name=f"hydrogen-n{n}-m{m}-l{num_levels}", tables=tabs, desired_tau={}, replicas=reps,
+                  num_levels=num_levels, seed=seed,
+                  meta={"rs": rs, "box": box, "kc": kc, "alpha": alpha, "tau": tau})
+
+
+def config2(dm_table: DavidTable, num_replicas: int = 1) -> System:
+    """32 e + 32 p, rs=1.75 (L = 8.9561 bohr), M=128, tau=0.125 Ha^-1, kc = 5 2pi/L (242 k)."""
+    return hydrogen_system(dm_table, num_replicas=num_replicas)
+
+
+def ocp_system(num_replicas: int = 16, n: int = 1024, rs: float = 1.0, m: int = 128, tau: float = 0.05,
+               num_levels: int = 4, kc_units: float = 10.0, seed: int = 2000) -> System:
+    """One-component plasma with a neutralising background: n electrons, straight-line Coulomb
+    pair table, Ulong subtraction, Ewald k-space action with the background terms."""
+    box = coulomb_box(n, rs)
+    kc = kcut(kc_units, box)
+    alpha = kc / 6.0
+    q_end = 0.75 * math.sqrt(3.0) * box
+    tabs = {"ee": coulomb_chord_table(1.0, tau, num_levels, q_grid=(GRID_LOG, 0.005, q_end))}
+    reps = []
+    for r in range(num_replicas):
+        rng = np.random.default_rng(seed + r)
+        reps.append(ReplicaSpec(num_particles=n, num_time_slices=m, box=(box, box, box), tau=tau,
+                                species_count=[n], lambdas=[LAMBDA_E], fit_keys=["ee"],
+                                beads=random_start(n, box, m, rng, jitter=0.05 * box), rng_stream=r,
+                                k_cutoff=kc, use_long_range=True, use_background=True,
+                                ewald={(0, 0): (1.0, alpha)}, ulong={(0, 0): (1.0, alpha)}))
+    return System(name=f"ocp-n{n}-m{m}-l{num_levels}", tables=tabs, desired_tau={}, replicas=reps,
+                  num_levels=num_levels, seed=seed,
+                  meta={"rs": rs, "box": box, "kc": kc, "alpha": alpha, "tau": tau})
+
+
+def config4(num_replicas: int = 16) -> System:
+    """OCP N=1024, rs=1 (L = 16.2479 bohr), M=128, kc = 10 2pi/L (2069 k), 16 replicas."""
+    return ocp_system(num_replicas=num_replicas)
