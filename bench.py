@@ -49,27 +49,24 @@ def load_system(lib, system):
 
 
 def run_gpu(system, precision, steps, warmup, moves_per_step):
+    """W untimed steps, then K steps queued back to back on the context's stream;
+    the timed region is bracketed by stream synchronisations (the stats take)."""
     lib = native.open_context(0)
     ids = load_system(lib, system)
     L, seed = system.num_levels, system.seed
     move = 0
     for _ in range(warmup):
-        lib.bisection_run(ids, L, seed, move, moves_per_step, precision, records=False)
+        lib.bisection_run(ids, L, seed, move, moves_per_step, precision, records=False, wait=False)
         move += moves_per_step
-    lib.synchronize()
-    tot = abi.BisectionStats()
-    dev_ms = []
+    lib.bisection_stats_take(ids)          # waits for the warmup; clears its counters
     t0 = time.perf_counter()
     for _ in range(steps):
-        _, st = lib.bisection_run(ids, L, seed, move, moves_per_step, precision, records=False)
+        lib.bisection_run(ids, L, seed, move, moves_per_step, precision, records=False, wait=False)
         move += moves_per_step
-        for f in ("moves", "accepted", "stages", "pair_evals", "k_evals"):
-            setattr(tot, f, getattr(tot, f) + getattr(st, f))
-        dev_ms.append(st.device_ms)
-    lib.synchronize()
+    tot = lib.bisection_stats_take(ids)    # waits for the timed steps
     wall = time.perf_counter() - t0
     lib.close()
-    return tot, wall, float(np.sum(dev_ms)), dev_ms
+    return tot, wall, tot.device_ms
 
 
 def delta_s_error(nrep: int = 16, nmoves: int = 40):
@@ -146,16 +143,37 @@ def cpu_baseline(seconds: float = 8.0, threads: int | None = None):
             "single_core_value": single[0][0] / single[0][1]}
 
 
+def calibrate_copy(mib: int):
+    """One elementwise kernel that reads and writes `mib` MiB each (out = in + 1, FP32): the
+    FETCH_SIZE / WRITE_SIZE counters of this dispatch scale the kernels' counter readings."""
+    import torch
+    x = torch.ones(mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        torch.add(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_mib": mib, "dispatches": 3, "check": float(y[-1].item())}))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--replicas", type=int, default=2048)
-    ap.add_argument("--moves-per-step", type=int, default=4)
+    ap.add_argument("--replicas", type=int, default=4096)
+    ap.add_argument("--moves-per-step", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=("mixed", "fp64"), default=None,
+                    help="time one precision only (profiling runs); the JSON line then lacks the other")
+    ap.add_argument("--no-dserr", action="store_true", help="skip the mixed-vs-FP64 dS comparison")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run an elementwise copy of MIB MiB (read MIB, write MIB) for counter scaling")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(replicas would shard across GPUs with no communication; see the design notes)",
@@ -165,8 +183,11 @@ def main():
     system = workloads.config1(num_replicas=args.replicas)
     n = system.replicas[0].num_particles
     res = {}
-    for name, prec in (("mixed", abi.MIXED), ("fp64", abi.FP64)):
-        tot, wall, dev_ms, per = run_gpu(system, prec, args.steps, args.warmup, args.moves_per_step)
+    runs = (("mixed", abi.MIXED), ("fp64", abi.FP64))
+    if args.only:
+        runs = tuple(r for r in runs if r[0] == args.only)
+    for name, prec in runs:
+        tot, wall, dev_ms = run_gpu(system, prec, args.steps, args.warmup, args.moves_per_step)
         res[name] = {
             "evals_per_s": tot.pair_evals / wall,
             "ms_per_step": 1e3 * wall / args.steps,
@@ -177,7 +198,10 @@ def main():
             "stages_per_move": tot.stages / max(1, tot.moves),
             "pair_evals": tot.pair_evals,
         }
-    err, nstages = delta_s_error()
+    if args.only:
+        print(json.dumps({"only": args.only, **res[args.only]}))
+        return
+    err, nstages = (None, 0) if args.no_dserr else delta_s_error()
     mixed = res["mixed"]
     achieved = mixed["bytes_per_launch"] / (mixed["kernel_ms_avg"] * 1e-3) / 1e9
     line = {

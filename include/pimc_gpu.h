@@ -287,11 +287,28 @@ typedef struct pimc_bisection_stats {
  * Each replica's moves are a Metropolis chain in move order; replicas are
  * independent.  records (may be NULL) is [nrep][num_moves].  The random
  * numbers of move m of a replica are a pure function of (seed, rng_stream, m),
- * so a chain is reproducible across calls and backends. */
+ * so a chain is reproducible across calls and backends.
+ * With records and stats both NULL the call is asynchronous: it returns once the
+ * work is queued on the context's stream, and its counters are read with
+ * pimc_bisection_stats_take.  Otherwise it returns when the moves are done.
+ * Replaces the per-move driver BisectionBlockClass::MakeMove
+ * (PIMC++/src/Moves/BisectionBlock.cc:310-451) with NoPermuteStageClass
+ * (PIMC++/src/Moves/NoPermuteClass.cc:26-68) and the stage loop of
+ * MultiStageClass::MakeMove (PIMC++/src/Moves/MultiStage.cc:42-110). */
 int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep,
                        const pimc_bisection_desc* desc, int64_t first_move, int32_t num_moves,
                        int32_t precision, pimc_move_record* records,
                        pimc_bisection_stats* stats);
+
+/* Waits for the context's stream, then stores in *stats the counters every
+ * pimc_bisection_run call since the previous take added for the listed
+ * replicas, and clears those replicas' counters.  device_ms is the compute time
+ * of all pimc_bisection_run calls on the context since the previous take.
+ * (The reference keeps such counters per move object, MoveBaseClass::NumAccepted
+ * and TimesCalled, PIMC++/src/Moves/MoveBase.h:70-77, and the time per action,
+ * ActionBaseClass::TimeSpent, PIMC++/src/Actions/ActionBase.h:37.) */
+int pimc_bisection_stats_take(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep,
+                              pimc_bisection_stats* stats);
 
 #ifdef __cplusplus
 }

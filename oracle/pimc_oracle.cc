@@ -513,6 +513,7 @@ struct LongRangeTables {
 };
 
 struct Replica {
+  pimc_bisection_stats acc{};   // counters accumulated until pimc_bisection_stats_take
   int N = 0, M = 0, nslices = 0, ns = 0, num_images = 0;
   double box[3], boxinv[3], isperiodic[3];
   double tau = 0;
@@ -1013,6 +1014,7 @@ int check_req(Ctx* c, const pimc_action_req& q, Replica** out) {
 // ==========================================================================
 struct pimc_ctx {
   Ctx c;
+  double acc_ms = 0.0;   // time of the bisection calls since the last take
 };
 
 extern "C" {
@@ -1370,13 +1372,45 @@ int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, con
   auto t0 = std::chrono::steady_clock::now();
   for (int r = 0; r < nrep; r++) {
     Replica& R = *get_rep(&ctx->c, replicas[r]);
+    pimc_bisection_stats one;
+    memset(&one, 0, sizeof one);
     for (int m = 0; m < num_moves; m++) {
       pimc_move_record* rec = records ? records + (size_t)r * num_moves + m : nullptr;
-      bisection_move(&ctx->c, R, *desc, (uint32_t)(first_move + m), rec, &st);
+      bisection_move(&ctx->c, R, *desc, (uint32_t)(first_move + m), rec, &one);
     }
+    st.moves += one.moves;
+    st.accepted += one.accepted;
+    st.stages += one.stages;
+    st.pair_evals += one.pair_evals;
+    st.k_evals += one.k_evals;
+    R.acc.moves += one.moves;
+    R.acc.accepted += one.accepted;
+    R.acc.stages += one.stages;
+    R.acc.pair_evals += one.pair_evals;
+    R.acc.k_evals += one.k_evals;
   }
   st.device_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  ctx->acc_ms += st.device_ms;
   if (stats) *stats = st;
+  return PIMC_OK;
+}
+
+int pimc_bisection_stats_take(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, pimc_bisection_stats* stats) {
+  if (!ctx || !stats || nrep < 0 || (nrep > 0 && !replicas)) return fail(PIMC_ERR_ARG, "bisection_stats_take: bad argument");
+  for (int r = 0; r < nrep; r++)
+    if (!get_rep(&ctx->c, replicas[r])) return fail(PIMC_ERR_ARG, "bisection_stats_take: bad replica");
+  memset(stats, 0, sizeof *stats);
+  stats->device_ms = ctx->acc_ms;
+  ctx->acc_ms = 0.0;
+  for (int r = 0; r < nrep; r++) {
+    Replica& R = *get_rep(&ctx->c, replicas[r]);
+    stats->moves += R.acc.moves;
+    stats->accepted += R.acc.accepted;
+    stats->stages += R.acc.stages;
+    stats->pair_evals += R.acc.pair_evals;
+    stats->k_evals += R.acc.k_evals;
+    R.acc = pimc_bisection_stats{};
+  }
   return PIMC_OK;
 }
 

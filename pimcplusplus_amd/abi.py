@@ -161,6 +161,7 @@ SIGNATURES = {
         C.c_int,
         [_CTX, _i32p, C.c_int32, C.POINTER(BisectionDesc), C.c_int64, C.c_int32, C.c_int32,
          C.POINTER(MoveRecord), C.POINTER(BisectionStats)]),
+    "pimc_bisection_stats_take": (C.c_int, [_CTX, _i32p, C.c_int32, C.POINTER(BisectionStats)]),
 }
 
 
@@ -426,16 +427,28 @@ class PimcLib:
     # -- bisection ---------------------------------------------------------
     def bisection_run(self, replicas, num_levels: int, seed: int, first_move: int, num_moves: int,
                       precision: int = FP64, species: int = 0, lowest_level: int = 0,
-                      records: bool = True):
+                      records: bool = True, wait: bool = True):
+        """Runs the moves; returns (records or None, stats).  wait=False (records must be
+        False) queues the moves and returns (None, None); read the counters with
+        bisection_stats_take."""
         reps = _i32(replicas)
         d = BisectionDesc(num_levels=num_levels, lowest_level=lowest_level, species=species,
                           reserved=0, seed=seed)
-        st = BisectionStats()
+        if not wait and records:
+            raise ValueError("records need wait=True")
+        st = BisectionStats() if wait else None
         rec = (MoveRecord * (reps.size * num_moves))() if records and num_moves > 0 else None
         self._check("pimc_bisection_run", self.lib.pimc_bisection_run(
             self.ctx, _p(reps, _i32p), reps.size, C.byref(d), first_move, num_moves, precision,
-            rec, C.byref(st)))
+            rec, C.byref(st) if st is not None else None))
         return (records_to_numpy(rec, reps.size, num_moves) if rec is not None else None), st
+
+    def bisection_stats_take(self, replicas) -> BisectionStats:
+        reps = _i32(replicas)
+        st = BisectionStats()
+        self._check("pimc_bisection_stats_take", self.lib.pimc_bisection_stats_take(
+            self.ctx, _p(reps, _i32p), reps.size, C.byref(st)))
+        return st
 
 
 def records_to_numpy(rec, nrep: int, nmoves: int) -> dict:

@@ -38,13 +38,21 @@ def hipcc() -> str:
     return path
 
 
+# Per-file flags.  The resident bisection kernel keeps its exp/log constants out
+# of loop preheaders (machine LICM would hoist them into registers that stay live
+# across the whole kernel and cap the occupancy).
+FILE_FLAGS = {
+    "pimc_bisection.hip": ["-mllvm", "-disable-machine-licm"],
+}
+
+
 def sources() -> list[str]:
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
 def _compile(src: str) -> str:
     obj = os.path.join(OBJDIR, os.path.basename(src)[:-4] + ".o")
-    cmd = [hipcc(), *HIPCC_FLAGS, "-c", src, "-o", obj]
+    cmd = [hipcc(), *HIPCC_FLAGS, *FILE_FLAGS.get(os.path.basename(src), []), "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{r.stderr}")

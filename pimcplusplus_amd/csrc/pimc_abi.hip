@@ -10,6 +10,7 @@
 #include <memory>
 #include <set>
 #include <string>
+#include <limits>
 #include <vector>
 
 #include "../../include/pimc_gpu.h"
@@ -39,8 +40,15 @@ int fail(int code, const std::string& msg) {
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() {
     if (p) (void)hipFree(p);
+  }
+  void swap(DevBuf& o) {
+    std::swap(p, o.p);
+    std::swap(bytes, o.bytes);
   }
   int ensure(size_t n) {
     if (n <= bytes) return PIMC_OK;
@@ -132,7 +140,8 @@ struct HostFit {
   FitDev dev{};            // pointers refer to `mem`
   DevBuf mem;
   int num_terms = 0;       // -1 for fits without the DavidPA expansion
-  size_t staged64 = 0, staged32 = 0;
+  size_t staged64 = 0, staged32 = 0;          // LDS bytes of one staged level
+  size_t staged64_lin = 0, staged32_lin = 0;  // the same for the linear-grid kernels (coefficients only)
   int num_levels() const { return dev.num_tau - dev.taupos; }
 };
 
@@ -230,14 +239,20 @@ struct pimc_ctx {
   DevBuf d_fits, d_reps;
   bool fits_dirty = true, reps_dirty = true;
   DevBuf d_reqs, d_partials, d_out, d_in0, d_in1, d_in2, d_idx, d_img, d_records, d_stats, d_repids;
+  // bisection counters accumulated per replica id ([acc_cap][5]) until pimc_bisection_stats_take
+  DevBuf d_acc;
+  int acc_cap = 0;
+  std::vector<int> h_repids;                 // replica list of the last upload to d_repids
+  // event pairs around bisection launches since the last take, and a pool of spare ones
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> timed, spare;
 
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;   // brackets the compute of timed calls
   // Host staging of the request upload; kept alive until the call that uses it synchronizes.
   std::vector<ReqDev> h_reqs;
 
   int sync_descriptors() {
     std::vector<FitDev> hf;
     std::vector<ReplicaDev> hr;
+    if (!(fits_dirty && !fits.empty()) && !(reps_dirty && !reps.empty())) return PIMC_OK;
     if (fits_dirty && !fits.empty()) {
       for (auto& f : fits) hf.push_back(f->dev);
       int rc = d_fits.ensure(hf.size() * sizeof(FitDev));
@@ -337,8 +352,11 @@ int pimc_ctx_destroy(pimc_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   hipStream_t s = ctx->stream;
-  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
-  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  for (auto* v : {&ctx->timed, &ctx->spare})
+    for (auto& e : *v) {
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
   delete ctx;
   if (s) (void)hipStreamDestroy(s);
   return PIMC_OK;
@@ -403,11 +421,18 @@ int pimc_fit_create_david(pimc_ctx* ctx, const pimc_david_fit_desc* d, int32_t* 
         if (fabs(cf.x) + fabs(cf.y) + fabs(cf.z) + fabs(cf.w) > kMixedLimit) big[(size_t)lev * nint + i] = 1;
       }
     }
+    // FP32 rows of flagged intervals hold NaN: the mixed kernels read the FP64 row there
+    for (int i = 0; i < nint; i++)
+      if (big[(size_t)lev * nint + i])
+        for (int c = 0; c < ncol; c++) {
+          const float qnan = std::numeric_limits<float>::quiet_NaN();
+          c32[((size_t)lev * nint + i) * ncol + c] = f4v{qnan, qnan, qnan, qnan};
+        }
   }
   auto f = std::make_unique<HostFit>();
   size_t bx = (size_t)ng * 8, bgeo = (size_t)nint * 16, b64 = c64.size() * 32, b32 = c32.size() * 16;
   size_t off_geo = 0, off_x = off_geo + bgeo, off64 = (off_x + bx + 255) & ~(size_t)255,
-         off32 = off64 + b64, offbig = off32 + b32, total = offbig + big.size();
+         off32 = off64 + b64, total = off32 + b32;
   int rc = f->mem.ensure(total);
   if (rc) return rc;
   unsigned char* base = f->mem.as<unsigned char>();
@@ -415,7 +440,6 @@ int pimc_fit_create_david(pimc_ctx* ctx, const pimc_david_fit_desc* d, int32_t* 
   HIP_TRY(hipMemcpy(base + off_x, g.x.data(), bx, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(base + off64, c64.data(), b64, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(base + off32, c32.data(), b32, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(base + offbig, big.data(), big.size(), hipMemcpyHostToDevice));
   FitDev& F = f->dev;
   F.grid_type = g.type;
   F.ng = ng;
@@ -435,10 +459,11 @@ int pimc_fit_create_david(pimc_ctx* ctx, const pimc_david_fit_desc* d, int32_t* 
   F.geo = (const d2v*)(base + off_geo);
   F.c64 = (const d4v*)(base + off64);
   F.c32 = (const f4v*)(base + off32);
-  F.big = (const unsigned char*)(base + offbig);
   f->num_terms = nt;
   f->staged64 = staged_bytes(nint, ncol, ng, false);
   f->staged32 = staged_bytes(nint, ncol, ng, true);
+  f->staged64_lin = staged_bytes(nint, ncol, ng, false, true);
+  f->staged32_lin = staged_bytes(nint, ncol, ng, true, true);
   ctx->fits.push_back(std::move(f));
   ctx->fits_dirty = true;
   *fit_id = (int32_t)ctx->fits.size() - 1;
@@ -949,7 +974,8 @@ int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, con
     return fail(PIMC_ERR_ARG, "bisection_run: bad levels");
   std::set<int> seen;
   int nt = -1;
-  size_t table_bytes = 0, lr_bytes = 0;
+  size_t table_bytes = 0, table_bytes_lin = 0, lr_bytes = 0;
+  bool has_bc = false, has_lr = false;
   for (int r = 0; r < nrep; r++) {
     HostReplica* H = ctx->rep(replicas[r]);
     if (!H) return fail(PIMC_ERR_ARG, "bisection_run: bad replica");
@@ -959,6 +985,7 @@ int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, con
     if (R.lr_on) {
       if (desc->lowest_level >= R.lr_levels) return fail(PIMC_ERR_ARG, "bisection_run: long-range tables lack the lowest level");
       lr_bytes = std::max(lr_bytes, (size_t)2 * (2 * (R.maxk0 + R.maxk1 + R.maxk2) + 3) * sizeof(d2v));
+      has_lr = true;
     }
     if (desc->species < 0 || desc->species >= R.ns || H->dev.sp.count[desc->species] < 1)
       return fail(PIMC_ERR_ARG, "bisection_run: bad species");
@@ -968,8 +995,10 @@ int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, con
         if (nt < 0) nt = F.num_terms;
         if (F.num_terms != nt) return fail(PIMC_ERR_UNSUPPORTED, "bisection_run: fits with different num_terms");
         table_bytes = std::max(table_bytes, precision == PIMC_MIXED ? F.staged32 : F.staged64);
+        table_bytes_lin = std::max(table_bytes_lin, precision == PIMC_MIXED ? F.staged32_lin : F.staged64_lin);
       }
       if (desc->num_levels > F.num_levels()) return fail(PIMC_ERR_ARG, "bisection_run: pair fit has too few tau levels");
+      has_bc = has_bc || F.dev.kind == 1 || F.dev.grid_type != PIMC_GRID_LINEAR;
       if (F.dev.kind == 1 && R.use_long_range && (!((R.ulong_mask >> f) & 1u) || desc->num_levels > R.ulong_levels))
         return fail(PIMC_ERR_STATE, "bisection_run: long-range pair fit without Ulong tables");
     }
@@ -980,10 +1009,28 @@ int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, con
   HIP_TRY(hipSetDevice(ctx->device));
   int rc = ctx->sync_descriptors();
   if (rc) return rc;
-  if ((rc = ctx->d_repids.ensure((size_t)nrep * sizeof(int))) || (rc = ctx->d_stats.ensure((size_t)nrep * 5 * sizeof(int64_t))))
-    return rc;
+  if ((rc = ctx->d_stats.ensure((size_t)nrep * 5 * sizeof(int64_t)))) return rc;
   if (records && (rc = ctx->d_records.ensure((size_t)nrep * num_moves * sizeof(pimc_move_record)))) return rc;
-  HIP_TRY(hipMemcpyAsync(ctx->d_repids.p, replicas, (size_t)nrep * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  if ((int)ctx->reps.size() > ctx->acc_cap) {
+    // grow the per-replica counters, keeping what earlier calls accumulated
+    const int cap = std::max((int)ctx->reps.size(), 2 * ctx->acc_cap);
+    DevBuf grown;
+    if ((rc = grown.ensure((size_t)cap * 5 * sizeof(int64_t)))) return rc;
+    HIP_TRY(hipMemsetAsync(grown.p, 0, (size_t)cap * 5 * sizeof(int64_t), ctx->stream));
+    if (ctx->acc_cap)
+      HIP_TRY(hipMemcpyAsync(grown.p, ctx->d_acc.p, (size_t)ctx->acc_cap * 5 * sizeof(int64_t),
+                             hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->d_acc.swap(grown);
+    ctx->acc_cap = cap;
+  }
+  if (ctx->h_repids.size() != (size_t)nrep || !std::equal(replicas, replicas + nrep, ctx->h_repids.begin())) {
+    // the replica list is uploaded once and reused while it does not change
+    if ((rc = ctx->d_repids.ensure((size_t)nrep * sizeof(int)))) return rc;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));   // earlier launches may still read the old list
+    ctx->h_repids.assign(replicas, replicas + nrep);
+    HIP_TRY(hipMemcpy(ctx->d_repids.p, ctx->h_repids.data(), (size_t)nrep * sizeof(int), hipMemcpyHostToDevice));
+  }
   LaunchBisection a{};
   a.reps = ctx->d_reps.as<ReplicaDev>();
   a.fits = ctx->d_fits.as<FitDev>();
@@ -997,16 +1044,24 @@ int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, con
   a.num_moves = num_moves;
   a.records = records ? ctx->d_records.as<pimc_move_record>() : nullptr;
   a.stats = ctx->d_stats.as<int64_t>();
-  a.lds_table_bytes = table_bytes;
+  a.acc = ctx->d_acc.as<int64_t>();
+  const bool lean = !has_bc && !has_lr;
+  a.lds_table_bytes = lean ? table_bytes_lin : table_bytes;
   a.lds_lr_bytes = lr_bytes;
-  if (!ctx->ev0) {
-    HIP_TRY(hipEventCreate(&ctx->ev0));
-    HIP_TRY(hipEventCreate(&ctx->ev1));
+  std::pair<hipEvent_t, hipEvent_t> ev;
+  if (!ctx->spare.empty()) {
+    ev = ctx->spare.back();
+    ctx->spare.pop_back();
+  } else {
+    HIP_TRY(hipEventCreate(&ev.first));
+    HIP_TRY(hipEventCreate(&ev.second));
   }
-  HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-  hipError_t e = launch_bisection(nt, precision == PIMC_MIXED, a, ctx->stream);
+  ctx->timed.push_back(ev);
+  HIP_TRY(hipEventRecord(ev.first, ctx->stream));
+  hipError_t e = launch_bisection(nt, precision == PIMC_MIXED, lean, a, ctx->stream);
   if (e != hipSuccess) return fail(PIMC_ERR_DEVICE, std::string("bisection launch: ") + hipGetErrorString(e));
-  HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+  HIP_TRY(hipEventRecord(ev.second, ctx->stream));
+  if (!records && !stats) return PIMC_OK;   // asynchronous: counters via pimc_bisection_stats_take
   if (records)
     HIP_TRY(hipMemcpyAsync(records, ctx->d_records.p, (size_t)nrep * num_moves * sizeof(pimc_move_record),
                            hipMemcpyDeviceToHost, ctx->stream));
@@ -1015,7 +1070,7 @@ int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, con
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   if (stats) {
     float ms = 0.0f;
-    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    HIP_TRY(hipEventElapsedTime(&ms, ev.first, ev.second));
     stats->device_ms = ms;
     for (int r = 0; r < nrep; r++) {
       stats->moves += st[5 * r + 0];
@@ -1025,6 +1080,36 @@ int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, con
       stats->k_evals += st[5 * r + 4];
     }
   }
+  return PIMC_OK;
+}
+
+int pimc_bisection_stats_take(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, pimc_bisection_stats* stats) {
+  if (!ctx || !stats || nrep < 0 || (nrep > 0 && !replicas)) return fail(PIMC_ERR_ARG, "bisection_stats_take: bad argument");
+  for (int r = 0; r < nrep; r++)
+    if (!ctx->rep(replicas[r])) return fail(PIMC_ERR_ARG, "bisection_stats_take: bad replica");
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  memset(stats, 0, sizeof *stats);
+  for (auto& e : ctx->timed) {
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, e.first, e.second));
+    stats->device_ms += ms;
+    ctx->spare.push_back(e);
+  }
+  ctx->timed.clear();
+  if (ctx->acc_cap == 0) return PIMC_OK;
+  std::vector<int64_t> acc((size_t)ctx->acc_cap * 5);
+  HIP_TRY(hipMemcpy(acc.data(), ctx->d_acc.p, acc.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+  for (int r = 0; r < nrep; r++) {
+    int64_t* c = acc.data() + (size_t)replicas[r] * 5;
+    stats->moves += c[0];
+    stats->accepted += c[1];
+    stats->stages += c[2];
+    stats->pair_evals += c[3];
+    stats->k_evals += c[4];
+    std::fill(c, c + 5, 0);
+  }
+  HIP_TRY(hipMemcpy(ctx->d_acc.p, acc.data(), acc.size() * sizeof(int64_t), hipMemcpyHostToDevice));
   return PIMC_OK;
 }
 

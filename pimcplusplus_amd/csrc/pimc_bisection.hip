@@ -19,56 +19,228 @@
 
 namespace pimc {
 
+// One link of one partner (ShortRangeClass.cc:153-167) in the slice walk of
+// sr_partners.  On entry dA, rA, uA, lA hold the partner's imaged displacement,
+// distance, u00 and Ulong at the link's first slice; on return, those of its
+// second slice (nearest image), which is also the next link's first slice.
+// DistDisp images the second slice consistently with the first; that image is
+// the nearest one unless the partner sits near half a box, where the second
+// slice's terms are evaluated again for this link only.
+template <int NT, bool MIXED, bool BC, bool LIN, typename Table>
+__device__ inline double sr_link(const RepCore& R, const FitDev& F, const Table& T, const BcView& Q,
+                                 const double* __restrict__ ul, const double pbox[3], const double hsafe[3],
+                                 double urmin, const PIMC_LDS double* m, const double p[3], double dA[3], double& rA,
+                                 double& uA, double& lA) {
+  double dn[3], raw[3];
+  bool same = true;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    raw[d] = p[d] - m[d];
+    {
+#pragma clang fp contract(off)
+      const double n = -rint(raw[d] * R.boxinv[d]);
+      dn[d] = raw[d] + n * pbox[d];
+    }
+    // DistDisp's consistent image m = nearbyint((dA - raw)/L) equals the nearest
+    // image n whenever |dA - dn| < L/2 (hsafe keeps a margin far above rounding)
+    same = same && (fabs(dA[d] - dn[d]) < hsafe[d]);
+  }
+  const double n2 = dn[0] * dn[0] + dn[1] * dn[1] + dn[2] * dn[2];
+  const double rn = MIXED ? sqrt_seeded(n2) : sqrt_dist(n2);
+  double un = 0.0, ln = 0.0;
+  if constexpr (!BC) un = slice_u00<LIN>(F, T, rn);
+  if (ul) ln = ulong_eval(R, ul, rn);
+  double db[3] = {dn[0], dn[1], dn[2]};
+  double rb = rn, ub = un, lb = ln;
+  if (!same) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+#pragma clang fp contract(off)
+      const double mm = rint((dA[d] - raw[d]) * R.boxinv[d]);
+      db[d] = raw[d] + mm * pbox[d];
+    }
+    const double b2 = db[0] * db[0] + db[1] * db[1] + db[2] * db[2];
+    rb = MIXED ? sqrt_seeded(b2) : sqrt_dist(b2);
+    if constexpr (!BC) ub = slice_u00<LIN>(F, T, rb);
+    if (ul) lb = ulong_eval(R, ul, rb);
+  }
+  const double e0 = dA[0] - db[0], e1 = dA[1] - db[1], e2 = dA[2] - db[2];
+  const double s2 = e0 * e0 + e1 * e1 + e2 * e2;
+  const double q = 0.5 * (rA + rb);
+  const double z = rA - rb;
+  double U;
+  if constexpr (BC) {
+    U = pair_u_bc(F, Q.q, Q.level, q, s2);
+  } else {
+    const double rmin = F.start;
+    U = (rA < rmin || rb < rmin) ? urmin : 0.5 * (uA + ub) + link_offdiag<NT, LIN>(F, T, q, z, s2);
+  }
+  if (ul) U -= 0.5 * (lA + lb);
+#pragma unroll
+  for (int d = 0; d < 3; d++) dA[d] = dn[d];
+  rA = rn;
+  uA = un;
+  lA = ln;
+  return U;
+}
+
 // ShortRange partner loop of one stage (ShortRangeClass.cc:149-169) for partners
-// [j0, j1): the partner's beads walk the links in order, the bead one link ahead
-// loaded while the current link is evaluated.
-template <int NT, bool KAHAN, typename Table>
-__device__ inline void sr_partners(const RepCore& R, const FitDev& F, const Table& T, const double* ul, int j0,
-                                   int j1, int ptcl, int slice1, int K, int step, const PIMC_LDS double* mold,
-                                   const PIMC_LDS double* mnew, Kahan& acc_old, Kahan& acc_new) {
+// [j0, j1): each thread walks one partner's K+1 slices in order, OLD and NEW
+// copies of the moved particle side by side, two links per iteration (K is a
+// power of two >= 2) with the partner's beads loaded two links ahead.
+template <int NT, bool MIXED, bool BC, bool LIN, typename Table>
+__device__ inline void sr_partners(const RepCore& R, const FitDev& F, const Table& T, const BcView& Q,
+                                   const double* ul, int j0, int j1, int ptcl, int slice1, int K, int step,
+                                   const PIMC_LDS double* mold, const PIMC_LDS double* mnew, Kahan& acc_old,
+                                   Kahan& acc_new) {
   const size_t row = (size_t)3 * R.N * step;     // distance between consecutive link slices
+  const double pbox[3] = {R.isper[0] * R.box[0], R.isper[1] * R.box[1], R.isper[2] * R.box[2]};
+  const double hsafe[3] = {0.4999999 * R.box[0], 0.4999999 * R.box[1], 0.4999999 * R.box[2]};
+  double urmin = 0.0;
+  if constexpr (!BC) urmin = u00_rmin(T);
+  auto load = [&](const PIMC_GLOBAL double* q, double v[3]) {
+    v[0] = q[0];
+    v[1] = q[R.N];
+    v[2] = q[2 * R.N];
+  };
+  auto link2 = [&](int k, const double p[3], double dO[3], double& rO, double& uO, double& lO, double dN[3],
+                   double& rN, double& uN, double& lN) {
+    const int at = 3 * (k + 1) * step;
+    const double uo = sr_link<NT, MIXED, BC, LIN>(R, F, T, Q, ul, pbox, hsafe, urmin, mold + at, p, dO, rO, uO, lO);
+    const double un = sr_link<NT, MIXED, BC, LIN>(R, F, T, Q, ul, pbox, hsafe, urmin, mnew + at, p, dN, rN, uN, lN);
+    if (MIXED) {
+      acc_old.add(uo);
+      acc_new.add(un);
+    } else {
+      acc_old.s += uo;
+      acc_new.s += un;
+    }
+  };
   for (int j = j0 + (int)threadIdx.x; j < j1; j += blockDim.x) {
     if (j == ptcl) continue;
-    const double* pj = R.path0 + (size_t)slice1 * 3 * R.N + j;
-    double pa[3] = {pj[0], pj[R.N], pj[2 * R.N]};
-    pj += row;
-    double pb[3] = {pj[0], pj[R.N], pj[2 * R.N]};
-    for (int k = 0; k < K; k++) {
-      double pn[3] = {0.0, 0.0, 0.0};
-      if (k + 1 < K) {
-        pj += row;
-        pn[0] = pj[0];
-        pn[1] = pj[R.N];
-        pn[2] = pj[2 * R.N];
-      }
-      double moa[3], mob[3], mna[3], mnb[3];
+    const PIMC_GLOBAL double* pj = (const PIMC_GLOBAL double*)R.path0 + (size_t)slice1 * 3 * R.N + j;
+    double p0[3], p1[3], p2[3];
+    load(pj, p0);
+    load(pj + row, p1);
+    load(pj + 2 * row, p2);
+    double dO[3], dN[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+#pragma clang fp contract(off)
+      const double ao = p0[d] - mold[d], an = p0[d] - mnew[d];
+      dO[d] = ao + (-rint(ao * R.boxinv[d])) * pbox[d];
+      dN[d] = an + (-rint(an * R.boxinv[d])) * pbox[d];
+    }
+    const double o2 = dO[0] * dO[0] + dO[1] * dO[1] + dO[2] * dO[2];
+    const double w2 = dN[0] * dN[0] + dN[1] * dN[1] + dN[2] * dN[2];
+    double rO = MIXED ? sqrt_seeded(o2) : sqrt_dist(o2);
+    double rN = MIXED ? sqrt_seeded(w2) : sqrt_dist(w2);
+    double uO = 0.0, uN = 0.0, lO = 0.0, lN = 0.0;
+    if constexpr (!BC) {
+      uO = slice_u00<LIN>(F, T, rO);
+      uN = slice_u00<LIN>(F, T, rN);
+    }
+    if (ul) {
+      lO = ulong_eval(R, ul, rO);
+      lN = ulong_eval(R, ul, rN);
+    }
+    pj += 2 * row;
+    for (int k = 0; k < K; k += 2) {
+      double p3[3] = {0.0, 0.0, 0.0}, p4[3] = {0.0, 0.0, 0.0};
+      if (k + 3 <= K) load(pj + row, p3);
+      link2(k, p1, dO, rO, uO, lO, dN, rN, uN, lN);
+      if (k + 4 <= K) load(pj + 2 * row, p4);
+      link2(k + 1, p2, dO, rO, uO, lO, dN, rN, uN, lN);
+      pj += 2 * row;
 #pragma unroll
       for (int d = 0; d < 3; d++) {
-        moa[d] = mold[3 * k * step + d];
-        mob[d] = mold[3 * (k + 1) * step + d];
-        mna[d] = mnew[3 * k * step + d];
-        mnb[d] = mnew[3 * (k + 1) * step + d];
-      }
-      const double uo = pair_link_u<NT>(R, F, T, moa, mob, pa, pb, ul);
-      const double un = pair_link_u<NT>(R, F, T, mna, mnb, pa, pb, ul);
-      if (KAHAN) {
-        acc_old.add(uo);
-        acc_new.add(un);
-      } else {
-        acc_old.s += uo;
-        acc_new.s += un;
-      }
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        pa[d] = pb[d];
-        pb[d] = pn[d];
+        p1[d] = p3[d];
+        p2[d] = p4[d];
       }
     }
   }
 }
 
-template <int NT, bool MIXED>
-__global__ void __launch_bounds__(kBlock) k_bisection(const LaunchBisection a) {
+// Waves per SIMD the register allocation targets: four blocks of four waves per CU.
+#ifndef PIMC_BISECTION_WAVES
+#define PIMC_BISECTION_WAVES 4
+#endif
+// BisectionStageClass::Sample for midpoint b of the stage (BisectionStage.cc:128-289):
+// NEW midpoint = rbar + Delta, Delta ~ N(0, sigma^2) per dimension, and the
+// log of the sampling probabilities of the NEW and OLD midpoints.
+__device__ inline void sample_midpoint(uint64_t seed, uint32_t stream, uint32_t move, int level, int b,
+                                             int skip, double sigma, double sigma2, double bx0, double bx1,
+                                             double bx2, double bi0, double bi1, double bi2, double ip0, double ip1,
+                                             double ip2, int num_images, PIMC_LDS double* mold,
+                                             PIMC_LDS double* mnew, PIMC_LDS double* lnew, PIMC_LDS double* lold) {
+  const int s = b * skip, half = skip >> 1;
+  // gaussians 3b .. 3b+2 of this (move, level) come from pairs (3b)>>1 and (3b)>>1 + 1
+  double g0, g1, g2, g3;
+  const uint32_t p0 = (uint32_t)(3 * b) >> 1;
+  gaussian_pair(seed, stream, move, level, p0, g0, g1);
+  gaussian_pair(seed, stream, move, level, p0 + 1, g2, g3);
+  const bool odd = (3 * b) & 1;
+  const double gd[3] = {odd ? g1 : g0, odd ? g2 : g1, odd ? g3 : g2};
+  const double bxs[3] = {bx0, bx1, bx2}, bis[3] = {bi0, bi1, bi2}, ips[3] = {ip0, ip1, ip2};
+  double GaussProd = 1.0, GaussProdOld = 1.0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double bx = bxs[d], bi = bis[d], ip = ips[d];
+    double rdiffOld = put_in_box_1(mold[3 * (s + skip) + d] - mold[3 * s + d], bx, bi, ip);
+    double rbarOld = mold[3 * s + d] + 0.5 * rdiffOld;
+    double DeltaOld = put_in_box_1(mold[3 * (s + half) + d] - rbarOld, bx, bi, ip);
+    double rdiff = put_in_box_1(mnew[3 * (s + skip) + d] - mnew[3 * s + d], bx, bi, ip);
+    double rbar = mnew[3 * s + d] + 0.5 * rdiff;
+    double Delta = put_in_box_1(gd[d] * sigma, bx, bi, ip);
+    double GaussSum = 0.0, GaussSumOld = 0.0;
+    for (int image = -num_images; image <= num_images; image++) {
+      double dist = Delta + (double)image * bx;
+      double distOld = DeltaOld + (double)image * bx;
+      GaussSum += exp(-0.5 * dist * dist / sigma2);
+      GaussSumOld += exp(-0.5 * distOld * distOld / sigma2);
+    }
+    GaussProd *= GaussSum;
+    GaussProdOld *= GaussSumOld;
+    mnew[3 * (s + half) + d] = rbar + Delta;
+  }
+  lnew[b] = log(GaussProd);
+  lold[b] = log(GaussProdOld);
+}
+
+// KineticClass::SingleAction of one link (KineticClass.cc:33-90).
+__device__ inline double kinetic_link_ool(const PIMC_LDS double* a, const PIMC_LDS double* b, double bx0,
+                                                double bx1, double bx2, double bi0, double bi1, double bi2,
+                                                double ip0, double ip1, double ip2, int num_images,
+                                                double FourLambdaTauInv) {
+  const double bxs[3] = {bx0, bx1, bx2}, bis[3] = {bi0, bi1, bi2}, ips[3] = {ip0, ip1, ip2};
+  double GaussProd = 1.0;
+#pragma unroll
+  for (int dim = 0; dim < 3; dim++) {
+    double vel = put_in_box_1(b[dim] - a[dim], bxs[dim], bis[dim], ips[dim]);
+    double GaussSum = 0.0;
+    for (int image = -num_images; image <= num_images; image++) {
+      double dist = vel + (double)image * bxs[dim];
+      GaussSum += exp(-dist * dist * FourLambdaTauInv);
+    }
+    GaussProd *= GaussSum;
+  }
+  return -log(GaussProd);
+}
+
+// log(u) of the stage's Metropolis draw.
+__device__ inline double log_accept_draw(uint64_t seed, uint32_t stream, uint32_t move, int level) {
+  return log(rng_uniform(seed, stream, move, kTagAccept, level));
+}
+
+// Kernel variants: kLeanDavid -- DavidPA fits on LINEAR grids only, no
+// long-range action (the Lennard-Jones systems); kGeneric -- any fits and
+// the long-range action, chosen at run time.
+enum { kLeanDavid = 0, kGeneric = 1 };
+
+template <int NT, bool MIXED, int VAR>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PIMC_BISECTION_WAVES)))
+k_bisection(const LaunchBisection a) {
+  constexpr bool HAS_BC = VAR == kGeneric, HAS_LR = VAR == kGeneric, LIN = VAR == kLeanDavid;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   PIMC_LDS unsigned char* smem = (PIMC_LDS unsigned char*)smem_raw;
   const ReplicaDev* RD = a.reps + a.rep_ids[blockIdx.x];
@@ -126,46 +298,21 @@ __global__ void __launch_bounds__(kBlock) k_bisection(const LaunchBisection a) {
       const double levelTau = 0.5 * R.tau * skip;
       const double sigma2 = (1.0 * lambda * levelTau);
       const double sigma = sqrt(sigma2);
-      for (int b = tid; b < nb; b += blockDim.x) {
-        const int s = b * skip;
-        // gaussians 3b .. 3b+2 of this (move, level) come from pairs (3b)>>1 and (3b)>>1 + 1
-        double g0, g1, g2, g3;
-        const uint32_t p0 = (uint32_t)(3 * b) >> 1;
-        gaussian_pair(a.seed, R.stream, move, level, p0, g0, g1);
-        gaussian_pair(a.seed, R.stream, move, level, p0 + 1, g2, g3);
-        const bool odd = (3 * b) & 1;
-        const double gd[3] = {odd ? g1 : g0, odd ? g2 : g1, odd ? g3 : g2};
-        double GaussProd = 1.0, GaussProdOld = 1.0;
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-          const double bx = R.box[d], bi = R.boxinv[d], ip = R.isper[d];
-          double rdiffOld = put_in_box_1(mold[3 * (s + skip) + d] - mold[3 * s + d], bx, bi, ip);
-          double rbarOld = mold[3 * s + d] + 0.5 * rdiffOld;
-          double DeltaOld = put_in_box_1(mold[3 * (s + half) + d] - rbarOld, bx, bi, ip);
-          double rdiff = put_in_box_1(mnew[3 * (s + skip) + d] - mnew[3 * s + d], bx, bi, ip);
-          double rbar = mnew[3 * s + d] + 0.5 * rdiff;
-          double Delta = put_in_box_1(gd[d] * sigma, bx, bi, ip);
-          double GaussSum = 0.0, GaussSumOld = 0.0;
-          for (int image = -R.num_images; image <= R.num_images; image++) {
-            double dist = Delta + (double)image * bx;
-            double distOld = DeltaOld + (double)image * bx;
-            GaussSum += exp(-0.5 * dist * dist / sigma2);
-            GaussSumOld += exp(-0.5 * distOld * distOld / sigma2);
-          }
-          GaussProd *= GaussSum;
-          GaussProdOld *= GaussSumOld;
-          mnew[3 * (s + half) + d] = rbar + Delta;
-        }
-        lnew[b] = log(GaussProd);
-        lold[b] = log(GaussProdOld);
-      }
+      for (int b = tid; b < nb; b += blockDim.x)
+        sample_midpoint(a.seed, R.stream, move, level, b, skip, sigma, sigma2, R.box[0], R.box[1], R.box[2],
+                        R.boxinv[0], R.boxinv[1], R.boxinv[2], R.isper[0], R.isper[1], R.isper[2], R.num_images,
+                        mold, mnew, lnew, lold);
       __syncthreads();
 
       // ---- Kinetic action of every link, OLD and NEW ----
       const double FourLambdaTauInv = 1.0 / (4.0 * lambda * (R.tau * (1 << level)));
       for (int k = tid; k < K; k += blockDim.x) {
-        kold[k] = kinetic_link(R, mold + 3 * k * step, mold + 3 * (k + 1) * step, FourLambdaTauInv);
-        knew[k] = kinetic_link(R, mnew + 3 * k * step, mnew + 3 * (k + 1) * step, FourLambdaTauInv);
+        kold[k] = kinetic_link_ool(mold + 3 * k * step, mold + 3 * (k + 1) * step, R.box[0], R.box[1], R.box[2],
+                                   R.boxinv[0], R.boxinv[1], R.boxinv[2], R.isper[0], R.isper[1], R.isper[2],
+                                   R.num_images, FourLambdaTauInv);
+        knew[k] = kinetic_link_ool(mnew + 3 * k * step, mnew + 3 * (k + 1) * step, R.box[0], R.box[1], R.box[2],
+                                   R.boxinv[0], R.boxinv[1], R.boxinv[2], R.isper[0], R.isper[1], R.isper[2],
+                                   R.num_images, FourLambdaTauInv);
       }
 
       // ---- Short-range pair action, OLD and NEW (ShortRangeClass.cc:145-171) ----
@@ -174,20 +321,24 @@ __global__ void __launch_bounds__(kBlock) k_bisection(const LaunchBisection a) {
         const FitDev F = load_uniform(&a.fits[S.fit[sp * R.ns + s2]]);
         const double* ul = ulong_for(R, F, sp * R.ns + s2, level);
         const int j0 = S.first[s2], j1 = S.first[s2] + S.count[s2];
-        if (F.kind == 1) {
-          const BcView T{qgrid_of(F), level};
-          sr_partners<NT, false>(R, F, T, ul, j0, j1, ptcl, slice1, K, step, mold, mnew, acc_old, acc_new);
-          continue;
+        if constexpr (HAS_BC) {
+          if (F.kind == 1) {
+            const BcView Q{qgrid_of(F), level};
+            sr_partners<NT, MIXED, true, false>(R, F, LevelTable64<true>{}, Q, ul, j0, j1, ptcl, slice1, K, step, mold,
+                                         mnew, acc_old, acc_new);
+            continue;
+          }
         }
         __syncthreads();
         auto T = [&] {
           if constexpr (MIXED)
-            return stage_level32(F, level + F.taupos, tbl);
+            return stage_level32<LIN>(F, level + F.taupos, tbl);
           else
-            return stage_level64(F, level + F.taupos, tbl);
+            return stage_level64<LIN>(F, level + F.taupos, tbl);
         }();
         __syncthreads();
-        sr_partners<NT, MIXED>(R, F, T, ul, j0, j1, ptcl, slice1, K, step, mold, mnew, acc_old, acc_new);
+        sr_partners<NT, MIXED, false, LIN>(R, F, T, BcView{}, nullptr, j0, j1, ptcl, slice1, K, step, mold, mnew,
+                                      acc_old, acc_new);
       }
       const double sr_old = block_sum(acc_old.value(), scratch);
       const double sr_new = block_sum(acc_new.value(), scratch);
@@ -196,8 +347,8 @@ __global__ void __launch_bounds__(kBlock) k_bisection(const LaunchBisection a) {
       // rho_k NEW = OLD + e^{ik r_new} - e^{ik r_old} over the window, computed on the
       // fly; copy OLD is read, nothing is written until the move is accepted.
       double lr_kold = 0.0, lr_knew = 0.0;
-      const bool with_lr = R.lr_on && level == a.lowest_level;
-      if (with_lr) {
+      const bool with_lr = HAS_LR && R.lr_on && level == a.lowest_level;
+      if (HAS_LR && with_lr) {
         const int stride = kstride(R), o1 = koff_of(R, 1), o2 = koff_of(R, 2);
         double part_o = 0.0, part_n = 0.0;
         for (int t = 0; t <= W; t += step) {
@@ -285,7 +436,7 @@ __global__ void __launch_bounds__(kBlock) k_bisection(const LaunchBisection a) {
         }
         const double currActionChange = newAction - oldAction;
         const double logAcceptProb = logSampleRatio - currActionChange + prev;
-        const double logu = log(rng_uniform(a.seed, R.stream, move, kTagAccept, level));
+        const double logu = log_accept_draw(a.seed, R.stream, move, level);
         const int ok = logAcceptProb >= logu;
         prev = currActionChange;
         ctl[2] = ok;
@@ -308,7 +459,7 @@ __global__ void __launch_bounds__(kBlock) k_bisection(const LaunchBisection a) {
     }
 
     // ---- AcceptCopy: rho_k OLD, NEW <- the updated rho_k; PutInBoxFast on the NEW beads, OLD <- NEW ----
-    if (accepted && R.lr_on) {
+    if (HAS_LR && accepted && R.lr_on) {
       const int step = 1 << a.lowest_level;
       const int stride = kstride(R), o1 = koff_of(R, 1), o2 = koff_of(R, 2);
       for (int t = 0; t <= W; t += step) {
@@ -371,6 +522,12 @@ __global__ void __launch_bounds__(kBlock) k_bisection(const LaunchBisection a) {
     st[2] = st_stages;
     st[3] = st_evals;
     st[4] = st_kevals;
+    int64_t* ac = a.acc + (size_t)a.rep_ids[blockIdx.x] * 5;   // one block per replica per launch
+    ac[0] += st_moves;
+    ac[1] += st_acc;
+    ac[2] += st_stages;
+    ac[3] += st_evals;
+    ac[4] += st_kevals;
   }
 }
 
@@ -379,23 +536,25 @@ static size_t bisection_lds_bytes(const LaunchBisection& a) {
   return a.lds_table_bytes + sizeof(double) * (6 * (W + 1) + W + 2 * W + kWaves) + 4 * sizeof(int) + a.lds_lr_bytes;
 }
 
-template <bool MIXED>
+template <bool MIXED, int VAR>
 static hipError_t launch_bis_t(int nt, const LaunchBisection& a, hipStream_t s) {
   const size_t lds = bisection_lds_bytes(a);
   dim3 grid(a.nrep);
   switch (nt) {
-    case 0: k_bisection<0, MIXED><<<grid, kBlock, lds, s>>>(a); break;
-    case 1: k_bisection<1, MIXED><<<grid, kBlock, lds, s>>>(a); break;
-    case 2: k_bisection<2, MIXED><<<grid, kBlock, lds, s>>>(a); break;
-    default: k_bisection<3, MIXED><<<grid, kBlock, lds, s>>>(a); break;
+    case 0: k_bisection<0, MIXED, VAR><<<grid, kBlock, lds, s>>>(a); break;
+    case 1: k_bisection<1, MIXED, VAR><<<grid, kBlock, lds, s>>>(a); break;
+    case 2: k_bisection<2, MIXED, VAR><<<grid, kBlock, lds, s>>>(a); break;
+    default: k_bisection<3, MIXED, VAR><<<grid, kBlock, lds, s>>>(a); break;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_bisection(int num_terms, bool mixed, const LaunchBisection& a, hipStream_t s) {
+hipError_t launch_bisection(int num_terms, bool mixed, bool lean, const LaunchBisection& a, hipStream_t s) {
   if (a.nrep <= 0 || a.num_moves <= 0) return hipSuccess;
   if (bisection_lds_bytes(a) > 64 * 1024) return hipErrorInvalidValue;
-  return mixed ? launch_bis_t<true>(num_terms, a, s) : launch_bis_t<false>(num_terms, a, s);
+  if (lean)
+    return mixed ? launch_bis_t<true, kLeanDavid>(num_terms, a, s) : launch_bis_t<false, kLeanDavid>(num_terms, a, s);
+  return mixed ? launch_bis_t<true, kGeneric>(num_terms, a, s) : launch_bis_t<false, kGeneric>(num_terms, a, s);
 }
 
 }  // namespace pimc

@@ -23,6 +23,7 @@ typedef double d2v __attribute__((ext_vector_type(2)));
 typedef double d4v __attribute__((ext_vector_type(4)));
 
 #define PIMC_LDS __attribute__((address_space(3)))
+#define PIMC_GLOBAL __attribute__((address_space(1)))
 
 // Pointer to const T, in LDS (InLds) or in global memory.
 template <bool InLds>
@@ -70,11 +71,10 @@ struct FitDev {
   const double* x;        // [ng] grid points
   const d2v* geo;         // [nint] {X_i, 1/(X_{i+1}-X_i)}
   const d4v* c64;         // [num_tau][nint][ncol]
-  const f4v* c32;         // [num_tau][nint][ncol]
-  // [num_tau][nint]: 1 where an interval's coefficients are too large for FP32
-  // (sum over |c0..c3| of a column above kMixedLimit); the mixed path evaluates
-  // those intervals from c64.
-  const unsigned char* big;
+  // [num_tau][nint][ncol] FP32 copy of c64; an interval whose coefficients are too
+  // large for FP32 (sum over |c0..c3| of a column above kMixedLimit) holds NaN in
+  // every column, and the mixed path evaluates it from c64.
+  const f4v* c32;
   // kind 1: bicubic U(q, t) per level; the q grid is the fields above
   int kind;
   int nt;
@@ -245,11 +245,12 @@ __device__ inline int clamp_to_int(double t, int n) {
 // LINEAR: ReverseMap = (int)nearbyint((x-Start)*deltainv - 0.5).  LOG: the
 // reference's floor(log(x/r0)/log(spacing)), corrected by one step against the
 // stored knots so that a last-bit difference in log() cannot change it.
-template <typename XPtr>
+// LIN: the caller guarantees a LINEAR grid (no log-grid code is generated).
+template <bool LIN = false, typename XPtr>
 __device__ inline int spline_low(const FitDev& F, XPtr xs, double x) {
 #pragma clang fp contract(off)
   int i;
-  if (F.grid_type == 0) {
+  if (LIN || F.grid_type == 0) {
     i = clamp_to_int(rint((x - F.start) * F.deltainv - 0.5), F.ng);
   } else {
     i = clamp_to_int(floor(log(x / F.r0) / F.logspacing), F.ng);
@@ -358,7 +359,6 @@ struct LevelTable32 {
   typename Ptr<InLds>::template c<d2v> geo;
   typename Ptr<InLds>::template c<f4v> c;            // FP32 coefficients
   typename Ptr<InLds>::template c<double> x;
-  typename Ptr<InLds>::template c<unsigned char> big; // [nint] FP64 fallback flags
   const d4v* c64;                                     // FP64 coefficients of this level (global memory)
 };
 
@@ -418,19 +418,23 @@ __device__ inline double sqrt_seeded(double a) {
 }
 
 // Columns [c0col, c0col + NC) of interval i at local coordinate b: FP32 Horner
-// from the FP32 table, or FP64 from the global FP64 table where flagged.
+// from the FP32 table, or FP64 from the global FP64 table where the interval is
+// flagged (its FP32 row holds NaN).
 template <int NC, bool InLds>
 __device__ inline void eval_cols_mixed(const LevelTable32<InLds>& T, int ncol, int i, int c0col, double b,
                                        double out[NC]) {
-  if (T.big[i]) {
-    const d4v* c = T.c64 + (size_t)i * ncol + c0col;
+  const int at = i * ncol + c0col;
+  f4v c[NC];
 #pragma unroll
-    for (int k = 0; k < NC; k++) out[k] = horner64(c[k], b);
+  for (int k = 0; k < NC; k++) c[k] = T.c[at + k];
+  if (__builtin_isnan(c[0].x)) {
+    const d4v* c64 = T.c64 + (size_t)at;
+#pragma unroll
+    for (int k = 0; k < NC; k++) out[k] = horner64(c64[k], b);
   } else {
     const float bf = (float)b;
-    const int at = i * ncol + c0col;
 #pragma unroll
-    for (int k = 0; k < NC; k++) out[k] = (double)horner32(T.c[at + k], bf);
+    for (int k = 0; k < NC; k++) out[k] = (double)horner32(c[k], bf);
   }
 }
 

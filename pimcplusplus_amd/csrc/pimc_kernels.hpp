@@ -77,7 +77,6 @@ __device__ inline LevelTable32<false> level_table32(const FitDev& F, int lev) {
   t.geo = F.geo;
   t.c = F.c32 + (size_t)lev * F.nint * F.ncol;
   t.x = F.x;
-  t.big = F.big + (size_t)lev * F.nint;
   t.c64 = F.c64 + (size_t)lev * F.nint * F.ncol;
   return t;
 }
@@ -122,9 +121,11 @@ struct IsMixed<LevelTable32<InLds>> {
   static constexpr bool value = true;
 };
 
-// Bytes of LDS one staged level of fit F takes: geo + coefficients + grid (+ flags).
-__host__ __device__ inline size_t staged_bytes(int nint, int ncol, int ng, bool mixed) {
-  size_t b = (size_t)nint * 16 + (size_t)nint * ncol * (mixed ? 16 : 32) + (size_t)ng * 8 + (mixed ? nint : 0);
+// Bytes of LDS one staged level of fit F takes: geo + coefficients + grid, or the
+// coefficients alone when the lookups use lin_low (lin).
+__host__ __device__ inline size_t staged_bytes(int nint, int ncol, int ng, bool mixed, bool lin = false) {
+  const size_t coef = (size_t)nint * ncol * (mixed ? 16 : 32);
+  const size_t b = lin ? coef : (size_t)nint * 16 + coef + (size_t)ng * 8;
   return (b + 15) & ~(size_t)15;
 }
 
@@ -143,15 +144,17 @@ __device__ inline void block_copy(PIMC_LDS T* dst, const T* __restrict__ src, in
   }
 }
 
-// Cooperative copy of level `lev` of F into LDS at `base` (FP64 view).
+// Cooperative copy of level `lev` of F into LDS at `base` (FP64 view).  LIN:
+// the lookups use lin_low, so the interval geometry and knots are not copied.
+template <bool LIN = false>
 __device__ inline LevelTable64<true> stage_level64(const FitDev& F, int lev, PIMC_LDS unsigned char* base) {
   PIMC_LDS d2v* geo = (PIMC_LDS d2v*)base;
-  PIMC_LDS d4v* c = (PIMC_LDS d4v*)(base + (size_t)F.nint * 16);
+  PIMC_LDS d4v* c = (PIMC_LDS d4v*)(LIN ? base : base + (size_t)F.nint * 16);
   const int nc = F.nint * F.ncol;
   PIMC_LDS double* xs = (PIMC_LDS double*)((PIMC_LDS unsigned char*)c + (size_t)nc * 32);
-  block_copy(geo, F.geo, F.nint);
+  if (!LIN) block_copy(geo, F.geo, F.nint);
   block_copy(c, F.c64 + (size_t)lev * nc, nc);
-  block_copy(xs, F.x, F.ng);
+  if (!LIN) block_copy(xs, F.x, F.ng);
   LevelTable64<true> t;
   t.geo = geo;
   t.c = c;
@@ -160,21 +163,19 @@ __device__ inline LevelTable64<true> stage_level64(const FitDev& F, int lev, PIM
 }
 
 // Cooperative copy of level `lev` of F into LDS at `base` (mixed view).
+template <bool LIN = false>
 __device__ inline LevelTable32<true> stage_level32(const FitDev& F, int lev, PIMC_LDS unsigned char* base) {
   PIMC_LDS d2v* geo = (PIMC_LDS d2v*)base;
-  PIMC_LDS f4v* c = (PIMC_LDS f4v*)(base + (size_t)F.nint * 16);
+  PIMC_LDS f4v* c = (PIMC_LDS f4v*)(LIN ? base : base + (size_t)F.nint * 16);
   const int nc = F.nint * F.ncol;
   PIMC_LDS double* xs = (PIMC_LDS double*)((PIMC_LDS unsigned char*)c + (size_t)nc * 16);
-  PIMC_LDS unsigned char* big = (PIMC_LDS unsigned char*)(xs + F.ng);
-  block_copy(geo, F.geo, F.nint);
+  if (!LIN) block_copy(geo, F.geo, F.nint);
   block_copy(c, F.c32 + (size_t)lev * nc, nc);
-  block_copy(xs, F.x, F.ng);
-  for (int i = threadIdx.x; i < F.nint; i += blockDim.x) big[i] = F.big[(size_t)lev * F.nint + i];
+  if (!LIN) block_copy(xs, F.x, F.ng);
   LevelTable32<true> t;
   t.geo = geo;
   t.c = c;
   t.x = xs;
-  t.big = big;
   t.c64 = F.c64 + (size_t)lev * nc;
   return t;
 }
@@ -206,6 +207,127 @@ __device__ inline double pair_link_u(const RepCore& R, const FitDev& F, const Ta
   // long-ranged fits: subtract the part the k-space action carries (ShortRangeClass.cc:165-166)
   if (ulong_tab) U -= 0.5 * (ulong_eval(R, ulong_tab, ra) + ulong_eval(R, ulong_tab, rb));
   return U;
+}
+
+// --------------------------------------------------------------------------
+// calcUsqzFast split by what it depends on (DavidPAClass.cc:468-511), for loops
+// that walk a partner's links in order: the diagonal column u00 depends on one
+// slice only and is evaluated once per slice, at |r| clamped to End (the
+// reference evaluates it at q +- z/2, the same distance up to rounding); the
+// off-diagonal sum depends on the link.
+// --------------------------------------------------------------------------
+// Interval and local coordinate of x on a LINEAR grid: t = (x - Start)/delta,
+// i = clamp(floor t) and b = t - i, for 0 <= x <= End.  The reference's
+// ReverseMap (nearbyint(t - 1/2)) differs only where t is an integer, where both
+// neighbouring cubics take the same value; b differs from (x - X_i)/h by
+// rounding of t.
+__device__ inline int lin_low(const FitDev& F, double x, double& b) {
+  const double t = (x - F.start) * F.deltainv;
+  int i = (int)t;
+  i = i < 0 ? 0 : (i > F.nint - 1 ? F.nint - 1 : i);
+  b = t - (double)i;
+  return i;
+}
+
+// Interval and local coordinate of x by the grid's own lookup.
+template <bool LIN, typename Table>
+__device__ inline int table_low(const FitDev& F, const Table& T, double x, double& b) {
+  if constexpr (LIN) {
+    return lin_low(F, x, b);
+  } else {
+    const int l = spline_low(F, T.x, x);
+    const d2v g = T.geo[l];
+    b = (x - g.x) * g.y;
+    return l;
+  }
+}
+
+template <bool LIN, bool InLds>
+__device__ inline double slice_u00(const FitDev& F, const LevelTable64<InLds>& T, double r) {
+  r = r > F.end ? F.end : r;
+  double b;
+  const int l = table_low<LIN>(F, T, r, b);
+  return horner64(T.c[l * F.ncol], b);
+}
+template <bool LIN, bool InLds>
+__device__ inline double slice_u00(const FitDev& F, const LevelTable32<InLds>& T, double r) {
+  r = r > F.end ? F.end : r;
+  double b;
+  const int l = table_low<LIN>(F, T, r, b);
+  double u[1];
+  eval_cols_mixed<1>(T, F.ncol, l, 0, b, u);
+  return u[0];
+}
+
+// sqrt for the short-range distances (0 <= a < 1e300): the hardware reciprocal
+// square root refined by one Goldschmidt and two Newton steps, without the
+// denormal scaling of the library sqrt; a = 0 gives ~1e-150.
+__device__ inline double sqrt_dist(double a) {
+  a = fmax(a, 1e-300);
+  const double y = __builtin_amdgcn_rsq(a);
+  double g = a * y, h = 0.5 * y;
+  const double r = fma(-h, g, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  double d = fma(-g, g, a);
+  g = fma(d, h, g);
+  d = fma(-g, g, a);
+  return fma(d, h, g);
+}
+
+// sum_{k=1..NT} sum_{j=0..k} u_kj(q) z^{2j} s^{2(k-j)} where s > 0 and q < End, else 0.
+template <int NT, bool LIN, typename Table>
+__device__ inline double link_offdiag(const FitDev& F, const Table& T, double q, double z, double s2) {
+  if constexpr (NT == 0) {
+    return 0.0;
+  } else {
+    if (!(s2 > 0.0 && q < F.end)) return 0.0;
+    constexpr int NOFF = NT * (NT + 3) / 2;
+    double bq;
+    const int lq = table_low<LIN>(F, T, q, bq);
+    double cv[NOFF];
+    if constexpr (IsMixed<Table>::value) {
+      eval_cols_mixed<NOFF>(T, F.ncol, lq, 1, bq, cv);
+    } else {
+#pragma unroll
+      for (int c = 0; c < NOFF; c++) cv[c] = horner64(T.c[lq * F.ncol + 1 + c], bq);
+    }
+    const double z2 = z * z;
+    double zp[NT + 1], sp[NT + 1];
+    zp[0] = 1.0;
+    sp[0] = 1.0;
+#pragma unroll
+    for (int t = 1; t <= NT; t++) {
+      zp[t] = zp[t - 1] * z2;
+      sp[t] = sp[t - 1] * s2;
+    }
+    double off = 0.0;
+    int col = 0;
+#pragma unroll
+    for (int k = 1; k <= NT; k++)
+#pragma unroll
+      for (int j = 0; j <= k; j++, col++) off += cv[col] * (zp[j] * sp[k - j]);
+    return off;
+  }
+}
+
+// u00(rmin): the value at the first knot (interval 0 at b = 0).
+template <bool InLds>
+__device__ inline double u00_rmin(const LevelTable64<InLds>& T) { return T.c[0].x; }
+template <bool InLds>
+__device__ inline double u00_rmin(const LevelTable32<InLds>& T) { return T.c64[0].x; }
+
+// Nearest-image displacement of one dimension (DistDisp's n = -nearbyint(a/L)).
+__device__ inline double near_image_1(double a, double box, double boxinv, double isper, double& n) {
+#pragma clang fp contract(off)
+  n = -rint(a * boxinv);
+  return a + n * isper * box;
+}
+// DistDisp's image of the second slice, consistent with the imaged first-slice displacement da.
+__device__ inline double cons_image_1(double da, double b, double box, double boxinv, double isper, double& m) {
+#pragma clang fp contract(off)
+  m = rint((da - b) * boxinv);
+  return b + m * isper * box;
 }
 
 // Ulong table of pair `pair` at `level` when ShortRange must subtract it, else nullptr.

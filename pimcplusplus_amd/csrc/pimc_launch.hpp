@@ -52,11 +52,13 @@ struct LaunchBisection {
   uint32_t first_move;
   int num_moves;
   pimc_move_record* records;  // [nrep][num_moves] or nullptr
-  int64_t* stats;             // [nrep][5]
+  int64_t* stats;             // [nrep][5]: this launch's counters per block
+  int64_t* acc;               // [replica id][5]: counters accumulated over launches
   size_t lds_table_bytes;     // max staged level size over the fits used
   size_t lds_lr_bytes;        // phase tables of the long-range stage term (0 without long range)
 };
-// Returns hipSuccess or the launch error.
-hipError_t launch_bisection(int num_terms, bool mixed, const LaunchBisection& a, hipStream_t s);
+// Returns hipSuccess or the launch error.  lean: every pair fit is a DavidPA
+// fit on a LINEAR grid and no replica evaluates the long-range action.
+hipError_t launch_bisection(int num_terms, bool mixed, bool lean, const LaunchBisection& a, hipStream_t s);
 
 }  // namespace pimc

@@ -175,3 +175,28 @@ def test_bisection_mixed_delta_s_within_1e6(oracle, gpu):
             assert np.max(np.abs(dso - dsg)) <= 1e-6
             compared += ns
     assert compared > 300
+
+
+def test_async_runs_match_sync_and_oracle(oracle, gpu):
+    sysm = workloads.config0(num_replicas=3)
+    io = workloads.load(oracle, sysm)
+    a = workloads.load(gpu, sysm)      # run synchronously
+    b = workloads.load(gpu, sysm)      # the same replicas, run asynchronously
+    tot = {f: 0 for f in ("moves", "accepted", "stages", "pair_evals")}
+    for c in range(4):
+        oracle.bisection_run(io, sysm.num_levels, sysm.seed, 25 * c, 25, records=False)
+        _, st = gpu.bisection_run(a, sysm.num_levels, sysm.seed, 25 * c, 25, FP64, records=False)
+        for f in tot:
+            tot[f] += getattr(st, f)
+        r, st = gpu.bisection_run(b, sysm.num_levels, sysm.seed, 25 * c, 25, FP64, records=False, wait=False)
+        assert r is None and st is None
+    tb = gpu.bisection_stats_take(b)
+    ta = gpu.bisection_stats_take(a)
+    to = oracle.bisection_stats_take(io)
+    for f in tot:
+        assert getattr(tb, f) == getattr(ta, f) == tot[f] == getattr(to, f), f
+    assert tb.moves == 300 and tb.device_ms > 0
+    for x, y, z in zip(a, b, io):
+        assert np.array_equal(gpu.path_get(x), gpu.path_get(y))
+        assert np.max(np.abs(gpu.path_get(y) - oracle.path_get(z))) < 1e-12
+    assert gpu.bisection_stats_take(b).moves == 0

@@ -72,3 +72,16 @@ def test_bisection_sample_moves_only_midpoints(oracle):
     changed = np.where(np.any(after != before, axis=2))
     assert set(changed[1]) == {1}
     assert set(changed[0]) == {6, 10}     # slice1 + 2 + 4b for level 1 in a window of 8
+
+
+def test_stats_take_accumulates_and_clears(oracle):
+    sysm = workloads.config0(num_replicas=2)
+    ids = workloads.load(oracle, sysm)
+    _, s1 = oracle.bisection_run(ids, sysm.num_levels, sysm.seed, 0, 20)
+    _, s2 = oracle.bisection_run(ids[:1], sysm.num_levels, sysm.seed, 20, 10, records=False)
+    t = oracle.bisection_stats_take(ids)
+    for f in ("moves", "accepted", "stages", "pair_evals", "k_evals"):
+        assert getattr(t, f) == getattr(s1, f) + getattr(s2, f), f
+    assert t.moves == 50 and t.device_ms > 0
+    t2 = oracle.bisection_stats_take(ids)
+    assert (t2.moves, t2.pair_evals, t2.device_ms) == (0, 0, 0.0)
