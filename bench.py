@@ -32,15 +32,34 @@ from pimcplusplus_amd import abi, native, workloads  # noqa: E402
 
 METRIC = "pair·slice action evals/s, mixed vs FP64 on GPU at ΔS err ≤1e-6"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FLOP_PER_EVAL = 180.0      # FP64 flops of one pair·slice evaluation (DistDisp, q/z/s², splines, sum)
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_bisection_lj_n256_m64_l4_traffic.md")
+
+
+def measured_traffic(replicas: int, moves_per_step: int):
+    """HBM bytes per launch from the committed counter summary (FETCH_SIZE and WRITE_SIZE of
+    the same workload, scaled by a copy kernel of known size), when it covers this workload."""
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    block = text.split("```json", 1)[1].split("```", 1)[0] if "```json" in text else ""
+    try:
+        d = json.loads(block)
+    except ValueError:
+        return None
+    want = f"{replicas} replicas, {moves_per_step} moves/launch"
+    return d.get("traffic_bytes_per_launch") if want in d.get("workload", "") else None
 
 
 def bytes_per_launch(st, n: int) -> float:
     """Algorithmic HBM bytes of the launches summarised by `st`: every executed
     stage reads the K+1 slices of N+1 FP64 beads (partners, plus the moved
-    particle's OLD and NEW copies counted once more): 24 (K+1)(N+1) bytes.
-    sum_stages K = pair_evals / (2 (N-1))."""
+    particle's OLD and NEW copies counted once more) and returns two FP64
+    actions: 24 (K+1)(N+1) + 16 bytes.  sum_stages K = pair_evals / (2 (N-1))."""
     sum_k = st.pair_evals / (2.0 * (n - 1))
-    return 24.0 * (n + 1) * (sum_k + st.stages)
+    return 24.0 * (n + 1) * (sum_k + st.stages) + 16.0 * st.stages
 
 
 def load_system(lib, system):
@@ -204,6 +223,7 @@ def main():
     err, nstages = (None, 0) if args.no_dserr else delta_s_error()
     mixed = res["mixed"]
     achieved = mixed["bytes_per_launch"] / (mixed["kernel_ms_avg"] * 1e-3) / 1e9
+    flops = mixed["device_evals_per_s"] * FLOP_PER_EVAL / 1e12
     line = {
         "metric": METRIC,
         "value": mixed["evals_per_s"],
@@ -215,7 +235,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 spline / f64 geometry+accumulation",
+        "dtype": "f32/f64 mixed",
         "data": "synthetic",
         "config": {"workload": "LJ N=256 M=64 tau=0.0125 level-4 bisection, independent replicas",
                    "replicas": args.replicas, "moves_per_step": args.moves_per_step,
@@ -228,10 +248,16 @@ def main():
         "acceptance": mixed["acceptance"],
         "stages_per_move": mixed["stages_per_move"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.replicas, args.moves_per_step),
                      "kernel": "k_bisection<NT=2,MIXED>",
-                     "note": "bytes = 24 (K+1)(N+1) per executed stage; the kernel's binding roof is "
-                             "VALU/LDS (spline evaluation), not HBM"},
+                     "note": "algorithmic bytes = 24 (K+1)(N+1) + 16 per executed stage, over the "
+                             "kernel's HIP-event time; traffic from the committed FETCH_SIZE/WRITE_SIZE "
+                             "summary. The binding roof is the vector ALU (see roofline_valu)"},
+        "roofline_valu": {"bound": "valu", "achieved": flops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                          "frac": flops / FP64_PEAK_TFLOPS,
+                          "note": f"{FLOP_PER_EVAL:.0f} algorithmic FP64 flop per pair·slice evaluation "
+                                  "over the kernel time; peak = MI355X FP64 vector"},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
