@@ -300,6 +300,42 @@ int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep,
                        int32_t precision, pimc_move_record* records,
                        pimc_bisection_stats* stats);
 
+/* Launch shape of the resident bisection kernel (the context's current plan is
+ * used by pimc_bisection_run and captured by pimc_sweep_graph_create). */
+typedef struct pimc_bisection_plan {
+  int32_t block_threads;     /* threads per replica: 256 or 512 */
+  int32_t link_split;        /* threads sharing one partner's links in the short-range loop: 1, 2 or 4 */
+  int32_t persistent;        /* 1: a persistent grid takes replicas from a work queue, longest first */
+  int32_t reserved;
+  double trial_ms;           /* pimc_bisection_tune: time of the chosen shape on the trial moves */
+} pimc_bisection_plan;
+
+/* The default plan is {256, 1, 0}. */
+int pimc_bisection_set_plan(pimc_ctx* ctx, const pimc_bisection_plan* plan);
+int pimc_bisection_get_plan(pimc_ctx* ctx, pimc_bisection_plan* plan);
+
+/* Autotuner: times every plan (block_threads x link_split x persistent) on
+ * trial_moves moves [first_move, first_move + trial_moves) of the listed
+ * replicas, restores their paths, permutations, rho_k and counters after each
+ * trial, and makes the fastest plan the context's plan (also returned in *out,
+ * which may be NULL).  Has no counterpart in the reference, whose CPU loops have
+ * no launch shape. */
+int pimc_bisection_tune(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep,
+                        const pimc_bisection_desc* desc, int64_t first_move, int32_t trial_moves,
+                        int32_t precision, pimc_bisection_plan* out);
+
+/* Sweep graphs: one sweep (num_moves moves of every listed replica with the
+ * context's plan) captured once as a device graph of kernels; every replay
+ * continues the chains, sweep i running moves first_move + i*num_moves onward.
+ * Launches are asynchronous; counters are read with pimc_bisection_stats_take.
+ * A graph reads the replicas' device state directly: do not change their
+ * paths with other calls while a graph's sweeps are queued. */
+int pimc_sweep_graph_create(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep,
+                            const pimc_bisection_desc* desc, int64_t first_move, int32_t num_moves,
+                            int32_t precision, int32_t* graph_id);
+int pimc_sweep_graph_launch(pimc_ctx* ctx, int32_t graph_id, int32_t num_sweeps);
+int pimc_sweep_graph_destroy(pimc_ctx* ctx, int32_t graph_id);
+
 /* Waits for the context's stream, then stores in *stats the counters every
  * pimc_bisection_run call since the previous take added for the listed
  * replicas, and clears those replicas' counters.  device_ms is the compute time

@@ -1012,9 +1012,20 @@ int check_req(Ctx* c, const pimc_action_req& q, Replica** out) {
 // ==========================================================================
 // C ABI
 // ==========================================================================
+// A recorded sweep: the oracle replays it as ordinary bisection calls.
+struct OracleSweep {
+  std::vector<int32_t> replicas;
+  pimc_bisection_desc desc{};
+  int64_t first_move = 0, sweeps = 0;
+  int32_t num_moves = 0, precision = 0;
+  bool live = false;
+};
+
 struct pimc_ctx {
   Ctx c;
   double acc_ms = 0.0;   // time of the bisection calls since the last take
+  pimc_bisection_plan plan{256, 1, 0, 0, 0.0};
+  std::vector<OracleSweep> sweeps;
 };
 
 extern "C" {
@@ -1345,7 +1356,7 @@ int pimc_dist_disp(pimc_ctx* ctx, int32_t replica, int32_t mode, int64_t n, cons
 int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, const pimc_bisection_desc* desc,
                        int64_t first_move, int32_t num_moves, int32_t precision, pimc_move_record* records,
                        pimc_bisection_stats* stats) {
-  (void)precision;
+  if (precision != PIMC_FP64 && precision != PIMC_MIXED) return fail(PIMC_ERR_ARG, "bisection_run: bad precision");
   if (!ctx || !replicas || !desc || nrep < 0 || num_moves < 0 || first_move < 0 ||
       first_move + num_moves > (int64_t)0xffffffffLL)
     return fail(PIMC_ERR_ARG, "bisection_run: bad argument");
@@ -1392,6 +1403,79 @@ int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, con
   st.device_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   ctx->acc_ms += st.device_ms;
   if (stats) *stats = st;
+  return PIMC_OK;
+}
+
+// Launch shapes mean nothing on the CPU: the oracle validates and stores them.
+int pimc_bisection_set_plan(pimc_ctx* ctx, const pimc_bisection_plan* plan) {
+  if (!ctx || !plan) return fail(PIMC_ERR_ARG, "bisection_set_plan: NULL argument");
+  if (plan->block_threads != 256 && plan->block_threads != 512)
+    return fail(PIMC_ERR_ARG, "bisection_set_plan: block_threads must be 256 or 512");
+  if (plan->link_split != 1 && plan->link_split != 2 && plan->link_split != 4)
+    return fail(PIMC_ERR_ARG, "bisection_set_plan: link_split must be 1, 2 or 4");
+  ctx->plan = *plan;
+  ctx->plan.persistent = plan->persistent ? 1 : 0;
+  return PIMC_OK;
+}
+
+int pimc_bisection_get_plan(pimc_ctx* ctx, pimc_bisection_plan* plan) {
+  if (!ctx || !plan) return fail(PIMC_ERR_ARG, "bisection_get_plan: NULL argument");
+  *plan = ctx->plan;
+  return PIMC_OK;
+}
+
+int pimc_bisection_tune(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, const pimc_bisection_desc* desc,
+                        int64_t first_move, int32_t trial_moves, int32_t precision, pimc_bisection_plan* out) {
+  if (!ctx || !desc || nrep <= 0 || !replicas || trial_moves <= 0 || first_move < 0 ||
+      first_move + trial_moves > (int64_t)0xffffffffLL)
+    return fail(PIMC_ERR_ARG, "bisection_tune: bad argument");
+  if (precision != PIMC_FP64 && precision != PIMC_MIXED) return fail(PIMC_ERR_ARG, "bisection_tune: bad precision");
+  for (int r = 0; r < nrep; r++)
+    if (!get_rep(&ctx->c, replicas[r])) return fail(PIMC_ERR_ARG, "bisection_tune: bad replica");
+  ctx->plan = pimc_bisection_plan{256, 1, 0, 0, 0.0};
+  if (out) *out = ctx->plan;
+  return PIMC_OK;
+}
+
+int pimc_sweep_graph_create(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, const pimc_bisection_desc* desc,
+                            int64_t first_move, int32_t num_moves, int32_t precision, int32_t* graph_id) {
+  if (!ctx || !desc || nrep <= 0 || !replicas || num_moves <= 0 || first_move < 0 || !graph_id ||
+      first_move > (int64_t)0xffffffffLL)
+    return fail(PIMC_ERR_ARG, "sweep_graph_create: bad argument");
+  // the same checks as a run, without moving anything
+  int rc = pimc_bisection_run(ctx, replicas, nrep, desc, first_move, 0, precision, nullptr, nullptr);
+  if (rc) return rc;
+  OracleSweep g;
+  g.replicas.assign(replicas, replicas + nrep);
+  g.desc = *desc;
+  g.first_move = first_move;
+  g.num_moves = num_moves;
+  g.precision = precision;
+  g.live = true;
+  ctx->sweeps.push_back(g);
+  *graph_id = (int32_t)ctx->sweeps.size() - 1;
+  return PIMC_OK;
+}
+
+int pimc_sweep_graph_launch(pimc_ctx* ctx, int32_t graph_id, int32_t num_sweeps) {
+  if (!ctx || graph_id < 0 || graph_id >= (int)ctx->sweeps.size() || !ctx->sweeps[graph_id].live || num_sweeps < 0)
+    return fail(PIMC_ERR_ARG, "sweep_graph_launch: bad argument");
+  OracleSweep& g = ctx->sweeps[graph_id];
+  if (g.first_move + (int64_t)g.num_moves * (g.sweeps + num_sweeps) > (int64_t)0xffffffffLL)
+    return fail(PIMC_ERR_ARG, "sweep_graph_launch: move numbers beyond 2^32");
+  for (int i = 0; i < num_sweeps; i++) {
+    int rc = pimc_bisection_run(ctx, g.replicas.data(), (int32_t)g.replicas.size(), &g.desc,
+                                g.first_move + g.sweeps * g.num_moves, g.num_moves, g.precision, nullptr, nullptr);
+    if (rc) return rc;
+    g.sweeps++;
+  }
+  return PIMC_OK;
+}
+
+int pimc_sweep_graph_destroy(pimc_ctx* ctx, int32_t graph_id) {
+  if (!ctx || graph_id < 0 || graph_id >= (int)ctx->sweeps.size() || !ctx->sweeps[graph_id].live)
+    return fail(PIMC_ERR_ARG, "sweep_graph_destroy: bad argument");
+  ctx->sweeps[graph_id].live = false;
   return PIMC_OK;
 }
 

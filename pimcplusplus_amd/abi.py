@@ -126,6 +126,11 @@ class BisectionStats(C.Structure):
     ]
 
 
+class BisectionPlan(C.Structure):
+    _fields_ = [("block_threads", C.c_int32), ("link_split", C.c_int32), ("persistent", C.c_int32),
+                ("reserved", C.c_int32), ("trial_ms", C.c_double)]
+
+
 # name -> (restype, argtypes); every pimc_* entry point of include/pimc_gpu.h
 _CTX = C.c_void_p
 SIGNATURES = {
@@ -162,6 +167,18 @@ SIGNATURES = {
         [_CTX, _i32p, C.c_int32, C.POINTER(BisectionDesc), C.c_int64, C.c_int32, C.c_int32,
          C.POINTER(MoveRecord), C.POINTER(BisectionStats)]),
     "pimc_bisection_stats_take": (C.c_int, [_CTX, _i32p, C.c_int32, C.POINTER(BisectionStats)]),
+    "pimc_bisection_set_plan": (C.c_int, [_CTX, C.POINTER(BisectionPlan)]),
+    "pimc_bisection_get_plan": (C.c_int, [_CTX, C.POINTER(BisectionPlan)]),
+    "pimc_bisection_tune": (
+        C.c_int,
+        [_CTX, _i32p, C.c_int32, C.POINTER(BisectionDesc), C.c_int64, C.c_int32, C.c_int32,
+         C.POINTER(BisectionPlan)]),
+    "pimc_sweep_graph_create": (
+        C.c_int,
+        [_CTX, _i32p, C.c_int32, C.POINTER(BisectionDesc), C.c_int64, C.c_int32, C.c_int32,
+         C.POINTER(C.c_int32)]),
+    "pimc_sweep_graph_launch": (C.c_int, [_CTX, C.c_int32, C.c_int32]),
+    "pimc_sweep_graph_destroy": (C.c_int, [_CTX, C.c_int32]),
 }
 
 
@@ -442,6 +459,40 @@ class PimcLib:
             self.ctx, _p(reps, _i32p), reps.size, C.byref(d), first_move, num_moves, precision,
             rec, C.byref(st) if st is not None else None))
         return (records_to_numpy(rec, reps.size, num_moves) if rec is not None else None), st
+
+    def set_plan(self, block_threads: int = 256, link_split: int = 1, persistent: bool = False):
+        p = BisectionPlan(block_threads=block_threads, link_split=link_split, persistent=int(persistent),
+                          reserved=0, trial_ms=0.0)
+        self._check("pimc_bisection_set_plan", self.lib.pimc_bisection_set_plan(self.ctx, C.byref(p)))
+
+    def get_plan(self) -> BisectionPlan:
+        p = BisectionPlan()
+        self._check("pimc_bisection_get_plan", self.lib.pimc_bisection_get_plan(self.ctx, C.byref(p)))
+        return p
+
+    def bisection_tune(self, replicas, num_levels: int, seed: int, first_move: int, trial_moves: int,
+                       precision: int = FP64, species: int = 0, lowest_level: int = 0) -> BisectionPlan:
+        reps = _i32(replicas)
+        d = BisectionDesc(num_levels=num_levels, lowest_level=lowest_level, species=species, reserved=0, seed=seed)
+        p = BisectionPlan()
+        self._check("pimc_bisection_tune", self.lib.pimc_bisection_tune(
+            self.ctx, _p(reps, _i32p), reps.size, C.byref(d), first_move, trial_moves, precision, C.byref(p)))
+        return p
+
+    def sweep_graph_create(self, replicas, num_levels: int, seed: int, first_move: int, num_moves: int,
+                           precision: int = FP64, species: int = 0, lowest_level: int = 0) -> int:
+        reps = _i32(replicas)
+        d = BisectionDesc(num_levels=num_levels, lowest_level=lowest_level, species=species, reserved=0, seed=seed)
+        gid = C.c_int32()
+        self._check("pimc_sweep_graph_create", self.lib.pimc_sweep_graph_create(
+            self.ctx, _p(reps, _i32p), reps.size, C.byref(d), first_move, num_moves, precision, C.byref(gid)))
+        return gid.value
+
+    def sweep_graph_launch(self, graph: int, num_sweeps: int = 1):
+        self._check("pimc_sweep_graph_launch", self.lib.pimc_sweep_graph_launch(self.ctx, graph, num_sweeps))
+
+    def sweep_graph_destroy(self, graph: int):
+        self._check("pimc_sweep_graph_destroy", self.lib.pimc_sweep_graph_destroy(self.ctx, graph))
 
     def bisection_stats_take(self, replicas) -> BisectionStats:
         reps = _i32(replicas)

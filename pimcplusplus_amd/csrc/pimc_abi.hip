@@ -231,6 +231,22 @@ void setup_kvecs(HostReplica& H, double kcut) {
 
 }  // namespace
 
+// A captured sweep and the device state its replays use.
+struct SweepGraph {
+  DevBuf ids, pos, stats, queue, first;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int num_moves = 0;
+  int64_t first_move = 0, sweeps = 0;
+  SweepGraph() = default;
+  SweepGraph(const SweepGraph&) = delete;
+  SweepGraph& operator=(const SweepGraph&) = delete;
+  ~SweepGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
 struct pimc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -242,17 +258,28 @@ struct pimc_ctx {
   // bisection counters accumulated per replica id ([acc_cap][5]) until pimc_bisection_stats_take
   DevBuf d_acc;
   int acc_cap = 0;
-  std::vector<int> h_repids;                 // replica list of the last upload to d_repids
+  std::vector<int> h_repids, h_reppos;       // launch list of the last upload to d_repids / d_reppos
+  DevBuf d_reppos, d_queue;
+  pimc_bisection_plan plan{256, 1, 0, 0, 0.0};   // launch shape of pimc_bisection_run
+  std::vector<std::unique_ptr<SweepGraph>> graphs;
   // event pairs around bisection launches since the last take, and a pool of spare ones
   std::vector<std::pair<hipEvent_t, hipEvent_t>> timed, spare;
 
   // Host staging of the request upload; kept alive until the call that uses it synchronizes.
   std::vector<ReqDev> h_reqs;
 
+  // A live sweep graph holds device addresses of the descriptor arrays and counters.
+  bool graphs_alive() const {
+    for (auto& g : graphs)
+      if (g) return true;
+    return false;
+  }
   int sync_descriptors() {
     std::vector<FitDev> hf;
     std::vector<ReplicaDev> hr;
     if (!(fits_dirty && !fits.empty()) && !(reps_dirty && !reps.empty())) return PIMC_OK;
+    if (graphs_alive() && (fits.size() * sizeof(FitDev) > d_fits.bytes || reps.size() * sizeof(ReplicaDev) > d_reps.bytes))
+      return fail(PIMC_ERR_STATE, "fits or replicas were added while a sweep graph exists; destroy the graphs first");
     if (fits_dirty && !fits.empty()) {
       for (auto& f : fits) hf.push_back(f->dev);
       int rc = d_fits.ensure(hf.size() * sizeof(FitDev));
@@ -962,92 +989,130 @@ int pimc_dist_disp(pimc_ctx* ctx, int32_t replica, int32_t mode, int64_t n, cons
   return PIMC_OK;
 }
 
-int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, const pimc_bisection_desc* desc,
-                       int64_t first_move, int32_t num_moves, int32_t precision, pimc_move_record* records,
-                       pimc_bisection_stats* stats) {
-  if (!ctx || !desc || nrep < 0 || (nrep > 0 && !replicas) || num_moves < 0 || first_move < 0 ||
-      first_move + num_moves > (int64_t)0xffffffffLL)
-    return fail(PIMC_ERR_ARG, "bisection_run: bad argument");
-  if (precision != PIMC_FP64 && precision != PIMC_MIXED) return fail(PIMC_ERR_ARG, "bisection_run: bad precision");
+}  // extern "C"
+
+namespace {
+
+// What a bisection launch over a replica list needs, validated.
+struct BisectionPrep {
+  int nt = 0;
+  bool lean = false;
+  size_t table_bytes = 0, lr_bytes = 0;
+  std::vector<int> order;    // list positions, longest replica first (persistent launches)
+  std::vector<int> ids;      // replica ids in launch order
+};
+
+int prepare_bisection(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, const pimc_bisection_desc* desc,
+                      int32_t precision, BisectionPrep& P) {
+  if (precision != PIMC_FP64 && precision != PIMC_MIXED) return fail(PIMC_ERR_ARG, "bisection: bad precision");
   if (desc->num_levels < 1 || desc->num_levels > PIMC_MAX_LEVELS || desc->lowest_level < 0 ||
       desc->lowest_level >= desc->num_levels)
-    return fail(PIMC_ERR_ARG, "bisection_run: bad levels");
+    return fail(PIMC_ERR_ARG, "bisection: bad levels");
   std::set<int> seen;
   int nt = -1;
   size_t table_bytes = 0, table_bytes_lin = 0, lr_bytes = 0;
   bool has_bc = false, has_lr = false;
   for (int r = 0; r < nrep; r++) {
     HostReplica* H = ctx->rep(replicas[r]);
-    if (!H) return fail(PIMC_ERR_ARG, "bisection_run: bad replica");
-    if (!seen.insert(replicas[r]).second) return fail(PIMC_ERR_ARG, "bisection_run: replica listed twice");
+    if (!H) return fail(PIMC_ERR_ARG, "bisection: bad replica");
+    if (!seen.insert(replicas[r]).second) return fail(PIMC_ERR_ARG, "bisection: replica listed twice");
     const RepCore& R = H->dev.core;
-    if ((1 << desc->num_levels) >= R.nslices) return fail(PIMC_ERR_ARG, "bisection_run: too few slices for num_levels");
+    if ((1 << desc->num_levels) >= R.nslices) return fail(PIMC_ERR_ARG, "bisection: too few slices for num_levels");
     if (R.lr_on) {
-      if (desc->lowest_level >= R.lr_levels) return fail(PIMC_ERR_ARG, "bisection_run: long-range tables lack the lowest level");
+      if (desc->lowest_level >= R.lr_levels) return fail(PIMC_ERR_ARG, "bisection: long-range tables lack the lowest level");
       lr_bytes = std::max(lr_bytes, (size_t)2 * (2 * (R.maxk0 + R.maxk1 + R.maxk2) + 3) * sizeof(d2v));
       has_lr = true;
     }
     if (desc->species < 0 || desc->species >= R.ns || H->dev.sp.count[desc->species] < 1)
-      return fail(PIMC_ERR_ARG, "bisection_run: bad species");
+      return fail(PIMC_ERR_ARG, "bisection: bad species");
     for (int f = 0; f < R.ns * R.ns; f++) {
       const HostFit& F = *ctx->fits[H->dev.sp.fit[f]];
       if (F.num_terms >= 0) {
         if (nt < 0) nt = F.num_terms;
-        if (F.num_terms != nt) return fail(PIMC_ERR_UNSUPPORTED, "bisection_run: fits with different num_terms");
+        if (F.num_terms != nt) return fail(PIMC_ERR_UNSUPPORTED, "bisection: fits with different num_terms");
         table_bytes = std::max(table_bytes, precision == PIMC_MIXED ? F.staged32 : F.staged64);
         table_bytes_lin = std::max(table_bytes_lin, precision == PIMC_MIXED ? F.staged32_lin : F.staged64_lin);
       }
-      if (desc->num_levels > F.num_levels()) return fail(PIMC_ERR_ARG, "bisection_run: pair fit has too few tau levels");
+      if (desc->num_levels > F.num_levels()) return fail(PIMC_ERR_ARG, "bisection: pair fit has too few tau levels");
       has_bc = has_bc || F.dev.kind == 1 || F.dev.grid_type != PIMC_GRID_LINEAR;
       if (F.dev.kind == 1 && R.use_long_range && (!((R.ulong_mask >> f) & 1u) || desc->num_levels > R.ulong_levels))
-        return fail(PIMC_ERR_STATE, "bisection_run: long-range pair fit without Ulong tables");
+        return fail(PIMC_ERR_STATE, "bisection: long-range pair fit without Ulong tables");
     }
   }
-  if (nt < 0) nt = 0;
-  if (stats) memset(stats, 0, sizeof *stats);
-  if (nrep == 0 || num_moves == 0) return PIMC_OK;
-  HIP_TRY(hipSetDevice(ctx->device));
-  int rc = ctx->sync_descriptors();
+  P.nt = nt < 0 ? 0 : nt;
+  P.lean = !has_bc && !has_lr;
+  P.table_bytes = P.lean ? table_bytes_lin : table_bytes;
+  P.lr_bytes = lr_bytes;
+  // longest first: the short-range work of a move grows with N, the long-range work with nk
+  P.order.resize(nrep);
+  for (int r = 0; r < nrep; r++) P.order[r] = r;
+  if (ctx->plan.persistent) {
+    auto cost = [&](int r) {
+      const RepCore& R = ctx->rep(replicas[r])->dev.core;
+      return (double)R.N + (R.lr_on ? 0.05 * R.nk : 0.0);
+    };
+    std::stable_sort(P.order.begin(), P.order.end(), [&](int x, int y) { return cost(x) > cost(y); });
+  }
+  P.ids.resize(nrep);
+  for (int i = 0; i < nrep; i++) P.ids[i] = replicas[P.order[i]];
+  return PIMC_OK;
+}
+
+// Grows the per-replica counters to cover every replica, keeping their values.
+int ensure_acc(pimc_ctx* ctx) {
+  if ((int)ctx->reps.size() <= ctx->acc_cap) return PIMC_OK;
+  if (ctx->graphs_alive())
+    return fail(PIMC_ERR_STATE, "replicas were added while a sweep graph exists; destroy the graphs first");
+  const int cap = std::max((int)ctx->reps.size(), 2 * ctx->acc_cap);
+  DevBuf grown;
+  int rc = grown.ensure((size_t)cap * 5 * sizeof(int64_t));
   if (rc) return rc;
-  if ((rc = ctx->d_stats.ensure((size_t)nrep * 5 * sizeof(int64_t)))) return rc;
-  if (records && (rc = ctx->d_records.ensure((size_t)nrep * num_moves * sizeof(pimc_move_record)))) return rc;
-  if ((int)ctx->reps.size() > ctx->acc_cap) {
-    // grow the per-replica counters, keeping what earlier calls accumulated
-    const int cap = std::max((int)ctx->reps.size(), 2 * ctx->acc_cap);
-    DevBuf grown;
-    if ((rc = grown.ensure((size_t)cap * 5 * sizeof(int64_t)))) return rc;
-    HIP_TRY(hipMemsetAsync(grown.p, 0, (size_t)cap * 5 * sizeof(int64_t), ctx->stream));
-    if (ctx->acc_cap)
-      HIP_TRY(hipMemcpyAsync(grown.p, ctx->d_acc.p, (size_t)ctx->acc_cap * 5 * sizeof(int64_t),
-                             hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    ctx->d_acc.swap(grown);
-    ctx->acc_cap = cap;
-  }
-  if (ctx->h_repids.size() != (size_t)nrep || !std::equal(replicas, replicas + nrep, ctx->h_repids.begin())) {
-    // the replica list is uploaded once and reused while it does not change
-    if ((rc = ctx->d_repids.ensure((size_t)nrep * sizeof(int)))) return rc;
-    HIP_TRY(hipStreamSynchronize(ctx->stream));   // earlier launches may still read the old list
-    ctx->h_repids.assign(replicas, replicas + nrep);
-    HIP_TRY(hipMemcpy(ctx->d_repids.p, ctx->h_repids.data(), (size_t)nrep * sizeof(int), hipMemcpyHostToDevice));
-  }
+  HIP_TRY(hipMemsetAsync(grown.p, 0, (size_t)cap * 5 * sizeof(int64_t), ctx->stream));
+  if (ctx->acc_cap)
+    HIP_TRY(hipMemcpyAsync(grown.p, ctx->d_acc.p, (size_t)ctx->acc_cap * 5 * sizeof(int64_t), hipMemcpyDeviceToDevice,
+                           ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->d_acc.swap(grown);
+  ctx->acc_cap = cap;
+  return PIMC_OK;
+}
+
+// Launch arguments over device buffers that stay valid while `ids`/`pos` do.
+LaunchBisection make_launch(pimc_ctx* ctx, const BisectionPrep& P, const pimc_bisection_desc* desc, int nrep,
+                            uint32_t first_move, int num_moves, const int* d_ids, const int* d_pos, int64_t* d_stats,
+                            pimc_move_record* d_records, int* d_queue, const uint32_t* d_first) {
   LaunchBisection a{};
   a.reps = ctx->d_reps.as<ReplicaDev>();
   a.fits = ctx->d_fits.as<FitDev>();
-  a.rep_ids = ctx->d_repids.as<int>();
+  a.rep_ids = d_ids;
+  a.rep_pos = d_pos;
   a.nrep = nrep;
   a.num_levels = desc->num_levels;
   a.lowest_level = desc->lowest_level;
   a.species = desc->species;
   a.seed = desc->seed;
-  a.first_move = (uint32_t)first_move;
+  a.first_move = first_move;
   a.num_moves = num_moves;
-  a.records = records ? ctx->d_records.as<pimc_move_record>() : nullptr;
-  a.stats = ctx->d_stats.as<int64_t>();
+  a.records = d_records;
+  a.stats = d_stats;
   a.acc = ctx->d_acc.as<int64_t>();
-  const bool lean = !has_bc && !has_lr;
-  a.lds_table_bytes = lean ? table_bytes_lin : table_bytes;
-  a.lds_lr_bytes = lr_bytes;
+  a.lds_table_bytes = P.table_bytes;
+  a.lds_lr_bytes = P.lr_bytes;
+  a.link_split = ctx->plan.link_split;
+  a.queue = ctx->plan.persistent ? d_queue : nullptr;
+  a.first_move_dev = d_first;
+  return a;
+}
+
+int persistent_grid(pimc_ctx* ctx, const BisectionPrep& P, const LaunchBisection& a, int precision, int nrep) {
+  const int per_cu = bisection_blocks_per_cu(P.nt, precision == PIMC_MIXED, P.lean, ctx->plan.block_threads,
+                                             bisection_lds_bytes_for(a));
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess || per_cu <= 0) return nrep;
+  return std::max(1, std::min(nrep, per_cu * prop.multiProcessorCount));
+}
+
+int launch_timed(pimc_ctx* ctx, const BisectionPrep& P, const LaunchBisection& a, int precision, int nrep) {
   std::pair<hipEvent_t, hipEvent_t> ev;
   if (!ctx->spare.empty()) {
     ev = ctx->spare.back();
@@ -1057,10 +1122,50 @@ int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, con
     HIP_TRY(hipEventCreate(&ev.second));
   }
   ctx->timed.push_back(ev);
+  const int grid = ctx->plan.persistent ? persistent_grid(ctx, P, a, precision, nrep) : nrep;
   HIP_TRY(hipEventRecord(ev.first, ctx->stream));
-  hipError_t e = launch_bisection(nt, precision == PIMC_MIXED, lean, a, ctx->stream);
+  hipError_t e = launch_bisection(P.nt, precision == PIMC_MIXED, P.lean, a, ctx->plan.block_threads, grid, ctx->stream);
   if (e != hipSuccess) return fail(PIMC_ERR_DEVICE, std::string("bisection launch: ") + hipGetErrorString(e));
   HIP_TRY(hipEventRecord(ev.second, ctx->stream));
+  return PIMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, const pimc_bisection_desc* desc,
+                       int64_t first_move, int32_t num_moves, int32_t precision, pimc_move_record* records,
+                       pimc_bisection_stats* stats) {
+  if (!ctx || !desc || nrep < 0 || (nrep > 0 && !replicas) || num_moves < 0 || first_move < 0 ||
+      first_move + num_moves > (int64_t)0xffffffffLL)
+    return fail(PIMC_ERR_ARG, "bisection_run: bad argument");
+  BisectionPrep P;
+  int rc = prepare_bisection(ctx, replicas, nrep, desc, precision, P);
+  if (rc) return rc;
+  if (stats) memset(stats, 0, sizeof *stats);
+  if (nrep == 0 || num_moves == 0) return PIMC_OK;
+  HIP_TRY(hipSetDevice(ctx->device));
+  if ((rc = ctx->sync_descriptors())) return rc;
+  if ((rc = ensure_acc(ctx))) return rc;
+  if ((rc = ctx->d_stats.ensure((size_t)nrep * 5 * sizeof(int64_t)))) return rc;
+  if (records && (rc = ctx->d_records.ensure((size_t)nrep * num_moves * sizeof(pimc_move_record)))) return rc;
+  if ((rc = ctx->d_queue.ensure(sizeof(int)))) return rc;
+  if (ctx->h_repids != P.ids || ctx->h_reppos != P.order) {
+    // the launch list is uploaded once and reused while it does not change
+    if ((rc = ctx->d_repids.ensure((size_t)nrep * sizeof(int))) || (rc = ctx->d_reppos.ensure((size_t)nrep * sizeof(int))))
+      return rc;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));   // earlier launches may still read the old list
+    ctx->h_repids = P.ids;
+    ctx->h_reppos = P.order;
+    HIP_TRY(hipMemcpy(ctx->d_repids.p, P.ids.data(), (size_t)nrep * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ctx->d_reppos.p, P.order.data(), (size_t)nrep * sizeof(int), hipMemcpyHostToDevice));
+  }
+  const LaunchBisection a =
+      make_launch(ctx, P, desc, nrep, (uint32_t)first_move, num_moves, ctx->d_repids.as<int>(), ctx->d_reppos.as<int>(),
+                  ctx->d_stats.as<int64_t>(), records ? ctx->d_records.as<pimc_move_record>() : nullptr,
+                  ctx->d_queue.as<int>(), nullptr);
+  if ((rc = launch_timed(ctx, P, a, precision, nrep))) return rc;
   if (!records && !stats) return PIMC_OK;   // asynchronous: counters via pimc_bisection_stats_take
   if (records)
     HIP_TRY(hipMemcpyAsync(records, ctx->d_records.p, (size_t)nrep * num_moves * sizeof(pimc_move_record),
@@ -1070,6 +1175,7 @@ int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, con
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   if (stats) {
     float ms = 0.0f;
+    const auto& ev = ctx->timed.back();
     HIP_TRY(hipEventElapsedTime(&ms, ev.first, ev.second));
     stats->device_ms = ms;
     for (int r = 0; r < nrep; r++) {
@@ -1080,6 +1186,186 @@ int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, con
       stats->k_evals += st[5 * r + 4];
     }
   }
+  return PIMC_OK;
+}
+
+int pimc_bisection_set_plan(pimc_ctx* ctx, const pimc_bisection_plan* plan) {
+  if (!ctx || !plan) return fail(PIMC_ERR_ARG, "bisection_set_plan: NULL argument");
+  if (plan->block_threads != 256 && plan->block_threads != 512)
+    return fail(PIMC_ERR_ARG, "bisection_set_plan: block_threads must be 256 or 512");
+  if (plan->link_split != 1 && plan->link_split != 2 && plan->link_split != 4)
+    return fail(PIMC_ERR_ARG, "bisection_set_plan: link_split must be 1, 2 or 4");
+  ctx->plan = *plan;
+  ctx->plan.persistent = plan->persistent ? 1 : 0;
+  return PIMC_OK;
+}
+
+int pimc_bisection_get_plan(pimc_ctx* ctx, pimc_bisection_plan* plan) {
+  if (!ctx || !plan) return fail(PIMC_ERR_ARG, "bisection_get_plan: NULL argument");
+  *plan = ctx->plan;
+  return PIMC_OK;
+}
+
+int pimc_bisection_tune(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, const pimc_bisection_desc* desc,
+                        int64_t first_move, int32_t trial_moves, int32_t precision, pimc_bisection_plan* out) {
+  if (!ctx || !desc || nrep <= 0 || !replicas || trial_moves <= 0 || first_move < 0 ||
+      first_move + trial_moves > (int64_t)0xffffffffLL)
+    return fail(PIMC_ERR_ARG, "bisection_tune: bad argument");
+  BisectionPrep P0;
+  int rc = prepare_bisection(ctx, replicas, nrep, desc, precision, P0);
+  if (rc) return rc;
+  HIP_TRY(hipSetDevice(ctx->device));
+  if ((rc = ctx->sync_descriptors())) return rc;
+  if ((rc = ensure_acc(ctx))) return rc;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  // save what trial moves change: both path copies, permutations, rho_k, the counters
+  auto state_of = [&](int r) {
+    HostReplica* H = ctx->rep(replicas[r]);
+    return std::vector<DevBuf*>{&H->path0, &H->path1, &H->perm0, &H->perm1, &H->rho0, &H->rho1};
+  };
+  std::vector<std::unique_ptr<DevBuf>> saved;
+  for (int r = 0; r < nrep; r++)
+    for (DevBuf* b : state_of(r)) {
+      saved.push_back(std::make_unique<DevBuf>());
+      if (b->bytes) {
+        if ((rc = saved.back()->ensure(b->bytes))) return rc;
+        HIP_TRY(hipMemcpyAsync(saved.back()->p, b->p, b->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+      }
+    }
+  std::vector<int64_t> acc((size_t)ctx->acc_cap * 5);
+  HIP_TRY(hipMemcpyAsync(acc.data(), ctx->d_acc.p, acc.size() * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  auto restore = [&]() -> int {
+    size_t i = 0;
+    for (int r = 0; r < nrep; r++)
+      for (DevBuf* b : state_of(r)) {
+        const DevBuf& sv = *saved[i++];
+        if (b->bytes) HIP_TRY(hipMemcpyAsync(b->p, sv.p, b->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+      }
+    HIP_TRY(hipMemcpyAsync(ctx->d_acc.p, acc.data(), acc.size() * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PIMC_OK;
+  };
+  const pimc_bisection_plan keep = ctx->plan;
+  pimc_bisection_plan best = keep;
+  double best_ms = 1e300;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  for (int block : {256, 512})
+    for (int ls : {1, 2, 4})
+      for (int pers : {0, 1}) {
+        ctx->plan = pimc_bisection_plan{block, ls, pers, 0, 0.0};
+        BisectionPrep P;
+        if ((rc = prepare_bisection(ctx, replicas, nrep, desc, precision, P))) break;
+        DevBuf ids, pos, st, q;
+        if ((rc = ids.ensure((size_t)nrep * 4)) || (rc = pos.ensure((size_t)nrep * 4)) ||
+            (rc = st.ensure((size_t)nrep * 40)) || (rc = q.ensure(4)))
+          break;
+        HIP_TRY(hipMemcpy(ids.p, P.ids.data(), (size_t)nrep * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(pos.p, P.order.data(), (size_t)nrep * 4, hipMemcpyHostToDevice));
+        const LaunchBisection a = make_launch(ctx, P, desc, nrep, (uint32_t)first_move, trial_moves, ids.as<int>(),
+                                              pos.as<int>(), st.as<int64_t>(), nullptr, q.as<int>(), nullptr);
+        const int grid = pers ? persistent_grid(ctx, P, a, precision, nrep) : nrep;
+        HIP_TRY(hipEventRecord(e0, ctx->stream));
+        hipError_t e = launch_bisection(P.nt, precision == PIMC_MIXED, P.lean, a, block, grid, ctx->stream);
+        if (e != hipSuccess) {
+          rc = fail(PIMC_ERR_DEVICE, std::string("bisection_tune launch: ") + hipGetErrorString(e));
+          break;
+        }
+        HIP_TRY(hipEventRecord(e1, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = ctx->plan;
+          best.trial_ms = ms;
+        }
+        if ((rc = restore())) break;
+      }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) {
+    ctx->plan = keep;
+    (void)restore();
+    return rc;
+  }
+  ctx->plan = best;
+  if (out) *out = best;
+  return PIMC_OK;
+}
+
+int pimc_sweep_graph_create(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, const pimc_bisection_desc* desc,
+                            int64_t first_move, int32_t num_moves, int32_t precision, int32_t* graph_id) {
+  if (!ctx || !desc || nrep <= 0 || !replicas || num_moves <= 0 || first_move < 0 || !graph_id ||
+      first_move > (int64_t)0xffffffffLL)
+    return fail(PIMC_ERR_ARG, "sweep_graph_create: bad argument");
+  BisectionPrep P;
+  int rc = prepare_bisection(ctx, replicas, nrep, desc, precision, P);
+  if (rc) return rc;
+  HIP_TRY(hipSetDevice(ctx->device));
+  if ((rc = ctx->sync_descriptors())) return rc;
+  if ((rc = ensure_acc(ctx))) return rc;
+  auto g = std::make_unique<SweepGraph>();
+  if ((rc = g->ids.ensure((size_t)nrep * 4)) || (rc = g->pos.ensure((size_t)nrep * 4)) ||
+      (rc = g->stats.ensure((size_t)nrep * 40)) || (rc = g->queue.ensure(4)) || (rc = g->first.ensure(4)))
+    return rc;
+  const uint32_t f0 = (uint32_t)first_move;
+  HIP_TRY(hipMemcpy(g->ids.p, P.ids.data(), (size_t)nrep * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(g->pos.p, P.order.data(), (size_t)nrep * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(g->first.p, &f0, 4, hipMemcpyHostToDevice));
+  const LaunchBisection a = make_launch(ctx, P, desc, nrep, 0u, num_moves, g->ids.as<int>(), g->pos.as<int>(),
+                                        g->stats.as<int64_t>(), nullptr, g->queue.as<int>(), g->first.as<uint32_t>());
+  const int grid = ctx->plan.persistent ? persistent_grid(ctx, P, a, precision, nrep) : nrep;
+  // one sweep: [queue reset] -> bisection moves -> first_move += num_moves; kernels only
+  HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = launch_bisection(P.nt, precision == PIMC_MIXED, P.lean, a, ctx->plan.block_threads, grid, ctx->stream);
+  if (e == hipSuccess) e = launch_sweep_advance(g->first.as<uint32_t>(), num_moves, ctx->stream);
+  hipGraph_t graph = nullptr;
+  hipError_t e2 = hipStreamEndCapture(ctx->stream, &graph);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return fail(PIMC_ERR_DEVICE, std::string("sweep_graph_create: capture failed: ") +
+                                     hipGetErrorString(e != hipSuccess ? e : e2));
+  }
+  g->graph = graph;
+  HIP_TRY(hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0));
+  g->num_moves = num_moves;
+  g->first_move = first_move;
+  ctx->graphs.push_back(std::move(g));
+  *graph_id = (int32_t)ctx->graphs.size() - 1;
+  return PIMC_OK;
+}
+
+int pimc_sweep_graph_launch(pimc_ctx* ctx, int32_t graph_id, int32_t num_sweeps) {
+  if (!ctx || graph_id < 0 || graph_id >= (int)ctx->graphs.size() || !ctx->graphs[graph_id] || num_sweeps < 0)
+    return fail(PIMC_ERR_ARG, "sweep_graph_launch: bad argument");
+  SweepGraph& g = *ctx->graphs[graph_id];
+  if (g.first_move + (int64_t)g.num_moves * (g.sweeps + num_sweeps) > (int64_t)0xffffffffLL)
+    return fail(PIMC_ERR_ARG, "sweep_graph_launch: move numbers beyond 2^32");
+  HIP_TRY(hipSetDevice(ctx->device));
+  std::pair<hipEvent_t, hipEvent_t> ev;
+  if (!ctx->spare.empty()) {
+    ev = ctx->spare.back();
+    ctx->spare.pop_back();
+  } else {
+    HIP_TRY(hipEventCreate(&ev.first));
+    HIP_TRY(hipEventCreate(&ev.second));
+  }
+  ctx->timed.push_back(ev);
+  HIP_TRY(hipEventRecord(ev.first, ctx->stream));
+  for (int i = 0; i < num_sweeps; i++) HIP_TRY(hipGraphLaunch(g.exec, ctx->stream));
+  HIP_TRY(hipEventRecord(ev.second, ctx->stream));
+  g.sweeps += num_sweeps;
+  return PIMC_OK;
+}
+
+int pimc_sweep_graph_destroy(pimc_ctx* ctx, int32_t graph_id) {
+  if (!ctx || graph_id < 0 || graph_id >= (int)ctx->graphs.size() || !ctx->graphs[graph_id])
+    return fail(PIMC_ERR_ARG, "sweep_graph_destroy: bad argument");
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->graphs[graph_id].reset();
   return PIMC_OK;
 }
 

@@ -85,12 +85,15 @@ __device__ inline double sr_link(const RepCore& R, const FitDev& F, const Table&
 }
 
 // ShortRange partner loop of one stage (ShortRangeClass.cc:149-169) for partners
-// [j0, j1): each thread walks one partner's K+1 slices in order, OLD and NEW
-// copies of the moved particle side by side, two links per iteration (K is a
-// power of two >= 2) with the partner's beads loaded two links ahead.
+// [j0, j1): each thread walks one partner's links in order, OLD and NEW copies
+// of the moved particle side by side, two links per iteration with the
+// partner's beads loaded two links ahead.  `ls` threads share a partner, each
+// taking a contiguous range of K/ls links (at least two; K is a power of two):
+// the block covers blockDim/ls partners per pass.  Consecutive lanes take
+// consecutive partners, so bead loads coalesce.
 template <int NT, bool MIXED, bool BC, bool LIN, typename Table>
 __device__ inline void sr_partners(const RepCore& R, const FitDev& F, const Table& T, const BcView& Q,
-                                   const double* ul, int j0, int j1, int ptcl, int slice1, int K, int step,
+                                   const double* ul, int j0, int j1, int ptcl, int slice1, int K, int step, int ls,
                                    const PIMC_LDS double* mold, const PIMC_LDS double* mnew, Kahan& acc_old,
                                    Kahan& acc_new) {
   const size_t row = (size_t)3 * R.N * step;     // distance between consecutive link slices
@@ -116,18 +119,26 @@ __device__ inline void sr_partners(const RepCore& R, const FitDev& F, const Tabl
       acc_new.s += un;
     }
   };
-  for (int j = j0 + (int)threadIdx.x; j < j1; j += blockDim.x) {
+  const int ppass = (int)blockDim.x / ls;             // partners per pass
+  const int sub = (int)threadIdx.x / ppass;           // this thread's link range
+  const int split = (K / 2 >= ls) ? ls : (K / 2);     // ranges in use: each >= 2 links
+  if (sub >= split) return;
+  const int kper = K / split;
+  const int kbeg = sub * kper, kend = kbeg + kper;
+  for (int j = j0 + (int)threadIdx.x % ppass; j < j1; j += ppass) {
     if (j == ptcl) continue;
-    const PIMC_GLOBAL double* pj = (const PIMC_GLOBAL double*)R.path0 + (size_t)slice1 * 3 * R.N + j;
+    const PIMC_GLOBAL double* pj = (const PIMC_GLOBAL double*)R.path0 + (size_t)(slice1 + kbeg * step) * 3 * R.N + j;
     double p0[3], p1[3], p2[3];
     load(pj, p0);
     load(pj + row, p1);
     load(pj + 2 * row, p2);
+    const PIMC_LDS double* mo0 = mold + 3 * kbeg * step;
+    const PIMC_LDS double* mn0 = mnew + 3 * kbeg * step;
     double dO[3], dN[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
 #pragma clang fp contract(off)
-      const double ao = p0[d] - mold[d], an = p0[d] - mnew[d];
+      const double ao = p0[d] - mo0[d], an = p0[d] - mn0[d];
       dO[d] = ao + (-rint(ao * R.boxinv[d])) * pbox[d];
       dN[d] = an + (-rint(an * R.boxinv[d])) * pbox[d];
     }
@@ -145,11 +156,11 @@ __device__ inline void sr_partners(const RepCore& R, const FitDev& F, const Tabl
       lN = ulong_eval(R, ul, rN);
     }
     pj += 2 * row;
-    for (int k = 0; k < K; k += 2) {
+    for (int k = kbeg; k < kend; k += 2) {
       double p3[3] = {0.0, 0.0, 0.0}, p4[3] = {0.0, 0.0, 0.0};
-      if (k + 3 <= K) load(pj + row, p3);
+      if (k + 3 <= kend) load(pj + row, p3);
       link2(k, p1, dO, rO, uO, lO, dN, rN, uN, lN);
-      if (k + 4 <= K) load(pj + 2 * row, p4);
+      if (k + 4 <= kend) load(pj + 2 * row, p4);
       link2(k + 1, p2, dO, rO, uO, lO, dN, rN, uN, lN);
       pj += 2 * row;
 #pragma unroll
@@ -237,13 +248,19 @@ __device__ inline double log_accept_draw(uint64_t seed, uint32_t stream, uint32_
 // the long-range action, chosen at run time.
 enum { kLeanDavid = 0, kGeneric = 1 };
 
+// The moves of replica a.rep_ids[item] (one block).
+// Byte offset of the control words (slice1, ptcl, accept, queue item) in the block's LDS.
+__host__ __device__ inline size_t bisection_ctl_offset(const LaunchBisection& a) {
+  const int W = 1 << a.num_levels;
+  return a.lds_table_bytes + sizeof(double) * (6 * (W + 1) + W + 2 * W + kMaxWaves);
+}
+
 template <int NT, bool MIXED, int VAR>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PIMC_BISECTION_WAVES)))
-k_bisection(const LaunchBisection a) {
+__device__ inline void bisection_replica(const LaunchBisection& a, int item, uint32_t first_move,
+                                         PIMC_LDS unsigned char* smem) {
   constexpr bool HAS_BC = VAR == kGeneric, HAS_LR = VAR == kGeneric, LIN = VAR == kLeanDavid;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  PIMC_LDS unsigned char* smem = (PIMC_LDS unsigned char*)smem_raw;
-  const ReplicaDev* RD = a.reps + a.rep_ids[blockIdx.x];
+  const ReplicaDev* RD = a.reps + a.rep_ids[item];
+  const int pos = a.rep_pos ? a.rep_pos[item] : item;   // the caller's index of this replica
   const RepCore R = load_uniform(&RD->core);
   const RepSpecies& S = RD->sp;
   const int L = a.num_levels;
@@ -258,8 +275,8 @@ k_bisection(const LaunchBisection a) {
   PIMC_LDS double* lold = lnew + W / 2;                                    // [W/2]
   PIMC_LDS double* kold = lold + W / 2;                                    // [W]
   PIMC_LDS double* knew = kold + W;                                        // [W]
-  PIMC_LDS double* scratch = knew + W;                                     // [kWaves]
-  PIMC_LDS int* ctl = (PIMC_LDS int*)(scratch + kWaves);                  // slice1, ptcl, accept
+  PIMC_LDS double* scratch = knew + W;                                     // [kMaxWaves]
+  PIMC_LDS int* ctl = (PIMC_LDS int*)(scratch + kMaxWaves);               // slice1, ptcl, accept, item
   PIMC_LDS d2v* lrC = (PIMC_LDS d2v*)(ctl + 4);                           // [2][kstride]: NEW, OLD rows
 
   int64_t st_moves = 0, st_acc = 0, st_stages = 0, st_evals = 0, st_kevals = 0;
@@ -268,7 +285,7 @@ k_bisection(const LaunchBisection a) {
   const double lambda = S.lambda[sp];
 
   for (int m = 0; m < a.num_moves; m++) {
-    const uint32_t move = a.first_move + (uint32_t)m;
+    const uint32_t move = first_move + (uint32_t)m;
     if (tid == 0) {
       ctl[0] = (int)floor(rng_uniform(a.seed, R.stream, move, kTagSlice, 0) * numLeft);
       ctl[1] = (int)floor(rng_uniform(a.seed, R.stream, move, kTagPtcl, 0) * S.count[sp]) + S.first[sp];
@@ -324,7 +341,7 @@ k_bisection(const LaunchBisection a) {
         if constexpr (HAS_BC) {
           if (F.kind == 1) {
             const BcView Q{qgrid_of(F), level};
-            sr_partners<NT, MIXED, true, false>(R, F, LevelTable64<true>{}, Q, ul, j0, j1, ptcl, slice1, K, step, mold,
+            sr_partners<NT, MIXED, true, false>(R, F, LevelTable64<true>{}, Q, ul, j0, j1, ptcl, slice1, K, step, a.link_split, mold,
                                          mnew, acc_old, acc_new);
             continue;
           }
@@ -337,7 +354,7 @@ k_bisection(const LaunchBisection a) {
             return stage_level64<LIN>(F, level + F.taupos, tbl);
         }();
         __syncthreads();
-        sr_partners<NT, MIXED, false, LIN>(R, F, T, BcView{}, nullptr, j0, j1, ptcl, slice1, K, step, mold, mnew,
+        sr_partners<NT, MIXED, false, LIN>(R, F, T, BcView{}, nullptr, j0, j1, ptcl, slice1, K, step, a.link_split, mold, mnew,
                                       acc_old, acc_new);
       }
       const double sr_old = block_sum(acc_old.value(), scratch);
@@ -443,7 +460,7 @@ k_bisection(const LaunchBisection a) {
         st_stages++;
         st_evals += 2 * (int64_t)(R.N - 1) * K;
         if (a.records) {
-          pimc_move_record* rec = a.records + (size_t)blockIdx.x * a.num_moves + m;
+          pimc_move_record* rec = a.records + (size_t)pos * a.num_moves + m;
           rec->old_action[stage] = oldAction;
           rec->new_action[stage] = newAction;
           rec->log_sample_ratio[stage] = logSampleRatio;
@@ -500,7 +517,7 @@ k_bisection(const LaunchBisection a) {
       st_moves++;
       st_acc += accepted ? 1 : 0;
       if (a.records) {
-        pimc_move_record* rec = a.records + (size_t)blockIdx.x * a.num_moves + m;
+        pimc_move_record* rec = a.records + (size_t)pos * a.num_moves + m;
         rec->slice1 = slice1;
         rec->ptcl = ptcl;
         rec->accepted = accepted ? 1 : 0;
@@ -516,13 +533,13 @@ k_bisection(const LaunchBisection a) {
     __syncthreads();
   }
   if (tid == 0) {
-    int64_t* st = a.stats + (size_t)blockIdx.x * 5;
+    int64_t* st = a.stats + (size_t)pos * 5;
     st[0] = st_moves;
     st[1] = st_acc;
     st[2] = st_stages;
     st[3] = st_evals;
     st[4] = st_kevals;
-    int64_t* ac = a.acc + (size_t)a.rep_ids[blockIdx.x] * 5;   // one block per replica per launch
+    int64_t* ac = a.acc + (size_t)a.rep_ids[item] * 5;   // one block per replica per launch
     ac[0] += st_moves;
     ac[1] += st_acc;
     ac[2] += st_stages;
@@ -531,30 +548,99 @@ k_bisection(const LaunchBisection a) {
   }
 }
 
-static size_t bisection_lds_bytes(const LaunchBisection& a) {
-  const int W = 1 << a.num_levels;
-  return a.lds_table_bytes + sizeof(double) * (6 * (W + 1) + W + 2 * W + kWaves) + 4 * sizeof(int) + a.lds_lr_bytes;
+// One block per replica (queue == nullptr), or a persistent grid whose blocks take
+// replicas from the work queue in list order (the host lists them longest first)
+// until the list is exhausted.  first_move_dev, when set, holds the first move
+// number (a captured sweep advances it on the device).
+template <int NT, bool MIXED, int VAR, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PIMC_BISECTION_WAVES)))
+k_bisection(const LaunchBisection a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  PIMC_LDS unsigned char* smem = (PIMC_LDS unsigned char*)smem_raw;
+  PIMC_LDS int* take = (PIMC_LDS int*)(smem + bisection_ctl_offset(a)) + 3;   // control word 3
+  const uint32_t first_move = a.first_move_dev ? *a.first_move_dev : a.first_move;
+  if (!a.queue) {
+    bisection_replica<NT, MIXED, VAR>(a, blockIdx.x, first_move, smem);
+    return;
+  }
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) *take = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int item = *take;
+    if (item >= a.nrep) break;   // uniform: every wave of the block leaves here
+    bisection_replica<NT, MIXED, VAR>(a, item, first_move, smem);
+  }
 }
 
-template <bool MIXED, int VAR>
-static hipError_t launch_bis_t(int nt, const LaunchBisection& a, hipStream_t s) {
+__global__ void k_queue_reset(int* queue) {
+  if (threadIdx.x == 0) *queue = 0;
+}
+__global__ void k_sweep_advance(uint32_t* first_move, int num_moves) {
+  if (threadIdx.x == 0) *first_move += (uint32_t)num_moves;
+}
+
+static size_t bisection_lds_bytes(const LaunchBisection& a) {
+  return bisection_ctl_offset(a) + 4 * sizeof(int) + a.lds_lr_bytes;
+}
+
+template <bool MIXED, int VAR, int BLOCK>
+static hipError_t launch_bis_b(int nt, const LaunchBisection& a, int grid, hipStream_t s) {
   const size_t lds = bisection_lds_bytes(a);
-  dim3 grid(a.nrep);
   switch (nt) {
-    case 0: k_bisection<0, MIXED, VAR><<<grid, kBlock, lds, s>>>(a); break;
-    case 1: k_bisection<1, MIXED, VAR><<<grid, kBlock, lds, s>>>(a); break;
-    case 2: k_bisection<2, MIXED, VAR><<<grid, kBlock, lds, s>>>(a); break;
-    default: k_bisection<3, MIXED, VAR><<<grid, kBlock, lds, s>>>(a); break;
+    case 0: k_bisection<0, MIXED, VAR, BLOCK><<<grid, BLOCK, lds, s>>>(a); break;
+    case 1: k_bisection<1, MIXED, VAR, BLOCK><<<grid, BLOCK, lds, s>>>(a); break;
+    case 2: k_bisection<2, MIXED, VAR, BLOCK><<<grid, BLOCK, lds, s>>>(a); break;
+    default: k_bisection<3, MIXED, VAR, BLOCK><<<grid, BLOCK, lds, s>>>(a); break;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_bisection(int num_terms, bool mixed, bool lean, const LaunchBisection& a, hipStream_t s) {
+template <bool MIXED, int VAR>
+static hipError_t launch_bis_t(int nt, const LaunchBisection& a, int block, int grid, hipStream_t s) {
+  return block == 512 ? launch_bis_b<MIXED, VAR, 512>(nt, a, grid, s) : launch_bis_b<MIXED, VAR, 256>(nt, a, grid, s);
+}
+
+hipError_t launch_bisection(int num_terms, bool mixed, bool lean, const LaunchBisection& a, int block, int grid,
+                            hipStream_t s) {
   if (a.nrep <= 0 || a.num_moves <= 0) return hipSuccess;
   if (bisection_lds_bytes(a) > 64 * 1024) return hipErrorInvalidValue;
+  if (block != 256 && block != 512) return hipErrorInvalidValue;
+  if (a.link_split != 1 && a.link_split != 2 && a.link_split != 4) return hipErrorInvalidValue;
+  if (a.queue) {
+    k_queue_reset<<<1, 64, 0, s>>>(a.queue);
+  } else {
+    grid = a.nrep;
+  }
   if (lean)
-    return mixed ? launch_bis_t<true, kLeanDavid>(num_terms, a, s) : launch_bis_t<false, kLeanDavid>(num_terms, a, s);
-  return mixed ? launch_bis_t<true, kGeneric>(num_terms, a, s) : launch_bis_t<false, kGeneric>(num_terms, a, s);
+    return mixed ? launch_bis_t<true, kLeanDavid>(num_terms, a, block, grid, s)
+                 : launch_bis_t<false, kLeanDavid>(num_terms, a, block, grid, s);
+  return mixed ? launch_bis_t<true, kGeneric>(num_terms, a, block, grid, s)
+               : launch_bis_t<false, kGeneric>(num_terms, a, block, grid, s);
+}
+
+hipError_t launch_sweep_advance(uint32_t* first_move, int num_moves, hipStream_t s) {
+  k_sweep_advance<<<1, 64, 0, s>>>(first_move, num_moves);
+  return hipGetLastError();
+}
+
+size_t bisection_lds_bytes_for(const LaunchBisection& a) { return bisection_lds_bytes(a); }
+
+// Resident blocks per CU of the bisection kernel for this launch shape (0 on error).
+int bisection_blocks_per_cu(int num_terms, bool mixed, bool lean, int block, size_t lds) {
+  int n = 0;
+  hipError_t e;
+  auto q = [&](auto kern) { return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, block, lds); };
+#define PIMC_OCC(NT_, B_)                                                                                   \
+  (lean ? (mixed ? q(k_bisection<NT_, true, kLeanDavid, B_>) : q(k_bisection<NT_, false, kLeanDavid, B_>)) \
+        : (mixed ? q(k_bisection<NT_, true, kGeneric, B_>) : q(k_bisection<NT_, false, kGeneric, B_>)))
+  const int nt = num_terms > 3 ? 3 : num_terms;
+  if (block == 512)
+    e = nt == 0 ? PIMC_OCC(0, 512) : nt == 1 ? PIMC_OCC(1, 512) : nt == 2 ? PIMC_OCC(2, 512) : PIMC_OCC(3, 512);
+  else
+    e = nt == 0 ? PIMC_OCC(0, 256) : nt == 1 ? PIMC_OCC(1, 256) : nt == 2 ? PIMC_OCC(2, 256) : PIMC_OCC(3, 256);
+#undef PIMC_OCC
+  return e == hipSuccess ? n : 0;
 }
 
 }  // namespace pimc

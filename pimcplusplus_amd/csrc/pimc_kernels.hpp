@@ -8,6 +8,7 @@ namespace pimc {
 
 constexpr int kBlock = 256;          // threads per block: 4 waves
 constexpr int kWaves = kBlock / 64;
+constexpr int kMaxWaves = 8;         // largest block of the bisection kernel: 512 threads
 
 // Request of the SingleAction entry points, as uploaded to the device.
 struct ReqDev {

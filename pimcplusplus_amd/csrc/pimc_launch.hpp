@@ -56,9 +56,18 @@ struct LaunchBisection {
   int64_t* acc;               // [replica id][5]: counters accumulated over launches
   size_t lds_table_bytes;     // max staged level size over the fits used
   size_t lds_lr_bytes;        // phase tables of the long-range stage term (0 without long range)
+  int link_split;             // threads per partner in the short-range loop: 1, 2 or 4
+  const int* rep_pos;         // [nrep] caller index of list entry i (records, stats), or nullptr
+  int* queue;                 // work-queue counter of a persistent launch, or nullptr
+  const uint32_t* first_move_dev;  // first move number in device memory, or nullptr
 };
 // Returns hipSuccess or the launch error.  lean: every pair fit is a DavidPA
 // fit on a LINEAR grid and no replica evaluates the long-range action.
-hipError_t launch_bisection(int num_terms, bool mixed, bool lean, const LaunchBisection& a, hipStream_t s);
+// block: 256 or 512 threads; grid: blocks of a persistent launch (a.queue set).
+hipError_t launch_bisection(int num_terms, bool mixed, bool lean, const LaunchBisection& a, int block, int grid,
+                            hipStream_t s);
+hipError_t launch_sweep_advance(uint32_t* first_move, int num_moves, hipStream_t s);
+size_t bisection_lds_bytes_for(const LaunchBisection& a);
+int bisection_blocks_per_cu(int num_terms, bool mixed, bool lean, int block, size_t lds);
 
 }  // namespace pimc
