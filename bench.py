@@ -67,25 +67,72 @@ def load_system(lib, system):
     return ids
 
 
-def run_gpu(system, precision, steps, warmup, moves_per_step):
+def run_gpu(system, precision, steps, warmup, moves_per_step, tune=False, graph=False):
     """W untimed steps, then K steps queued back to back on the context's stream;
-    the timed region is bracketed by stream synchronisations (the stats take)."""
+    the timed region is bracketed by stream synchronisations (the stats take).
+    tune: the autotuner picks the launch plan first (on moves the chain then
+    repeats); graph: each step is a replay of one captured sweep."""
     lib = native.open_context(0)
     ids = load_system(lib, system)
     L, seed = system.num_levels, system.seed
+    plan = lib.bisection_tune(ids, L, seed, 0, moves_per_step, precision) if tune else lib.get_plan()
     move = 0
+    if graph:
+        g = lib.sweep_graph_create(ids, L, seed, 0, moves_per_step, precision)
+        step = lambda: lib.sweep_graph_launch(g, 1)  # noqa: E731
+    else:
+        def step():
+            nonlocal move
+            lib.bisection_run(ids, L, seed, move, moves_per_step, precision, records=False, wait=False)
+            move += moves_per_step
     for _ in range(warmup):
-        lib.bisection_run(ids, L, seed, move, moves_per_step, precision, records=False, wait=False)
-        move += moves_per_step
+        step()
     lib.bisection_stats_take(ids)          # waits for the warmup; clears its counters
     t0 = time.perf_counter()
     for _ in range(steps):
-        lib.bisection_run(ids, L, seed, move, moves_per_step, precision, records=False, wait=False)
-        move += moves_per_step
+        step()
     tot = lib.bisection_stats_take(ids)    # waits for the timed steps
     wall = time.perf_counter() - t0
+    if graph:
+        lib.sweep_graph_destroy(g)
     lib.close()
-    return tot, wall, tot.device_ms
+    return tot, wall, tot.device_ms, {"block_threads": plan.block_threads, "link_split": plan.link_split,
+                                      "persistent": bool(plan.persistent)}
+
+
+def other_workload(name: str, replicas: int):
+    """The configurations besides the headline one, for reports."""
+    if name == "uneven":
+        return workloads.config3(num_replicas=128)
+    if name == "hydrogen":
+        from pimcplusplus_amd.tables import load_npz
+        return workloads.config2(load_npz(os.path.join(ROOT, "tests", "golden", "testcoul_fit2.npz")),
+                                 num_replicas=replicas)
+    if name == "ocp":
+        return workloads.config4(num_replicas=16)
+    if name == "lj64":
+        return workloads.config0(num_replicas=replicas)
+    raise ValueError(name)
+
+
+def report_other(args):
+    """One JSON line for a non-headline workload: mixed and FP64 throughput, tuned plan,
+    one captured sweep per step."""
+    system = other_workload(args.workload, args.replicas)
+    out = {"metric": METRIC, "unit": "pair·slice evals/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "higher_is_better": True, "data": "synthetic",
+           "config": {"workload": system.name, "replicas": len(system.replicas),
+                      "moves_per_step": args.moves_per_step, "num_levels": system.num_levels,
+                      "driver": "tuned plan, captured sweep graph"}}
+    for name, prec in (("mixed", abi.MIXED), ("fp64", abi.FP64)):
+        tot, wall, dev_ms, plan = run_gpu(system, prec, args.steps, args.warmup, args.moves_per_step,
+                                          tune=True, graph=True)
+        out[name] = {"evals_per_s": tot.pair_evals / wall, "ms_per_step": 1e3 * wall / args.steps,
+                     "kernel_ms_avg": dev_ms / args.steps, "k_evals_per_s": tot.k_evals / wall,
+                     "acceptance": tot.accepted / max(1, tot.moves), "plan": plan}
+    out["value"] = out["mixed"]["evals_per_s"]
+    out["mixed_over_fp64"] = out["mixed"]["evals_per_s"] / out["fp64"]["evals_per_s"]
+    print(json.dumps(out))
 
 
 def delta_s_error(nrep: int = 16, nmoves: int = 40):
@@ -189,6 +236,8 @@ def main():
     ap.add_argument("--no-dserr", action="store_true", help="skip the mixed-vs-FP64 dS comparison")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run an elementwise copy of MIB MiB (read MIB, write MIB) for counter scaling")
+    ap.add_argument("--workload", default="lj256", choices=("lj256", "lj64", "uneven", "hydrogen", "ocp"),
+                    help="lj256 (default) is the headline configuration; the others print a report line")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
@@ -199,6 +248,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.workload != "lj256":
+        report_other(args)
+        return
     system = workloads.config1(num_replicas=args.replicas)
     n = system.replicas[0].num_particles
     res = {}
@@ -206,7 +258,7 @@ def main():
     if args.only:
         runs = tuple(r for r in runs if r[0] == args.only)
     for name, prec in runs:
-        tot, wall, dev_ms = run_gpu(system, prec, args.steps, args.warmup, args.moves_per_step)
+        tot, wall, dev_ms, _ = run_gpu(system, prec, args.steps, args.warmup, args.moves_per_step)
         res[name] = {
             "evals_per_s": tot.pair_evals / wall,
             "ms_per_step": 1e3 * wall / args.steps,

@@ -108,6 +108,8 @@ typedef struct pimc_david_fit_desc {
   double smallest_tau;
   double desired_tau;
   const double* y;         /* [num_tau][num_points][num_cols] */
+  const double* dy;        /* the beta-derivative table dukj in the same layout, or NULL
+                              (needed by pimc_estimator_energy only; DavidPAClass.cc:840-877) */
 } pimc_david_fit_desc;
 
 /* Creates a fit; replaces ReadPAFit + DavidPAClass::Read for table data. */
@@ -140,6 +142,7 @@ typedef struct pimc_coulomb_bc_fit_desc {
   double smallest_beta;    /* the level-0 tau */
   double z1z2;             /* Coulomb potential Z1 Z2 / r beyond the grid */
   const double* umat;      /* [num_betas][q_points][t_points] */
+  const double* dumat;     /* dU/dbeta in the same layout (dUsplines), or NULL */
 } pimc_coulomb_bc_fit_desc;
 
 int pimc_fit_create_coulomb_bc(pimc_ctx* ctx, const pimc_coulomb_bc_fit_desc* desc, int32_t* fit_id);
@@ -173,12 +176,16 @@ int pimc_replica_num_kvecs(pimc_ctx* ctx, int32_t replica, int32_t* nk);
 int pimc_replica_kvecs(pimc_ctx* ctx, int32_t replica, double* kvecs, int32_t* kidx);
 
 /* Long-range tables of species pair (sa, sb) on this replica's k set
- * (PairActionFitClass Ulong_k, Ulong_r0, Ushort_k0, Ulong_k0, PAFitBase.h:38-46).
- * ulong_k is [num_levels][nk]; the scalars are [num_levels]. */
+ * (PairActionFitClass Ulong_k, Ulong_r0, Ushort_k0, Ulong_k0, PAFitBase.h:38-46,
+ * and their beta derivatives dUlong_k, dUlong_r0, dUshort_k0, dUlong_k0).
+ * ulong_k and dulong_k are [num_levels][nk]; the scalars are [num_levels].
+ * The four derivative tables may be NULL together (needed by the energy estimator only). */
 int pimc_replica_set_long_range(pimc_ctx* ctx, int32_t replica, int32_t sa, int32_t sb,
                                 int32_t num_levels, const double* ulong_k,
                                 const double* ulong_r0, const double* ushort_k0,
-                                const double* ulong_k0);
+                                const double* ulong_k0, const double* dulong_k,
+                                const double* dulong_r0, const double* dushort_k0,
+                                const double* dulong_k0);
 
 /* Ulong(level)(r) of species pair (sa, sb) (PairActionFitClass::Ulong, PAFitBase.h:38): a
  * QuinticSpline per level on LinearGrid(grid_start, grid_end, num_points), given as the
@@ -186,7 +193,9 @@ int pimc_replica_set_long_range(pimc_ctx* ctx, int32_t replica, int32_t sa, int3
  * coefs [num_levels][num_points][6] = {Y, B, C, D, E, F} of each knot,
  * S(x) = ((((F P + E) P + D) P + C) P + B) P + Y with P = x - X_I. */
 int pimc_replica_set_ulong(pimc_ctx* ctx, int32_t replica, int32_t sa, int32_t sb, int32_t num_levels,
-                           double grid_start, double grid_end, int32_t num_points, const double* coefs);
+                           double grid_start, double grid_end, int32_t num_points, const double* coefs,
+                           const double* dcoefs);
+/* dcoefs: dUlong(level)(r) in the same layout, or NULL (needed by the energy estimator only). */
 
 /* Path data. mode: PIMC_OLDMODE, PIMC_NEWMODE or PIMC_BOTHMODES (set only).
  * beads is [(M+1)][N][3]. */
@@ -245,6 +254,40 @@ int pimc_action_long_range(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t n
  * reference computes them (n = -nearbyint(dispA/L), m = nearbyint((dispA-dispB)/L)). */
 int pimc_dist_disp(pimc_ctx* ctx, int32_t replica, int32_t mode, int64_t n,
                    const int32_t* idx, double* out, int32_t* images);
+
+/* ------------------------------------------------------------------ */
+/* Estimators (ActionsClass::Energy, PairCorrelationClass)             */
+/* ------------------------------------------------------------------ */
+
+/* ActionsClass::Energy (PIMC++/src/Actions/ActionsClass.cc:813-871) of copy `mode`
+ * over the whole path (slices [0, M], level 0):
+ *   kinetic   KineticClass::d_dBeta        (PIMC++/src/Actions/KineticClass.cc:187-256)
+ *   du_short  ShortRangeClass::d_dBeta     (PIMC++/src/Actions/ShortRangeClass.cc:256-295;
+ *             DavidPAClass::dU via calcUsqz, DavidPAClass.cc:115-126, 291-463;
+ *             PAcoulombBCFitClass::dU, PAcoulombBCFit.cc:178-192)
+ *   du_long   LongRangeClass::d_dBeta      (PIMC++/src/Actions/LongRangeClass.cc:374-442), 0 without long range
+ *   v_short   sum over slices (1/2 at the ends) of ShortRangePotClass::V
+ *             (PIMC++/src/Actions/ShortRangePotClass.cc:27-47); NaN when a pair fit is not a DavidPA fit
+ *   total     kinetic + du_short + du_long (EnergyClass::Accumulate's sum without node terms,
+ *             PIMC++/src/Observables/ObservableEnergy.cc:31-60)
+ * Needs the fits' beta-derivative tables (and the long-range ones when long range is used). */
+typedef struct pimc_energy {
+  double kinetic;
+  double du_short;
+  double du_long;
+  double v_short;
+  double total;
+} pimc_energy;
+
+int pimc_estimator_energy(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, int32_t mode, pimc_energy* out);
+
+/* PairCorrelationClass::Accumulate (PIMC++/src/Observables/ObservableCorrelation.cc:156-221):
+ * adds to hist[num_points-1] the count of minimum-image distances between
+ * particles of species s1 and s2 (pairs counted once when s1 == s2) on slices
+ * [0, M-1] of copy `mode` that fall below `end`, binned by the ReverseMap of
+ * LinearGrid(start, end, num_points).  Distances below `start` are not counted. */
+int pimc_estimator_pair_correlation(pimc_ctx* ctx, int32_t replica, int32_t mode, int32_t s1, int32_t s2,
+                                    double start, double end, int32_t num_points, int64_t* hist);
 
 /* ------------------------------------------------------------------ */
 /* Bisection moves (BisectionBlockClass with PermuteType NONE)          */

@@ -43,6 +43,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <limits>
 #include <vector>
 
 namespace {
@@ -269,6 +270,11 @@ struct MultiCubicSpline {
 struct PairFit {
   virtual ~PairFit() {}
   virtual double U(double q, double z, double s2, int level) const = 0;
+  // beta derivative (PairActionFitClass::dU); valid where has_dU()
+  virtual double dU(double q, double z, double s2, int level) const = 0;
+  virtual bool has_dU() const = 0;
+  // the pair potential the fit was made from (PairActionFitClass::V); NaN where not stored
+  virtual double V(double r) const { (void)r; return std::numeric_limits<double>::quiet_NaN(); }
   virtual bool IsLongRange() const = 0;
   // bisection levels the fit covers
   virtual int num_levels() const = 0;
@@ -413,8 +419,19 @@ struct BicubicSpline {
 // ------------------------------------------------------------------------
 struct CoulombBCFit : PairFit {
   Grid qgrid, tgrid;
-  std::vector<BicubicSpline> Usplines;
+  std::vector<BicubicSpline> Usplines, dUsplines;
   double SmallestBeta = 0, Z1Z2 = 0;
+  // PAcoulombBCFitClass::dU (PAcoulombBCFit.cc:178-192): beyond the grid the Coulomb potential
+  double dU(double q, double z, double s2, int level) const override {
+    (void)z;
+    if (q <= (qgrid.end * 1.0000001)) {
+      if (q == 0) return (dUsplines[level](0.0, 0.0));
+      double t = sqrt(s2) / (2.0 * q);
+      return (dUsplines[level](q, t));
+    }
+    return Z1Z2 / q;
+  }
+  bool has_dU() const override { return !dUsplines.empty(); }
   double U(double q, double z, double s2, int level) const override {
     if (q <= (qgrid.end * 1.0000001)) {
       if (q == 0) return (Usplines[level](0.0, 0.0));
@@ -454,9 +471,10 @@ struct QuinticTable {
 struct DavidFit : PairFit {
   Grid grid;
   std::vector<MultiCubicSpline> ukj;  // per tau level
+  std::vector<MultiCubicSpline> dukj; // beta derivatives, per tau level (empty when not given)
   int num_tau = 0, ncols = 0, num_terms = 0, taupos = 0;
   double smallest_tau = 0;
-  mutable std::vector<double> temp;   // TempukjArray
+  mutable std::vector<double> temp, dtemp;   // TempukjArray, TempdukjArray
 
   // calcUsqzFast (DavidPAClass.cc:468-511)
   double calc_usqz_fast(double s, double q, double z, int level) const {
@@ -498,6 +516,61 @@ struct DavidFit : PairFit {
     double s = sqrt(s2);
     return calc_usqz_fast(s, q, z, level);
   }
+  // calcUsqz (DavidPAClass.cc:291-463): U, dU (dukj plus the potential the .dm
+  // file leaves out of the diagonal) and V, with r, r' clamped into the grid.
+  void calc_usqz(double s, double q, double z, int level, double& U, double& dUo, double& Vo) const {
+    level = level + taupos;
+    const MultiCubicSpline& sp = ukj[level];
+    const MultiCubicSpline& dsp = dukj[level];
+    double rmin = sp.grid->start;
+    U = 0.0;
+    dUo = 0.0;
+    double r = q + 0.5 * z;
+    double rprime = q - 0.5 * z;
+    if (r > sp.grid->end) r = sp.grid->end;
+    if (rprime > sp.grid->end) rprime = sp.grid->end;
+    if (rprime < rmin) rprime = rmin;
+    if (r < rmin) r = rmin;
+    Vo = 0.5 * (sp.eval(0, r) + sp.eval(0, rprime));
+    U += 0.5 * (sp.eval(1, r) + sp.eval(1, rprime));
+    dUo += 0.5 * (dsp.eval(1, r) + dsp.eval(1, rprime));
+    dUo += Vo;
+    if (s > 0.0 && q < sp.grid->end) {
+      double zsquared = z * z;
+      double ssquared = s * s;
+      double ssquaredinverse = 1.0 / ssquared;
+      double Sto2k = ssquared;
+      sp.eval_all(q, temp.data());
+      dsp.eval_all(q, dtemp.data());
+      for (int k = 1; k <= num_terms; k++) {
+        double Zto2j = 1;
+        double currS = Sto2k;
+        for (int j = 0; j <= k; j++) {
+          double Ucof = temp[k * (k + 1) / 2 + j + 1];
+          double dUcof = dtemp[k * (k + 1) / 2 + j + 1];
+          U += (Ucof)*Zto2j * currS;
+          dUo += (dUcof)*Zto2j * currS;
+          Zto2j *= zsquared;
+          currS = currS * ssquaredinverse;
+        }
+        Sto2k = Sto2k * ssquared;
+      }
+    }
+  }
+  // DavidPAClass::dU (DavidPAClass.cc:115-126)
+  double dU(double q, double z, double s2, int level) const override {
+    double s = sqrt(s2);
+    double u, du, v;
+    calc_usqz(s, q, z, level, u, du, v);
+    return du;
+  }
+  bool has_dU() const override { return !dukj.empty(); }
+  // DavidPAClass::V (DavidPAClass.cc:104-113): column 0 of the first level, r clamped to the grid
+  double V(double r) const override {
+    if (r > ukj[0].grid->end) r = ukj[0].grid->end;
+    if (r < ukj[0].grid->start) r = ukj[0].grid->start;
+    return ukj[0].eval(0, r);
+  }
   bool IsLongRange() const override { return false; }   // DavidPAClass.cc:127-130
   int num_levels() const override { return num_tau - taupos; }
 };
@@ -506,10 +579,11 @@ struct DavidFit : PairFit {
 // Replica: PathClass state + action configuration
 // ------------------------------------------------------------------------
 struct LongRangeTables {
-  bool set = false;
+  bool set = false, dset = false;
   int num_levels = 0;
   std::vector<double> ulong_k;  // [levels][nk]
   std::vector<double> ulong_r0, ushort_k0, ulong_k0;
+  std::vector<double> dulong_k, dulong_r0, dushort_k0, dulong_k0;   // beta derivatives
 };
 
 struct Replica {
@@ -532,6 +606,7 @@ struct Replica {
   std::vector<cplx> rho[2];      // [slice][species][k]
   std::vector<LongRangeTables> lr;  // [ns*ns]
   std::vector<QuinticTable> ulong;  // [ns*ns] PairActionFitClass::Ulong of the pair
+  std::vector<QuinticTable> dulong; // [ns*ns] dUlong
   uint32_t stream = 0;
   int join = 0;
 
@@ -1069,6 +1144,13 @@ int pimc_fit_create_david(pimc_ctx* ctx, const pimc_david_fit_desc* d, int32_t* 
   f->ukj.resize(d->num_tau);
   for (int l = 0; l < d->num_tau; l++)
     f->ukj[l].init(&f->grid, d->y + (size_t)l * d->num_points * d->num_cols, d->num_cols, sd.data(), ed.data());
+  if (d->dy) {
+    // dukj: column 0 the potential, the same end conditions (DavidPAClass.cc:840-877)
+    f->dtemp.resize(d->num_cols);
+    f->dukj.resize(d->num_tau);
+    for (int l = 0; l < d->num_tau; l++)
+      f->dukj[l].init(&f->grid, d->dy + (size_t)l * d->num_points * d->num_cols, d->num_cols, sd.data(), ed.data());
+  }
   // TauPos (DavidPAClass.cc:859-868)
   f->taupos = -1;
   double tau = d->smallest_tau;
@@ -1096,13 +1178,19 @@ int pimc_fit_create_coulomb_bc(pimc_ctx* ctx, const pimc_coulomb_bc_fit_desc* d,
   f->Usplines.resize(d->num_betas);
   for (int b = 0; b < d->num_betas; b++)
     f->Usplines[b].init(&f->qgrid, &f->tgrid, d->umat + (size_t)b * d->q_points * d->t_points);
+  if (d->dumat) {
+    f->dUsplines.resize(d->num_betas);
+    for (int b = 0; b < d->num_betas; b++)
+      f->dUsplines[b].init(&f->qgrid, &f->tgrid, d->dumat + (size_t)b * d->q_points * d->t_points);
+  }
   ctx->c.fits.push_back(std::move(f));
   *fit_id = (int32_t)ctx->c.fits.size() - 1;
   return PIMC_OK;
 }
 
 int pimc_replica_set_ulong(pimc_ctx* ctx, int32_t replica, int32_t sa, int32_t sb, int32_t num_levels,
-                           double grid_start, double grid_end, int32_t num_points, const double* coefs) {
+                           double grid_start, double grid_end, int32_t num_points, const double* coefs,
+                           const double* dcoefs) {
   Replica* R = get_rep(&ctx->c, replica);
   if (!R || sa < 0 || sb < 0 || sa >= R->ns || sb >= R->ns || num_levels < 1 || num_points < 2 || !coefs ||
       !(grid_start < grid_end))
@@ -1114,6 +1202,11 @@ int pimc_replica_set_ulong(pimc_ctx* ctx, int32_t replica, int32_t sa, int32_t s
   t.c.assign(coefs, coefs + (size_t)num_levels * num_points * 6);
   R->ulong[sa * R->ns + sb] = t;
   R->ulong[sb * R->ns + sa] = t;
+  if (dcoefs) {
+    t.c.assign(dcoefs, dcoefs + (size_t)num_levels * num_points * 6);
+    R->dulong[sa * R->ns + sb] = t;
+    R->dulong[sb * R->ns + sa] = t;
+  }
   return PIMC_OK;
 }
 
@@ -1173,6 +1266,7 @@ int pimc_replica_create(pimc_ctx* ctx, const pimc_replica_desc* d, int32_t* repl
   }
   R->lr.resize(R->ns * R->ns);
   R->ulong.resize(R->ns * R->ns);
+  R->dulong.resize(R->ns * R->ns);
   ctx->c.reps.push_back(std::move(R));
   *replica_id = (int32_t)ctx->c.reps.size() - 1;
   return PIMC_OK;
@@ -1194,11 +1288,14 @@ int pimc_replica_kvecs(pimc_ctx* ctx, int32_t replica, double* kvecs, int32_t* k
 
 int pimc_replica_set_long_range(pimc_ctx* ctx, int32_t replica, int32_t sa, int32_t sb,
                                 int32_t num_levels, const double* ulong_k, const double* ulong_r0,
-                                const double* ushort_k0, const double* ulong_k0) {
+                                const double* ushort_k0, const double* ulong_k0, const double* dulong_k,
+                                const double* dulong_r0, const double* dushort_k0, const double* dulong_k0) {
   Replica* R = get_rep(&ctx->c, replica);
   if (!R || sa < 0 || sb < 0 || sa >= R->ns || sb >= R->ns || num_levels < 1 || !ulong_k || !ulong_r0 ||
       !ushort_k0 || !ulong_k0)
     return fail(PIMC_ERR_ARG, "replica_set_long_range: bad argument");
+  const int nd = !!dulong_k + !!dulong_r0 + !!dushort_k0 + !!dulong_k0;
+  if (nd != 0 && nd != 4) return fail(PIMC_ERR_ARG, "replica_set_long_range: give all four derivative tables or none");
   if (R->nk == 0) return fail(PIMC_ERR_STATE, "replica_set_long_range: replica has no k set");
   LongRangeTables t;
   t.set = true;
@@ -1207,6 +1304,13 @@ int pimc_replica_set_long_range(pimc_ctx* ctx, int32_t replica, int32_t sa, int3
   t.ulong_r0.assign(ulong_r0, ulong_r0 + num_levels);
   t.ushort_k0.assign(ushort_k0, ushort_k0 + num_levels);
   t.ulong_k0.assign(ulong_k0, ulong_k0 + num_levels);
+  if (nd == 4) {
+    t.dset = true;
+    t.dulong_k.assign(dulong_k, dulong_k + (size_t)num_levels * R->nk);
+    t.dulong_r0.assign(dulong_r0, dulong_r0 + num_levels);
+    t.dushort_k0.assign(dushort_k0, dushort_k0 + num_levels);
+    t.dulong_k0.assign(dulong_k0, dulong_k0 + num_levels);
+  }
   R->lr[sa * R->ns + sb] = t;
   R->lr[sb * R->ns + sa] = t;
   return PIMC_OK;
@@ -1349,6 +1453,210 @@ int pimc_dist_disp(pimc_ctx* ctx, int32_t replica, int32_t mode, int64_t n, cons
         images[6 * i + d] = (int32_t)nimg[d];
         images[6 * i + 3 + d] = (int32_t)mimg[d];
       }
+  }
+  return PIMC_OK;
+}
+
+// KineticClass::d_dBeta (KineticClass.cc:187-256)
+static double kinetic_dbeta(Replica& R, int mode, int slice1, int slice2, int level) {
+  double spring = 0.0;
+  double levelTau = ldexp(R.tau, level);
+  int skip = 1 << level;
+  for (int ptcl = 0; ptcl < R.N; ptcl++) {
+    double lambda = R.lambda[R.species_of[ptcl]];
+    if (lambda != 0.0) {
+      double FourLambdaTauInv = 1.0 / (4.0 * lambda * levelTau);
+      for (int slice = slice1; slice < slice2; slice += skip) {
+        spring += (0.5 * 3) / levelTau;
+        double vel[3];
+        R.velocity(mode, slice, slice + skip, ptcl, vel);
+        double Z = 1.0;
+        double GaussSum[3] = {0.0, 0.0, 0.0}, numSum[3] = {0.0, 0.0, 0.0};
+        for (int dim = 0; dim < 3; dim++) {
+          for (int image = -R.num_images; image <= R.num_images; image++) {
+            double dist = vel[dim] + (double)image * R.box[dim];
+            double d2overFLT = dist * dist * FourLambdaTauInv;
+            double expPart = exp(-d2overFLT);
+            GaussSum[dim] += expPart;
+            numSum[dim] += -d2overFLT / levelTau * expPart;
+          }
+          Z *= GaussSum[dim];
+        }
+        double scalarnumSum = 0.0;
+        for (int dim = 0; dim < 3; dim++) {
+          double numProd = 1.0;
+          for (int dim2 = 0; dim2 < 3; dim2++) numProd *= (dim2 != dim) ? GaussSum[dim2] : numSum[dim2];
+          scalarnumSum += numProd;
+        }
+        spring += scalarnumSum / Z;
+      }
+    }
+  }
+  return spring;
+}
+
+// ShortRangeClass::d_dBeta (ShortRangeClass.cc:256-295)
+static double short_range_dbeta(Ctx* c, Replica& R, int mode, int slice1, int slice2, int level) {
+  int skip = 1 << level;
+  double dU = 0.0;
+  for (int ptcl1 = 0; ptcl1 < R.N; ptcl1++) {
+    int species1 = R.species_of[ptcl1];
+    for (int ptcl2 = 0; ptcl2 < ptcl1; ptcl2++) {
+      int pr = species1 * R.ns + R.species_of[ptcl2];
+      const PairFit& pa = *c->fits[R.pair_fit[pr]];
+      for (int slice = slice1; slice < slice2; slice += skip) {
+        double r[3], rp[3], rmag, rpmag;
+        R.dist_disp(mode, slice, slice + skip, ptcl1, ptcl2, rmag, rpmag, r, rp, nullptr, nullptr);
+        double d0 = r[0] - rp[0], d1 = r[1] - rp[1], d2 = r[2] - rp[2];
+        double s2 = d0 * d0 + d1 * d1 + d2 * d2;
+        double q = 0.5 * (rmag + rpmag);
+        double z = (rmag - rpmag);
+        dU += pa.dU(q, z, s2, level);
+        if (pa.IsLongRange() && R.use_long_range)
+          dU -= 0.5 * (R.dulong[pr].eval(level, rmag) + R.dulong[pr].eval(level, rpmag));
+      }
+    }
+  }
+  return dU;
+}
+
+// LongRangeClass::d_dBeta (LongRangeClass.cc:374-442) with rho_k recomputed from the
+// positions (CalcRho_ks_Fast) into a scratch copy.
+static double long_range_dbeta(Replica& R, int mode, int slice1, int slice2, int level) {
+  double homo = 0.0, hetero = 0.0, background = 0.0, k0Homo = 0.0, k0Hetero = 0.0;
+  int skip = (1 << level);
+  std::vector<cplx> saved = R.rho[mode];
+  for (int slice = slice1; slice <= slice2; slice += skip) {
+    double factor = ((slice == slice1) || (slice == slice2)) ? 0.5 : 1.0;
+    for (int species = 0; species < R.ns; species++) {
+      R.calc_rho_fast(mode, slice, species);
+      const LongRangeTables& PA = R.lr[species * R.ns + species];
+      if (PA.set) {
+        for (int ki = 0; ki < R.nk; ki++) {
+          double rhok2 = std::norm(R.R(mode, slice, species, ki));
+          homo += factor * 0.5 * 2.0 * rhok2 * PA.dulong_k[(size_t)level * R.nk + ki];
+        }
+        int N = R.count[species];
+        homo -= factor * 0.5 * N * PA.dulong_r0[level];
+        background -= factor * 0.5 * N * N * PA.dushort_k0[level];
+        k0Homo += factor * 0.5 * N * N * PA.dulong_k0[level];
+      }
+    }
+    for (int species1 = 0; species1 < R.ns; species1++)
+      for (int species2 = species1 + 1; species2 < R.ns; species2++) {
+        const LongRangeTables& PA = R.lr[species1 * R.ns + species2];
+        if (PA.set) {
+          for (int ki = 0; ki < R.nk; ki++) {
+            const cplx a = R.R(mode, slice, species1, ki), b = R.R(mode, slice, species2, ki);
+            double rhorho = a.real() * b.real() + a.imag() * b.imag();
+            hetero += factor * 2.0 * rhorho * PA.dulong_k[(size_t)level * R.nk + ki];
+          }
+          int N1 = R.count[species1], N2 = R.count[species2];
+          background -= factor * N1 * N2 * PA.dushort_k0[level];
+          k0Hetero += factor * N1 * N2 * PA.dulong_k0[level];
+        }
+      }
+  }
+  R.rho[mode] = saved;
+  double dU = homo + hetero;
+  if (R.use_background)
+    dU += background;
+  else
+    dU += (k0Homo + k0Hetero);
+  return dU;
+}
+
+// ShortRangePotClass::V(slice) (ShortRangePotClass.cc:27-47) with DistDisp(slice, p1, p2)
+// (PathClass.h:638-660); DavidPA fits only.
+static double short_range_v(Ctx* c, Replica& R, int mode, int slice) {
+  double val = 0.0;
+  for (int ptcl1 = 0; ptcl1 < R.N; ptcl1++) {
+    int species1 = R.species_of[ptcl1];
+    for (int ptcl2 = 0; ptcl2 < ptcl1; ptcl2++) {
+      const double* a = R.P(mode, slice, ptcl1);
+      const double* b = R.P(mode, slice, ptcl2);
+      double disp[3];
+      for (int d = 0; d < 3; d++) {
+        disp[d] = b[d] - a[d];
+        double n = nearbyint(disp[d] * R.boxinv[d]);
+        disp[d] -= n * R.isperiodic[d] * R.box[d];
+      }
+      double dist = sqrt(disp[0] * disp[0] + disp[1] * disp[1] + disp[2] * disp[2]);
+      val += c->fits[R.pair_fit[species1 * R.ns + R.species_of[ptcl2]]]->V(dist);
+    }
+  }
+  return val;
+}
+
+int pimc_estimator_energy(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, int32_t mode, pimc_energy* out) {
+  if (!ctx || nrep < 0 || (nrep > 0 && (!replicas || !out)) || mode < 0 || mode > 1)
+    return fail(PIMC_ERR_ARG, "estimator_energy: bad argument");
+  for (int r = 0; r < nrep; r++) {
+    Replica* R = get_rep(&ctx->c, replicas[r]);
+    if (!R) return fail(PIMC_ERR_ARG, "estimator_energy: bad replica");
+    for (int f = 0; f < R->ns * R->ns; f++) {
+      const PairFit& pa = *ctx->c.fits[R->pair_fit[f]];
+      if (!pa.has_dU()) return fail(PIMC_ERR_STATE, "estimator_energy: a pair fit has no beta-derivative table");
+      if (pa.IsLongRange() && R->use_long_range && (!R->dulong[f].set || R->dulong[f].levels < 1))
+        return fail(PIMC_ERR_STATE, "estimator_energy: long-range pair fit without dUlong tables");
+      if (R->long_range && R->use_long_range && R->lr[f].set && !R->lr[f].dset)
+        return fail(PIMC_ERR_STATE, "estimator_energy: long-range tables without beta derivatives");
+    }
+  }
+  for (int r = 0; r < nrep; r++) {
+    Replica& R = *get_rep(&ctx->c, replicas[r]);
+    const int M = R.M;
+    pimc_energy e{};
+    e.kinetic = kinetic_dbeta(R, mode, 0, M, 0);
+    e.du_short = short_range_dbeta(&ctx->c, R, mode, 0, M, 0);
+    e.du_long = (R.long_range && R.use_long_range) ? long_range_dbeta(R, mode, 0, M, 0) : 0.0;
+    double v = 0.0;
+    for (int slice = 0; slice <= M; slice++) {
+      double factor = ((slice == 0) || (slice == M)) ? 0.5 : 1.0;
+      v += factor * short_range_v(&ctx->c, R, mode, slice);
+    }
+    e.v_short = v;
+    e.total = e.kinetic + e.du_short + e.du_long;
+    out[r] = e;
+  }
+  return PIMC_OK;
+}
+
+// PairCorrelationClass::Accumulate (ObservableCorrelation.cc:156-221)
+int pimc_estimator_pair_correlation(pimc_ctx* ctx, int32_t replica, int32_t mode, int32_t s1, int32_t s2,
+                                    double start, double end, int32_t num_points, int64_t* hist) {
+  Replica* Rp = ctx ? get_rep(&ctx->c, replica) : nullptr;
+  if (!Rp || !hist || mode < 0 || mode > 1 || num_points < 2 || !(start < end) || s1 < 0 || s2 < 0 ||
+      s1 >= Rp->ns || s2 >= Rp->ns)
+    return fail(PIMC_ERR_ARG, "estimator_pair_correlation: bad argument");
+  Replica& R = *Rp;
+  Grid grid;
+  grid.init_linear(start, end, num_points);
+  auto add = [&](int slice, int p1, int p2) {
+    const double* a = R.P(mode, slice, p1);
+    const double* b = R.P(mode, slice, p2);
+    double disp[3];
+    for (int d = 0; d < 3; d++) {
+      disp[d] = b[d] - a[d];
+      double n = nearbyint(disp[d] * R.boxinv[d]);
+      disp[d] -= n * R.isperiodic[d] * R.box[d];
+    }
+    double dist = sqrt(disp[0] * disp[0] + disp[1] * disp[1] + disp[2] * disp[2]);
+    if (dist < grid.end && dist >= grid.start) {
+      int index = grid.reverse_map(dist);
+      if (index >= 0 && index < num_points - 1) hist[index]++;
+    }
+  };
+  const int f1 = R.first[s1], l1 = R.first[s1] + R.count[s1] - 1;
+  const int f2 = R.first[s2], l2 = R.first[s2] + R.count[s2] - 1;
+  for (int slice = 0; slice < R.nslices - 1; slice++) {
+    if (s1 == s2) {
+      for (int p1 = f1; p1 <= l1; p1++)
+        for (int p2 = p1 + 1; p2 <= l1; p2++) add(slice, p1, p2);
+    } else {
+      for (int p1 = f1; p1 <= l1; p1++)
+        for (int p2 = f2; p2 <= l2; p2++) add(slice, p1, p2);
+    }
   }
   return PIMC_OK;
 }

@@ -39,6 +39,7 @@ class DavidFitDesc(C.Structure):
         ("smallest_tau", C.c_double),
         ("desired_tau", C.c_double),
         ("y", _f64p),
+        ("dy", _f64p),
     ]
 
 
@@ -57,6 +58,7 @@ class CoulombBCFitDesc(C.Structure):
         ("smallest_beta", C.c_double),
         ("z1z2", C.c_double),
         ("umat", _f64p),
+        ("dumat", _f64p),
     ]
 
 
@@ -131,6 +133,11 @@ class BisectionPlan(C.Structure):
                 ("reserved", C.c_int32), ("trial_ms", C.c_double)]
 
 
+class Energy(C.Structure):
+    _fields_ = [("kinetic", C.c_double), ("du_short", C.c_double), ("du_long", C.c_double),
+                ("v_short", C.c_double), ("total", C.c_double)]
+
+
 # name -> (restype, argtypes); every pimc_* entry point of include/pimc_gpu.h
 _CTX = C.c_void_p
 SIGNATURES = {
@@ -144,11 +151,17 @@ SIGNATURES = {
     "pimc_fit_create_coulomb_bc": (C.c_int, [_CTX, C.POINTER(CoulombBCFitDesc), _i32p]),
     "pimc_replica_create": (C.c_int, [_CTX, C.POINTER(ReplicaDesc), _i32p]),
     "pimc_replica_set_ulong": (
-        C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32, _f64p]),
+        C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32, _f64p,
+                  _f64p]),
     "pimc_replica_num_kvecs": (C.c_int, [_CTX, C.c_int32, _i32p]),
     "pimc_replica_kvecs": (C.c_int, [_CTX, C.c_int32, _f64p, _i32p]),
     "pimc_replica_set_long_range": (
-        C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _f64p, _f64p, _f64p, _f64p]),
+        C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _f64p, _f64p, _f64p, _f64p,
+                  _f64p, _f64p, _f64p, _f64p]),
+    "pimc_estimator_energy": (C.c_int, [_CTX, _i32p, C.c_int32, C.c_int32, C.POINTER(Energy)]),
+    "pimc_estimator_pair_correlation": (
+        C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32,
+                  C.POINTER(C.c_int64)]),
     "pimc_path_set": (C.c_int, [_CTX, C.c_int32, C.c_int32, _f64p]),
     "pimc_path_get": (C.c_int, [_CTX, C.c_int32, C.c_int32, _f64p]),
     "pimc_path_set_beads": (C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _f64p]),
@@ -254,41 +267,60 @@ class PimcLib:
 
     # -- fits --------------------------------------------------------------
     def fit_create_david(self, grid_type: int, grid_start: float, grid_end: float, y: np.ndarray,
-                         num_terms: int, smallest_tau: float, desired_tau: float) -> int:
-        """y is [num_tau][num_points][num_cols] (column 0 potential, 1 u00, 2.. u_kj)."""
+                         num_terms: int, smallest_tau: float, desired_tau: float, dy=None) -> int:
+        """y is [num_tau][num_points][num_cols] (column 0 potential, 1 u00, 2.. u_kj);
+        dy the beta-derivative table in the same layout, or None."""
         y = _f64(y)
         if y.ndim != 3:
             raise ValueError("y must be [num_tau][num_points][num_cols]")
+        dyp = None
+        if dy is not None:
+            dy = _f64(dy)
+            if dy.shape != y.shape:
+                raise ValueError("dy must have the shape of y")
+            dyp = _p(dy, _f64p)
         d = DavidFitDesc(grid_type=grid_type, num_points=y.shape[1], grid_start=grid_start,
                          grid_end=grid_end, num_tau=y.shape[0], num_cols=y.shape[2],
                          num_terms=num_terms, reserved=0, smallest_tau=smallest_tau,
-                         desired_tau=desired_tau, y=_p(y, _f64p))
+                         desired_tau=desired_tau, y=_p(y, _f64p), dy=dyp)
         fid = C.c_int32()
         self._check("pimc_fit_create_david", self.lib.pimc_fit_create_david(self.ctx, C.byref(d), C.byref(fid)))
         return fid.value
 
     def fit_create_coulomb_bc(self, q_grid: tuple, t_grid: tuple, umat: np.ndarray, smallest_beta: float,
-                              z1z2: float) -> int:
-        """q_grid, t_grid = (grid_type, start, end); umat is [num_betas][nq][nt]."""
+                              z1z2: float, dumat=None) -> int:
+        """q_grid, t_grid = (grid_type, start, end); umat (and dumat, dU/dbeta, or None) is [num_betas][nq][nt]."""
         u = _f64(umat)
         if u.ndim != 3:
             raise ValueError("umat must be [num_betas][nq][nt]")
+        du = None
+        if dumat is not None:
+            du = _f64(dumat)
+            if du.shape != u.shape:
+                raise ValueError("dumat must have the shape of umat")
         d = CoulombBCFitDesc(q_grid_type=q_grid[0], q_points=u.shape[1], q_start=q_grid[1], q_end=q_grid[2],
                              t_grid_type=t_grid[0], t_points=u.shape[2], t_start=t_grid[1], t_end=t_grid[2],
                              num_betas=u.shape[0], reserved=0, smallest_beta=smallest_beta, z1z2=z1z2,
-                             umat=_p(u, _f64p))
+                             umat=_p(u, _f64p), dumat=_p(du, _f64p) if du is not None else None)
         fid = C.c_int32()
         self._check("pimc_fit_create_coulomb_bc",
                     self.lib.pimc_fit_create_coulomb_bc(self.ctx, C.byref(d), C.byref(fid)))
         return fid.value
 
-    def set_ulong(self, rep: int, sa: int, sb: int, grid_start: float, grid_end: float, coefs: np.ndarray):
-        """coefs [num_levels][num_points][6] = {Y, B, C, D, E, F} per knot."""
+    def set_ulong(self, rep: int, sa: int, sb: int, grid_start: float, grid_end: float, coefs: np.ndarray,
+                  dcoefs=None):
+        """coefs [num_levels][num_points][6] = {Y, B, C, D, E, F} per knot; dcoefs: dUlong, or None."""
         c = _f64(coefs)
         if c.ndim != 3 or c.shape[2] != 6:
             raise ValueError("coefs must be [num_levels][num_points][6]")
+        dc = None
+        if dcoefs is not None:
+            dc = _f64(dcoefs)
+            if dc.shape != c.shape:
+                raise ValueError("dcoefs must have the shape of coefs")
         self._check("pimc_replica_set_ulong", self.lib.pimc_replica_set_ulong(
-            self.ctx, rep, sa, sb, c.shape[0], grid_start, grid_end, c.shape[1], _p(c, _f64p)))
+            self.ctx, rep, sa, sb, c.shape[0], grid_start, grid_end, c.shape[1], _p(c, _f64p),
+            _p(dc, _f64p) if dc is not None else None))
 
     def fit_eval_u(self, fit: int, level: int, q, z, s2, precision: int = FP64) -> np.ndarray:
         q, z, s2 = _f64(q), _f64(z), _f64(s2)
@@ -341,14 +373,35 @@ class PimcLib:
         self._check("pimc_replica_kvecs", self.lib.pimc_replica_kvecs(self.ctx, rep, _p(kv, _f64p), _p(ki, _i32p)))
         return kv, ki
 
-    def set_long_range(self, rep: int, sa: int, sb: int, ulong_k, ulong_r0, ushort_k0, ulong_k0):
+    def set_long_range(self, rep: int, sa: int, sb: int, ulong_k, ulong_r0, ushort_k0, ulong_k0,
+                       dulong_k=None, dulong_r0=None, dushort_k0=None, dulong_k0=None):
         uk = _f64(ulong_k)
         r0, sk0, lk0 = _f64(ulong_r0), _f64(ushort_k0), _f64(ulong_k0)
         nl = r0.size
         if uk.shape != (nl, self.replicas[rep].nk) or sk0.size != nl or lk0.size != nl:
             raise ValueError("long-range table shapes")
+        d = [dulong_k, dulong_r0, dushort_k0, dulong_k0]
+        if any(x is not None for x in d):
+            d = [_f64(x) for x in d]
+            if d[0].shape != uk.shape or any(x.size != nl for x in d[1:]):
+                raise ValueError("long-range derivative table shapes")
+        dp = [_p(x, _f64p) if x is not None else None for x in d]
         self._check("pimc_replica_set_long_range", self.lib.pimc_replica_set_long_range(
-            self.ctx, rep, sa, sb, nl, _p(uk, _f64p), _p(r0, _f64p), _p(sk0, _f64p), _p(lk0, _f64p)))
+            self.ctx, rep, sa, sb, nl, _p(uk, _f64p), _p(r0, _f64p), _p(sk0, _f64p), _p(lk0, _f64p), *dp))
+
+    def energy(self, replicas, mode: int = OLDMODE) -> list:
+        reps = _i32(replicas)
+        out = (Energy * max(1, reps.size))()
+        self._check("pimc_estimator_energy", self.lib.pimc_estimator_energy(
+            self.ctx, _p(reps, _i32p), reps.size, mode, out))
+        return [out[i] for i in range(reps.size)]
+
+    def pair_correlation(self, rep: int, s1: int, s2: int, start: float, end: float, num_points: int,
+                         mode: int = OLDMODE, hist=None) -> np.ndarray:
+        h = np.zeros(num_points - 1, dtype=np.int64) if hist is None else hist
+        self._check("pimc_estimator_pair_correlation", self.lib.pimc_estimator_pair_correlation(
+            self.ctx, rep, mode, s1, s2, start, end, num_points, h.ctypes.data_as(C.POINTER(C.c_int64))))
+        return h
 
     def _path_shape(self, rep: int):
         info = self.replicas[rep]

@@ -137,8 +137,8 @@ void spline_d2(const std::vector<double>& x, const double* y, int stride, double
 }
 
 struct HostFit {
-  FitDev dev{};            // pointers refer to `mem`
-  DevBuf mem;
+  FitDev dev{};            // pointers refer to `mem` (and `dmem` for the beta-derivative tables)
+  DevBuf mem, dmem;
   int num_terms = 0;       // -1 for fits without the DavidPA expansion
   size_t staged64 = 0, staged32 = 0;          // LDS bytes of one staged level
   size_t staged64_lin = 0, staged32_lin = 0;  // the same for the linear-grid kernels (coefficients only)
@@ -191,6 +191,8 @@ struct HostReplica {
   ReplicaDev dev{};
   DevBuf path0, path1, perm0, perm1;
   DevBuf rho0, rho1, kidx_dev, ulong_k, lr_const, ulong;
+  DevBuf dulong, dulong_k, dlr_const;   // beta derivatives (energy estimator)
+  unsigned dulong_mask = 0, dlr_mask = 0;
   std::vector<double> kvecs;     // [nk][3]
   std::vector<int> kidx;         // [nk][3] index + maxk
   bool use_long_range = false;
@@ -395,6 +397,19 @@ int pimc_ctx_synchronize(pimc_ctx* ctx) {
   return PIMC_OK;
 }
 
+// Interval cubics {y_lo, ..} in the local coordinate of one table column (see FitDev),
+// from the column's natural-start, zero-end-slope spline (DavidPAClass.cc:760-763).
+void column_cubics(const HostGrid& g, const double* y, int stride, std::vector<double>& d2, d4v* out, int out_stride) {
+  spline_d2(g.x, y, stride, 5.0e30, 0.0, d2);
+  for (int i = 0; i < g.n - 1; i++) {
+    double h = g.x[i + 1] - g.x[i];
+    double h2over6 = h * h * 0.1666666666666666666;
+    double ylo = y[(size_t)i * stride], yhi = y[(size_t)(i + 1) * stride];
+    double Dlo = d2[i] * h2over6, Dhi = d2[i + 1] * h2over6;
+    out[(size_t)i * out_stride] = d4v{ylo, yhi - ylo - 2.0 * Dlo - Dhi, 3.0 * Dlo, Dhi - Dlo};
+  }
+}
+
 int pimc_fit_create_david(pimc_ctx* ctx, const pimc_david_fit_desc* d, int32_t* fit_id) {
   if (!ctx || !d || !fit_id || !d->y) return fail(PIMC_ERR_ARG, "fit_create_david: NULL argument");
   if (d->num_points < 3 || d->num_tau < 1 || d->num_cols < 2)
@@ -467,6 +482,22 @@ int pimc_fit_create_david(pimc_ctx* ctx, const pimc_david_fit_desc* d, int32_t* 
   HIP_TRY(hipMemcpy(base + off_x, g.x.data(), bx, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(base + off64, c64.data(), b64, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(base + off32, c32.data(), b32, hipMemcpyHostToDevice));
+  if (d->dy) {
+    // energy-estimator table: V (column 0 of y), du00 and the off-diagonal dukj columns
+    const int ecol = ncol + 1;
+    std::vector<d4v> e64((size_t)ntau * nint * ecol);
+    for (int lev = 0; lev < ntau; lev++) {
+      const double* y = d->y + (size_t)lev * ng * nc;
+      const double* dy = d->dy + (size_t)lev * ng * nc;
+      d4v* out = e64.data() + (size_t)lev * nint * ecol;
+      column_cubics(g, y, nc, d2, out, ecol);
+      for (int c = 1; c < ecol; c++) column_cubics(g, dy + c, nc, d2, out + c, ecol);
+    }
+    if ((rc = f->dmem.ensure(e64.size() * sizeof(d4v)))) return rc;
+    HIP_TRY(hipMemcpy(f->dmem.p, e64.data(), e64.size() * sizeof(d4v), hipMemcpyHostToDevice));
+    f->dev.e64 = f->dmem.as<d4v>();
+    f->dev.ecol = ecol;
+  }
   FitDev& F = f->dev;
   F.grid_type = g.type;
   F.ng = ng;
@@ -518,6 +549,21 @@ int pimc_fit_create_coulomb_bc(pimc_ctx* ctx, const pimc_coulomb_bc_fit_desc* d,
     for (size_t i = 0; i < z.size(); i++) bc[(size_t)b * z.size() + i] = d4v{z[i], dx[i], dy[i], dxy[i]};
   }
   auto f = std::make_unique<HostFit>();
+  if (d->dumat) {
+    // dU/dbeta bicubic: the same node-derivative construction on dumat
+    std::vector<d4v> dbc(bc.size());
+    for (int b = 0; b < nb; b++) {
+      const double* u = d->dumat + (size_t)b * nq * nt;
+      std::copy(u, u + z.size(), z.begin());
+      for (int iy = 0; iy < nt; iy++) bc_slopes(qg.x, z.data() + iy, dx.data() + iy, nt);
+      for (int ix = 0; ix < nq; ix++) bc_slopes(tg.x, z.data() + (size_t)ix * nt, dy.data() + (size_t)ix * nt, 1);
+      for (int iy = 0; iy < nt; iy++) bc_slopes(qg.x, dy.data() + iy, dxy.data() + iy, nt);
+      for (size_t i = 0; i < z.size(); i++) dbc[(size_t)b * z.size() + i] = d4v{z[i], dx[i], dy[i], dxy[i]};
+    }
+    if ((rc = f->dmem.ensure(dbc.size() * sizeof(d4v)))) return rc;
+    HIP_TRY(hipMemcpy(f->dmem.p, dbc.data(), dbc.size() * sizeof(d4v), hipMemcpyHostToDevice));
+    f->dev.dbc = f->dmem.as<d4v>();
+  }
   size_t bq = (size_t)nq * 8, bt = (size_t)nt * 8, bb = bc.size() * sizeof(d4v);
   size_t off_q = 0, off_t = (bq + 255) & ~(size_t)255, off_b = (off_t + bt + 255) & ~(size_t)255, total = off_b + bb;
   if ((rc = f->mem.ensure(total))) return rc;
@@ -553,7 +599,8 @@ int pimc_fit_create_coulomb_bc(pimc_ctx* ctx, const pimc_coulomb_bc_fit_desc* d,
 }
 
 int pimc_replica_set_ulong(pimc_ctx* ctx, int32_t replica, int32_t sa, int32_t sb, int32_t num_levels,
-                           double grid_start, double grid_end, int32_t num_points, const double* coefs) {
+                           double grid_start, double grid_end, int32_t num_points, const double* coefs,
+                           const double* dcoefs) {
   HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
   if (!H || num_levels < 1 || num_levels > PIMC_MAX_LEVELS || num_points < 2 || !coefs || !(grid_start < grid_end))
     return fail(PIMC_ERR_ARG, "replica_set_ulong: bad argument");
@@ -580,6 +627,19 @@ int pimc_replica_set_ulong(pimc_ctx* ctx, int32_t replica, int32_t sa, int32_t s
   for (int pr : {sa * R.ns + sb, sb * R.ns + sa}) {
     HIP_TRY(hipMemcpy(H->ulong.as<double>() + pr * per_pair, coefs, per_pair * sizeof(double), hipMemcpyHostToDevice));
     R.ulong_mask |= 1u << pr;
+  }
+  if (dcoefs) {
+    if (!R.dulong) {
+      int rc = H->dulong.ensure((size_t)R.ns * R.ns * per_pair * sizeof(double));
+      if (rc) return rc;
+      HIP_TRY(hipMemset(H->dulong.p, 0, (size_t)R.ns * R.ns * per_pair * sizeof(double)));
+      R.dulong = H->dulong.as<double>();
+    }
+    for (int pr : {sa * R.ns + sb, sb * R.ns + sa}) {
+      HIP_TRY(hipMemcpy(H->dulong.as<double>() + pr * per_pair, dcoefs, per_pair * sizeof(double),
+                        hipMemcpyHostToDevice));
+      H->dulong_mask |= 1u << pr;
+    }
   }
   ctx->reps_dirty = true;
   return PIMC_OK;
@@ -712,10 +772,13 @@ int pimc_replica_kvecs(pimc_ctx* ctx, int32_t replica, double* kvecs, int32_t* k
 }
 int pimc_replica_set_long_range(pimc_ctx* ctx, int32_t replica, int32_t sa, int32_t sb, int32_t num_levels,
                                 const double* ulong_k, const double* ulong_r0, const double* ushort_k0,
-                                const double* ulong_k0) {
+                                const double* ulong_k0, const double* dulong_k, const double* dulong_r0,
+                                const double* dushort_k0, const double* dulong_k0) {
   HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
   if (!H || num_levels < 1 || num_levels > PIMC_MAX_LEVELS || !ulong_k || !ulong_r0 || !ushort_k0 || !ulong_k0)
     return fail(PIMC_ERR_ARG, "replica_set_long_range: bad argument");
+  const int nd = !!dulong_k + !!dulong_r0 + !!dushort_k0 + !!dulong_k0;
+  if (nd != 0 && nd != 4) return fail(PIMC_ERR_ARG, "replica_set_long_range: give all four derivative tables or none");
   RepCore& R = H->dev.core;
   if (sa < 0 || sb < 0 || sa >= R.ns || sb >= R.ns) return fail(PIMC_ERR_ARG, "replica_set_long_range: bad species");
   if (R.nk == 0) return fail(PIMC_ERR_STATE, "replica_set_long_range: replica has no k set");
@@ -742,6 +805,26 @@ int pimc_replica_set_long_range(pimc_ctx* ctx, int32_t replica, int32_t sa, int3
     HIP_TRY(hipMemcpy(c + num_levels, ushort_k0, num_levels * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c + 2 * num_levels, ulong_k0, num_levels * sizeof(double), hipMemcpyHostToDevice));
     R.lr_mask |= 1u << pr;
+  }
+  if (nd == 4) {
+    const size_t bk = (size_t)npair * num_levels * R.nk * sizeof(double), bc = (size_t)npair * 3 * num_levels * sizeof(double);
+    if (!R.dulong_k) {
+      int rc;
+      if ((rc = H->dulong_k.ensure(bk)) || (rc = H->dlr_const.ensure(bc))) return rc;
+      HIP_TRY(hipMemset(H->dulong_k.p, 0, bk));
+      HIP_TRY(hipMemset(H->dlr_const.p, 0, bc));
+      R.dulong_k = H->dulong_k.as<double>();
+      R.dlr_const = H->dlr_const.as<double>();
+    }
+    for (int pr : {sa * R.ns + sb, sb * R.ns + sa}) {
+      HIP_TRY(hipMemcpy(H->dulong_k.as<double>() + (size_t)pr * num_levels * R.nk, dulong_k,
+                        (size_t)num_levels * R.nk * sizeof(double), hipMemcpyHostToDevice));
+      double* c = H->dlr_const.as<double>() + (size_t)pr * 3 * num_levels;
+      HIP_TRY(hipMemcpy(c, dulong_r0, num_levels * sizeof(double), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(c + num_levels, dushort_k0, num_levels * sizeof(double), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(c + 2 * num_levels, dulong_k0, num_levels * sizeof(double), hipMemcpyHostToDevice));
+      H->dlr_mask |= 1u << pr;
+    }
   }
   R.lr_on = (H->use_long_range && R.lr_mask) ? 1 : 0;
   R.lr_terms = 0;
@@ -1366,6 +1449,111 @@ int pimc_sweep_graph_destroy(pimc_ctx* ctx, int32_t graph_id) {
     return fail(PIMC_ERR_ARG, "sweep_graph_destroy: bad argument");
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   ctx->graphs[graph_id].reset();
+  return PIMC_OK;
+}
+
+int pimc_estimator_energy(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, int32_t mode, pimc_energy* out) {
+  if (!ctx || nrep < 0 || (nrep > 0 && (!replicas || !out)) || mode < 0 || mode > 1)
+    return fail(PIMC_ERR_ARG, "estimator_energy: bad argument");
+  for (int r = 0; r < nrep; r++) {
+    HostReplica* H = ctx->rep(replicas[r]);
+    if (!H) return fail(PIMC_ERR_ARG, "estimator_energy: bad replica");
+    const RepCore& R = H->dev.core;
+    for (int f = 0; f < R.ns * R.ns; f++) {
+      const FitDev& F = ctx->fits[H->dev.sp.fit[f]]->dev;
+      if ((F.kind == 0 && !F.e64) || (F.kind == 1 && !F.dbc))
+        return fail(PIMC_ERR_STATE, "estimator_energy: a pair fit has no beta-derivative table");
+      if (F.kind == 1 && R.use_long_range && !((H->dulong_mask >> f) & 1u))
+        return fail(PIMC_ERR_STATE, "estimator_energy: long-range pair fit without dUlong tables");
+    }
+    if (R.lr_on && (R.lr_mask & ~H->dlr_mask))
+      return fail(PIMC_ERR_STATE, "estimator_energy: long-range tables without beta derivatives");
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  int rc = ctx->sync_descriptors();
+  if (rc) return rc;
+  for (int r = 0; r < nrep; r++) {
+    HostReplica* H = ctx->rep(replicas[r]);
+    const RepCore& R = H->dev.core;
+    int nt = 0;
+    bool all_david = true;
+    for (int f = 0; f < R.ns * R.ns; f++) {
+      const HostFit& F = *ctx->fits[H->dev.sp.fit[f]];
+      if (F.dev.kind == 0)
+        nt = std::max(nt, F.num_terms);
+      else
+        all_david = false;
+    }
+    for (int f = 0; f < R.ns * R.ns; f++) {
+      const HostFit& F = *ctx->fits[H->dev.sp.fit[f]];
+      if (F.dev.kind == 0 && F.num_terms != nt)
+        return fail(PIMC_ERR_UNSUPPORTED, "estimator_energy: fits with different num_terms in one replica");
+    }
+    const long long npairs = (long long)R.N * (R.N - 1) / 2;
+    const int M = R.nslices - 1;
+    const int pair_blocks = (int)std::max(1LL, std::min(2048LL, (npairs + kBlock - 1) / kBlock));
+    const int kin_blocks = (int)std::max(1LL, std::min(1024LL, ((long long)R.N * M + kBlock - 1) / kBlock));
+    const int lr_slices = R.lr_on ? R.nslices : 0;
+    const size_t n_pp = 2 * (size_t)pair_blocks, n_kp = kin_blocks, n_lp = 3 * (size_t)std::max(1, lr_slices);
+    if ((rc = ctx->d_out.ensure((n_pp + n_kp + n_lp) * sizeof(double)))) return rc;
+    double* dp = ctx->d_out.as<double>();
+    hipError_t e = launch_energy(nt, ctx->d_reps.as<ReplicaDev>(), ctx->d_fits.as<FitDev>(), replicas[r], mode,
+                                 pair_blocks, kin_blocks, lr_slices, dp, dp + n_pp, dp + n_pp + n_kp, ctx->stream);
+    if (e != hipSuccess) return fail(PIMC_ERR_DEVICE, std::string("estimator_energy launch: ") + hipGetErrorString(e));
+    std::vector<double> h(n_pp + n_kp + n_lp);
+    HIP_TRY(hipMemcpyAsync(h.data(), dp, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    pimc_energy en{};
+    double v = 0.0;
+    for (int b = 0; b < pair_blocks; b++) {
+      en.du_short += h[2 * b];
+      v += h[2 * b + 1];
+    }
+    for (int b = 0; b < kin_blocks; b++) en.kinetic += h[n_pp + b];
+    if (lr_slices) {
+      double hh = 0.0, bg = 0.0, k0 = 0.0;
+      for (int sl = 0; sl < lr_slices; sl++) {
+        hh += h[n_pp + n_kp + 3 * sl];
+        bg += h[n_pp + n_kp + 3 * sl + 1];
+        k0 += h[n_pp + n_kp + 3 * sl + 2];
+      }
+      en.du_long = hh + (R.use_background ? bg : k0);
+    }
+    en.v_short = all_david ? v : std::numeric_limits<double>::quiet_NaN();
+    en.total = en.kinetic + en.du_short + en.du_long;
+    out[r] = en;
+  }
+  return PIMC_OK;
+}
+
+int pimc_estimator_pair_correlation(pimc_ctx* ctx, int32_t replica, int32_t mode, int32_t s1, int32_t s2, double start,
+                                    double end, int32_t num_points, int64_t* hist) {
+  HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
+  if (!H || !hist || mode < 0 || mode > 1 || num_points < 2 || !(start < end))
+    return fail(PIMC_ERR_ARG, "estimator_pair_correlation: bad argument");
+  const RepCore& R = H->dev.core;
+  if (s1 < 0 || s2 < 0 || s1 >= R.ns || s2 >= R.ns) return fail(PIMC_ERR_ARG, "estimator_pair_correlation: bad species");
+  HIP_TRY(hipSetDevice(ctx->device));
+  int rc = ctx->sync_descriptors();
+  if (rc) return rc;
+  const int nbins = num_points - 1;
+  if ((rc = ctx->d_img.ensure((size_t)nbins * sizeof(unsigned long long)))) return rc;
+  HIP_TRY(hipMemsetAsync(ctx->d_img.p, 0, (size_t)nbins * sizeof(unsigned long long), ctx->stream));
+  const RepSpecies& S = H->dev.sp;
+  const int same = s1 == s2;
+  const long long npairs = same ? (long long)S.count[s1] * (S.count[s1] - 1) / 2 : (long long)S.count[s1] * S.count[s2];
+  const long long nitems = npairs * (R.nslices - 1);
+  const int blocks = (int)std::max(1LL, std::min(4096LL, (nitems + kBlock - 1) / kBlock));
+  const double delta = (end - start) / (double)(num_points - 1);
+  hipError_t e = launch_pair_correlation(ctx->d_reps.as<ReplicaDev>(), replica, mode, S.first[s1], S.count[s1],
+                                         S.first[s2], S.count[s2], same, start, 1.0 / delta, end, nbins,
+                                         ctx->d_img.as<unsigned long long>(), blocks, ctx->stream);
+  if (e != hipSuccess) return fail(PIMC_ERR_DEVICE, std::string("estimator_pair_correlation launch: ") + hipGetErrorString(e));
+  std::vector<unsigned long long> h(nbins);
+  HIP_TRY(hipMemcpyAsync(h.data(), ctx->d_img.p, (size_t)nbins * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                         ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (int i = 0; i < nbins; i++) hist[i] += (int64_t)h[i];
   return PIMC_OK;
 }
 

@@ -80,6 +80,12 @@ struct FitDev {
   int nt;
   GridDev tgrid;
   const d4v* bc;          // [num_tau][ng][nt] BCpoint {z, dzdx, dzdy, d2zdxdy}
+  // beta derivatives for the energy estimator (nullptr when not given):
+  // kind 0: [num_tau][nint][ecol] cubics of V (column 0 of the table), du00 and the
+  // off-diagonal dukj columns; kind 1: dU bicubic nodes like bc
+  const d4v* e64;
+  int ecol;
+  const d4v* dbc;
   double smallest_beta;
   double z1z2;
 };
@@ -121,6 +127,9 @@ struct RepCore {
   unsigned ulong_mask;     // bit sa*ns+sb set where the pair has Ulong tables
   double ulong_start, ulong_delta, ulong_deltainv;
   const double* ulong;     // [ns*ns][ulong_levels][ulong_n][6]
+  const double* dulong;    // dUlong, same layout, or nullptr
+  const double* dulong_k;  // [ns*ns][lr_levels][nk] dUlong_k, or nullptr
+  const double* dlr_const; // [ns*ns][3][lr_levels]: dUlong_r0, dUshort_k0, dUlong_k0
   __device__ __host__ inline double* path(int mode) const { return mode ? path1 : path0; }
   __device__ __host__ inline int* perm(int mode) const { return mode ? perm1 : perm0; }
   __device__ __host__ inline d2v* rho(int mode) const { return mode ? rho1 : rho0; }

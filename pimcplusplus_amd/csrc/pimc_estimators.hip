@@ -1,0 +1,343 @@
+// pimc_estimators.hip -- the thermodynamic energy and pair-correlation estimators.
+//
+//   kinetic   KineticClass::d_dBeta          PIMC++/src/Actions/KineticClass.cc:187-256
+//   du_short  ShortRangeClass::d_dBeta       PIMC++/src/Actions/ShortRangeClass.cc:256-295
+//             DavidPAClass::dU / calcUsqz    Common/PairAction/DavidPAClass.cc:115-126, 291-463
+//             PAcoulombBCFitClass::dU        Common/PairAction/PAcoulombBCFit.cc:178-192
+//   v_short   ShortRangePotClass::V          PIMC++/src/Actions/ShortRangePotClass.cc:27-47
+//   du_long   LongRangeClass::d_dBeta        PIMC++/src/Actions/LongRangeClass.cc:374-442
+//   g(r)      PairCorrelationClass::Accumulate  PIMC++/src/Observables/ObservableCorrelation.cc:156-221
+//
+// They run over the whole path and are off the timed path: FP64 throughout, tables
+// read from global memory.  Every block writes its partial sums; the host adds
+// them in block order, so results do not depend on scheduling.
+#include "pimc_kernels.hpp"
+#include "pimc_launch.hpp"
+
+namespace pimc {
+
+// Interval and local coordinate of x in a fit's r grid (clamped to [0, nint-1]).
+__device__ inline int fit_low(const FitDev& F, double x, double& b) {
+  const int l = spline_low(F, F.x, x);
+  const d2v g = F.geo[l];
+  b = (x - g.x) * g.y;
+  return l;
+}
+
+// calcUsqz's dU (DavidPAClass.cc:291-463) for bisection level `level`: r, r'
+// clamped into the grid; dU = (du00(r)+du00(r'))/2 + (V(r)+V(r'))/2 + sum of the
+// dukj off-diagonal terms where s > 0 and q < End.
+template <int NT>
+__device__ inline double david_du(const FitDev& F, int level, double q, double z, double s2) {
+  const d4v* E = F.e64 + (size_t)(level + F.taupos) * F.nint * F.ecol;
+  const double rmin = F.start;
+  double r = q + 0.5 * z, rp = q - 0.5 * z;
+  if (r > F.end) r = F.end;
+  if (rp > F.end) rp = F.end;
+  if (rp < rmin) rp = rmin;
+  if (r < rmin) r = rmin;
+  double br, bp;
+  const int lr = fit_low(F, r, br), lp = fit_low(F, rp, bp);
+  const double V = 0.5 * (horner64(E[lr * F.ecol], br) + horner64(E[lp * F.ecol], bp));
+  double dU = 0.5 * (horner64(E[lr * F.ecol + 1], br) + horner64(E[lp * F.ecol + 1], bp));
+  dU += V;
+  if constexpr (NT > 0) {
+    const double s = sqrt(s2);
+    if (s > 0.0 && q < F.end) {
+      const double zsquared = z * z, ssquared = s * s, ssquaredinverse = 1.0 / ssquared;
+      double bq;
+      const int lq = fit_low(F, q, bq);
+      double Sto2k = ssquared;
+      int col = 2;
+#pragma unroll
+      for (int k = 1; k <= NT; k++) {
+        double Zto2j = 1.0, currS = Sto2k;
+#pragma unroll
+        for (int j = 0; j <= k; j++, col++) {
+          dU += horner64(E[lq * F.ecol + col], bq) * Zto2j * currS;
+          Zto2j *= zsquared;
+          currS = currS * ssquaredinverse;
+        }
+        Sto2k = Sto2k * ssquared;
+      }
+    }
+  }
+  return dU;
+}
+
+__device__ inline double bc_du(const FitDev& F, int level, double q, double s2) {
+  const GridDev Q = qgrid_of(F);
+  const d4v* tab = F.dbc + (size_t)level * F.ng * F.nt;
+  if (q <= (F.end * 1.0000001)) {
+    if (q == 0) return bicubic(F, Q, tab, 0.0, 0.0);
+    return bicubic(F, Q, tab, q, sqrt(s2) / (2.0 * q));
+  }
+  return F.z1z2 / q;
+}
+
+// DavidPAClass::V: column 0 of the first stored level at r clamped to the grid.
+__device__ inline double david_v(const FitDev& F, double r) {
+  if (r > F.end) r = F.end;
+  if (r < F.start) r = F.start;
+  double b;
+  const int l = fit_low(F, r, b);
+  return horner64(F.e64[l * F.ecol], b);
+}
+
+// Pair p of the N(N-1)/2 pairs (ptcl1 > ptcl2), in the reference's loop order.
+__device__ inline void pair_of(long long p, int& p1, int& p2) {
+  long long i = (long long)((1.0 + sqrt(1.0 + 8.0 * (double)p)) * 0.5);
+  while (i * (i - 1) / 2 > p) i--;
+  while ((i + 1) * i / 2 <= p) i++;
+  p1 = (int)i;
+  p2 = (int)(p - i * (i - 1) / 2);
+}
+
+// Nearest-image distance of one slice (DistDisp(slice, p1, p2), PathClass.h:638-660).
+__device__ inline double slice_dist(const RepCore& R, const double a[3], const double b[3]) {
+#pragma clang fp contract(off)
+  double d2 = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    double disp = b[d] - a[d];
+    const double n = rint(disp * R.boxinv[d]);
+    disp -= n * R.isper[d] * R.box[d];
+    d2 += disp * disp;
+  }
+  return sqrt(d2);
+}
+
+// dU and V of every pair over the whole path of copy `mode` (slices [0, M], level 0).
+// partials[block] = {du, v}.
+template <int NT>
+__global__ void __launch_bounds__(kBlock) k_energy_pairs(const ReplicaDev* reps, const FitDev* fits, int rep, int mode,
+                                                         double* partials) {
+  __shared__ double scratch_raw[kWaves];
+  PIMC_LDS double* scratch = (PIMC_LDS double*)scratch_raw;
+  const ReplicaDev* RD = reps + rep;
+  const RepCore R = load_uniform(&RD->core);
+  const RepSpecies& S = RD->sp;
+  const int M = R.nslices - 1;
+  const long long npairs = (long long)R.N * (R.N - 1) / 2;
+  double du = 0.0, v = 0.0;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npairs;
+       p += (long long)gridDim.x * blockDim.x) {
+    int p1, p2;
+    pair_of(p, p1, p2);
+    int s1 = 0, s2 = 0;
+    for (int s = 1; s < R.ns; s++) {
+      if (p1 >= S.first[s]) s1 = s;
+      if (p2 >= S.first[s]) s2 = s;
+    }
+    const int pr = s1 * R.ns + s2;
+    const FitDev& F = fits[S.fit[pr]];
+    const double* dul = (F.kind == 1 && R.use_long_range && R.dulong)
+                            ? R.dulong + (size_t)pr * R.ulong_levels * R.ulong_n * 6
+                            : nullptr;
+    double a1[3], a2[3];
+    load_bead(R, mode, 0, p1, a1);
+    load_bead(R, mode, 0, p2, a2);
+    if (F.kind == 0) v += 0.5 * david_v(F, slice_dist(R, a1, a2));
+    for (int slice = 0; slice < M; slice++) {
+      double b1[3], b2[3];
+      load_bead(R, mode, slice + 1, p1, b1);
+      load_bead(R, mode, slice + 1, p2, b2);
+      double da[3], db[3];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        double n, m;
+        dist_disp_1(a2[d] - a1[d], b2[d] - b1[d], R.box[d], R.boxinv[d], R.isper[d], da[d], db[d], n, m);
+      }
+      const double rmag = sqrt(da[0] * da[0] + da[1] * da[1] + da[2] * da[2]);
+      const double rpmag = sqrt(db[0] * db[0] + db[1] * db[1] + db[2] * db[2]);
+      const double e0 = da[0] - db[0], e1 = da[1] - db[1], e2 = da[2] - db[2];
+      const double s2v = e0 * e0 + e1 * e1 + e2 * e2;
+      const double q = 0.5 * (rmag + rpmag), z = rmag - rpmag;
+      du += F.kind == 1 ? bc_du(F, 0, q, s2v) : david_du<NT>(F, 0, q, z, s2v);
+      if (dul) du -= 0.5 * (ulong_eval(R, dul, rmag) + ulong_eval(R, dul, rpmag));
+      if (F.kind == 0) v += (slice + 1 == M ? 0.5 : 1.0) * david_v(F, slice_dist(R, b1, b2));
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        a1[d] = b1[d];
+        a2[d] = b2[d];
+      }
+    }
+  }
+  const double tdu = block_sum(du, scratch);
+  const double tv = block_sum(v, scratch);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = tdu;
+    partials[2 * blockIdx.x + 1] = tv;
+  }
+}
+
+// KineticClass::d_dBeta over every particle and link of copy `mode`.
+__global__ void __launch_bounds__(kBlock) k_energy_kinetic(const ReplicaDev* reps, int rep, int mode, double* partials) {
+  __shared__ double scratch_raw[kWaves];
+  PIMC_LDS double* scratch = (PIMC_LDS double*)scratch_raw;
+  const ReplicaDev* RD = reps + rep;
+  const RepCore R = load_uniform(&RD->core);
+  const RepSpecies& S = RD->sp;
+  const int M = R.nslices - 1;
+  const double levelTau = R.tau;
+  const long long nitems = (long long)R.N * M;
+  double spring = 0.0;
+  for (long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x; it < nitems;
+       it += (long long)gridDim.x * blockDim.x) {
+    const int ptcl = (int)(it / M), slice = (int)(it % M);
+    int sp = 0;
+    for (int s = 1; s < R.ns; s++)
+      if (ptcl >= S.first[s]) sp = s;
+    const double lambda = S.lambda[sp];
+    if (lambda == 0.0) continue;
+    const double FourLambdaTauInv = 1.0 / (4.0 * lambda * levelTau);
+    spring += (0.5 * 3) / levelTau;
+    double a[3], b[3];
+    load_bead(R, mode, slice, ptcl, a);
+    load_bead(R, mode, slice + 1, ptcl, b);
+    double GaussSum[3], numSum[3], Z = 1.0;
+#pragma unroll
+    for (int dim = 0; dim < 3; dim++) {
+      const double vel = put_in_box_1(b[dim] - a[dim], R.box[dim], R.boxinv[dim], R.isper[dim]);
+      GaussSum[dim] = 0.0;
+      numSum[dim] = 0.0;
+      for (int image = -R.num_images; image <= R.num_images; image++) {
+        const double dist = vel + (double)image * R.box[dim];
+        const double d2overFLT = dist * dist * FourLambdaTauInv;
+        const double expPart = exp(-d2overFLT);
+        GaussSum[dim] += expPart;
+        numSum[dim] += -d2overFLT / levelTau * expPart;
+      }
+      Z *= GaussSum[dim];
+    }
+    double scalarnumSum = 0.0;
+#pragma unroll
+    for (int dim = 0; dim < 3; dim++) {
+      double numProd = 1.0;
+#pragma unroll
+      for (int dim2 = 0; dim2 < 3; dim2++) numProd *= (dim2 != dim) ? GaussSum[dim2] : numSum[dim2];
+      scalarnumSum += numProd;
+    }
+    spring += scalarnumSum / Z;
+  }
+  const double t = block_sum(spring, scratch);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// LongRangeClass::d_dBeta: one block per slice of [0, M], rho_k of copy `mode`.
+// partials[slice] = {homo + hetero, background, k0} with the slice's factor.
+__global__ void __launch_bounds__(kBlock) k_energy_long_range(const ReplicaDev* reps, int rep, int mode,
+                                                              double* partials) {
+  __shared__ double scratch_raw[kWaves];
+  PIMC_LDS double* scratch = (PIMC_LDS double*)scratch_raw;
+  const ReplicaDev* RD = reps + rep;
+  const RepCore R = load_uniform(&RD->core);
+  const RepSpecies& S = RD->sp;
+  const int M = R.nslices - 1;
+  const int slice = blockIdx.x;
+  const double factor = (slice == 0 || slice == M) ? 0.5 : 1.0;
+  const d2v* rho = R.rho(mode) + (size_t)slice * R.ns * R.nk;
+  const int L = R.lr_levels;
+  double part = 0.0;
+  for (int k = threadIdx.x; k < R.nk; k += blockDim.x) {
+    for (int s = 0; s < R.ns; s++) {
+      const int pr = s * R.ns + s;
+      if (!((R.lr_mask >> pr) & 1u)) continue;
+      const d2v a = rho[(size_t)s * R.nk + k];
+      part += factor * 0.5 * 2.0 * (a.x * a.x + a.y * a.y) * R.dulong_k[((size_t)pr * L + 0) * R.nk + k];
+    }
+    for (int s1 = 0; s1 < R.ns; s1++)
+      for (int s2 = s1 + 1; s2 < R.ns; s2++) {
+        const int pr = s1 * R.ns + s2;
+        if (!((R.lr_mask >> pr) & 1u)) continue;
+        const d2v a = rho[(size_t)s1 * R.nk + k], b = rho[(size_t)s2 * R.nk + k];
+        part += factor * 2.0 * (a.x * b.x + a.y * b.y) * R.dulong_k[((size_t)pr * L + 0) * R.nk + k];
+      }
+  }
+  const double t = block_sum(part, scratch);
+  if (threadIdx.x == 0) {
+    double homo = 0.0, background = 0.0, k0 = 0.0;
+    for (int s = 0; s < R.ns; s++) {
+      const int pr = s * R.ns + s;
+      if (!((R.lr_mask >> pr) & 1u)) continue;
+      const int N = S.count[s];
+      homo -= factor * 0.5 * N * R.dlr_const[(size_t)(pr * 3 + 0) * L + 0];
+      background -= factor * 0.5 * N * N * R.dlr_const[(size_t)(pr * 3 + 1) * L + 0];
+      k0 += factor * 0.5 * N * N * R.dlr_const[(size_t)(pr * 3 + 2) * L + 0];
+    }
+    for (int s1 = 0; s1 < R.ns; s1++)
+      for (int s2 = s1 + 1; s2 < R.ns; s2++) {
+        const int pr = s1 * R.ns + s2;
+        if (!((R.lr_mask >> pr) & 1u)) continue;
+        const int N1 = S.count[s1], N2 = S.count[s2];
+        background -= factor * N1 * N2 * R.dlr_const[(size_t)(pr * 3 + 1) * L + 0];
+        k0 += factor * N1 * N2 * R.dlr_const[(size_t)(pr * 3 + 2) * L + 0];
+      }
+    partials[3 * slice] = t + homo;
+    partials[3 * slice + 1] = background;
+    partials[3 * slice + 2] = k0;
+  }
+}
+
+// PairCorrelationClass::Accumulate into hist (64-bit counters).
+__global__ void __launch_bounds__(kBlock) k_pair_correlation(const ReplicaDev* reps, int rep, int mode, int f1,
+                                                             int n1, int f2, int n2, int same, double start,
+                                                             double deltainv, double end, int nbins,
+                                                             unsigned long long* hist) {
+  const ReplicaDev* RD = reps + rep;
+  const RepCore R = load_uniform(&RD->core);
+  const int M = R.nslices - 1;
+  const long long npairs = same ? (long long)n1 * (n1 - 1) / 2 : (long long)n1 * n2;
+  const long long nitems = npairs * M;
+  for (long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x; it < nitems;
+       it += (long long)gridDim.x * blockDim.x) {
+    const int slice = (int)(it / npairs);
+    const long long p = it % npairs;
+    int p1, p2;
+    if (same) {
+      int hi, lo;
+      pair_of(p, hi, lo);   // hi > lo
+      p1 = f1 + lo;
+      p2 = f1 + hi;
+    } else {
+      p1 = f1 + (int)(p / n2);
+      p2 = f2 + (int)(p % n2);
+    }
+    double a[3], b[3];
+    load_bead(R, mode, slice, p1, a);
+    load_bead(R, mode, slice, p2, b);
+    const double dist = slice_dist(R, a, b);
+    if (dist < end && dist >= start) {
+      int index;
+      {
+#pragma clang fp contract(off)
+        index = clamp_to_int(rint((dist - start) * deltainv - 0.5), nbins + 1);
+      }
+      if (index >= 0 && index < nbins) atomicAdd(hist + index, 1ull);
+    }
+  }
+}
+
+hipError_t launch_energy(int num_terms, const ReplicaDev* reps, const FitDev* fits, int rep, int mode,
+                         int pair_blocks, int kin_blocks, int lr_slices, double* pair_partials,
+                         double* kin_partials, double* lr_partials, hipStream_t s) {
+  switch (num_terms) {
+    case 0: k_energy_pairs<0><<<pair_blocks, kBlock, 0, s>>>(reps, fits, rep, mode, pair_partials); break;
+    case 1: k_energy_pairs<1><<<pair_blocks, kBlock, 0, s>>>(reps, fits, rep, mode, pair_partials); break;
+    case 2: k_energy_pairs<2><<<pair_blocks, kBlock, 0, s>>>(reps, fits, rep, mode, pair_partials); break;
+    default: k_energy_pairs<3><<<pair_blocks, kBlock, 0, s>>>(reps, fits, rep, mode, pair_partials); break;
+  }
+  k_energy_kinetic<<<kin_blocks, kBlock, 0, s>>>(reps, rep, mode, kin_partials);
+  if (lr_slices > 0) k_energy_long_range<<<lr_slices, kBlock, 0, s>>>(reps, rep, mode, lr_partials);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_correlation(const ReplicaDev* reps, int rep, int mode, int f1, int n1, int f2, int n2,
+                                   int same, double start, double deltainv, double end, int nbins,
+                                   unsigned long long* hist, int blocks, hipStream_t s) {
+  k_pair_correlation<<<blocks, kBlock, 0, s>>>(reps, rep, mode, f1, n1, f2, n2, same, start, deltainv, end, nbins,
+                                               hist);
+  return hipGetLastError();
+}
+
+}  // namespace pimc

@@ -41,6 +41,15 @@ void launch_rho_all(const ReplicaDev* reps, int rep, int nslices, int ns, int nk
 void launch_long_range(bool mixed, const ReplicaDev* reps, const ReqDev* reqs, int nreq, const int* strides,
                        double* out, hipStream_t s);
 
+// Energy estimator: partial sums per block (pairs: {du, v}; kinetic; long range per
+// slice: {homo+hetero, background, k0}); lr_slices = 0 skips the long-range kernel.
+hipError_t launch_energy(int num_terms, const ReplicaDev* reps, const FitDev* fits, int rep, int mode,
+                         int pair_blocks, int kin_blocks, int lr_slices, double* pair_partials,
+                         double* kin_partials, double* lr_partials, hipStream_t s);
+hipError_t launch_pair_correlation(const ReplicaDev* reps, int rep, int mode, int f1, int n1, int f2, int n2,
+                                   int same, double start, double deltainv, double end, int nbins,
+                                   unsigned long long* hist, int blocks, hipStream_t s);
+
 // The GPU-resident bisection chain: one block per replica, num_moves moves each.
 struct LaunchBisection {
   const ReplicaDev* reps;

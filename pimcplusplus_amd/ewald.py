@@ -34,6 +34,14 @@ def ewald_tables(kvecs: np.ndarray, box, z1z2: float, alpha: float, tau: float, 
     return ulong_k, ulong_r0, ushort_k0, ulong_k0
 
 
+def ewald_dbeta_tables(kvecs: np.ndarray, box, z1z2: float, alpha: float, tau: float, num_levels: int):
+    """The beta derivatives (dUlong_k, dUlong_r0, dUshort_k0, dUlong_k0): every table above is
+    tau_l times a potential, so its derivative is that potential at every level."""
+    ulong_k, ulong_r0, ushort_k0, ulong_k0 = ewald_tables(kvecs, box, z1z2, alpha, tau, num_levels)
+    taus = tau * 2.0 ** np.arange(num_levels)
+    return ulong_k / taus[:, None], ulong_r0 / taus, ushort_k0 / taus, ulong_k0 / taus
+
+
 def kset_size(n: int) -> int:
     """Half-space k vectors with |k| < n 2 pi / L in a cubic box: (#{m in Z^3 : 0 < |m|^2 < n^2}) / 2."""
     r = np.arange(-n, n + 1)
@@ -57,3 +65,11 @@ def ulong_quintic(z1z2: float, alpha: float, tau: float, num_levels: int, r_end:
     for lev in range(num_levels):
         out[lev] = tau * 2 ** lev * z1z2 * base
     return 0.0, float(x[-1]), out
+
+
+def dulong_quintic(z1z2: float, alpha: float, tau: float, num_levels: int, r_end: float, n: int = 1000):
+    """dUlong(level)(r) = Z1Z2 erf(alpha r)/r on the grid of ulong_quintic (the derivative of
+    tau_l Vlong with respect to tau_l)."""
+    start, end, coefs = ulong_quintic(z1z2, alpha, tau, num_levels, r_end, n)
+    taus = tau * 2.0 ** np.arange(num_levels)
+    return start, end, coefs / taus[:, None, None]

@@ -222,9 +222,12 @@ def lj_david_table(eps: float, sigma: float, r_min: float, r_max: float, num_poi
         y[lev, :, 1] = tau * vr
         if num_terms == 2:
             y[lev, :, 2:7] = switch[:, None] * straight_line_ukj(r, tau, f1, f2, f3, f4)
-        # the straight-line action is linear in tau: dU/dtau = U/tau; column 0 stays V
+        # the straight-line action is linear in tau: dU/dtau = U/tau; column 0 stays V.  The
+        # diagonal column holds d(u00)/dtau - V, the .dm convention that calcUsqz
+        # compensates by adding V back (DavidPAClass.cc:362-366): here 0.
         dy[lev, :, 0] = vr
         dy[lev, :, 1:] = y[lev, :, 1:] / tau
+        dy[lev, :, 1] = 0.0
     y[:, num_points - 1, :] = 0.0    # the .dm reader's convention (DavidPAClass.cc:759)
     dy[:, num_points - 1, :] = 0.0
     return DavidTable(grid_type=GRID_LINEAR, grid_start=float(r_min), grid_end=float(r_max), y=y,
@@ -241,6 +244,7 @@ class CoulombBCTable:
     umat: np.ndarray           # [num_betas][nq][nt]
     smallest_beta: float
     z1z2: float
+    dumat: np.ndarray | None = None   # dU/dbeta, same layout
 
     def grid_points(self, which: str) -> np.ndarray:
         gtype, start, end = self.q_grid if which == "q" else self.t_grid
@@ -279,6 +283,33 @@ def david_u(table: DavidTable, level_index: int, q, z, s2):
     return out
 
 
+def david_du(table: DavidTable, level_index: int, q, z, s2):
+    """DavidPAClass::dU (calcUsqz, DavidPAClass.cc:291-463) of a DavidTable level with its
+    beta-derivative table, evaluated with scipy splines: r and r' clamped into the grid,
+    dU = (du00(r) + du00(r'))/2 + (V(r) + V(r'))/2 + off-diagonal dukj terms."""
+    from scipy.interpolate import CubicSpline
+    if table.dy is None:
+        raise ValueError("table has no beta-derivative columns")
+    x = table.grid_points()
+    y, dy = table.y[level_index], table.dy[level_index]
+    bc = ((2, 0.0), (1, 0.0))
+    q, z, s2 = (np.asarray(v, dtype=np.float64) for v in (q, z, s2))
+    rmin, end = x[0], x[-1]
+    r = np.clip(q + 0.5 * z, rmin, end)
+    rp = np.clip(q - 0.5 * z, rmin, end)
+    v0 = CubicSpline(x, y[:, 0], bc_type=bc)
+    d1 = CubicSpline(x, dy[:, 1], bc_type=bc)
+    out = 0.5 * (d1(r) + d1(rp)) + 0.5 * (v0(r) + v0(rp))
+    use = (s2 > 0) & (q < end)
+    z2 = z * z
+    qc = np.minimum(q, end)
+    for k in range(1, table.num_terms + 1):
+        for j in range(0, k + 1):
+            c = k * (k + 1) // 2 + j + 1
+            out = out + np.where(use, CubicSpline(x, dy[:, c], bc_type=bc)(qc) * z2 ** j * s2 ** (k - j), 0.0)
+    return out
+
+
 def coulomb_bc_from_david(table: DavidTable, tau: float, num_betas: int, z1z2: float,
                           q_grid=(GRID_LOG, 0.0025, 9.5), nq: int = 200,
                           t_grid=(GRID_LINEAR, 0.0, 1.0), nt: int = 20) -> CoulombBCTable:
@@ -288,13 +319,17 @@ def coulomb_bc_from_david(table: DavidTable, tau: float, num_betas: int, z1z2: f
     t = _grid(*t_grid, nt)
     taus = table.smallest_tau * 2.0 ** np.arange(table.num_tau)
     umat = np.zeros((num_betas, nq, nt))
+    dumat = np.zeros((num_betas, nq, nt)) if table.dy is not None else None
     Q, T = np.meshgrid(q, t, indexing="ij")
     for b in range(num_betas):
         lev = int(np.argmin(np.abs(taus - tau * 2 ** b)))
         if abs(taus[lev] - tau * 2 ** b) > 1e-4:
             raise ValueError(f"table has no tau level {tau * 2 ** b}")
-        umat[b] = david_u(table, lev, Q.ravel(), np.zeros(Q.size), (2 * Q * T).ravel() ** 2).reshape(nq, nt)
-    return CoulombBCTable(q_grid, t_grid, umat, tau, z1z2)
+        args = (Q.ravel(), np.zeros(Q.size), (2 * Q * T).ravel() ** 2)
+        umat[b] = david_u(table, lev, *args).reshape(nq, nt)
+        if dumat is not None:
+            dumat[b] = david_du(table, lev, *args).reshape(nq, nt)
+    return CoulombBCTable(q_grid, t_grid, umat, tau, z1z2, dumat)
 
 
 def coulomb_chord_table(z1z2: float, tau: float, num_betas: int, q_grid=(GRID_LOG, 0.01, 10.0), nq: int = 200,
@@ -308,4 +343,5 @@ def coulomb_chord_table(z1z2: float, tau: float, num_betas: int, q_grid=(GRID_LO
     umat = np.zeros((num_betas, nq, nt))
     for b in range(num_betas):
         umat[b] = tau * 2 ** b * z1z2 * g[None, :] / q[:, None]
-    return CoulombBCTable(q_grid, t_grid, umat, tau, z1z2)
+    dumat = np.repeat((z1z2 * g[None, :] / q[:, None])[None], num_betas, axis=0)   # linear in tau
+    return CoulombBCTable(q_grid, t_grid, umat, tau, z1z2, dumat)

@@ -15,7 +15,7 @@ import numpy as np
 
 import math
 
-from .ewald import ewald_tables, ulong_quintic
+from .ewald import dulong_quintic, ewald_dbeta_tables, ewald_tables, ulong_quintic
 from .tables import (GRID_LINEAR, GRID_LOG, CoulombBCTable, DavidTable, coulomb_bc_from_david,
                      coulomb_chord_table, lj_david_table)
 
@@ -141,10 +141,11 @@ def load(lib, system: System):
     fit_ids = {}
     for key, t in system.tables.items():
         if isinstance(t, CoulombBCTable):
-            fit_ids[key] = lib.fit_create_coulomb_bc(t.q_grid, t.t_grid, t.umat, t.smallest_beta, t.z1z2)
+            fit_ids[key] = lib.fit_create_coulomb_bc(t.q_grid, t.t_grid, t.umat, t.smallest_beta, t.z1z2,
+                                                     dumat=t.dumat)
         else:
             fit_ids[key] = lib.fit_create_david(t.grid_type, t.grid_start, t.grid_end, t.y, t.num_terms,
-                                                t.smallest_tau, system.desired_tau[key])
+                                                t.smallest_tau, system.desired_tau[key], dy=t.dy)
     ids = []
     for spec in system.replicas:
         rid = lib.replica_create(spec.num_particles, spec.num_time_slices, spec.box, spec.tau,
@@ -157,12 +158,14 @@ def load(lib, system: System):
             r_end = 0.75 * math.sqrt(sum(b * b for b in spec.box))
             for (sa, sb), (z1z2, alpha) in spec.ulong.items():
                 start, end, coefs = ulong_quintic(z1z2, alpha, spec.tau, system.num_levels, r_end)
-                lib.set_ulong(rid, sa, sb, start, end, coefs)
+                _, _, dcoefs = dulong_quintic(z1z2, alpha, spec.tau, system.num_levels, r_end)
+                lib.set_ulong(rid, sa, sb, start, end, coefs, dcoefs)
         if spec.ewald:
             kv, _ = lib.kvecs(rid)
             for (sa, sb), (z1z2, alpha) in spec.ewald.items():
-                lib.set_long_range(rid, sa, sb, *ewald_tables(kv, spec.box, z1z2, alpha, spec.tau,
-                                                              system.num_levels))
+                lib.set_long_range(rid, sa, sb,
+                                   *ewald_tables(kv, spec.box, z1z2, alpha, spec.tau, system.num_levels),
+                                   *ewald_dbeta_tables(kv, spec.box, z1z2, alpha, spec.tau, system.num_levels))
             lib.rho_k_update_all(rid)
         ids.append(rid)
     return ids

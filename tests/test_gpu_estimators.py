@@ -1,0 +1,111 @@
+"""GPU estimators against the oracle (FP64, 1e-12; histograms exact), and the
+thermodynamic energy of free particles against its closed form 3N/(2 beta)."""
+
+import math
+
+import numpy as np
+import pytest
+
+from pimcplusplus_amd import abi, tables, workloads
+from tests.helpers import GOLDEN
+from tests.test_oracle_estimators import free_table
+
+pytestmark = pytest.mark.gpu
+
+
+def both(oracle, gpu, system):
+    return workloads.load(oracle, system), workloads.load(gpu, system)
+
+
+def close(a, b, rel=1e-12):
+    return abs(a - b) <= rel * (1.0 + abs(b))
+
+
+def check_energy(oracle, gpu, system, moves=30):
+    io, ig = both(oracle, gpu, system)
+    oracle.bisection_run(io, system.num_levels, system.seed, 0, moves)
+    gpu.bisection_run(ig, system.num_levels, system.seed, 0, moves, abi.FP64)
+    for mode in (abi.OLDMODE, abi.NEWMODE):
+        eo = oracle.energy(io, mode)
+        eg = gpu.energy(ig, mode)
+        for a, b in zip(eg, eo):
+            for f in ("kinetic", "du_short", "du_long", "total"):
+                assert close(getattr(a, f), getattr(b, f)), (f, getattr(a, f), getattr(b, f))
+            if math.isnan(b.v_short):
+                assert math.isnan(a.v_short)
+            else:
+                assert close(a.v_short, b.v_short)
+    return eo
+
+
+def test_energy_lj(oracle, gpu):
+    check_energy(oracle, gpu, workloads.config0(num_replicas=3))
+
+
+def test_energy_charged_lj_long_range(oracle, gpu):
+    e = check_energy(oracle, gpu, workloads.charged_lj_system(n_plus=12, n_minus=12, m=16, num_replicas=2))
+    assert e[0].du_long != 0.0
+
+
+def test_energy_hydrogen(oracle, gpu):
+    t = tables.load_npz(f"{GOLDEN}/testcoul_fit2.npz")
+    e = check_energy(oracle, gpu, workloads.hydrogen_system(t, num_replicas=2, n_pairs=8, m=32, num_levels=3))
+    assert math.isnan(e[0].v_short) and e[0].du_long != 0.0
+
+
+def test_energy_ocp_background(oracle, gpu):
+    check_energy(oracle, gpu, workloads.ocp_system(num_replicas=2, n=64, m=16, num_levels=3, kc_units=4.0))
+
+
+def test_pair_correlation_matches_oracle(oracle, gpu):
+    sysm = workloads.charged_lj_system(n_plus=12, n_minus=20, m=16, num_replicas=1)
+    io, ig = both(oracle, gpu, sysm)
+    oracle.bisection_run(io, sysm.num_levels, sysm.seed, 0, 40, species=1)
+    gpu.bisection_run(ig, sysm.num_levels, sysm.seed, 0, 40, abi.FP64, species=1)
+    L = sysm.replicas[0].box[0]
+    for s1, s2 in ((0, 0), (1, 1), (0, 1)):
+        ho = oracle.pair_correlation(io[0], s1, s2, 0.0, L / 2, 101)
+        hg = gpu.pair_correlation(ig[0], s1, s2, 0.0, L / 2, 101)
+        assert np.array_equal(ho, hg) and ho.sum() > 0
+
+
+def test_free_particle_energy_is_three_n_over_two_beta(gpu):
+    # distinguishable free particles, thermal wavelength << box: <E> = 3N/(2 beta)
+    n, m, tau, lam, box, nrep = 6, 16, 0.1, 0.5, 40.0, 512
+    t = free_table(box, smallest_tau=tau, num_tau=4)
+    rng = np.random.default_rng(7)
+    reps = []
+    for r in range(nrep):
+        beads = np.repeat(rng.uniform(-box / 2, box / 2, size=(1, n, 3)), m + 1, axis=0)
+        reps.append(workloads.ReplicaSpec(num_particles=n, num_time_slices=m, box=(box, box, box), tau=tau,
+                                          species_count=[n], lambdas=[lam], fit_keys=["free"], beads=beads,
+                                          rng_stream=r))
+    sysm = workloads.System("free", {"free": t}, {"free": tau}, reps, 3, 4242)
+    ids = workloads.load(gpu, sysm)
+    move, samples = 0, []
+    for block in range(60):
+        gpu.bisection_run(ids, 3, sysm.seed, move, 40, abi.FP64, records=False, wait=False)
+        move += 40
+        if block >= 20:
+            samples.append([e.total / m for e in gpu.energy(ids)])
+    s = np.array(samples)               # [samples][replicas]
+    per_rep = s.mean(axis=0)            # replicas are independent
+    mean, err = per_rep.mean(), per_rep.std(ddof=1) / math.sqrt(nrep)
+    exact = 3.0 * n / (2.0 * m * tau)
+    assert abs(mean - exact) < 4.0 * err, (mean, err, exact)
+    assert err < 0.05 * exact
+
+
+def test_mixed_and_fp64_energies_agree_within_error_bars(gpu):
+    sysm = workloads.config0(num_replicas=256)
+    res = {}
+    for prec in (abi.FP64, abi.MIXED):
+        ids = workloads.load(gpu, sysm)
+        vals = []
+        for block in range(8):
+            gpu.bisection_run(ids, sysm.num_levels, sysm.seed, 50 * block, 50, prec, records=False, wait=False)
+            vals.append([e.total for e in gpu.energy(ids)])
+        res[prec] = np.array(vals[2:]).mean(axis=0)
+    d = res[abi.MIXED] - res[abi.FP64]
+    err = math.sqrt(res[abi.FP64].var(ddof=1) / len(d) + res[abi.MIXED].var(ddof=1) / len(d))
+    assert abs(d.mean()) < 4.0 * err + 1e-9
