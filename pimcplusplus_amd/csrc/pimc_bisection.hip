@@ -119,8 +119,9 @@ __device__ inline void sr_partners(const RepCore& R, const FitDev& F, const Tabl
       acc_new.s += un;
     }
   };
-  const int ppass = (int)blockDim.x / ls;             // partners per pass
-  const int sub = (int)threadIdx.x / ppass;           // this thread's link range
+  const int ppass = (int)blockDim.x / ls;             // partners per pass: a multiple of the wave size
+  // this thread's link range; whole waves share one, so it is wave-uniform (scalar)
+  const int sub = __builtin_amdgcn_readfirstlane((int)threadIdx.x / ppass);
   const int split = (K / 2 >= ls) ? ls : (K / 2);     // ranges in use: each >= 2 links
   if (sub >= split) return;
   const int kper = K / split;
@@ -552,14 +553,14 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, uin
 // replicas from the work queue in list order (the host lists them longest first)
 // until the list is exhausted.  first_move_dev, when set, holds the first move
 // number (a captured sweep advances it on the device).
-template <int NT, bool MIXED, int VAR, int BLOCK>
+template <int NT, bool MIXED, int VAR, int BLOCK, bool PERSIST>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PIMC_BISECTION_WAVES)))
 k_bisection(const LaunchBisection a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   PIMC_LDS unsigned char* smem = (PIMC_LDS unsigned char*)smem_raw;
   PIMC_LDS int* take = (PIMC_LDS int*)(smem + bisection_ctl_offset(a)) + 3;   // control word 3
   const uint32_t first_move = a.first_move_dev ? *a.first_move_dev : a.first_move;
-  if (!a.queue) {
+  if constexpr (!PERSIST) {
     bisection_replica<NT, MIXED, VAR>(a, blockIdx.x, first_move, smem);
     return;
   }
@@ -584,16 +585,22 @@ static size_t bisection_lds_bytes(const LaunchBisection& a) {
   return bisection_ctl_offset(a) + 4 * sizeof(int) + a.lds_lr_bytes;
 }
 
-template <bool MIXED, int VAR, int BLOCK>
-static hipError_t launch_bis_b(int nt, const LaunchBisection& a, int grid, hipStream_t s) {
+template <bool MIXED, int VAR, int BLOCK, bool PERSIST>
+static hipError_t launch_bis_p(int nt, const LaunchBisection& a, int grid, hipStream_t s) {
   const size_t lds = bisection_lds_bytes(a);
   switch (nt) {
-    case 0: k_bisection<0, MIXED, VAR, BLOCK><<<grid, BLOCK, lds, s>>>(a); break;
-    case 1: k_bisection<1, MIXED, VAR, BLOCK><<<grid, BLOCK, lds, s>>>(a); break;
-    case 2: k_bisection<2, MIXED, VAR, BLOCK><<<grid, BLOCK, lds, s>>>(a); break;
-    default: k_bisection<3, MIXED, VAR, BLOCK><<<grid, BLOCK, lds, s>>>(a); break;
+    case 0: k_bisection<0, MIXED, VAR, BLOCK, PERSIST><<<grid, BLOCK, lds, s>>>(a); break;
+    case 1: k_bisection<1, MIXED, VAR, BLOCK, PERSIST><<<grid, BLOCK, lds, s>>>(a); break;
+    case 2: k_bisection<2, MIXED, VAR, BLOCK, PERSIST><<<grid, BLOCK, lds, s>>>(a); break;
+    default: k_bisection<3, MIXED, VAR, BLOCK, PERSIST><<<grid, BLOCK, lds, s>>>(a); break;
   }
   return hipGetLastError();
+}
+
+template <bool MIXED, int VAR, int BLOCK>
+static hipError_t launch_bis_b(int nt, const LaunchBisection& a, int grid, hipStream_t s) {
+  return a.queue ? launch_bis_p<MIXED, VAR, BLOCK, true>(nt, a, grid, s)
+                 : launch_bis_p<MIXED, VAR, BLOCK, false>(nt, a, grid, s);
 }
 
 template <bool MIXED, int VAR>
@@ -631,9 +638,10 @@ int bisection_blocks_per_cu(int num_terms, bool mixed, bool lean, int block, siz
   int n = 0;
   hipError_t e;
   auto q = [&](auto kern) { return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, block, lds); };
-#define PIMC_OCC(NT_, B_)                                                                                   \
-  (lean ? (mixed ? q(k_bisection<NT_, true, kLeanDavid, B_>) : q(k_bisection<NT_, false, kLeanDavid, B_>)) \
-        : (mixed ? q(k_bisection<NT_, true, kGeneric, B_>) : q(k_bisection<NT_, false, kGeneric, B_>)))
+#define PIMC_OCC(NT_, B_)                                                                       \
+  (lean ? (mixed ? q(k_bisection<NT_, true, kLeanDavid, B_, true>)                                 \
+                 : q(k_bisection<NT_, false, kLeanDavid, B_, true>))                               \
+        : (mixed ? q(k_bisection<NT_, true, kGeneric, B_, true>) : q(k_bisection<NT_, false, kGeneric, B_, true>)))
   const int nt = num_terms > 3 ? 3 : num_terms;
   if (block == 512)
     e = nt == 0 ? PIMC_OCC(0, 512) : nt == 1 ? PIMC_OCC(1, 512) : nt == 2 ? PIMC_OCC(2, 512) : PIMC_OCC(3, 512);
