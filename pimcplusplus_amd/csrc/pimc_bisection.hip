@@ -180,21 +180,15 @@ __device__ inline void sr_partners(const RepCore& R, const FitDev& F, const Tabl
 // BisectionStageClass::Sample for midpoint b of the stage (BisectionStage.cc:128-289):
 // NEW midpoint = rbar + Delta, Delta ~ N(0, sigma^2) per dimension, and the
 // log of the sampling probabilities of the NEW and OLD midpoints.
-__device__ inline void sample_midpoint(uint64_t seed, uint32_t stream, uint32_t move, int level, int b,
-                                             int skip, double sigma, double sigma2, double bx0, double bx1,
-                                             double bx2, double bi0, double bi1, double bi2, double ip0, double ip1,
-                                             double ip2, int num_images, PIMC_LDS double* mold,
-                                             PIMC_LDS double* mnew, PIMC_LDS double* lnew, PIMC_LDS double* lold) {
+__device__ inline void sample_midpoint(const PIMC_LDS double* gauss, int b, int skip, double sigma, double sigma2,
+                                       double bx0, double bx1, double bx2, double bi0, double bi1, double bi2,
+                                       double ip0, double ip1, double ip2, int num_images, PIMC_LDS double* mold,
+                                       PIMC_LDS double* mnew, PIMC_LDS double* lnew, PIMC_LDS double* lold) {
   const int s = b * skip, half = skip >> 1;
-  // gaussians 3b .. 3b+2 of this (move, level) come from pairs (3b)>>1 and (3b)>>1 + 1
-  double g0, g1, g2, g3;
-  const uint32_t p0 = (uint32_t)(3 * b) >> 1;
-  gaussian_pair(seed, stream, move, level, p0, g0, g1);
-  gaussian_pair(seed, stream, move, level, p0 + 1, g2, g3);
-  const bool odd = (3 * b) & 1;
-  const double gd[3] = {odd ? g1 : g0, odd ? g2 : g1, odd ? g3 : g2};
+  // gaussians 3b .. 3b+2 of this (move, level): elements 3b.. of the level's pair sequence
+  const double gd[3] = {gauss[3 * b], gauss[3 * b + 1], gauss[3 * b + 2]};
   const double bxs[3] = {bx0, bx1, bx2}, bis[3] = {bi0, bi1, bi2}, ips[3] = {ip0, ip1, ip2};
-  double GaussProd = 1.0, GaussProdOld = 1.0;
+  double GaussProd = 1.0, GaussProdOld = 1.0, expo = 0.0, expoOld = 0.0;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     const double bx = bxs[d], bi = bis[d], ip = ips[d];
@@ -204,39 +198,63 @@ __device__ inline void sample_midpoint(uint64_t seed, uint32_t stream, uint32_t 
     double rdiff = put_in_box_1(mnew[3 * (s + skip) + d] - mnew[3 * s + d], bx, bi, ip);
     double rbar = mnew[3 * s + d] + 0.5 * rdiff;
     double Delta = put_in_box_1(gd[d] * sigma, bx, bi, ip);
-    double GaussSum = 0.0, GaussSumOld = 0.0;
-    for (int image = -num_images; image <= num_images; image++) {
-      double dist = Delta + (double)image * bx;
-      double distOld = DeltaOld + (double)image * bx;
-      GaussSum += exp(-0.5 * dist * dist / sigma2);
-      GaussSumOld += exp(-0.5 * distOld * distOld / sigma2);
+    if (num_images == 0) {
+      // log of a product of single-image Gaussians: the sum of their exponents
+      expo += -0.5 * Delta * Delta / sigma2;
+      expoOld += -0.5 * DeltaOld * DeltaOld / sigma2;
+    } else {
+      double GaussSum = 0.0, GaussSumOld = 0.0;
+      for (int image = -num_images; image <= num_images; image++) {
+        double dist = Delta + (double)image * bx;
+        double distOld = DeltaOld + (double)image * bx;
+        GaussSum += exp(-0.5 * dist * dist / sigma2);
+        GaussSumOld += exp(-0.5 * distOld * distOld / sigma2);
+      }
+      GaussProd *= GaussSum;
+      GaussProdOld *= GaussSumOld;
     }
-    GaussProd *= GaussSum;
-    GaussProdOld *= GaussSumOld;
     mnew[3 * (s + half) + d] = rbar + Delta;
   }
-  lnew[b] = log(GaussProd);
-  lold[b] = log(GaussProdOld);
+  lnew[b] = num_images == 0 ? expo : log(GaussProd);
+  lold[b] = num_images == 0 ? expoOld : log(GaussProdOld);
+}
+
+// The Gaussian pairs of every level of a move and the Metropolis draws log(u),
+// computed by the whole block at the start of the move (they are pure functions
+// of (seed, stream, move, level, index)).  Level l's pairs start at
+// gauss_offset(W, l) doubles; level l needs the pairs holding gaussians 0 .. 3 nb - 1.
+__host__ __device__ inline int gauss_pairs(int W, int level) {
+  const int nb = W >> (level + 1);
+  return ((3 * nb - 1) >> 1) + 1;
+}
+__host__ __device__ inline int gauss_offset(int W, int level) {
+  int off = 0;
+  for (int l = 0; l < level; l++) off += 2 * gauss_pairs(W, l);
+  return off;
 }
 
 // KineticClass::SingleAction of one link (KineticClass.cc:33-90).
 __device__ inline double kinetic_link_ool(const PIMC_LDS double* a, const PIMC_LDS double* b, double bx0,
-                                                double bx1, double bx2, double bi0, double bi1, double bi2,
-                                                double ip0, double ip1, double ip2, int num_images,
-                                                double FourLambdaTauInv) {
+                                          double bx1, double bx2, double bi0, double bi1, double bi2,
+                                          double ip0, double ip1, double ip2, int num_images,
+                                          double FourLambdaTauInv) {
   const double bxs[3] = {bx0, bx1, bx2}, bis[3] = {bi0, bi1, bi2}, ips[3] = {ip0, ip1, ip2};
-  double GaussProd = 1.0;
+  double GaussProd = 1.0, expo = 0.0;
 #pragma unroll
   for (int dim = 0; dim < 3; dim++) {
     double vel = put_in_box_1(b[dim] - a[dim], bxs[dim], bis[dim], ips[dim]);
-    double GaussSum = 0.0;
-    for (int image = -num_images; image <= num_images; image++) {
-      double dist = vel + (double)image * bxs[dim];
-      GaussSum += exp(-dist * dist * FourLambdaTauInv);
+    if (num_images == 0) {
+      expo += vel * vel * FourLambdaTauInv;   // -log(exp(-x)) summed over dimensions
+    } else {
+      double GaussSum = 0.0;
+      for (int image = -num_images; image <= num_images; image++) {
+        double dist = vel + (double)image * bxs[dim];
+        GaussSum += exp(-dist * dist * FourLambdaTauInv);
+      }
+      GaussProd *= GaussSum;
     }
-    GaussProd *= GaussSum;
   }
-  return -log(GaussProd);
+  return num_images == 0 ? expo : -log(GaussProd);
 }
 
 // log(u) of the stage's Metropolis draw.
@@ -253,7 +271,7 @@ enum { kLeanDavid = 0, kGeneric = 1 };
 // Byte offset of the control words (slice1, ptcl, accept, queue item) in the block's LDS.
 __host__ __device__ inline size_t bisection_ctl_offset(const LaunchBisection& a) {
   const int W = 1 << a.num_levels;
-  return a.lds_table_bytes + sizeof(double) * (6 * (W + 1) + W + 2 * W + kMaxWaves);
+  return a.lds_table_bytes + sizeof(double) * (6 * (W + 1) + W + 2 * W + 2 * kMaxWaves);
 }
 
 template <int NT, bool MIXED, int VAR>
@@ -276,9 +294,11 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, uin
   PIMC_LDS double* lold = lnew + W / 2;                                    // [W/2]
   PIMC_LDS double* kold = lold + W / 2;                                    // [W]
   PIMC_LDS double* knew = kold + W;                                        // [W]
-  PIMC_LDS double* scratch = knew + W;                                     // [kMaxWaves]
-  PIMC_LDS int* ctl = (PIMC_LDS int*)(scratch + kMaxWaves);               // slice1, ptcl, accept, item
-  PIMC_LDS d2v* lrC = (PIMC_LDS d2v*)(ctl + 4);                           // [2][kstride]: NEW, OLD rows
+  PIMC_LDS double* scratch = knew + W;                                     // [2 kMaxWaves]
+  PIMC_LDS int* ctl = (PIMC_LDS int*)(scratch + 2 * kMaxWaves);           // slice1, ptcl, accept, item
+  PIMC_LDS double* gauss = (PIMC_LDS double*)(ctl + 4);                   // [gauss_offset(W, L)]
+  PIMC_LDS double* logu_tab = gauss + gauss_offset(W, L);                 // [PIMC_MAX_LEVELS]
+  PIMC_LDS d2v* lrC = (PIMC_LDS d2v*)(logu_tab + PIMC_MAX_LEVELS);        // [2][kstride]: NEW, OLD rows
 
   int64_t st_moves = 0, st_acc = 0, st_stages = 0, st_evals = 0, st_kevals = 0;
   typedef typename std::conditional<MIXED, float, double>::type LrT;   // phase-sum arithmetic
@@ -290,6 +310,28 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, uin
     if (tid == 0) {
       ctl[0] = (int)floor(rng_uniform(a.seed, R.stream, move, kTagSlice, 0) * numLeft);
       ctl[1] = (int)floor(rng_uniform(a.seed, R.stream, move, kTagPtcl, 0) * S.count[sp]) + S.first[sp];
+    }
+    {
+      // the move's Gaussian pairs (levels lowest .. L-1) and Metropolis draws, one per thread
+      const int npairs_tot = (gauss_offset(W, L) - gauss_offset(W, a.lowest_level)) / 2;
+      const int nlev = L - a.lowest_level;
+      for (int t = tid; t < npairs_tot + nlev; t += blockDim.x) {
+        if (t < npairs_tot) {
+          int level = a.lowest_level, p = t;
+          while (p >= gauss_pairs(W, level)) {
+            p -= gauss_pairs(W, level);
+            level++;
+          }
+          double g0, g1;
+          gaussian_pair(a.seed, R.stream, move, level, (uint32_t)p, g0, g1);
+          PIMC_LDS double* g = gauss + gauss_offset(W, level) + 2 * p;
+          g[0] = g0;
+          g[1] = g1;
+        } else {
+          const int level = a.lowest_level + (t - npairs_tot);
+          logu_tab[level] = log_accept_draw(a.seed, R.stream, move, level);
+        }
+      }
     }
     __syncthreads();
     const int slice1 = ctl[0];
@@ -317,7 +359,7 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, uin
       const double sigma2 = (1.0 * lambda * levelTau);
       const double sigma = sqrt(sigma2);
       for (int b = tid; b < nb; b += blockDim.x)
-        sample_midpoint(a.seed, R.stream, move, level, b, skip, sigma, sigma2, R.box[0], R.box[1], R.box[2],
+        sample_midpoint(gauss + gauss_offset(W, level), b, skip, sigma, sigma2, R.box[0], R.box[1], R.box[2],
                         R.boxinv[0], R.boxinv[1], R.boxinv[2], R.isper[0], R.isper[1], R.isper[2], R.num_images,
                         mold, mnew, lnew, lold);
       __syncthreads();
@@ -358,8 +400,8 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, uin
         sr_partners<NT, MIXED, false, LIN>(R, F, T, BcView{}, nullptr, j0, j1, ptcl, slice1, K, step, a.link_split, mold, mnew,
                                       acc_old, acc_new);
       }
-      const double sr_old = block_sum(acc_old.value(), scratch);
-      const double sr_new = block_sum(acc_new.value(), scratch);
+      double sr_old, sr_new;
+      block_sum2(acc_old.value(), acc_new.value(), scratch, sr_old, sr_new);
 
       // ---- Long-range action at the lowest level (LongRangeClass.cc:273-371) ----
       // rho_k NEW = OLD + e^{ik r_new} - e^{ik r_old} over the window, computed on the
@@ -420,8 +462,7 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, uin
               }
           }
         }
-        lr_kold = block_sum(part_o, scratch);
-        lr_knew = block_sum(part_n, scratch);
+        block_sum2(part_o, part_n, scratch, lr_kold, lr_knew);
       }
 
       // ---- Metropolis test (StageClass.cc:69-72) ----
@@ -454,7 +495,7 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, uin
         }
         const double currActionChange = newAction - oldAction;
         const double logAcceptProb = logSampleRatio - currActionChange + prev;
-        const double logu = log_accept_draw(a.seed, R.stream, move, level);
+        const double logu = logu_tab[level];
         const int ok = logAcceptProb >= logu;
         prev = currActionChange;
         ctl[2] = ok;
@@ -582,7 +623,9 @@ __global__ void k_sweep_advance(uint32_t* first_move, int num_moves) {
 }
 
 static size_t bisection_lds_bytes(const LaunchBisection& a) {
-  return bisection_ctl_offset(a) + 4 * sizeof(int) + a.lds_lr_bytes;
+  const int W = 1 << a.num_levels;
+  return bisection_ctl_offset(a) + 4 * sizeof(int) + sizeof(double) * (gauss_offset(W, a.num_levels) + PIMC_MAX_LEVELS) +
+         a.lds_lr_bytes;
 }
 
 template <bool MIXED, int VAR, int BLOCK, bool PERSIST>

@@ -51,6 +51,26 @@ __device__ inline double block_sum(double v, PIMC_LDS double* scratch) {
   return t;
 }
 
+// Block sums of two values in one pass (valid in thread 0), in a fixed order.
+__device__ inline void block_sum2(double a, double b, PIMC_LDS double* scratch, double& ta, double& tb) {
+  a = wave_sum(a);
+  b = wave_sum(b);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+  if (lane == 0) {
+    scratch[wave] = a;
+    scratch[nw + wave] = b;
+  }
+  __syncthreads();
+  ta = 0.0;
+  tb = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < nw; w++) {
+      ta += scratch[w];
+      tb += scratch[nw + w];
+    }
+  __syncthreads();
+}
+
 // Kahan-compensated FP64 accumulation of the mixed path's terms.
 struct Kahan {
   double s = 0.0, c = 0.0;
