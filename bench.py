@@ -97,7 +97,7 @@ def run_gpu(system, precision, steps, warmup, moves_per_step, tune=False, graph=
         lib.sweep_graph_destroy(g)
     lib.close()
     return tot, wall, tot.device_ms, {"block_threads": plan.block_threads, "link_split": plan.link_split,
-                                      "persistent": bool(plan.persistent)}
+                                      "persistent": bool(plan.persistent), "cluster": plan.cluster}
 
 
 def other_workload(name: str, replicas: int):

@@ -349,15 +349,18 @@ typedef struct pimc_bisection_plan {
   int32_t block_threads;     /* threads per replica: 256 or 512 */
   int32_t link_split;        /* threads sharing one partner's links in the short-range loop: 1, 2 or 4 */
   int32_t persistent;        /* 1: a persistent grid takes replicas from a work queue, longest first */
-  int32_t reserved;
+  int32_t cluster;           /* blocks per replica, 1..32 (0 means 1); above 1 the blocks split the
+                                partners and k-vectors of every stage and exchange partial sums; the
+                                nrep * cluster blocks must all be resident at once; not persistent */
   double trial_ms;           /* pimc_bisection_tune: time of the chosen shape on the trial moves */
 } pimc_bisection_plan;
 
-/* The default plan is {256, 1, 0}. */
+/* The default plan is {256, 1, 0, 1}. */
 int pimc_bisection_set_plan(pimc_ctx* ctx, const pimc_bisection_plan* plan);
 int pimc_bisection_get_plan(pimc_ctx* ctx, pimc_bisection_plan* plan);
 
-/* Autotuner: times every plan (block_threads x link_split x persistent) on
+/* Autotuner: times every plan (block_threads x link_split x {one block per
+ * replica, persistent, clusters of 2..32 blocks that fit the device at once}) on
  * trial_moves moves [first_move, first_move + trial_moves) of the listed
  * replicas, restores their paths, permutations, rho_k and counters after each
  * trial, and makes the fastest plan the context's plan (also returned in *out,

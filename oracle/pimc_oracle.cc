@@ -1099,7 +1099,7 @@ struct OracleSweep {
 struct pimc_ctx {
   Ctx c;
   double acc_ms = 0.0;   // time of the bisection calls since the last take
-  pimc_bisection_plan plan{256, 1, 0, 0, 0.0};
+  pimc_bisection_plan plan{256, 1, 0, 1, 0.0};
   std::vector<OracleSweep> sweeps;
 };
 
@@ -1721,8 +1721,12 @@ int pimc_bisection_set_plan(pimc_ctx* ctx, const pimc_bisection_plan* plan) {
     return fail(PIMC_ERR_ARG, "bisection_set_plan: block_threads must be 256 or 512");
   if (plan->link_split != 1 && plan->link_split != 2 && plan->link_split != 4)
     return fail(PIMC_ERR_ARG, "bisection_set_plan: link_split must be 1, 2 or 4");
+  if (plan->cluster < 0 || plan->cluster > 32) return fail(PIMC_ERR_ARG, "bisection_set_plan: cluster must be 0..32");
+  if (plan->cluster > 1 && plan->persistent)
+    return fail(PIMC_ERR_ARG, "bisection_set_plan: a cluster plan is not persistent");
   ctx->plan = *plan;
   ctx->plan.persistent = plan->persistent ? 1 : 0;
+  ctx->plan.cluster = plan->cluster > 1 ? plan->cluster : 1;
   return PIMC_OK;
 }
 
@@ -1740,7 +1744,7 @@ int pimc_bisection_tune(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, co
   if (precision != PIMC_FP64 && precision != PIMC_MIXED) return fail(PIMC_ERR_ARG, "bisection_tune: bad precision");
   for (int r = 0; r < nrep; r++)
     if (!get_rep(&ctx->c, replicas[r])) return fail(PIMC_ERR_ARG, "bisection_tune: bad replica");
-  ctx->plan = pimc_bisection_plan{256, 1, 0, 0, 0.0};
+  ctx->plan = pimc_bisection_plan{256, 1, 0, 1, 0.0};
   if (out) *out = ctx->plan;
   return PIMC_OK;
 }

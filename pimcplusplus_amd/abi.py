@@ -130,7 +130,7 @@ class BisectionStats(C.Structure):
 
 class BisectionPlan(C.Structure):
     _fields_ = [("block_threads", C.c_int32), ("link_split", C.c_int32), ("persistent", C.c_int32),
-                ("reserved", C.c_int32), ("trial_ms", C.c_double)]
+                ("cluster", C.c_int32), ("trial_ms", C.c_double)]
 
 
 class Energy(C.Structure):
@@ -513,9 +513,9 @@ class PimcLib:
             rec, C.byref(st) if st is not None else None))
         return (records_to_numpy(rec, reps.size, num_moves) if rec is not None else None), st
 
-    def set_plan(self, block_threads: int = 256, link_split: int = 1, persistent: bool = False):
+    def set_plan(self, block_threads: int = 256, link_split: int = 1, persistent: bool = False, cluster: int = 1):
         p = BisectionPlan(block_threads=block_threads, link_split=link_split, persistent=int(persistent),
-                          reserved=0, trial_ms=0.0)
+                          cluster=cluster, trial_ms=0.0)
         self._check("pimc_bisection_set_plan", self.lib.pimc_bisection_set_plan(self.ctx, C.byref(p)))
 
     def get_plan(self) -> BisectionPlan:

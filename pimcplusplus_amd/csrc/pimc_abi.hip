@@ -235,7 +235,7 @@ void setup_kvecs(HostReplica& H, double kcut) {
 
 // A captured sweep and the device state its replays use.
 struct SweepGraph {
-  DevBuf ids, pos, stats, queue, first;
+  DevBuf ids, pos, stats, queue, first, bar, part;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int num_moves = 0;
@@ -262,7 +262,11 @@ struct pimc_ctx {
   int acc_cap = 0;
   std::vector<int> h_repids, h_reppos;       // launch list of the last upload to d_repids / d_reppos
   DevBuf d_reppos, d_queue;
-  pimc_bisection_plan plan{256, 1, 0, 0, 0.0};   // launch shape of pimc_bisection_run
+  // cluster launches: barrier counters, partial sums, the sticky abort flag, and
+  // the bound of one barrier wait in device wall-clock ticks
+  DevBuf d_bar, d_part, d_abort;
+  uint64_t wait_ticks = 0;
+  pimc_bisection_plan plan{256, 1, 0, 1, 0.0};   // launch shape of pimc_bisection_run
   std::vector<std::unique_ptr<SweepGraph>> graphs;
   // event pairs around bisection launches since the last take, and a pool of spare ones
   std::vector<std::pair<hipEvent_t, hipEvent_t>> timed, spare;
@@ -372,6 +376,13 @@ int pimc_ctx_create(int32_t device, pimc_ctx** out) {
   auto c = std::make_unique<pimc_ctx>();
   c->device = device;
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || khz <= 0)
+    khz = 100000;   // the MI300-series wall clock runs at 100 MHz
+  c->wait_ticks = (uint64_t)khz * 1000u * 2u;   // two seconds
+  int rc = c->d_abort.ensure(sizeof(int));
+  if (rc) return rc;
+  HIP_TRY(hipMemset(c->d_abort.p, 0, sizeof(int)));
   *out = c.release();
   return PIMC_OK;
 }
@@ -1163,7 +1174,8 @@ int ensure_acc(pimc_ctx* ctx) {
 // Launch arguments over device buffers that stay valid while `ids`/`pos` do.
 LaunchBisection make_launch(pimc_ctx* ctx, const BisectionPrep& P, const pimc_bisection_desc* desc, int nrep,
                             uint32_t first_move, int num_moves, const int* d_ids, const int* d_pos, int64_t* d_stats,
-                            pimc_move_record* d_records, int* d_queue, const uint32_t* d_first) {
+                            pimc_move_record* d_records, int* d_queue, const uint32_t* d_first, int* d_bar = nullptr,
+                            double* d_part = nullptr) {
   LaunchBisection a{};
   a.reps = ctx->d_reps.as<ReplicaDev>();
   a.fits = ctx->d_fits.as<FitDev>();
@@ -1184,12 +1196,49 @@ LaunchBisection make_launch(pimc_ctx* ctx, const BisectionPrep& P, const pimc_bi
   a.link_split = ctx->plan.link_split;
   a.queue = ctx->plan.persistent ? d_queue : nullptr;
   a.first_move_dev = d_first;
+  a.cluster = std::max(1, ctx->plan.cluster);
+  a.bar = d_bar;
+  a.part = d_part;
+  a.abort_flag = ctx->d_abort.as<int>();
+  a.wait_ticks = ctx->wait_ticks;
   return a;
+}
+
+// Device buffers of a cluster launch over nrep replicas with the context's plan.
+int ensure_cluster_buffers(pimc_ctx* ctx, int nrep, DevBuf& bar, DevBuf& part) {
+  const int c = std::max(1, ctx->plan.cluster);
+  if (c == 1) return PIMC_OK;
+  int rc = bar.ensure((size_t)nrep * sizeof(int));
+  if (!rc) rc = part.ensure((size_t)nrep * 2 * c * 4 * sizeof(double));
+  return rc;
+}
+
+// A cluster launch needs all nrep * cluster blocks resident at once.
+int check_cluster_fits(pimc_ctx* ctx, const BisectionPrep& P, const LaunchBisection& a, int precision, int nrep) {
+  if (a.cluster <= 1) return PIMC_OK;
+  const int per_cu = bisection_blocks_per_cu(P.nt, precision == PIMC_MIXED, P.lean, ctx->plan.block_threads,
+                                             bisection_lds_bytes_for(a), 2);
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, ctx->device));
+  if ((int64_t)nrep * a.cluster > (int64_t)per_cu * prop.multiProcessorCount)
+    return fail(PIMC_ERR_ARG, "bisection: replicas x cluster (" + std::to_string((int64_t)nrep * a.cluster) +
+                                  " blocks) exceed the resident blocks of the device (" +
+                                  std::to_string(per_cu * prop.multiProcessorCount) + ")");
+  return PIMC_OK;
+}
+
+// Reports (and clears) a cluster barrier wait that timed out in an earlier launch.
+int take_abort(pimc_ctx* ctx) {
+  int flag = 0;
+  HIP_TRY(hipMemcpy(&flag, ctx->d_abort.p, sizeof(int), hipMemcpyDeviceToHost));
+  if (!flag) return PIMC_OK;
+  HIP_TRY(hipMemset(ctx->d_abort.p, 0, sizeof(int)));
+  return fail(PIMC_ERR_DEVICE, "bisection: a cluster barrier wait timed out; the replicas' paths are undefined");
 }
 
 int persistent_grid(pimc_ctx* ctx, const BisectionPrep& P, const LaunchBisection& a, int precision, int nrep) {
   const int per_cu = bisection_blocks_per_cu(P.nt, precision == PIMC_MIXED, P.lean, ctx->plan.block_threads,
-                                             bisection_lds_bytes_for(a));
+                                             bisection_lds_bytes_for(a), 1);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess || per_cu <= 0) return nrep;
   return std::max(1, std::min(nrep, per_cu * prop.multiProcessorCount));
@@ -1206,6 +1255,8 @@ int launch_timed(pimc_ctx* ctx, const BisectionPrep& P, const LaunchBisection& a
   }
   ctx->timed.push_back(ev);
   const int grid = ctx->plan.persistent ? persistent_grid(ctx, P, a, precision, nrep) : nrep;
+  int rc = check_cluster_fits(ctx, P, a, precision, nrep);
+  if (rc) return rc;
   HIP_TRY(hipEventRecord(ev.first, ctx->stream));
   hipError_t e = launch_bisection(P.nt, precision == PIMC_MIXED, P.lean, a, ctx->plan.block_threads, grid, ctx->stream);
   if (e != hipSuccess) return fail(PIMC_ERR_DEVICE, std::string("bisection launch: ") + hipGetErrorString(e));
@@ -1234,6 +1285,7 @@ int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, con
   if ((rc = ctx->d_stats.ensure((size_t)nrep * 5 * sizeof(int64_t)))) return rc;
   if (records && (rc = ctx->d_records.ensure((size_t)nrep * num_moves * sizeof(pimc_move_record)))) return rc;
   if ((rc = ctx->d_queue.ensure(sizeof(int)))) return rc;
+  if ((rc = ensure_cluster_buffers(ctx, nrep, ctx->d_bar, ctx->d_part))) return rc;
   if (ctx->h_repids != P.ids || ctx->h_reppos != P.order) {
     // the launch list is uploaded once and reused while it does not change
     if ((rc = ctx->d_repids.ensure((size_t)nrep * sizeof(int))) || (rc = ctx->d_reppos.ensure((size_t)nrep * sizeof(int))))
@@ -1247,7 +1299,7 @@ int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, con
   const LaunchBisection a =
       make_launch(ctx, P, desc, nrep, (uint32_t)first_move, num_moves, ctx->d_repids.as<int>(), ctx->d_reppos.as<int>(),
                   ctx->d_stats.as<int64_t>(), records ? ctx->d_records.as<pimc_move_record>() : nullptr,
-                  ctx->d_queue.as<int>(), nullptr);
+                  ctx->d_queue.as<int>(), nullptr, ctx->d_bar.as<int>(), ctx->d_part.as<double>());
   if ((rc = launch_timed(ctx, P, a, precision, nrep))) return rc;
   if (!records && !stats) return PIMC_OK;   // asynchronous: counters via pimc_bisection_stats_take
   if (records)
@@ -1256,6 +1308,7 @@ int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, con
   std::vector<int64_t> st((size_t)nrep * 5);
   HIP_TRY(hipMemcpyAsync(st.data(), ctx->d_stats.p, st.size() * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if ((rc = take_abort(ctx))) return rc;
   if (stats) {
     float ms = 0.0f;
     const auto& ev = ctx->timed.back();
@@ -1278,8 +1331,13 @@ int pimc_bisection_set_plan(pimc_ctx* ctx, const pimc_bisection_plan* plan) {
     return fail(PIMC_ERR_ARG, "bisection_set_plan: block_threads must be 256 or 512");
   if (plan->link_split != 1 && plan->link_split != 2 && plan->link_split != 4)
     return fail(PIMC_ERR_ARG, "bisection_set_plan: link_split must be 1, 2 or 4");
+  if (plan->cluster < 0 || plan->cluster > kMaxCluster)
+    return fail(PIMC_ERR_ARG, "bisection_set_plan: cluster must be 0.." + std::to_string(kMaxCluster));
+  if (plan->cluster > 1 && plan->persistent)
+    return fail(PIMC_ERR_ARG, "bisection_set_plan: a cluster plan is not persistent");
   ctx->plan = *plan;
   ctx->plan.persistent = plan->persistent ? 1 : 0;
+  ctx->plan.cluster = std::max(1, plan->cluster);
   return PIMC_OK;
 }
 
@@ -1335,38 +1393,47 @@ int pimc_bisection_tune(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, co
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
+  // every (block, link split) with one block per replica, a persistent grid, and
+  // clusters of 2 .. kMaxCluster blocks per replica where all blocks fit at once
+  std::vector<pimc_bisection_plan> plans;
   for (int block : {256, 512})
-    for (int ls : {1, 2, 4})
-      for (int pers : {0, 1}) {
-        ctx->plan = pimc_bisection_plan{block, ls, pers, 0, 0.0};
-        BisectionPrep P;
-        if ((rc = prepare_bisection(ctx, replicas, nrep, desc, precision, P))) break;
-        DevBuf ids, pos, st, q;
-        if ((rc = ids.ensure((size_t)nrep * 4)) || (rc = pos.ensure((size_t)nrep * 4)) ||
-            (rc = st.ensure((size_t)nrep * 40)) || (rc = q.ensure(4)))
-          break;
-        HIP_TRY(hipMemcpy(ids.p, P.ids.data(), (size_t)nrep * 4, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(pos.p, P.order.data(), (size_t)nrep * 4, hipMemcpyHostToDevice));
-        const LaunchBisection a = make_launch(ctx, P, desc, nrep, (uint32_t)first_move, trial_moves, ids.as<int>(),
-                                              pos.as<int>(), st.as<int64_t>(), nullptr, q.as<int>(), nullptr);
-        const int grid = pers ? persistent_grid(ctx, P, a, precision, nrep) : nrep;
-        HIP_TRY(hipEventRecord(e0, ctx->stream));
-        hipError_t e = launch_bisection(P.nt, precision == PIMC_MIXED, P.lean, a, block, grid, ctx->stream);
-        if (e != hipSuccess) {
-          rc = fail(PIMC_ERR_DEVICE, std::string("bisection_tune launch: ") + hipGetErrorString(e));
-          break;
-        }
-        HIP_TRY(hipEventRecord(e1, ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-        float ms = 0.0f;
-        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best_ms) {
-          best_ms = ms;
-          best = ctx->plan;
-          best.trial_ms = ms;
-        }
-        if ((rc = restore())) break;
-      }
+    for (int ls : {1, 2, 4}) {
+      for (int pers : {0, 1}) plans.push_back(pimc_bisection_plan{block, ls, pers, 1, 0.0});
+      for (int c = 2; c <= kMaxCluster; c *= 2) plans.push_back(pimc_bisection_plan{block, ls, 0, c, 0.0});
+    }
+  for (const pimc_bisection_plan& plan : plans) {
+    ctx->plan = plan;
+    BisectionPrep P;
+    if ((rc = prepare_bisection(ctx, replicas, nrep, desc, precision, P))) break;
+    DevBuf ids, pos, st, q, bar, part;
+    if ((rc = ids.ensure((size_t)nrep * 4)) || (rc = pos.ensure((size_t)nrep * 4)) ||
+        (rc = st.ensure((size_t)nrep * 40)) || (rc = q.ensure(4)) || (rc = ensure_cluster_buffers(ctx, nrep, bar, part)))
+      break;
+    HIP_TRY(hipMemcpy(ids.p, P.ids.data(), (size_t)nrep * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(pos.p, P.order.data(), (size_t)nrep * 4, hipMemcpyHostToDevice));
+    const LaunchBisection a =
+        make_launch(ctx, P, desc, nrep, (uint32_t)first_move, trial_moves, ids.as<int>(), pos.as<int>(),
+                    st.as<int64_t>(), nullptr, q.as<int>(), nullptr, bar.as<int>(), part.as<double>());
+    if (plan.cluster > 1 && check_cluster_fits(ctx, P, a, precision, nrep) != PIMC_OK) continue;   // no trial
+    const int grid = plan.persistent ? persistent_grid(ctx, P, a, precision, nrep) : nrep;
+    HIP_TRY(hipEventRecord(e0, ctx->stream));
+    hipError_t e = launch_bisection(P.nt, precision == PIMC_MIXED, P.lean, a, plan.block_threads, grid, ctx->stream);
+    if (e != hipSuccess) {
+      rc = fail(PIMC_ERR_DEVICE, std::string("bisection_tune launch: ") + hipGetErrorString(e));
+      break;
+    }
+    HIP_TRY(hipEventRecord(e1, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if ((rc = take_abort(ctx))) break;
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = ctx->plan;
+      best.trial_ms = ms;
+    }
+    if ((rc = restore())) break;
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) {
@@ -1392,14 +1459,17 @@ int pimc_sweep_graph_create(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep
   if ((rc = ensure_acc(ctx))) return rc;
   auto g = std::make_unique<SweepGraph>();
   if ((rc = g->ids.ensure((size_t)nrep * 4)) || (rc = g->pos.ensure((size_t)nrep * 4)) ||
-      (rc = g->stats.ensure((size_t)nrep * 40)) || (rc = g->queue.ensure(4)) || (rc = g->first.ensure(4)))
+      (rc = g->stats.ensure((size_t)nrep * 40)) || (rc = g->queue.ensure(4)) || (rc = g->first.ensure(4)) ||
+      (rc = ensure_cluster_buffers(ctx, nrep, g->bar, g->part)))
     return rc;
   const uint32_t f0 = (uint32_t)first_move;
   HIP_TRY(hipMemcpy(g->ids.p, P.ids.data(), (size_t)nrep * 4, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(g->pos.p, P.order.data(), (size_t)nrep * 4, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(g->first.p, &f0, 4, hipMemcpyHostToDevice));
-  const LaunchBisection a = make_launch(ctx, P, desc, nrep, 0u, num_moves, g->ids.as<int>(), g->pos.as<int>(),
-                                        g->stats.as<int64_t>(), nullptr, g->queue.as<int>(), g->first.as<uint32_t>());
+  const LaunchBisection a =
+      make_launch(ctx, P, desc, nrep, 0u, num_moves, g->ids.as<int>(), g->pos.as<int>(), g->stats.as<int64_t>(),
+                  nullptr, g->queue.as<int>(), g->first.as<uint32_t>(), g->bar.as<int>(), g->part.as<double>());
+  if ((rc = check_cluster_fits(ctx, P, a, precision, nrep))) return rc;
   const int grid = ctx->plan.persistent ? persistent_grid(ctx, P, a, precision, nrep) : nrep;
   // one sweep: [queue reset] -> bisection moves -> first_move += num_moves; kernels only
   HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
@@ -1564,6 +1634,8 @@ int pimc_bisection_stats_take(pimc_ctx* ctx, const int32_t* replicas, int32_t nr
   HIP_TRY(hipSetDevice(ctx->device));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   memset(stats, 0, sizeof *stats);
+  int rc = take_abort(ctx);
+  if (rc) return rc;
   for (auto& e : ctx->timed) {
     float ms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&ms, e.first, e.second));

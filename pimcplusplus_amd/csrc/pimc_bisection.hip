@@ -274,8 +274,61 @@ __host__ __device__ inline size_t bisection_ctl_offset(const LaunchBisection& a)
   return a.lds_table_bytes + sizeof(double) * (6 * (W + 1) + W + 2 * W + 2 * kMaxWaves);
 }
 
-template <int NT, bool MIXED, int VAR>
-__device__ inline void bisection_replica(const LaunchBisection& a, int item, uint32_t first_move,
+// ---- Several blocks per replica (a cluster) ----
+// The `cluster` blocks of one replica run the same moves in lock step: each
+// samples the same midpoints (same draws), takes its share of the partners and
+// k-vectors, and the members exchange their partial stage sums through global
+// memory.  Every member then holds the same sums, added in member order, and
+// takes the same Metropolis decision.
+
+// Thread 0 of every member adds one to the replica's counter and waits until all
+// members have arrived n times.  The wait is bounded (a.wait_ticks of the device
+// wall clock, or until another block gives up): on a timeout the launch's abort
+// flag is set and false is returned, so no wave waits forever.
+__device__ inline bool cluster_barrier(const LaunchBisection& a, int item, int n) {
+  int* bar = a.bar + item;
+  __hip_atomic_fetch_add(bar, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  const int target = a.cluster * n;
+  const uint64_t t0 = wall_clock64();
+  while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    if (__hip_atomic_load(a.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+    if (wall_clock64() - t0 > a.wait_ticks) {
+      __hip_atomic_store(a.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return true;
+}
+
+// Replaces v[0..3] (this member's {SR old, SR new, LR old, LR new}) by the sums over
+// the members, in member order.  Slots alternate between two buffers: a member
+// can only overwrite a buffer after every member has passed the next barrier,
+// that is, after every member has read it.  Thread 0 only.
+__device__ inline bool cluster_sum(const LaunchBisection& a, int item, int member, int nbar, double v[4]) {
+  double* slot = a.part + ((size_t)item * 2 + (nbar & 1)) * a.cluster * 4;
+#pragma unroll
+  for (int i = 0; i < 4; i++) slot[member * 4 + i] = v[i];
+  if (!cluster_barrier(a, item, nbar + 1)) return false;
+  double t[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int m = 0; m < a.cluster; m++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) t[i] += slot[m * 4 + i];
+#pragma unroll
+  for (int i = 0; i < 4; i++) v[i] = t[i];
+  return true;
+}
+
+// This member's share [b, e) of the range [lo, hi).
+__device__ inline void cluster_share(int lo, int hi, int member, int cluster, int& b, int& e) {
+  const int per = (hi - lo + cluster - 1) / cluster;
+  b = min(hi, lo + member * per);
+  e = min(hi, b + per);
+}
+
+template <int NT, bool MIXED, int VAR, bool CL>
+__device__ inline void bisection_replica(const LaunchBisection& a, int item, int member, uint32_t first_move,
                                          PIMC_LDS unsigned char* smem) {
   constexpr bool HAS_BC = VAR == kGeneric, HAS_LR = VAR == kGeneric, LIN = VAR == kLeanDavid;
   const ReplicaDev* RD = a.reps + a.rep_ids[item];
@@ -301,6 +354,10 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, uin
   PIMC_LDS d2v* lrC = (PIMC_LDS d2v*)(logu_tab + PIMC_MAX_LEVELS);        // [2][kstride]: NEW, OLD rows
 
   int64_t st_moves = 0, st_acc = 0, st_stages = 0, st_evals = 0, st_kevals = 0;
+  int nbar = 0;   // cluster barriers passed (the same count on every member)
+  // this member's k-vectors [kb, ke) of the long-range sums and updates
+  int kb = 0, ke = R.nk;
+  if constexpr (CL) cluster_share(0, R.nk, member, a.cluster, kb, ke);
   typedef typename std::conditional<MIXED, float, double>::type LrT;   // phase-sum arithmetic
   const int numLeft = R.nslices - W;
   const double lambda = S.lambda[sp];
@@ -345,7 +402,7 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, uin
     __syncthreads();
 
     double prev = 0.0;   // thread 0 only
-    bool accepted = true;
+    bool accepted = true, aborted = false;
     int stage = 0;
     for (int level = L - 1; level >= a.lowest_level; level--, stage++) {
       const int skip = 1 << (level + 1);
@@ -380,7 +437,8 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, uin
       for (int s2 = 0; s2 < R.ns; s2++) {
         const FitDev F = load_uniform(&a.fits[S.fit[sp * R.ns + s2]]);
         const double* ul = ulong_for(R, F, sp * R.ns + s2, level);
-        const int j0 = S.first[s2], j1 = S.first[s2] + S.count[s2];
+        int j0 = S.first[s2], j1 = S.first[s2] + S.count[s2];
+        if constexpr (CL) cluster_share(j0, j1, member, a.cluster, j0, j1);
         if constexpr (HAS_BC) {
           if (F.kind == 1) {
             const BcView Q{qgrid_of(F), level};
@@ -423,7 +481,7 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, uin
           const double factor = (t == 0 || t == W) ? 0.5 : 1.0;
           const int slice = slice1 + t;
           const d2v* rho = R.rho0 + (size_t)slice * R.ns * R.nk;
-          for (int k = tid; k < R.nk; k += blockDim.x) {
+          for (int k = kb + tid; k < ke; k += blockDim.x) {
             const int ki[3] = {R.kidx[4 * k], R.kidx[4 * k + 1], R.kidx[4 * k + 2]};
             LrT nr, ni, orr, oi;
             kterm<LrT>(lrC, o1, o2, ki, nr, ni);
@@ -466,7 +524,20 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, uin
       }
 
       // ---- Metropolis test (StageClass.cc:69-72) ----
-      if (tid == 0) {
+      bool cl_ok = true;
+      if constexpr (CL) {
+        if (tid == 0) {
+          double v[4] = {sr_old, sr_new, lr_kold, lr_knew};
+          cl_ok = cluster_sum(a, item, member, nbar, v);
+          sr_old = v[0];
+          sr_new = v[1];
+          lr_kold = v[2];
+          lr_knew = v[3];
+        }
+        nbar++;
+      }
+      if (tid == 0 && !cl_ok) ctl[2] = -1;   // a member gave up waiting: abandon the launch
+      if (tid == 0 && cl_ok) {
         double logSampleProb = 0.0, logOldSampleProb = 0.0;
         for (int b = 0; b < nb; b++) {
           logOldSampleProb += 0.0 + lold[b];
@@ -501,7 +572,7 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, uin
         ctl[2] = ok;
         st_stages++;
         st_evals += 2 * (int64_t)(R.N - 1) * K;
-        if (a.records) {
+        if (a.records && member == 0) {
           pimc_move_record* rec = a.records + (size_t)pos * a.num_moves + m;
           rec->old_action[stage] = oldAction;
           rec->new_action[stage] = newAction;
@@ -510,12 +581,14 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, uin
         }
       }
       __syncthreads();
-      if (!ctl[2]) {
+      if (ctl[2] != 1) {
         accepted = false;
+        aborted = ctl[2] < 0;
         stage++;
         break;
       }
     }
+    if (CL && aborted) return;
 
     // ---- AcceptCopy: rho_k OLD, NEW <- the updated rho_k; PutInBoxFast on the NEW beads, OLD <- NEW ----
     if (HAS_LR && accepted && R.lr_on) {
@@ -531,7 +604,7 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, uin
         }
         __syncthreads();
         const size_t base = ((size_t)(slice1 + t) * R.ns + sp) * R.nk;
-        for (int k = tid; k < R.nk; k += blockDim.x) {
+        for (int k = kb + tid; k < ke; k += blockDim.x) {
           const int ki[3] = {R.kidx[4 * k], R.kidx[4 * k + 1], R.kidx[4 * k + 2]};
           LrT nr, ni, orr, oi;
           kterm<LrT>(lrC, o1, o2, ki, nr, ni);
@@ -546,7 +619,7 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, uin
         }
       }
     }
-    if (accepted) {
+    if (accepted && member == 0) {
       for (int t = tid; t <= W; t += blockDim.x) {
         double v[3];
 #pragma unroll
@@ -555,7 +628,18 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, uin
         store_bead(R, 1, slice1 + t, ptcl, v);
       }
     }
-    if (tid == 0) {
+    if constexpr (CL) {
+      // the other members read the moved beads in the next move: publish them
+      if (accepted) {
+        if (member == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (tid == 0) ctl[2] = cluster_barrier(a, item, nbar + 1) ? 1 : -1;
+        nbar++;
+        __syncthreads();
+        if (ctl[2] < 0) return;
+      }
+    }
+    if (tid == 0 && member == 0) {
       st_moves++;
       st_acc += accepted ? 1 : 0;
       if (a.records) {
@@ -574,7 +658,7 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, uin
     }
     __syncthreads();
   }
-  if (tid == 0) {
+  if (tid == 0 && member == 0) {
     int64_t* st = a.stats + (size_t)pos * 5;
     st[0] = st_moves;
     st[1] = st_acc;
@@ -590,33 +674,44 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, uin
   }
 }
 
-// One block per replica (queue == nullptr), or a persistent grid whose blocks take
-// replicas from the work queue in list order (the host lists them longest first)
-// until the list is exhausted.  first_move_dev, when set, holds the first move
-// number (a captured sweep advances it on the device).
-template <int NT, bool MIXED, int VAR, int BLOCK, bool PERSIST>
+// Launch modes: kOneBlock -- one block per replica; kPersistent -- a persistent
+// grid whose blocks take replicas from the work queue in list order (the host
+// lists them longest first) until the list is exhausted; kCluster -- a.cluster
+// blocks per replica (blocks c*i .. c*i + c - 1 run replica i).  first_move_dev,
+// when set, holds the first move number (a captured sweep advances it on the
+// device).
+enum { kOneBlock = 0, kPersistent = 1, kCluster = 2 };
+
+template <int NT, bool MIXED, int VAR, int BLOCK, int MODE>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PIMC_BISECTION_WAVES)))
 k_bisection(const LaunchBisection a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   PIMC_LDS unsigned char* smem = (PIMC_LDS unsigned char*)smem_raw;
   PIMC_LDS int* take = (PIMC_LDS int*)(smem + bisection_ctl_offset(a)) + 3;   // control word 3
   const uint32_t first_move = a.first_move_dev ? *a.first_move_dev : a.first_move;
-  if constexpr (!PERSIST) {
-    bisection_replica<NT, MIXED, VAR>(a, blockIdx.x, first_move, smem);
-    return;
-  }
-  for (;;) {
-    __syncthreads();
-    if (threadIdx.x == 0) *take = atomicAdd(a.queue, 1);
-    __syncthreads();
-    const int item = *take;
-    if (item >= a.nrep) break;   // uniform: every wave of the block leaves here
-    bisection_replica<NT, MIXED, VAR>(a, item, first_move, smem);
+  if constexpr (MODE == kOneBlock) {
+    bisection_replica<NT, MIXED, VAR, false>(a, blockIdx.x, 0, first_move, smem);
+  } else if constexpr (MODE == kCluster) {
+    bisection_replica<NT, MIXED, VAR, true>(a, (int)blockIdx.x / a.cluster, (int)blockIdx.x % a.cluster, first_move,
+                                            smem);
+  } else {
+    for (;;) {
+      __syncthreads();
+      if (threadIdx.x == 0) *take = atomicAdd(a.queue, 1);
+      __syncthreads();
+      const int item = *take;
+      if (item >= a.nrep) break;   // uniform: every wave of the block leaves here
+      bisection_replica<NT, MIXED, VAR, false>(a, item, 0, first_move, smem);
+    }
   }
 }
 
-__global__ void k_queue_reset(int* queue) {
-  if (threadIdx.x == 0) *queue = 0;
+// Clears the work-queue counter and the cluster barrier counters of a launch
+// (a kernel, so that a captured sweep holds no memset node).
+__global__ void k_launch_reset(int* queue, int* bar, int nbar) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (queue && i == 0) *queue = 0;
+  if (bar && i < nbar) bar[i] = 0;
 }
 __global__ void k_sweep_advance(uint32_t* first_move, int num_moves) {
   if (threadIdx.x == 0) *first_move += (uint32_t)num_moves;
@@ -628,22 +723,23 @@ static size_t bisection_lds_bytes(const LaunchBisection& a) {
          a.lds_lr_bytes;
 }
 
-template <bool MIXED, int VAR, int BLOCK, bool PERSIST>
+template <bool MIXED, int VAR, int BLOCK, int MODE>
 static hipError_t launch_bis_p(int nt, const LaunchBisection& a, int grid, hipStream_t s) {
   const size_t lds = bisection_lds_bytes(a);
   switch (nt) {
-    case 0: k_bisection<0, MIXED, VAR, BLOCK, PERSIST><<<grid, BLOCK, lds, s>>>(a); break;
-    case 1: k_bisection<1, MIXED, VAR, BLOCK, PERSIST><<<grid, BLOCK, lds, s>>>(a); break;
-    case 2: k_bisection<2, MIXED, VAR, BLOCK, PERSIST><<<grid, BLOCK, lds, s>>>(a); break;
-    default: k_bisection<3, MIXED, VAR, BLOCK, PERSIST><<<grid, BLOCK, lds, s>>>(a); break;
+    case 0: k_bisection<0, MIXED, VAR, BLOCK, MODE><<<grid, BLOCK, lds, s>>>(a); break;
+    case 1: k_bisection<1, MIXED, VAR, BLOCK, MODE><<<grid, BLOCK, lds, s>>>(a); break;
+    case 2: k_bisection<2, MIXED, VAR, BLOCK, MODE><<<grid, BLOCK, lds, s>>>(a); break;
+    default: k_bisection<3, MIXED, VAR, BLOCK, MODE><<<grid, BLOCK, lds, s>>>(a); break;
   }
   return hipGetLastError();
 }
 
 template <bool MIXED, int VAR, int BLOCK>
 static hipError_t launch_bis_b(int nt, const LaunchBisection& a, int grid, hipStream_t s) {
-  return a.queue ? launch_bis_p<MIXED, VAR, BLOCK, true>(nt, a, grid, s)
-                 : launch_bis_p<MIXED, VAR, BLOCK, false>(nt, a, grid, s);
+  if (a.cluster > 1) return launch_bis_p<MIXED, VAR, BLOCK, kCluster>(nt, a, grid, s);
+  return a.queue ? launch_bis_p<MIXED, VAR, BLOCK, kPersistent>(nt, a, grid, s)
+                 : launch_bis_p<MIXED, VAR, BLOCK, kOneBlock>(nt, a, grid, s);
 }
 
 template <bool MIXED, int VAR>
@@ -657,8 +753,14 @@ hipError_t launch_bisection(int num_terms, bool mixed, bool lean, const LaunchBi
   if (bisection_lds_bytes(a) > 64 * 1024) return hipErrorInvalidValue;
   if (block != 256 && block != 512) return hipErrorInvalidValue;
   if (a.link_split != 1 && a.link_split != 2 && a.link_split != 4) return hipErrorInvalidValue;
-  if (a.queue) {
-    k_queue_reset<<<1, 64, 0, s>>>(a.queue);
+  if (a.cluster > 1) {
+    // every block of the grid must be resident at once: the caller checks the grid
+    // against the resident capacity; the barrier's bounded wait is the backstop
+    if (a.queue || !a.bar || !a.part || !a.abort_flag || a.cluster > kMaxCluster) return hipErrorInvalidValue;
+    grid = a.nrep * a.cluster;
+    k_launch_reset<<<(a.nrep + 255) / 256, 256, 0, s>>>(nullptr, a.bar, a.nrep);
+  } else if (a.queue) {
+    k_launch_reset<<<1, 64, 0, s>>>(a.queue, nullptr, 0);
   } else {
     grid = a.nrep;
   }
@@ -677,20 +779,25 @@ hipError_t launch_sweep_advance(uint32_t* first_move, int num_moves, hipStream_t
 size_t bisection_lds_bytes_for(const LaunchBisection& a) { return bisection_lds_bytes(a); }
 
 // Resident blocks per CU of the bisection kernel for this launch shape (0 on error).
-int bisection_blocks_per_cu(int num_terms, bool mixed, bool lean, int block, size_t lds) {
-  int n = 0;
-  hipError_t e;
+template <int MODE>
+static hipError_t occupancy_of(int& n, int nt, bool mixed, bool lean, int block, size_t lds) {
   auto q = [&](auto kern) { return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, block, lds); };
-#define PIMC_OCC(NT_, B_)                                                                       \
-  (lean ? (mixed ? q(k_bisection<NT_, true, kLeanDavid, B_, true>)                                 \
-                 : q(k_bisection<NT_, false, kLeanDavid, B_, true>))                               \
-        : (mixed ? q(k_bisection<NT_, true, kGeneric, B_, true>) : q(k_bisection<NT_, false, kGeneric, B_, true>)))
-  const int nt = num_terms > 3 ? 3 : num_terms;
+#define PIMC_OCC(NT_, B_)                                                                                 \
+  (lean ? (mixed ? q(k_bisection<NT_, true, kLeanDavid, B_, MODE>)                                        \
+                 : q(k_bisection<NT_, false, kLeanDavid, B_, MODE>))                                      \
+        : (mixed ? q(k_bisection<NT_, true, kGeneric, B_, MODE>) : q(k_bisection<NT_, false, kGeneric, B_, MODE>)))
   if (block == 512)
-    e = nt == 0 ? PIMC_OCC(0, 512) : nt == 1 ? PIMC_OCC(1, 512) : nt == 2 ? PIMC_OCC(2, 512) : PIMC_OCC(3, 512);
-  else
-    e = nt == 0 ? PIMC_OCC(0, 256) : nt == 1 ? PIMC_OCC(1, 256) : nt == 2 ? PIMC_OCC(2, 256) : PIMC_OCC(3, 256);
+    return nt == 0 ? PIMC_OCC(0, 512) : nt == 1 ? PIMC_OCC(1, 512) : nt == 2 ? PIMC_OCC(2, 512) : PIMC_OCC(3, 512);
+  return nt == 0 ? PIMC_OCC(0, 256) : nt == 1 ? PIMC_OCC(1, 256) : nt == 2 ? PIMC_OCC(2, 256) : PIMC_OCC(3, 256);
 #undef PIMC_OCC
+}
+
+int bisection_blocks_per_cu(int num_terms, bool mixed, bool lean, int block, size_t lds, int mode) {
+  int n = 0;
+  const int nt = num_terms > 3 ? 3 : num_terms;
+  const hipError_t e = mode == kCluster      ? occupancy_of<kCluster>(n, nt, mixed, lean, block, lds)
+                       : mode == kPersistent ? occupancy_of<kPersistent>(n, nt, mixed, lean, block, lds)
+                                             : occupancy_of<kOneBlock>(n, nt, mixed, lean, block, lds);
   return e == hipSuccess ? n : 0;
 }
 

@@ -400,16 +400,17 @@ __device__ inline int koff_of(const RepCore& R, int d) {
   return d == 0 ? 0 : (d == 1 ? 2 * R.maxk0 + 1 : 2 * (R.maxk0 + R.maxk1) + 2);
 }
 
+// cos and sin of phi = r k_box are taken as cospi/sinpi of phi/pi: |phi| stays
+// within a few pi here, and the pi-scaled forms need no long argument reduction
+// (whose tables and registers would otherwise be live inside the chain kernel).
 template <typename T>
 __device__ inline void phase_row(double phi, int maxk, PIMC_LDS d2v* C) {
   T c, s;
+  constexpr double kInvPi = 0.31830988618379067154;
   if constexpr (sizeof(T) == 8) {
-    c = cos(phi);
-    s = sin(phi);
+    sincospi(phi * kInvPi, &s, &c);
   } else {
-    const float pf = (float)phi;
-    c = cosf(pf);
-    s = sinf(pf);
+    sincospif((float)(phi * kInvPi), &s, &c);
   }
   T re = 1, im = 0;
   C[maxk] = d2v{1.0, 0.0};
@@ -424,9 +425,13 @@ __device__ inline void phase_row(double phi, int maxk, PIMC_LDS d2v* C) {
 }
 
 // The three rows of one position r (dimension d at koff_of(d)); lanes 0..2 of the caller.
+// d is a per-lane value: the dimension's fields are selected, not indexed (a
+// run-time index into the register-resident R would move it to scratch).
 template <typename T>
 __device__ inline void phase_rows_lane(const RepCore& R, const double r[3], int d, PIMC_LDS d2v* C) {
-  phase_row<T>(r[d] * R.kbox[d], kmax_of(R, d), C + koff_of(R, d));
+  const double x = d == 0 ? r[0] : (d == 1 ? r[1] : r[2]);
+  const double kb = d == 0 ? R.kbox[0] : (d == 1 ? R.kbox[1] : R.kbox[2]);
+  phase_row<T>(x * kb, kmax_of(R, d), C + koff_of(R, d));
 }
 
 template <typename T>

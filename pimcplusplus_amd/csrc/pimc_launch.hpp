@@ -69,14 +69,23 @@ struct LaunchBisection {
   const int* rep_pos;         // [nrep] caller index of list entry i (records, stats), or nullptr
   int* queue;                 // work-queue counter of a persistent launch, or nullptr
   const uint32_t* first_move_dev;  // first move number in device memory, or nullptr
+  // several blocks per replica (cluster > 1; not with a queue): grid = nrep * cluster
+  int cluster;
+  int* bar;                   // [nrep] barrier counters, cleared by the launch
+  double* part;               // [nrep][2][cluster][4] partial stage sums
+  int* abort_flag;            // set when a barrier wait timed out (sticky until the host clears it)
+  uint64_t wait_ticks;        // bound of one barrier wait, in device wall-clock ticks
 };
+constexpr int kMaxCluster = 32;
 // Returns hipSuccess or the launch error.  lean: every pair fit is a DavidPA
 // fit on a LINEAR grid and no replica evaluates the long-range action.
-// block: 256 or 512 threads; grid: blocks of a persistent launch (a.queue set).
+// block: 256 or 512 threads; grid: blocks of a persistent launch (a.queue set);
+// a cluster launch runs nrep * cluster blocks.
 hipError_t launch_bisection(int num_terms, bool mixed, bool lean, const LaunchBisection& a, int block, int grid,
                             hipStream_t s);
 hipError_t launch_sweep_advance(uint32_t* first_move, int num_moves, hipStream_t s);
 size_t bisection_lds_bytes_for(const LaunchBisection& a);
-int bisection_blocks_per_cu(int num_terms, bool mixed, bool lean, int block, size_t lds);
+// mode: 0 one block per replica, 1 persistent, 2 cluster.
+int bisection_blocks_per_cu(int num_terms, bool mixed, bool lean, int block, size_t lds, int mode);
 
 }  // namespace pimc

@@ -75,3 +75,78 @@ def test_sweep_graph_replays_match_eager_runs(gpu, uneven, persistent):
     for x, y in zip(a, b):
         assert np.array_equal(gpu.path_get(x), gpu.path_get(y))
     gpu.sweep_graph_destroy(g)
+
+
+# Several blocks per replica: the members split partners and k-vectors and add
+# their partial sums in member order, so the chains equal the oracle's to 1e-12
+# with the same accept/reject sequence.
+CLUSTER_PLANS = [(256, 1, 2), (256, 4, 4), (512, 2, 8), (256, 1, 16)]
+
+
+@pytest.mark.parametrize("block,split,cluster", CLUSTER_PLANS)
+def test_cluster_plans_reproduce_the_oracle(gpu, uneven, oracle_chain, block, split, cluster):
+    ro, so, paths = oracle_chain
+    print(f"cluster plan block={block} split={split} cluster={cluster}", flush=True)
+    gpu.set_plan(block, split, False, cluster)
+    ig = workloads.load(gpu, uneven)
+    rg, sg = gpu.bisection_run(ig, uneven.num_levels, uneven.seed, 0, 40, abi.FP64)
+    compare_chains(ro, rg)
+    assert (sg.moves, sg.accepted, sg.pair_evals) == (so.moves, so.accepted, so.pair_evals)
+    for i, p in zip(ig, paths):
+        assert np.max(np.abs(gpu.path_get(i) - p)) < 1e-12
+        assert np.array_equal(gpu.path_get(i, 0), gpu.path_get(i, 1))
+
+
+@pytest.mark.parametrize("cluster", [3, 8])
+def test_cluster_plans_with_long_range(oracle, gpu, cluster):
+    sysm = workloads.charged_lj_system(n_plus=24, n_minus=24, m=32, kc_units=4.0, num_replicas=3)
+    io = workloads.load(oracle, sysm)
+    gpu.set_plan(256, 2, False, cluster)
+    ig = workloads.load(gpu, sysm)
+    print(f"charged LJ, cluster={cluster}", flush=True)
+    ro, so = oracle.bisection_run(io, sysm.num_levels, sysm.seed, 0, 150)
+    rg, sg = gpu.bisection_run(ig, sysm.num_levels, sysm.seed, 0, 150, abi.FP64)
+    compare_chains(ro, rg)
+    assert so.accepted == sg.accepted > 0 and so.k_evals == sg.k_evals > 0
+    for a, b in zip(io, ig):
+        for mode in (0, 1):
+            assert np.max(np.abs(oracle.rho_k(a, mode) - gpu.rho_k(b, mode))) < 1e-10
+        assert np.max(np.abs(oracle.path_get(a) - gpu.path_get(b))) < 1e-12
+
+
+def test_cluster_plan_ocp_background(oracle, gpu):
+    sysm = workloads.ocp_system(num_replicas=2, n=128, m=32, num_levels=3, kc_units=6.0, seed=2100)
+    io = workloads.load(oracle, sysm)
+    gpu.set_plan(512, 1, False, 16)
+    ig = workloads.load(gpu, sysm)
+    print("OCP N=128, cluster=16", flush=True)
+    ro, _ = oracle.bisection_run(io, sysm.num_levels, sysm.seed, 0, 100)
+    rg, _ = gpu.bisection_run(ig, sysm.num_levels, sysm.seed, 0, 100, abi.FP64)
+    compare_chains(ro, rg)
+    for a, b in zip(io, ig):
+        assert np.max(np.abs(oracle.path_get(a) - gpu.path_get(b))) < 1e-12
+
+
+def test_cluster_sweep_graph_matches_eager_runs(gpu, uneven):
+    gpu.set_plan(256, 2, False, 4)
+    a = workloads.load(gpu, uneven)
+    b = workloads.load(gpu, uneven)
+    print("cluster=4 sweep graph", flush=True)
+    g = gpu.sweep_graph_create(a, uneven.num_levels, uneven.seed, 0, 10, abi.MIXED)
+    for sweep in range(3):
+        gpu.sweep_graph_launch(g, 1)
+        gpu.bisection_run(b, uneven.num_levels, uneven.seed, 10 * sweep, 10, abi.MIXED, records=False)
+        gpu.synchronize()
+        for x, y in zip(a, b):
+            assert np.array_equal(gpu.path_get(x), gpu.path_get(y)), f"sweep {sweep}"
+    ta, tb = gpu.bisection_stats_take(a), gpu.bisection_stats_take(b)
+    assert (ta.moves, ta.accepted, ta.pair_evals) == (tb.moves, tb.accepted, tb.pair_evals)
+    gpu.sweep_graph_destroy(g)
+
+
+def test_cluster_plan_too_large_for_the_device_is_refused(gpu):
+    sysm = workloads.config1(num_replicas=64)
+    ids = workloads.load(gpu, sysm)
+    gpu.set_plan(512, 1, False, 32)   # 2048 blocks of 512 threads cannot all be resident
+    with pytest.raises(RuntimeError):
+        gpu.bisection_run(ids, sysm.num_levels, sysm.seed, 0, 1, abi.FP64)

@@ -6,6 +6,7 @@ and every move is accepted: this ties Sample, the kinetic action and the
 Metropolis bookkeeping (prevActionChange) together."""
 
 import numpy as np
+import pytest
 
 from pimcplusplus_amd import workloads
 from tests.helpers import oracle_cdll
@@ -85,3 +86,19 @@ def test_stats_take_accumulates_and_clears(oracle):
     assert t.moves == 50 and t.device_ms > 0
     t2 = oracle.bisection_stats_take(ids)
     assert (t2.moves, t2.pair_evals, t2.device_ms) == (0, 0, 0.0)
+
+
+def test_plan_validation(oracle):
+    oracle.set_plan(512, 4, True)
+    p = oracle.get_plan()
+    assert (p.block_threads, p.link_split, p.persistent, p.cluster) == (512, 4, 1, 1)
+    oracle.set_plan(256, 2, False, 8)
+    assert oracle.get_plan().cluster == 8
+    oracle.set_plan(256, 1, False, 0)          # 0 means one block per replica
+    assert oracle.get_plan().cluster == 1
+    for bad in (dict(block_threads=128), dict(link_split=3), dict(cluster=33), dict(cluster=-1),
+                dict(cluster=2, persistent=True)):
+        args = dict(block_threads=256, link_split=1, persistent=False, cluster=1)
+        args.update(bad)
+        with pytest.raises(RuntimeError):
+            oracle.set_plan(**args)
