@@ -1087,11 +1087,15 @@ int pimc_dist_disp(pimc_ctx* ctx, int32_t replica, int32_t mode, int64_t n, cons
 
 namespace {
 
+// LDS for the staged long-range phase tables of a bisection launch.
+constexpr size_t kLrPhaseBytes = 24 * 1024;
+
 // What a bisection launch over a replica list needs, validated.
 struct BisectionPrep {
   int nt = 0;
   bool lean = false;
   size_t table_bytes = 0, lr_bytes = 0;
+  int lr_chunk = 1;
   std::vector<int> order;    // list positions, longest replica first (persistent launches)
   std::vector<int> ids;      // replica ids in launch order
 };
@@ -1114,7 +1118,7 @@ int prepare_bisection(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, cons
     if ((1 << desc->num_levels) >= R.nslices) return fail(PIMC_ERR_ARG, "bisection: too few slices for num_levels");
     if (R.lr_on) {
       if (desc->lowest_level >= R.lr_levels) return fail(PIMC_ERR_ARG, "bisection: long-range tables lack the lowest level");
-      lr_bytes = std::max(lr_bytes, (size_t)2 * (2 * (R.maxk0 + R.maxk1 + R.maxk2) + 3) * sizeof(d2v));
+      lr_bytes = std::max(lr_bytes, (size_t)2 * (2 * (R.maxk0 + R.maxk1 + R.maxk2) + 3) * sizeof(d2v));   // per slice
       has_lr = true;
     }
     if (desc->species < 0 || desc->species >= R.ns || H->dev.sp.count[desc->species] < 1)
@@ -1136,7 +1140,12 @@ int prepare_bisection(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, cons
   P.nt = nt < 0 ? 0 : nt;
   P.lean = !has_bc && !has_lr;
   P.table_bytes = P.lean ? table_bytes_lin : table_bytes;
-  P.lr_bytes = lr_bytes;
+  // phase tables of up to 24 KB of slices at a time (all W/2^lowest + 1 slices when they
+  // fit), within the 64 KB a block may use next to the staged spline table
+  const int nslc = (1 << (desc->num_levels - desc->lowest_level)) + 1;
+  const size_t room = P.table_bytes + 8192 < 64 * 1024 ? 64 * 1024 - P.table_bytes - 8192 : 0;
+  P.lr_chunk = lr_bytes ? std::max(1, std::min(nslc, (int)(std::min(kLrPhaseBytes, room) / lr_bytes))) : 1;
+  P.lr_bytes = lr_bytes * P.lr_chunk;
   // longest first: the short-range work of a move grows with N, the long-range work with nk
   P.order.resize(nrep);
   for (int r = 0; r < nrep; r++) P.order[r] = r;
@@ -1193,6 +1202,7 @@ LaunchBisection make_launch(pimc_ctx* ctx, const BisectionPrep& P, const pimc_bi
   a.acc = ctx->d_acc.as<int64_t>();
   a.lds_table_bytes = P.table_bytes;
   a.lds_lr_bytes = P.lr_bytes;
+  a.lr_chunk = P.lr_chunk;
   a.link_split = ctx->plan.link_split;
   a.queue = ctx->plan.persistent ? d_queue : nullptr;
   a.first_move_dev = d_first;

@@ -274,6 +274,20 @@ __host__ __device__ inline size_t bisection_ctl_offset(const LaunchBisection& a)
   return a.lds_table_bytes + sizeof(double) * (6 * (W + 1) + W + 2 * W + 2 * kMaxWaves);
 }
 
+// Phase rows (per-dimension e^{i n phi}, both copies) of the slices t = (c0 + j) step,
+// j = 0 .. nc-1, into lrC: row pair j holds copy NEW at (2j) stride, OLD at (2j+1)
+// stride.  One lane per (slice, copy, dimension).
+template <typename LrT>
+__device__ inline void phase_rows_chunk(const RepCore& R, const PIMC_LDS double* mnew, const PIMC_LDS double* mold,
+                                        int c0, int nc, int step, int stride, PIMC_LDS d2v* lrC) {
+  for (int i = threadIdx.x; i < nc * 6; i += blockDim.x) {
+    const int j = i / 6, c = (i % 6) / 3, d = i % 3;
+    const PIMC_LDS double* src = (c == 0 ? mnew : mold) + 3 * (c0 + j) * step;
+    const double x[3] = {src[0], src[1], src[2]};
+    phase_rows_lane<LrT>(R, x, d, lrC + (2 * j + c) * stride);
+  }
+}
+
 // ---- Several blocks per replica (a cluster) ----
 // The `cluster` blocks of one replica run the same moves in lock step: each
 // samples the same midpoints (same draws), takes its share of the partners and
@@ -351,7 +365,7 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
   PIMC_LDS int* ctl = (PIMC_LDS int*)(scratch + 2 * kMaxWaves);           // slice1, ptcl, accept, item
   PIMC_LDS double* gauss = (PIMC_LDS double*)(ctl + 4);                   // [gauss_offset(W, L)]
   PIMC_LDS double* logu_tab = gauss + gauss_offset(W, L);                 // [PIMC_MAX_LEVELS]
-  PIMC_LDS d2v* lrC = (PIMC_LDS d2v*)(logu_tab + PIMC_MAX_LEVELS);        // [2][kstride]: NEW, OLD rows
+  PIMC_LDS d2v* lrC = (PIMC_LDS d2v*)(logu_tab + PIMC_MAX_LEVELS);        // [lr_chunk][2][kstride]: NEW, OLD rows
 
   int64_t st_moves = 0, st_acc = 0, st_stages = 0, st_evals = 0, st_kevals = 0;
   int nbar = 0;   // cluster barriers passed (the same count on every member)
@@ -468,24 +482,22 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
       const bool with_lr = HAS_LR && R.lr_on && level == a.lowest_level;
       if (HAS_LR && with_lr) {
         const int stride = kstride(R), o1 = koff_of(R, 1), o2 = koff_of(R, 2);
+        const int nslc = W / step + 1;   // slices t = 0, step, .., W; phase rows staged lr_chunk at a time
         double part_o = 0.0, part_n = 0.0;
-        for (int t = 0; t <= W; t += step) {
+        for (int c0 = 0; c0 < nslc; c0 += a.lr_chunk) {
+          const int nc = min(a.lr_chunk, nslc - c0);
           __syncthreads();
-          if (tid < 6) {
-            const int d = tid % 3;
-            PIMC_LDS double* src = (tid < 3 ? mnew : mold) + 3 * t;
-            const double x[3] = {src[0], src[1], src[2]};
-            phase_rows_lane<LrT>(R, x, d, lrC + (tid < 3 ? 0 : stride));
-          }
+          phase_rows_chunk<LrT>(R, mnew, mold, c0, nc, step, stride, lrC);
           __syncthreads();
-          const double factor = (t == 0 || t == W) ? 0.5 : 1.0;
-          const int slice = slice1 + t;
-          const d2v* rho = R.rho0 + (size_t)slice * R.ns * R.nk;
           for (int k = kb + tid; k < ke; k += blockDim.x) {
-            const int ki[3] = {R.kidx[4 * k], R.kidx[4 * k + 1], R.kidx[4 * k + 2]};
+           const int ki[3] = {R.kidx[4 * k], R.kidx[4 * k + 1], R.kidx[4 * k + 2]};
+           for (int j = 0; j < nc; j++) {
+            const int t = (c0 + j) * step;
+            const double factor = (t == 0 || t == W) ? 0.5 : 1.0;
+            const d2v* rho = R.rho0 + (size_t)(slice1 + t) * R.ns * R.nk;
             LrT nr, ni, orr, oi;
-            kterm<LrT>(lrC, o1, o2, ki, nr, ni);
-            kterm<LrT>(lrC + stride, o1, o2, ki, orr, oi);
+            kterm<LrT>(lrC + 2 * j * stride, o1, o2, ki, nr, ni);
+            kterm<LrT>(lrC + (2 * j + 1) * stride, o1, o2, ki, orr, oi);
             d2v ro[kMaxSpecies], rn[kMaxSpecies];
 #pragma unroll
             for (int s = 0; s < kMaxSpecies; s++) {
@@ -518,6 +530,7 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
                   part_n += factor * 2.0 * (rn[s1].x * rn[s2].x + rn[s1].y * rn[s2].y) * u;
                 }
               }
+           }
           }
         }
         block_sum2(part_o, part_n, scratch, lr_kold, lr_knew);
@@ -594,28 +607,27 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
     if (HAS_LR && accepted && R.lr_on) {
       const int step = 1 << a.lowest_level;
       const int stride = kstride(R), o1 = koff_of(R, 1), o2 = koff_of(R, 2);
-      for (int t = 0; t <= W; t += step) {
+      const int nslc = W / step + 1;
+      for (int c0 = 0; c0 < nslc; c0 += a.lr_chunk) {
+        const int nc = min(a.lr_chunk, nslc - c0);
         __syncthreads();
-        if (tid < 6) {
-          const int d = tid % 3;
-          PIMC_LDS double* src = (tid < 3 ? mnew : mold) + 3 * t;
-          const double x[3] = {src[0], src[1], src[2]};
-          phase_rows_lane<LrT>(R, x, d, lrC + (tid < 3 ? 0 : stride));
-        }
+        phase_rows_chunk<LrT>(R, mnew, mold, c0, nc, step, stride, lrC);
         __syncthreads();
-        const size_t base = ((size_t)(slice1 + t) * R.ns + sp) * R.nk;
         for (int k = kb + tid; k < ke; k += blockDim.x) {
           const int ki[3] = {R.kidx[4 * k], R.kidx[4 * k + 1], R.kidx[4 * k + 2]};
-          LrT nr, ni, orr, oi;
-          kterm<LrT>(lrC, o1, o2, ki, nr, ni);
-          kterm<LrT>(lrC + stride, o1, o2, ki, orr, oi);
-          d2v v = R.rho0[base + k];
-          v.x += (double)nr;
-          v.y += (double)ni;
-          v.x -= (double)orr;
-          v.y -= (double)oi;
-          R.rho0[base + k] = v;
-          R.rho1[base + k] = v;
+          for (int j = 0; j < nc; j++) {
+            const size_t base = ((size_t)(slice1 + (c0 + j) * step) * R.ns + sp) * R.nk;
+            LrT nr, ni, orr, oi;
+            kterm<LrT>(lrC + 2 * j * stride, o1, o2, ki, nr, ni);
+            kterm<LrT>(lrC + (2 * j + 1) * stride, o1, o2, ki, orr, oi);
+            d2v v = R.rho0[base + k];
+            v.x += (double)nr;
+            v.y += (double)ni;
+            v.x -= (double)orr;
+            v.y -= (double)oi;
+            R.rho0[base + k] = v;
+            R.rho1[base + k] = v;
+          }
         }
       }
     }

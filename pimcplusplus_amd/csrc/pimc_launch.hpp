@@ -65,6 +65,7 @@ struct LaunchBisection {
   int64_t* acc;               // [replica id][5]: counters accumulated over launches
   size_t lds_table_bytes;     // max staged level size over the fits used
   size_t lds_lr_bytes;        // phase tables of the long-range stage term (0 without long range)
+  int lr_chunk;               // slices whose phase tables are staged together (lds_lr_bytes holds them)
   int link_split;             // threads per partner in the short-range loop: 1, 2 or 4
   const int* rep_pos;         // [nrep] caller index of list entry i (records, stats), or nullptr
   int* queue;                 // work-queue counter of a persistent launch, or nullptr
