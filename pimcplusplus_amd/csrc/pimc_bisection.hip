@@ -291,17 +291,28 @@ __device__ inline void phase_rows_chunk(const RepCore& R, const PIMC_LDS double*
 // ---- Several blocks per replica (a cluster) ----
 // The `cluster` blocks of one replica run the same moves in lock step: each
 // samples the same midpoints (same draws), takes its share of the partners and
-// k-vectors, and the members exchange their partial stage sums through global
-// memory.  Every member then holds the same sums, added in member order, and
-// takes the same Metropolis decision.
+// k-vectors, and the members exchange their partial stage sums.  Every member then
+// holds the same sums, added in member order, and takes the same Metropolis
+// decision.  Every member also writes the accepted beads itself (the same values),
+// so no member reads a bead another block wrote.
+//
+// The exchange needs no cache maintenance: the partial sums are written with
+// device-scope atomic stores (write-through to the coherent level), the writer
+// waits for those stores to complete before it counts its arrival, and readers
+// load the sums with device-scope atomic loads after seeing every arrival.  (An
+// agent-scope release/acquire pair would write back and invalidate a whole L2
+// at every stage.)
 
-// Thread 0 of every member adds one to the replica's counter and waits until all
-// members have arrived n times.  The wait is bounded (a.wait_ticks of the device
-// wall clock, or until another block gives up): on a timeout the launch's abort
-// flag is set and false is returned, so no wave waits forever.
+// Thread 0 of every member counts its arrival on the replica's counter and waits
+// until all members have arrived n times.  The wait is bounded (a.wait_ticks of
+// the device wall clock, or until another block gives up): on a timeout the
+// launch's abort flag is set and false is returned, so no wave waits forever.
 __device__ inline bool cluster_barrier(const LaunchBisection& a, int item, int n) {
   int* bar = a.bar + item;
-  __hip_atomic_fetch_add(bar, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);   // this wave's earlier stores have completed
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __hip_atomic_fetch_add(bar, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const int target = a.cluster * n;
   const uint64_t t0 = wall_clock64();
   while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
@@ -312,7 +323,7 @@ __device__ inline bool cluster_barrier(const LaunchBisection& a, int item, int n
     }
     __builtin_amdgcn_s_sleep(1);
   }
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
   return true;
 }
 
@@ -323,12 +334,12 @@ __device__ inline bool cluster_barrier(const LaunchBisection& a, int item, int n
 __device__ inline bool cluster_sum(const LaunchBisection& a, int item, int member, int nbar, double v[4]) {
   double* slot = a.part + ((size_t)item * 2 + (nbar & 1)) * a.cluster * 4;
 #pragma unroll
-  for (int i = 0; i < 4; i++) slot[member * 4 + i] = v[i];
+  for (int i = 0; i < 4; i++) __hip_atomic_store(slot + member * 4 + i, v[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (!cluster_barrier(a, item, nbar + 1)) return false;
   double t[4] = {0.0, 0.0, 0.0, 0.0};
   for (int m = 0; m < a.cluster; m++)
 #pragma unroll
-    for (int i = 0; i < 4; i++) t[i] += slot[m * 4 + i];
+    for (int i = 0; i < 4; i++) t[i] += __hip_atomic_load(slot + m * 4 + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
   for (int i = 0; i < 4; i++) v[i] = t[i];
   return true;
@@ -631,24 +642,13 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
         }
       }
     }
-    if (accepted && member == 0) {
+    if (accepted) {
       for (int t = tid; t <= W; t += blockDim.x) {
         double v[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) v[d] = put_in_box_fast_1(mnew[3 * t + d], R.box[d]);
         store_bead(R, 0, slice1 + t, ptcl, v);
         store_bead(R, 1, slice1 + t, ptcl, v);
-      }
-    }
-    if constexpr (CL) {
-      // the other members read the moved beads in the next move: publish them
-      if (accepted) {
-        if (member == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __syncthreads();
-        if (tid == 0) ctl[2] = cluster_barrier(a, item, nbar + 1) ? 1 : -1;
-        nbar++;
-        __syncthreads();
-        if (ctl[2] < 0) return;
       }
     }
     if (tid == 0 && member == 0) {
