@@ -215,6 +215,22 @@ int pimc_reject_copy(pimc_ctx* ctx, int32_t replica, int32_t slice1, int32_t sli
 /* PathClass::MoveJoin(old_join, new_join). */
 int pimc_move_join(pimc_ctx* ctx, int32_t replica, int32_t old_join, int32_t new_join);
 
+/* PermuteTableClass::ConstructHTable (PIMC++/src/Moves/PermuteTableClass.cc:122-150)
+ * for the particles of `species`, read from copy `mode`:
+ *   HTable(i,j) = exp((|PutInBox(r_i(s2) - r_i(s1))|^2 - |PutInBox(r_j(s2) - r_i(s1))|^2)
+ *                     / (4 lambda tau (s2 - s1)))
+ * stored as float, or 0 where the exponent is below log(epsilon).  htable is
+ * [n][n] row-major (i = row), n = the species' particle count; i and j are
+ * indices within the species. */
+int pimc_permute_htable(pimc_ctx* ctx, int32_t replica, int32_t mode, int32_t species,
+                        int32_t slice1, int32_t slice2, double epsilon, float* htable);
+/* CycleClass::Apply (PIMC++/src/Moves/PermuteTableClass.cc:172-187) on the NEW copy
+ * at one slice: particle first_ptcl + cycle[i] takes the bead and permutation entry
+ * of first_ptcl + cycle[i+1], and the last takes those of first_ptcl + cycle[0].
+ * length 1..4, entries distinct. */
+int pimc_cycle_apply(pimc_ctx* ctx, int32_t replica, int32_t first_ptcl, int32_t slice,
+                     int32_t length, const int32_t* cycle);
+
 /* PathClass::UpdateRho_ks(): recompute rho_k of every slice and species from
  * the OLD copy, then NEW <- OLD (PathClass.cc:709-723). */
 int pimc_rho_k_update_all(pimc_ctx* ctx, int32_t replica);

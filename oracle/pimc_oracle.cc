@@ -21,6 +21,7 @@
 //   PIMC++/src/Moves/StageClass.cc:49-95,124-136, MultiStage.cc:42-110,
 //   PIMC++/src/Moves/BisectionStage.cc:128-289, BisectionBlock.cc:66-104,310-451,
 //   PIMC++/src/Moves/NoPermuteClass.cc:26-68, Common/Random/Random.h:45-46,156-180
+//   PIMC++/src/Moves/PermuteTableClass.cc:122-150,172-187   ConstructHTable, CycleClass::Apply
 //
 // Deliberate, documented departures:
 //   * LogGrid::ReverseMap: the reference's floor(log(r/r0)/log(spacing)) is
@@ -1375,6 +1376,60 @@ int pimc_move_join(pimc_ctx* ctx, int32_t replica, int32_t old_join, int32_t new
     return fail(PIMC_ERR_ARG, "move_join: bad argument");
   move_join(*R, old_join, new_join);
   R->join = new_join;
+  return PIMC_OK;
+}
+
+// PermuteTableClass::ConstructHTable (PermuteTableClass.cc:122-150)
+int pimc_permute_htable(pimc_ctx* ctx, int32_t replica, int32_t mode, int32_t species, int32_t slice1,
+                        int32_t slice2, double epsilon, float* htable) {
+  Replica* R = get_rep(&ctx->c, replica);
+  if (!R || !htable || mode < 0 || mode > 1) return fail(PIMC_ERR_ARG, "permute_htable: bad argument");
+  if (species < 0 || species >= R->ns) return fail(PIMC_ERR_ARG, "permute_htable: bad species");
+  if (slice1 < 0 || slice2 <= slice1 || slice2 >= R->nslices) return fail(PIMC_ERR_ARG, "permute_htable: bad slices");
+  if (!(epsilon > 0.0)) return fail(PIMC_ERR_ARG, "permute_htable: epsilon must be positive");
+  const double logEps = log(epsilon);
+  const int firstPtcl = R->first[species];
+  const double lambda = R->lambda[species];
+  const double beta = R->tau * (double)(slice2 - slice1);
+  const double fourLambdaBetaInv = 1.0 / (4.0 * lambda * beta);
+  const int N = R->count[species];
+  for (int i = 0; i < N; i++) {
+    double disp_ii[3];
+    for (int d = 0; d < 3; d++) disp_ii[d] = R->P(mode, slice2, i + firstPtcl)[d] - R->P(mode, slice1, i + firstPtcl)[d];
+    R->put_in_box(disp_ii);
+    const double dist_ii = disp_ii[0] * disp_ii[0] + disp_ii[1] * disp_ii[1] + disp_ii[2] * disp_ii[2];
+    for (int j = 0; j < N; j++) {
+      double disp_ij[3];
+      for (int d = 0; d < 3; d++) disp_ij[d] = R->P(mode, slice2, j + firstPtcl)[d] - R->P(mode, slice1, i + firstPtcl)[d];
+      R->put_in_box(disp_ij);
+      const double dist_ij = disp_ij[0] * disp_ij[0] + disp_ij[1] * disp_ij[1] + disp_ij[2] * disp_ij[2];
+      const double toExp = (-dist_ij + dist_ii) * fourLambdaBetaInv;
+      htable[(size_t)i * N + j] = toExp < logEps ? 0.0f : (float)exp(toExp);
+    }
+  }
+  return PIMC_OK;
+}
+
+// CycleClass::Apply (PermuteTableClass.cc:172-187), on the NEW copy
+int pimc_cycle_apply(pimc_ctx* ctx, int32_t replica, int32_t first_ptcl, int32_t slice, int32_t length,
+                     const int32_t* cycle) {
+  Replica* R = get_rep(&ctx->c, replica);
+  if (!R || !cycle || length < 1 || length > 4) return fail(PIMC_ERR_ARG, "cycle_apply: bad argument");
+  if (slice < 0 || slice >= R->nslices || first_ptcl < 0) return fail(PIMC_ERR_ARG, "cycle_apply: bad slice or first_ptcl");
+  for (int i = 0; i < length; i++) {
+    if (cycle[i] < 0 || first_ptcl + cycle[i] >= R->N) return fail(PIMC_ERR_ARG, "cycle_apply: particle out of range");
+    for (int k = 0; k < i; k++)
+      if (cycle[k] == cycle[i]) return fail(PIMC_ERR_ARG, "cycle_apply: repeated particle");
+  }
+  double tempPos[3];
+  memcpy(tempPos, R->P(1, slice, cycle[0] + first_ptcl), sizeof tempPos);
+  const int tempPtcl = R->perm[1][cycle[0] + first_ptcl];
+  for (int i = 0; i < length - 1; i++) {
+    memcpy(R->P(1, slice, cycle[i] + first_ptcl), R->P(1, slice, cycle[i + 1] + first_ptcl), 3 * sizeof(double));
+    R->perm[1][cycle[i] + first_ptcl] = R->perm[1][cycle[i + 1] + first_ptcl];
+  }
+  memcpy(R->P(1, slice, cycle[length - 1] + first_ptcl), tempPos, sizeof tempPos);
+  R->perm[1][cycle[length - 1] + first_ptcl] = tempPtcl;
   return PIMC_OK;
 }
 

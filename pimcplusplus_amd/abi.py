@@ -24,6 +24,7 @@ GRID_LINEAR, GRID_LOG = 0, 1
 
 _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)
+_f32p = C.POINTER(C.c_float)
 
 
 class DavidFitDesc(C.Structure):
@@ -169,6 +170,9 @@ SIGNATURES = {
     "pimc_accept_copy": (C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32, _i32p, C.c_int32]),
     "pimc_reject_copy": (C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32, _i32p, C.c_int32]),
     "pimc_move_join": (C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32]),
+    "pimc_permute_htable": (C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                                      _f32p]),
+    "pimc_cycle_apply": (C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _i32p]),
     "pimc_rho_k_update_all": (C.c_int, [_CTX, C.c_int32]),
     "pimc_rho_k_get": (C.c_int, [_CTX, C.c_int32, C.c_int32, _f64p]),
     "pimc_action_short_range": (C.c_int, [_CTX, C.POINTER(ActionReq), C.c_int32, C.c_int32, _f64p]),
@@ -439,6 +443,20 @@ class PimcLib:
 
     def move_join(self, rep: int, old_join: int, new_join: int):
         self._check("pimc_move_join", self.lib.pimc_move_join(self.ctx, rep, old_join, new_join))
+
+    def permute_htable(self, rep: int, species: int, slice1: int, slice2: int, epsilon: float,
+                       n: int, mode: int = OLDMODE) -> np.ndarray:
+        """PermuteTableClass::ConstructHTable of `species` (n particles): float [n, n]."""
+        out = np.zeros((n, n), dtype=np.float32)
+        self._check("pimc_permute_htable", self.lib.pimc_permute_htable(
+            self.ctx, rep, mode, species, slice1, slice2, epsilon, out.ctypes.data_as(_f32p)))
+        return out
+
+    def cycle_apply(self, rep: int, first_ptcl: int, slice_: int, cycle):
+        """CycleClass::Apply on the NEW copy at one slice."""
+        c = _i32(cycle)
+        self._check("pimc_cycle_apply", self.lib.pimc_cycle_apply(self.ctx, rep, first_ptcl, slice_, c.size,
+                                                                  _p(c, _i32p)))
 
     def rho_k_update_all(self, rep: int):
         self._check("pimc_rho_k_update_all", self.lib.pimc_rho_k_update_all(self.ctx, rep))

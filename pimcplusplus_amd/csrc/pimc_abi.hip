@@ -942,6 +942,55 @@ int pimc_move_join(pimc_ctx* ctx, int32_t replica, int32_t old_join, int32_t new
   return PIMC_OK;
 }
 
+int pimc_permute_htable(pimc_ctx* ctx, int32_t replica, int32_t mode, int32_t species, int32_t slice1,
+                        int32_t slice2, double epsilon, float* htable) {
+  HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
+  if (!H || !htable || mode < 0 || mode > 1) return fail(PIMC_ERR_ARG, "permute_htable: bad argument");
+  const RepCore& R = H->dev.core;
+  if (species < 0 || species >= R.ns) return fail(PIMC_ERR_ARG, "permute_htable: bad species");
+  if (slice1 < 0 || slice2 <= slice1 || slice2 >= R.nslices) return fail(PIMC_ERR_ARG, "permute_htable: bad slices");
+  if (!(epsilon > 0.0)) return fail(PIMC_ERR_ARG, "permute_htable: epsilon must be positive");
+  const int first = H->dev.sp.first[species], n = H->dev.sp.count[species];
+  // PermuteTableClass.cc:124-129
+  const double beta = R.tau * (double)(slice2 - slice1);
+  const double four_lambda_beta_inv = 1.0 / (4.0 * H->dev.sp.lambda[species] * beta);
+  const double log_eps = log(epsilon);
+  HIP_TRY(hipSetDevice(ctx->device));
+  int rc = ctx->sync_descriptors();
+  if (rc) return rc;
+  const size_t bytes = (size_t)n * n * sizeof(float);
+  DevBuf out;
+  if ((rc = out.ensure(bytes))) return rc;
+  launch_permute_htable(ctx->d_reps.as<ReplicaDev>(), replica, mode, first, n, slice1, slice2, four_lambda_beta_inv,
+                        log_eps, out.as<float>(), ctx->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(htable, out.p, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return PIMC_OK;
+}
+
+int pimc_cycle_apply(pimc_ctx* ctx, int32_t replica, int32_t first_ptcl, int32_t slice, int32_t length,
+                     const int32_t* cycle) {
+  HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
+  if (!H || !cycle || length < 1 || length > 4) return fail(PIMC_ERR_ARG, "cycle_apply: bad argument");
+  const RepCore& R = H->dev.core;
+  if (slice < 0 || slice >= R.nslices || first_ptcl < 0) return fail(PIMC_ERR_ARG, "cycle_apply: bad slice or first_ptcl");
+  int ptcls[4];
+  for (int i = 0; i < length; i++) {
+    ptcls[i] = first_ptcl + cycle[i];
+    if (cycle[i] < 0 || ptcls[i] >= R.N) return fail(PIMC_ERR_ARG, "cycle_apply: particle out of range");
+    for (int k = 0; k < i; k++)
+      if (ptcls[k] == ptcls[i]) return fail(PIMC_ERR_ARG, "cycle_apply: repeated particle");
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  int rc = ctx->sync_descriptors();
+  if (rc) return rc;
+  launch_cycle_apply(ctx->d_reps.as<ReplicaDev>(), replica, slice, ptcls, length, ctx->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return PIMC_OK;
+}
+
 int pimc_rho_k_update_all(pimc_ctx* ctx, int32_t replica) {
   HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
   if (!H) return fail(PIMC_ERR_ARG, "rho_k_update_all: bad replica");

@@ -283,4 +283,66 @@ void launch_move_join(const ReplicaDev* reps, int rep, int N, int old_join, int 
   k_move_join_2<<<(total + 255) / 256, 256, 0, s>>>(reps, rep, lo, hi);
 }
 
+// --------------------------------------------------------------------------
+// PermuteTableClass::ConstructHTable (PermuteTableClass.cc:122-150): one thread
+// per (i, j) of the species' particles; displacements through PutInBox, dot
+// products summed over x, y, z in order.
+// --------------------------------------------------------------------------
+__global__ void k_permute_htable(const ReplicaDev* __restrict__ reps, int rep, int mode, int first, int n, int slice1,
+                                 int slice2, double four_lambda_beta_inv, double log_eps, float* __restrict__ out) {
+  const RepCore& R = reps[rep].core;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)n * n) return;
+  const int i = (int)(t / n), j = (int)(t % n);
+  double a[3], b[3], c[3];
+  load_bead(R, mode, slice1, first + i, a);
+  load_bead(R, mode, slice2, first + i, b);
+  load_bead(R, mode, slice2, first + j, c);
+  double dii = 0.0, dij = 0.0;
+  for (int d = 0; d < 3; d++) {
+#pragma clang fp contract(off)
+    const double u = put_in_box_1(b[d] - a[d], R.box[d], R.boxinv[d], R.isper[d]);
+    const double w = put_in_box_1(c[d] - a[d], R.box[d], R.boxinv[d], R.isper[d]);
+    dii += u * u;
+    dij += w * w;
+  }
+  const double to_exp = (-dij + dii) * four_lambda_beta_inv;
+  out[t] = to_exp < log_eps ? 0.0f : (float)exp(to_exp);
+}
+void launch_permute_htable(const ReplicaDev* reps, int rep, int mode, int first, int n, int slice1, int slice2,
+                           double four_lambda_beta_inv, double log_eps, float* out, hipStream_t s) {
+  const long long total = (long long)n * n;
+  if (total <= 0) return;
+  k_permute_htable<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(reps, rep, mode, first, n, slice1, slice2,
+                                                                  four_lambda_beta_inv, log_eps, out);
+}
+
+// --------------------------------------------------------------------------
+// CycleClass::Apply (PermuteTableClass.cc:172-187) on the NEW copy at one slice:
+// a rotation of at most four beads and permutation entries, in the reference's
+// order, by one thread.
+// --------------------------------------------------------------------------
+__global__ void k_cycle_apply(const ReplicaDev* __restrict__ reps, int rep, int slice, ReqDev cyc) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const RepCore& R = reps[rep].core;
+  const int len = cyc.nptcls;
+  double temp[3];
+  load_bead(R, 1, slice, cyc.ptcls[0], temp);
+  const int temp_perm = R.perm1[cyc.ptcls[0]];
+  for (int i = 0; i < len - 1; i++) {
+    double v[3];
+    load_bead(R, 1, slice, cyc.ptcls[i + 1], v);
+    store_bead(R, 1, slice, cyc.ptcls[i], v);
+    R.perm1[cyc.ptcls[i]] = R.perm1[cyc.ptcls[i + 1]];
+  }
+  store_bead(R, 1, slice, cyc.ptcls[len - 1], temp);
+  R.perm1[cyc.ptcls[len - 1]] = temp_perm;
+}
+void launch_cycle_apply(const ReplicaDev* reps, int rep, int slice, const int* ptcls, int n, hipStream_t s) {
+  ReqDev c{};
+  c.nptcls = n;
+  for (int i = 0; i < n; i++) c.ptcls[i] = ptcls[i];
+  k_cycle_apply<<<1, 64, 0, s>>>(reps, rep, slice, c);
+}
+
 }  // namespace pimc
