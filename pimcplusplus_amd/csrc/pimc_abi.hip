@@ -517,6 +517,11 @@ int pimc_fit_create_david(pimc_ctx* ctx, const pimc_david_fit_desc* d, int32_t* 
   F.ncol = ncol;
   F.noff = noff;
   F.num_terms = nt;
+  // the first interval above every flagged one (all levels)
+  F.i32 = 0;
+  for (int lev = 0; lev < ntau; lev++)
+    for (int i = 0; i < nint; i++)
+      if (big[(size_t)lev * nint + i]) F.i32 = std::max(F.i32, i + 1);
   F.taupos = taupos;
   F.start = g.start;
   F.end = g.end;

@@ -31,7 +31,7 @@ __device__ inline double sr_link(const RepCore& R, const FitDev& F, const Table&
                                  const double* __restrict__ ul, const double pbox[3], const double hsafe[3],
                                  double urmin, const PIMC_LDS double* m, const double p[3], double dA[3], double& rA,
                                  double& uA, double& lA) {
-  double dn[3], raw[3];
+  double dn[3], raw[3], e[3];
   bool same = true;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
@@ -43,7 +43,8 @@ __device__ inline double sr_link(const RepCore& R, const FitDev& F, const Table&
     }
     // DistDisp's consistent image m = nearbyint((dA - raw)/L) equals the nearest
     // image n whenever |dA - dn| < L/2 (hsafe keeps a margin far above rounding)
-    same = same && (fabs(dA[d] - dn[d]) < hsafe[d]);
+    e[d] = dA[d] - dn[d];
+    same = same && (fabs(e[d]) < hsafe[d]);
   }
   const double n2 = dn[0] * dn[0] + dn[1] * dn[1] + dn[2] * dn[2];
   const double rn = MIXED ? sqrt_seeded(n2) : sqrt_dist(n2);
@@ -58,14 +59,14 @@ __device__ inline double sr_link(const RepCore& R, const FitDev& F, const Table&
 #pragma clang fp contract(off)
       const double mm = rint((dA[d] - raw[d]) * R.boxinv[d]);
       db[d] = raw[d] + mm * pbox[d];
+      e[d] = dA[d] - db[d];
     }
     const double b2 = db[0] * db[0] + db[1] * db[1] + db[2] * db[2];
     rb = MIXED ? sqrt_seeded(b2) : sqrt_dist(b2);
     if constexpr (!BC) ub = slice_u00<LIN>(F, T, rb);
     if (ul) lb = ulong_eval(R, ul, rb);
   }
-  const double e0 = dA[0] - db[0], e1 = dA[1] - db[1], e2 = dA[2] - db[2];
-  const double s2 = e0 * e0 + e1 * e1 + e2 * e2;
+  const double s2 = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
   const double q = 0.5 * (rA + rb);
   const double z = rA - rb;
   double U;

@@ -88,6 +88,9 @@ struct FitDev {
   const d4v* dbc;
   double smallest_beta;
   double z1z2;
+  // kind 0: the FP32 rows of intervals i >= i32 are valid (not NaN) at every
+  // level; the mixed loops evaluate the FP64 rows of intervals below it
+  int i32;
 };
 
 // Largest |c0|+|c1|+|c2|+|c3| of a column the mixed path evaluates in FP32.
@@ -417,13 +420,32 @@ __device__ inline double pair_u64(const FitDev& F, const LevelTable64<InLds>& T,
 
 // sqrt in FP64 from an FP32 reciprocal-square-root seed and one Goldschmidt
 // step (relative error ~1e-15); used by the mixed path for pair distances.
+// Valid for 0 <= a < 1e30 (squared pair distances): a = 0 gives 0, and a below
+// 1e-30 (distances below 1e-15, inside any pair table's core) is not refined.
 __device__ inline double sqrt_seeded(double a) {
-  if (!(a > 1e-30) || !(a < 1e30)) return sqrt(a);
-  double y = (double)__builtin_amdgcn_rsqf((float)a);
+  double y = (double)__builtin_amdgcn_rsqf(fmaxf((float)a, 1e-30f));
   double g = a * y;
   double h = 0.5 * y;
   double d = fma(-g, h, 0.5);
   return fma(g, d, g);
+}
+
+// Columns [c0col, c0col + NC) of interval i at local coordinate b: FP32 Horner
+// from the FP32 table, or FP64 from the global FP64 table when use64 (the caller
+// decides from the lookup coordinate, before any table load).
+template <int NC, bool InLds>
+__device__ inline void eval_cols_split(const LevelTable32<InLds>& T, int ncol, int i, int c0col, double b, bool use64,
+                                       double out[NC]) {
+  const int at = i * ncol + c0col;
+  if (use64) {
+    const d4v* c64 = T.c64 + (size_t)at;
+#pragma unroll
+    for (int k = 0; k < NC; k++) out[k] = horner64(c64[k], b);
+  } else {
+    const float bf = (float)b;
+#pragma unroll
+    for (int k = 0; k < NC; k++) out[k] = (double)horner32(T.c[at + k], bf);
+  }
 }
 
 // Columns [c0col, c0col + NC) of interval i at local coordinate b: FP32 Horner

@@ -242,10 +242,11 @@ __device__ inline double pair_link_u(const RepCore& R, const FitDev& F, const Ta
 // ReverseMap (nearbyint(t - 1/2)) differs only where t is an integer, where both
 // neighbouring cubics take the same value; b differs from (x - X_i)/h by
 // rounding of t.
+// x < Start (a distance inside the core, whose value the caller replaces by
+// u00(rmin)) maps to an in-range interval through the unsigned clamp.
 __device__ inline int lin_low(const FitDev& F, double x, double& b) {
   const double t = (x - F.start) * F.deltainv;
-  int i = (int)t;
-  i = i < 0 ? 0 : (i > F.nint - 1 ? F.nint - 1 : i);
+  const int i = (int)min((unsigned)(int)t, (unsigned)(F.nint - 1));
   b = t - (double)i;
   return i;
 }
@@ -265,18 +266,18 @@ __device__ inline int table_low(const FitDev& F, const Table& T, double x, doubl
 
 template <bool LIN, bool InLds>
 __device__ inline double slice_u00(const FitDev& F, const LevelTable64<InLds>& T, double r) {
-  r = r > F.end ? F.end : r;
+  r = fmin(r, F.end);
   double b;
   const int l = table_low<LIN>(F, T, r, b);
   return horner64(T.c[l * F.ncol], b);
 }
 template <bool LIN, bool InLds>
 __device__ inline double slice_u00(const FitDev& F, const LevelTable32<InLds>& T, double r) {
-  r = r > F.end ? F.end : r;
+  r = fmin(r, F.end);
   double b;
   const int l = table_low<LIN>(F, T, r, b);
   double u[1];
-  eval_cols_mixed<1>(T, F.ncol, l, 0, b, u);
+  eval_cols_split<1>(T, F.ncol, l, 0, b, l < F.i32, u);
   return u[0];
 }
 
@@ -308,12 +309,20 @@ __device__ inline double link_offdiag(const FitDev& F, const Table& T, double q,
     const int lq = table_low<LIN>(F, T, q, bq);
     double cv[NOFF];
     if constexpr (IsMixed<Table>::value) {
-      eval_cols_mixed<NOFF>(T, F.ncol, lq, 1, bq, cv);
+      eval_cols_split<NOFF>(T, F.ncol, lq, 1, bq, lq < F.i32, cv);
     } else {
 #pragma unroll
       for (int c = 0; c < NOFF; c++) cv[c] = horner64(T.c[lq * F.ncol + 1 + c], bq);
     }
     const double z2 = z * z;
+    if constexpr (NT == 1) {
+      return fma(cv[1], z2, cv[0] * s2);   // u10 s^2 + u11 z^2
+    } else if constexpr (NT == 2) {
+      // u10 s^2 + u11 z^2 + u20 s^4 + u21 z^2 s^2 + u22 z^4, grouped in s^2 and z^2
+      const double a = fma(cv[4], z2, cv[1]);
+      const double c = fma(cv[3], z2, fma(cv[2], s2, cv[0]));
+      return fma(a, z2, c * s2);
+    }
     double zp[NT + 1], sp[NT + 1];
     zp[0] = 1.0;
     sp[0] = 1.0;
