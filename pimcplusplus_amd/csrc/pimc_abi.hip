@@ -580,13 +580,27 @@ int pimc_fit_create_coulomb_bc(pimc_ctx* ctx, const pimc_coulomb_bc_fit_desc* d,
     HIP_TRY(hipMemcpy(f->dmem.p, dbc.data(), dbc.size() * sizeof(d4v), hipMemcpyHostToDevice));
     f->dev.dbc = f->dmem.as<d4v>();
   }
+  // interval records {X_i, h_i, 1/h_i, X_{i+1}} of both grids
+  auto records = [](const HostGrid& g) {
+    std::vector<d4v> r(g.n - 1);
+    for (int i = 0; i + 1 < g.n; i++) {
+      const double h = g.x[i + 1] - g.x[i];
+      r[i] = d4v{g.x[i], h, 1.0 / h, g.x[i + 1]};
+    }
+    return r;
+  };
+  const std::vector<d4v> qiv = records(qg), tiv = records(tg);
   size_t bq = (size_t)nq * 8, bt = (size_t)nt * 8, bb = bc.size() * sizeof(d4v);
-  size_t off_q = 0, off_t = (bq + 255) & ~(size_t)255, off_b = (off_t + bt + 255) & ~(size_t)255, total = off_b + bb;
+  size_t bqi = qiv.size() * sizeof(d4v), bti = tiv.size() * sizeof(d4v);
+  size_t off_q = 0, off_t = (bq + 255) & ~(size_t)255, off_b = (off_t + bt + 255) & ~(size_t)255;
+  size_t off_qi = (off_b + bb + 255) & ~(size_t)255, off_ti = (off_qi + bqi + 255) & ~(size_t)255, total = off_ti + bti;
   if ((rc = f->mem.ensure(total))) return rc;
   unsigned char* base = f->mem.as<unsigned char>();
   HIP_TRY(hipMemcpy(base + off_q, qg.x.data(), bq, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(base + off_t, tg.x.data(), bt, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(base + off_b, bc.data(), bb, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(base + off_qi, qiv.data(), bqi, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(base + off_ti, tiv.data(), bti, hipMemcpyHostToDevice));
   FitDev& F = f->dev;
   F.kind = 1;
   F.grid_type = qg.type;
@@ -602,8 +616,13 @@ int pimc_fit_create_coulomb_bc(pimc_ctx* ctx, const pimc_coulomb_bc_fit_desc* d,
   F.logspacing = qg.type == PIMC_GRID_LOG ? log(qg.spacing) : 0.0;
   F.x = (const double*)(base + off_q);
   F.nt = nt;
-  F.tgrid = GridDev{tg.type, nt, tg.start, tg.end, tg.deltainv, tg.r0,
-                    tg.type == PIMC_GRID_LOG ? log(tg.spacing) : 0.0, (const double*)(base + off_t)};
+  auto grid_dev = [](const HostGrid& g, const unsigned char* x, const unsigned char* iv) {
+    const bool lg = g.type == PIMC_GRID_LOG;
+    return GridDev{g.type, g.n, g.start, g.end, g.deltainv, g.r0, lg ? log(g.spacing) : 0.0, (const double*)x,
+                   (const d4v*)iv, lg ? 1.0 / g.r0 : 0.0, lg ? (float)(1.0 / log2(g.spacing)) : 0.0f};
+  };
+  F.tgrid = grid_dev(tg, base + off_t, base + off_ti);
+  F.qgrid = grid_dev(qg, base + off_q, base + off_qi);
   F.bc = (const d4v*)(base + off_b);
   F.smallest_beta = d->smallest_beta;
   F.z1z2 = d->z1z2;

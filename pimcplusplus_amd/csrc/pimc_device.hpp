@@ -49,6 +49,9 @@ struct GridDev {
   double deltainv;    // linear
   double r0, logspacing;
   const double* x;    // [n] points
+  const d4v* iv;      // [n-1] interval records {X_i, h_i, 1/h_i, X_{i+1}}
+  double r0inv;       // log: 1/r0
+  float log2inv;      // log: 1/log2(spacing)
 };
 
 // One DavidPAClass fit (kind 0) or PAcoulombBCFitClass fit (kind 1).  Per tau level and grid interval i the spline columns
@@ -79,6 +82,7 @@ struct FitDev {
   int kind;
   int nt;
   GridDev tgrid;
+  GridDev qgrid;          // kind 1: the q grid with its interval records
   const d4v* bc;          // [num_tau][ng][nt] BCpoint {z, dzdx, dzdy, d2zdxdy}
   // beta derivatives for the energy estimator (nullptr when not given):
   // kind 0: [num_tau][nint][ecol] cubics of V (column 0 of the table), du00 and the
@@ -277,36 +281,43 @@ __device__ inline int spline_low(const FitDev& F, XPtr xs, double x) {
   return hi - 1;
 }
 
-// The same lookup on a GridDev (BicubicSpline: ix = ReverseMap clamped to [0, n-2]).
-__device__ inline int grid_low(const GridDev& G, double x) {
-#pragma clang fp contract(off)
-  int i;
-  if (G.type == 0) {
-    i = clamp_to_int(rint((x - G.start) * G.deltainv - 0.5), G.n);
-  } else {
-    i = clamp_to_int(floor(log(x / G.r0) / G.logspacing), G.n);
-    if (i >= 1 && i <= G.n - 2 && x < G.x[i])
-      i -= 1;
-    else if (i >= -1 && i <= G.n - 3 && x >= G.x[i + 1])
-      i += 1;
+// The interval ix of x on G clamped to [0, n-2] (BicubicSpline's ix) and its
+// record {X_i, h_i, 1/h_i, X_{i+1}}: an estimate from the grid's closed form (an
+// FP32 log2 on a LOG grid), corrected by one step against the record's knots, so
+// the interval is the one whose knots bracket x.  The estimate is within one
+// interval of it for every grid these tables use (FP32 log2 errs by ~1e-7).
+__device__ inline int grid_iv(const GridDev& G, double x, d4v& rec) {
+  float t;
+  if (G.type == 0)
+    t = (float)((x - G.start) * G.deltainv);
+  else
+    t = __log2f((float)(x * G.r0inv)) * G.log2inv;
+  t = fminf(fmaxf(t, 0.0f), (float)(G.n - 2));   // also maps log2(0) = -inf to 0
+  int i = (int)t;
+  rec = G.iv[i];
+  if (x < rec.x && i > 0) {
+    i -= 1;
+    rec = G.iv[i];
+  } else if (x >= rec.w && i < G.n - 2) {
+    i += 1;
+    rec = G.iv[i];
   }
-  if (i < 0) i = 0;
-  if (i > G.n - 2) i = G.n - 2;
   return i;
 }
 
 // --------------------------------------------------------------------------
 // PAcoulombBCFitClass::U (PAcoulombBCFit.cc:159-176) with BicubicSpline
-// evaluation (BicubicSpline.h:266-300, 356-413); the table is read from global memory.
+// evaluation (BicubicSpline.h:266-300, 356-413); the table is read from global
+// memory.  The local coordinates are (x - X_i) times the stored 1/h_i.
 // --------------------------------------------------------------------------
 __device__ inline double bicubic(const FitDev& F, const GridDev& Q, const d4v* __restrict__ tab, double x,
                                  double y) {
-  const int ix = grid_low(Q, x);
-  const int iy = grid_low(F.tgrid, y);
-  const double h = Q.x[ix + 1] - Q.x[ix];
-  const double k = F.tgrid.x[iy + 1] - F.tgrid.x[iy];
-  const double u = (x - Q.x[ix]) / h;
-  const double v = (y - F.tgrid.x[iy]) / k;
+  d4v qr, tr;
+  const int ix = grid_iv(Q, x, qr);
+  const int iy = grid_iv(F.tgrid, y, tr);
+  const double h = qr.y, k = tr.y;
+  const double u = (x - qr.x) * qr.z;
+  const double v = (y - tr.x) * tr.z;
   const double a0 = (u - 1.0) * (u - 1.0) * (1.0 + 2.0 * u), a1 = u * u * (3.0 - 2.0 * u);
   const double a2 = h * (u * (u - 1.0) * (u - 1.0)), a3 = h * (u * u * (u - 1.0));
   const double b0 = (v - 1.0) * (v - 1.0) * (1.0 + 2.0 * v), b1 = v * v * (3.0 - 2.0 * v);

@@ -116,18 +116,7 @@ struct BcView {
   GridDev q;
   int level;
 };
-__device__ inline GridDev qgrid_of(const FitDev& F) {
-  GridDev g;
-  g.type = F.grid_type;
-  g.n = F.ng;
-  g.start = F.start;
-  g.end = F.end;
-  g.deltainv = F.deltainv;
-  g.r0 = F.r0;
-  g.logspacing = F.logspacing;
-  g.x = F.x;
-  return g;
-}
+__device__ inline GridDev qgrid_of(const FitDev& F) { return F.qgrid; }
 template <int NT>
 __device__ inline double pair_u(const FitDev& F, const BcView& T, double q, double z, double s2) {
   return pair_u_bc(F, T.q, T.level, q, s2);
