@@ -251,6 +251,29 @@ class CoulombBCTable:
         n = self.umat.shape[1] if which == "q" else self.umat.shape[2]
         return _grid(gtype, start, end, n)
 
+    def udiag(self, level: int, derivative: bool = False):
+        """The diagonal action U(r, r; beta_l) (or dU/dbeta) as a vectorised function of r
+        (PAcoulombBCFitClass::Udiag / dUdiag, Common/PairAction/PAcoulombBCFit.cc:384-438): the
+        bicubic at t = 0, which is the natural cubic spline through the t = 0 column along q,
+        inside the q grid; beyond it the Coulomb tail beta_l Z1Z2 / r (Z1Z2 / r for dU) plus
+        the end mismatch decaying as (r_end / r)^3."""
+        from scipy.interpolate import CubicSpline
+        q = self.grid_points("q")
+        table = self.dumat if derivative else self.umat
+        if table is None:
+            raise ValueError("this table has no dU/dbeta data")
+        sp = CubicSpline(q, table[level][:, 0], bc_type="natural")
+        rend = q[-1]
+        scale = 1.0 if derivative else self.smallest_beta * 2.0 ** level
+        delta = float(sp(rend)) - scale * self.z1z2 / rend
+
+        def f(r):
+            r = np.asarray(r, dtype=float)
+            inside = r <= rend * 1.0000001
+            rr = np.where(inside, rend, r)
+            return np.where(inside, sp(np.minimum(r, rend)), scale * self.z1z2 / rr + delta * rend ** 3 / rr ** 3)
+        return f
+
 
 def _grid(gtype: int, start: float, end: float, n: int) -> np.ndarray:
     if gtype == GRID_LINEAR:

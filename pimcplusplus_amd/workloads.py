@@ -15,6 +15,7 @@ import numpy as np
 
 import math
 
+from .breakup import optimized_breakup, quintic_data
 from .ewald import dulong_quintic, ewald_dbeta_tables, ewald_tables, ulong_quintic
 from .tables import (GRID_LINEAR, GRID_LOG, CoulombBCTable, DavidTable, coulomb_bc_from_david,
                      coulomb_chord_table, lj_david_table)
@@ -45,6 +46,9 @@ class ReplicaSpec:
     # (sa, sb) -> (z1z2, alpha): Ulong(r) = tau_l z1z2 erf(alpha r)/r subtracted from long-ranged
     # pair fits, a quintic on LinearGrid(0, 0.75 |box|, 1000)
     ulong: dict = field(default_factory=dict)
+    # (sa, sb) -> key of a CoulombBCTable in System.tables: the optimized breakup of that
+    # pair's action gives both the k-space tables and Ulong(r)
+    breakup: dict = field(default_factory=dict)
 
 
 @dataclass
@@ -160,6 +164,16 @@ def load(lib, system: System):
                 start, end, coefs = ulong_quintic(z1z2, alpha, spec.tau, system.num_levels, r_end)
                 _, _, dcoefs = dulong_quintic(z1z2, alpha, spec.tau, system.num_levels, r_end)
                 lib.set_ulong(rid, sa, sb, start, end, coefs, dcoefs)
+        if spec.breakup:
+            kv, _ = lib.kvecs(rid)
+            for (sa, sb), key in spec.breakup.items():
+                tab = system.tables[key]
+                u, du = _breakup_tables(tab, spec.box, spec.k_cutoff, kv, system.num_levels)
+                lib.set_ulong(rid, sa, sb, 0.0, float(u.r_grid[-1]), quintic_data(u.r_grid, u.ulong_r),
+                              quintic_data(du.r_grid, du.ulong_r))
+                lib.set_long_range(rid, sa, sb, u.ulong_k, u.ulong_r0, u.ushort_k0, u.ulong_k0,
+                                   du.ulong_k, du.ulong_r0, du.ushort_k0, du.ulong_k0)
+            lib.rho_k_update_all(rid)
         if spec.ewald:
             kv, _ = lib.kvecs(rid)
             for (sa, sb), (z1z2, alpha) in spec.ewald.items():
@@ -213,6 +227,30 @@ def kcut(units: float, box: float) -> float:
     return (units - 1e-9) * 2.0 * math.pi / box
 
 
+_BREAKUP_CACHE: dict = {}
+
+
+def _breakup_tables(tab, box, kc: float, kvecs: np.ndarray, num_levels: int):
+    """Optimized breakups of U and dU/dbeta of a Coulomb pair table for one box and k set
+    (LongRangeClass::OptimizedBreakup_U / _dU), cached across replicas with the same box."""
+    key = (id(tab), tuple(box), kc, num_levels, kvecs.shape[0])
+    if key not in _BREAKUP_CACHE:
+        q_end = float(tab.grid_points("q")[-1])
+        u = optimized_breakup([tab.udiag(l) for l in range(num_levels)], box, kc, kvecs, tail_end=q_end)
+        du = optimized_breakup([tab.udiag(l, derivative=True) for l in range(num_levels)], box, kc, kvecs,
+                               tail_end=q_end)
+        _BREAKUP_CACHE[key] = (u, du)
+    return _BREAKUP_CACHE[key]
+
+
+def _long_range_tables(kind: str, z: list, alpha: float, keys: dict) -> dict:
+    if kind == "breakup":
+        return {"breakup": keys}
+    if kind == "ewald":
+        return {"ewald": _coulomb_pairs(z, alpha), "ulong": _coulomb_pairs(z, alpha)}
+    raise ValueError(f"long_range must be 'breakup' or 'ewald', not {kind!r}")
+
+
 def _coulomb_pairs(z: list, alpha: float) -> dict:
     ns = len(z)
     return {(a, b): (z[a] * z[b], alpha) for a in range(ns) for b in range(a, ns)}
@@ -220,12 +258,14 @@ def _coulomb_pairs(z: list, alpha: float) -> dict:
 
 def hydrogen_system(dm_table: DavidTable, num_replicas: int = 1, n_pairs: int = 32, rs: float = 1.75,
                     m: int = 128, tau: float = 0.125, num_levels: int = 4, kc_units: float = 5.0,
-                    seed: int = 1002) -> System:
+                    seed: int = 1002, long_range: str = "breakup") -> System:
     """Hydrogen plasma: n_pairs electrons (species 0, moved) and n_pairs protons (species 1).
     e-p: the squarer table `dm_table` (testcoul.dm, Z1Z2 = -1) resampled on the
     PAcoulombBCFitClass (q, t) layout; e-e and p-p: straight-line Coulomb tables.  Every pair
-    is long-ranged: Ulong = tau_l Z1Z2 erf(alpha r)/r is subtracted in the short-range action
-    and the Ewald k-space action is added at the lowest bisection level (alpha = kc / 6)."""
+    is long-ranged: its long part Ulong is subtracted in the short-range action and the
+    k-space action is added at the lowest bisection level.  long_range "breakup": the
+    optimized breakup of each pair table (as PIMC++); "ewald": the closed-form Ewald split
+    erf(alpha r)/r with alpha = kc / 6."""
     n = 2 * n_pairs
     box = coulomb_box(n_pairs, rs)
     kc = kcut(kc_units, box)
@@ -245,9 +285,8 @@ def hydrogen_system(dm_table: DavidTable, num_replicas: int = 1, n_pairs: int = 
                                 species_count=[n_pairs, n_pairs],
                                 lambdas=[LAMBDA_E, LAMBDA_E / PROTON_MASS],
                                 fit_keys=["ee", "ep", "ep", "pp"], beads=beads, rng_stream=r,
-                                k_cutoff=kc, use_long_range=True,
-                                ewald=_coulomb_pairs([-1.0, 1.0], alpha),
-                                ulong=_coulomb_pairs([-1.0, 1.0], alpha)))
+                                k_cutoff=kc, use_long_range=True, **_long_range_tables(
+                                    long_range, [-1.0, 1.0], alpha, {(0, 0): "ee", (0, 1): "ep", (1, 1): "pp"})))
     return System(name=f"hydrogen-n{n}-m{m}-l{num_levels}", tables=tabs, desired_tau={}, replicas=reps,
                   num_levels=num_levels, seed=seed,
                   meta={"rs": rs, "box": box, "kc": kc, "alpha": alpha, "tau": tau})
@@ -259,9 +298,10 @@ def config2(dm_table: DavidTable, num_replicas: int = 1) -> System:
 
 
 def ocp_system(num_replicas: int = 16, n: int = 1024, rs: float = 1.0, m: int = 128, tau: float = 0.05,
-               num_levels: int = 4, kc_units: float = 10.0, seed: int = 2000) -> System:
+               num_levels: int = 4, kc_units: float = 10.0, seed: int = 2000, long_range: str = "breakup") -> System:
     """One-component plasma with a neutralising background: n electrons, straight-line Coulomb
-    pair table, Ulong subtraction, Ewald k-space action with the background terms."""
+    pair table, Ulong subtraction, k-space action with the background terms (long_range as in
+    hydrogen_system)."""
     box = coulomb_box(n, rs)
     kc = kcut(kc_units, box)
     alpha = kc / 6.0
@@ -274,7 +314,7 @@ def ocp_system(num_replicas: int = 16, n: int = 1024, rs: float = 1.0, m: int = 
                                 species_count=[n], lambdas=[LAMBDA_E], fit_keys=["ee"],
                                 beads=random_start(n, box, m, rng, jitter=0.05 * box), rng_stream=r,
                                 k_cutoff=kc, use_long_range=True, use_background=True,
-                                ewald={(0, 0): (1.0, alpha)}, ulong={(0, 0): (1.0, alpha)}))
+                                **_long_range_tables(long_range, [1.0], alpha, {(0, 0): "ee"})))
     return System(name=f"ocp-n{n}-m{m}-l{num_levels}", tables=tabs, desired_tau={}, replicas=reps,
                   num_levels=num_levels, seed=seed,
                   meta={"rs": rs, "box": box, "kc": kc, "alpha": alpha, "tau": tau})
