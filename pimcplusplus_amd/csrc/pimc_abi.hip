@@ -379,7 +379,11 @@ int pimc_ctx_create(int32_t device, pimc_ctx** out) {
   int khz = 0;
   if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || khz <= 0)
     khz = 100000;   // the MI300-series wall clock runs at 100 MHz
-  c->wait_ticks = (uint64_t)khz * 1000u * 2u;   // two seconds
+  // bound of one cluster barrier wait: two seconds, or PIMC_CLUSTER_WAIT_US microseconds
+  // (a test can make every wait give up at once to exercise the abort path)
+  double wait_us = 2.0e6;
+  if (const char* env = getenv("PIMC_CLUSTER_WAIT_US")) wait_us = std::max(0.0, atof(env));
+  c->wait_ticks = (uint64_t)((double)khz * 1.0e-3 * wait_us);
   int rc = c->d_abort.ensure(sizeof(int));
   if (rc) return rc;
   HIP_TRY(hipMemset(c->d_abort.p, 0, sizeof(int)));

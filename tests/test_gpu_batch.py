@@ -150,3 +150,32 @@ def test_cluster_plan_too_large_for_the_device_is_refused(gpu):
     gpu.set_plan(512, 1, False, 32)   # 2048 blocks of 512 threads cannot all be resident
     with pytest.raises(RuntimeError):
         gpu.bisection_run(ids, sysm.num_levels, sysm.seed, 0, 1, abi.FP64)
+
+
+def test_cluster_barrier_timeout_is_reported_and_the_library_recovers(uneven, monkeypatch):
+    """A cluster whose barrier waits give up at once (the bound forced to zero from the host):
+    every block leaves the kernel, the run reports the error, and a context with the normal
+    bound runs the same replicas again."""
+    from pimcplusplus_amd import native
+    monkeypatch.setenv("PIMC_CLUSTER_WAIT_US", "0")
+    lib = native.open_context(0)
+    try:
+        lib.set_plan(256, 1, False, 4)
+        ids = workloads.load(lib, uneven)
+        print("forced cluster barrier timeout", flush=True)
+        with pytest.raises(RuntimeError, match="timed out"):
+            lib.bisection_run(ids, uneven.num_levels, uneven.seed, 0, 5, abi.FP64)
+        lib.set_plan(256, 1, False, 1)      # one block per replica: no barrier
+        _, st = lib.bisection_run(ids, uneven.num_levels, uneven.seed, 5, 5, abi.FP64)
+        assert st.moves == 5 * len(ids)
+    finally:
+        lib.close()
+    monkeypatch.delenv("PIMC_CLUSTER_WAIT_US")
+    lib = native.open_context(0)
+    try:
+        lib.set_plan(256, 1, False, 4)
+        ids = workloads.load(lib, uneven)
+        _, st = lib.bisection_run(ids, uneven.num_levels, uneven.seed, 0, 5, abi.FP64)
+        assert st.moves == 5 * len(ids)
+    finally:
+        lib.close()
