@@ -197,6 +197,16 @@ int pimc_replica_set_ulong(pimc_ctx* ctx, int32_t replica, int32_t sa, int32_t s
                            const double* dcoefs);
 /* dcoefs: dUlong(level)(r) in the same layout, or NULL (needed by the energy estimator only). */
 
+/* The long-range part of the pair potential of (sa, sb), for the potential-energy
+ * estimator: PairActionFitClass's Vlong (a QuinticSpline on the grid of the pair's Ulong
+ * tables, coefs [num_points][6] as above), Vlong_k [nk], Vlong_r0, Vshort_k0 and Vlong_k0
+ * (PAFitBase.h:39-53), as LongRangeClass::OptimizedBreakup_V builds them
+ * (PIMC++/src/Actions/LongRangeClass.cc:979-1136).  grid_start, grid_end and num_points
+ * must match the replica's Ulong grid when one is set. */
+int pimc_replica_set_vlong(pimc_ctx* ctx, int32_t replica, int32_t sa, int32_t sb, const double* vlong_k,
+                           double vlong_r0, double vshort_k0, double vlong_k0, double grid_start,
+                           double grid_end, int32_t num_points, const double* coefs);
+
 /* Path data. mode: PIMC_OLDMODE, PIMC_NEWMODE or PIMC_BOTHMODES (set only).
  * beads is [(M+1)][N][3]. */
 int pimc_path_set(pimc_ctx* ctx, int32_t replica, int32_t mode, const double* beads);
@@ -283,7 +293,11 @@ int pimc_dist_disp(pimc_ctx* ctx, int32_t replica, int32_t mode, int64_t n,
  *             PAcoulombBCFitClass::dU, PAcoulombBCFit.cc:178-192)
  *   du_long   LongRangeClass::d_dBeta      (PIMC++/src/Actions/LongRangeClass.cc:374-442), 0 without long range
  *   v_short   sum over slices (1/2 at the ends) of ShortRangePotClass::V
- *             (PIMC++/src/Actions/ShortRangePotClass.cc:27-47); NaN when a pair fit is not a DavidPA fit
+ *             (PIMC++/src/Actions/ShortRangePotClass.cc:27-47): V(r) of each pair (the fit's
+ *             column 0 for DavidPA fits, Z1Z2/r for Coulomb fits), minus Vlong(r) for a
+ *             long-ranged fit when long range is used; NaN when such a pair has no Vlong tables
+ *   v_long    the same sum of LongRangePotClass::V (PIMC++/src/Actions/LongRangePotClass.cc:32-85);
+ *             0 without long range, NaN when a long-ranged pair has no Vlong tables
  *   total     kinetic + du_short + du_long (EnergyClass::Accumulate's sum without node terms,
  *             PIMC++/src/Observables/ObservableEnergy.cc:31-60)
  * Needs the fits' beta-derivative tables (and the long-range ones when long range is used). */
@@ -292,6 +306,7 @@ typedef struct pimc_energy {
   double du_short;
   double du_long;
   double v_short;
+  double v_long;
   double total;
 } pimc_energy;
 

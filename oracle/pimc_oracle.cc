@@ -422,6 +422,8 @@ struct CoulombBCFit : PairFit {
   Grid qgrid, tgrid;
   std::vector<BicubicSpline> Usplines, dUsplines;
   double SmallestBeta = 0, Z1Z2 = 0;
+  // PairActionFitClass::V (PAFitBase.h:82): the Coulomb potential
+  double V(double r) const override { return Z1Z2 / r; }
   // PAcoulombBCFitClass::dU (PAcoulombBCFit.cc:178-192): beyond the grid the Coulomb potential
   double dU(double q, double z, double s2, int level) const override {
     (void)z;
@@ -608,6 +610,15 @@ struct Replica {
   std::vector<LongRangeTables> lr;  // [ns*ns]
   std::vector<QuinticTable> ulong;  // [ns*ns] PairActionFitClass::Ulong of the pair
   std::vector<QuinticTable> dulong; // [ns*ns] dUlong
+  // [ns*ns] PairActionFitClass Vlong (a one-level QuinticTable), Vlong_k, Vlong_r0,
+  // Vshort_k0, Vlong_k0 (PAFitBase.h:39-53)
+  struct VlongTables {
+    bool set = false;
+    QuinticTable spline;
+    std::vector<double> vk;
+    double r0 = 0, short_k0 = 0, long_k0 = 0;
+  };
+  std::vector<VlongTables> vlong;
   uint32_t stream = 0;
   int join = 0;
 
@@ -1211,6 +1222,28 @@ int pimc_replica_set_ulong(pimc_ctx* ctx, int32_t replica, int32_t sa, int32_t s
   return PIMC_OK;
 }
 
+int pimc_replica_set_vlong(pimc_ctx* ctx, int32_t replica, int32_t sa, int32_t sb, const double* vlong_k,
+                           double vlong_r0, double vshort_k0, double vlong_k0, double grid_start, double grid_end,
+                           int32_t num_points, const double* coefs) {
+  Replica* R = get_rep(&ctx->c, replica);
+  if (!R || sa < 0 || sb < 0 || sa >= R->ns || sb >= R->ns || num_points < 2 || !coefs || !(grid_start < grid_end))
+    return fail(PIMC_ERR_ARG, "replica_set_vlong: bad argument");
+  if (R->nk > 0 && !vlong_k) return fail(PIMC_ERR_ARG, "replica_set_vlong: vlong_k is NULL");
+  Replica::VlongTables t;
+  t.set = true;
+  t.spline.set = true;
+  t.spline.grid.init_linear(grid_start, grid_end, num_points);
+  t.spline.levels = 1;
+  t.spline.c.assign(coefs, coefs + (size_t)num_points * 6);
+  if (R->nk > 0) t.vk.assign(vlong_k, vlong_k + R->nk);
+  t.r0 = vlong_r0;
+  t.short_k0 = vshort_k0;
+  t.long_k0 = vlong_k0;
+  R->vlong[sa * R->ns + sb] = t;
+  R->vlong[sb * R->ns + sa] = t;
+  return PIMC_OK;
+}
+
 int pimc_fit_eval_u(pimc_ctx* ctx, int32_t fit_id, int32_t level, int64_t n, const double* q,
                     const double* z, const double* s2, int32_t precision, double* out) {
   (void)precision;
@@ -1268,6 +1301,7 @@ int pimc_replica_create(pimc_ctx* ctx, const pimc_replica_desc* d, int32_t* repl
   R->lr.resize(R->ns * R->ns);
   R->ulong.resize(R->ns * R->ns);
   R->dulong.resize(R->ns * R->ns);
+  R->vlong.resize(R->ns * R->ns);
   ctx->c.reps.push_back(std::move(R));
   *replica_id = (int32_t)ctx->c.reps.size() - 1;
   return PIMC_OK;
@@ -1637,10 +1671,48 @@ static double short_range_v(Ctx* c, Replica& R, int mode, int slice) {
         disp[d] -= n * R.isperiodic[d] * R.box[d];
       }
       double dist = sqrt(disp[0] * disp[0] + disp[1] * disp[1] + disp[2] * disp[2]);
-      val += c->fits[R.pair_fit[species1 * R.ns + R.species_of[ptcl2]]]->V(dist);
+      const int f = species1 * R.ns + R.species_of[ptcl2];
+      const PairFit& pa = *c->fits[R.pair_fit[f]];
+      val += pa.V(dist);
+      if (pa.IsLongRange() && R.use_long_range) val -= R.vlong[f].spline.eval(0, dist);
     }
   }
   return val;
+}
+
+// LongRangePotClass::V (LongRangePotClass.cc:32-85) with the stored rho_k of the slice.
+static double long_range_v(Replica& R, int mode, int slice) {
+  double homo = 0.0, hetero = 0.0, background = 0.0, k0Homo = 0.0, k0Hetero = 0.0;
+  for (int species = 0; species < R.ns; species++) {
+    const Replica::VlongTables& T = R.vlong[species * R.ns + species];
+    if (!T.set) continue;   // no long-range potential for this pair (tables zero in the reference)
+    for (int ki = 0; ki < R.nk; ki++) {
+      const cplx z = R.R(mode, slice, species, ki);
+      homo += 0.5 * 2.0 * (z.real() * z.real() + z.imag() * z.imag()) * T.vk[ki];
+    }
+    const int N = R.count[species];
+    homo -= 0.5 * N * T.r0;
+    background -= 0.5 * N * N * T.short_k0;
+    k0Homo += 0.5 * N * N * T.long_k0;
+  }
+  for (int species1 = 0; species1 < R.ns; species1++)
+    for (int species2 = species1 + 1; species2 < R.ns; species2++) {
+      const Replica::VlongTables& T = R.vlong[species1 * R.ns + species2];
+      if (!T.set) continue;
+      for (int ki = 0; ki < R.nk; ki++) {
+        const cplx a = R.R(mode, slice, species1, ki), b = R.R(mode, slice, species2, ki);
+        hetero += 2.0 * (a.real() * b.real() + a.imag() * b.imag()) * T.vk[ki];
+      }
+      const int N1 = R.count[species1], N2 = R.count[species2];
+      background -= N1 * N2 * T.short_k0;
+      k0Hetero += N1 * N2 * T.long_k0;
+    }
+  double Vlong = homo + hetero;
+  if (R.use_background)
+    Vlong += background;
+  else
+    Vlong += (k0Homo + k0Hetero);
+  return Vlong;
 }
 
 int pimc_estimator_energy(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, int32_t mode, pimc_energy* out) {
@@ -1665,12 +1737,21 @@ int pimc_estimator_energy(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, 
     e.kinetic = kinetic_dbeta(R, mode, 0, M, 0);
     e.du_short = short_range_dbeta(&ctx->c, R, mode, 0, M, 0);
     e.du_long = (R.long_range && R.use_long_range) ? long_range_dbeta(R, mode, 0, M, 0) : 0.0;
-    double v = 0.0;
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    bool v_known = true, vlong_known = true;
+    for (int f = 0; f < R.ns * R.ns; f++) {
+      const PairFit& pa = *ctx->c.fits[R.pair_fit[f]];
+      if (pa.IsLongRange() && R.use_long_range && !R.vlong[f].set) v_known = false;
+      if (R.long_range && R.use_long_range && R.lr[f].set && !R.vlong[f].set) vlong_known = false;
+    }
+    double v = 0.0, vl = 0.0;
     for (int slice = 0; slice <= M; slice++) {
       double factor = ((slice == 0) || (slice == M)) ? 0.5 : 1.0;
-      v += factor * short_range_v(&ctx->c, R, mode, slice);
+      if (v_known) v += factor * short_range_v(&ctx->c, R, mode, slice);
+      if (R.long_range && R.use_long_range && vlong_known) vl += factor * long_range_v(R, mode, slice);
     }
-    e.v_short = v;
+    e.v_short = v_known ? v : nan;
+    e.v_long = vlong_known ? vl : nan;
     e.total = e.kinetic + e.du_short + e.du_long;
     out[r] = e;
   }

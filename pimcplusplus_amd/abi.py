@@ -136,7 +136,7 @@ class BisectionPlan(C.Structure):
 
 class Energy(C.Structure):
     _fields_ = [("kinetic", C.c_double), ("du_short", C.c_double), ("du_long", C.c_double),
-                ("v_short", C.c_double), ("total", C.c_double)]
+                ("v_short", C.c_double), ("v_long", C.c_double), ("total", C.c_double)]
 
 
 # name -> (restype, argtypes); every pimc_* entry point of include/pimc_gpu.h
@@ -154,6 +154,9 @@ SIGNATURES = {
     "pimc_replica_set_ulong": (
         C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32, _f64p,
                   _f64p]),
+    "pimc_replica_set_vlong": (
+        C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32, _f64p, C.c_double, C.c_double, C.c_double, C.c_double,
+                  C.c_double, C.c_int32, _f64p]),
     "pimc_replica_num_kvecs": (C.c_int, [_CTX, C.c_int32, _i32p]),
     "pimc_replica_kvecs": (C.c_int, [_CTX, C.c_int32, _f64p, _i32p]),
     "pimc_replica_set_long_range": (
@@ -325,6 +328,18 @@ class PimcLib:
         self._check("pimc_replica_set_ulong", self.lib.pimc_replica_set_ulong(
             self.ctx, rep, sa, sb, c.shape[0], grid_start, grid_end, c.shape[1], _p(c, _f64p),
             _p(dc, _f64p) if dc is not None else None))
+
+    def set_vlong(self, rep: int, sa: int, sb: int, vlong_k, vlong_r0: float, vshort_k0: float, vlong_k0: float,
+                  grid_start: float, grid_end: float, coefs: np.ndarray):
+        """The pair's long-range potential: Vlong_k [nk], the three constants, and Vlong(r) as
+        quintic data coefs [num_points][6] on LinearGrid(grid_start, grid_end, num_points)."""
+        vk = _f64(vlong_k)
+        c = _f64(coefs)
+        if c.ndim != 2 or c.shape[1] != 6:
+            raise ValueError("coefs must be [num_points][6]")
+        self._check("pimc_replica_set_vlong", self.lib.pimc_replica_set_vlong(
+            self.ctx, rep, sa, sb, _p(vk, _f64p), vlong_r0, vshort_k0, vlong_k0, grid_start, grid_end, c.shape[0],
+            _p(c, _f64p)))
 
     def fit_eval_u(self, fit: int, level: int, q, z, s2, precision: int = FP64) -> np.ndarray:
         q, z, s2 = _f64(q), _f64(z), _f64(s2)

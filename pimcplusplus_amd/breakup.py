@@ -19,7 +19,10 @@ This module computes the same quantities with its own numerics:
 
 The tables it returns are what the library's pimc_replica_set_long_range and
 pimc_replica_set_ulong take: Ulong_k, Ulong_r0, Ushort_k0, Ulong_k0 per level and
-the quintic-spline data of Ulong(r) on LinearGrid(0, 0.75 |box|, 1000).
+the quintic-spline data of Ulong(r) on LinearGrid(0, 0.75 |box|, 1000).  The same
+routine with one "level", U = V = Z1Z2/r and the tail fixed to Z1Z2/r gives the
+potential's tables for pimc_replica_set_vlong (LongRangeClass::OptimizedBreakup_V,
+LongRangeClass.cc:979-1132).
 """
 
 from __future__ import annotations
@@ -171,12 +174,16 @@ class BreakupTables:
 
 
 def optimized_breakup(udiag_levels, box, kc: float, kvecs: np.ndarray, num_knots: int = 20,
-                      tail_end: float | None = None) -> BreakupTables:
+                      tail_end: float | None = None, tail_coef=None, short_of=None) -> BreakupTables:
     """The long-range tables of one pair from its diagonal action U(r, level).
 
     udiag_levels: callables U_l(r) (vectorised), one per tau level; each must extend past
     r_c = L/2 (a Coulomb tail).  tail_end: the end of the 1/r-polynomial fit (the pair
-    table's q_end; default 2 r_c).  kvecs: the simulation's k vectors [nk][3]."""
+    table's q_end; default 2 r_c).  kvecs: the simulation's k vectors [nk][3].
+    tail_coef: fixed (c1, c2, c3) instead of the fit (the potential's breakup uses
+    (Z1Z2, 0, 0), PAcoulombBCFit.cc:511-520, 561-566).  short_of: the function whose
+    difference with the long part is integrated for the short part's k = 0 transform, if
+    not U itself (the dU breakup integrates V - dUlong, LongRangeClass.cc:181-186)."""
     box = np.asarray(box, dtype=float)
     omega = float(np.prod(box))
     rc = 0.5 * float(box.min())
@@ -194,7 +201,9 @@ def optimized_breakup(udiag_levels, box, kc: float, kvecs: np.ndarray, num_knots
     out = BreakupTables(np.zeros((nl, ksim.size)), np.zeros(nl), np.zeros(nl), np.zeros(nl), rg,
                         np.zeros((nl, rg.size)), np.zeros(nl), np.zeros((nl, basis.size)), basis)
     for lev, U in enumerate(udiag_levels):
-        coef = tail_fit(U, rc, tail_end if tail_end else 2.0 * rc)
+        coef = np.asarray(tail_coef, dtype=float) if tail_coef is not None else \
+            tail_fit(U, rc, tail_end if tail_end else 2.0 * rc)
+        S = short_of if short_of is not None else U
         t, chi2 = fit_breakup(cnk, wk, xk_tail(kk, rc, coef) / omega)
         out.chi2[lev] = chi2
         out.t[lev] = t
@@ -206,7 +215,7 @@ def optimized_breakup(udiag_levels, box, kc: float, kvecs: np.ndarray, num_knots
         # (LongRangeClass.cc:622-635): Gauss-Legendre on every knot interval and on 64
         # equal pieces of [rc, rmax]
         out.ushort_k0[lev] = 4.0 * math.pi / omega * _integrate(
-            lambda r: r * r * (U(r) - t @ np.stack([basis.h(n, r) for n in range(basis.size)])),
+            lambda r: r * r * (S(r) - t @ np.stack([basis.h(n, r) for n in range(basis.size)])),
             basis.delta * np.arange(num_knots))
         w_in = _integrate(lambda r: r * r * (t @ np.stack([basis.h(n, r) for n in range(basis.size)])),
                           basis.delta * np.arange(num_knots))

@@ -192,6 +192,7 @@ struct HostReplica {
   DevBuf path0, path1, perm0, perm1;
   DevBuf rho0, rho1, kidx_dev, ulong_k, lr_const, ulong;
   DevBuf dulong, dulong_k, dlr_const;   // beta derivatives (energy estimator)
+  DevBuf vlong, vlong_k, vlr_const;     // long-range potential (potential-energy estimator)
   unsigned dulong_mask = 0, dlr_mask = 0;
   std::vector<double> kvecs;     // [nk][3]
   std::vector<int> kidx;         // [nk][3] index + maxk
@@ -646,13 +647,13 @@ int pimc_replica_set_ulong(pimc_ctx* ctx, int32_t replica, int32_t sa, int32_t s
   RepCore& R = H->dev.core;
   if (sa < 0 || sb < 0 || sa >= R.ns || sb >= R.ns) return fail(PIMC_ERR_ARG, "replica_set_ulong: bad species");
   const double delta = (grid_end - grid_start) / (double)(num_points - 1);
-  if (R.ulong_n != 0 && (R.ulong_n != num_points || R.ulong_levels != num_levels || R.ulong_start != grid_start ||
-                         R.ulong_delta != delta))
+  if ((R.ulong_n != 0 && (R.ulong_n != num_points || R.ulong_start != grid_start || R.ulong_delta != delta)) ||
+      (R.ulong && R.ulong_levels != num_levels))
     return fail(PIMC_ERR_ARG, "replica_set_ulong: every pair of a replica needs the same grid and levels");
   HIP_TRY(hipSetDevice(ctx->device));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   const size_t per_pair = (size_t)num_levels * num_points * 6;
-  if (R.ulong_n == 0) {
+  if (!R.ulong) {
     int rc = H->ulong.ensure((size_t)R.ns * R.ns * per_pair * sizeof(double));
     if (rc) return rc;
     HIP_TRY(hipMemset(H->ulong.p, 0, (size_t)R.ns * R.ns * per_pair * sizeof(double)));
@@ -679,6 +680,53 @@ int pimc_replica_set_ulong(pimc_ctx* ctx, int32_t replica, int32_t sa, int32_t s
                         hipMemcpyHostToDevice));
       H->dulong_mask |= 1u << pr;
     }
+  }
+  ctx->reps_dirty = true;
+  return PIMC_OK;
+}
+
+int pimc_replica_set_vlong(pimc_ctx* ctx, int32_t replica, int32_t sa, int32_t sb, const double* vlong_k,
+                           double vlong_r0, double vshort_k0, double vlong_k0, double grid_start, double grid_end,
+                           int32_t num_points, const double* coefs) {
+  HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
+  if (!H || !coefs || num_points < 2 || !(grid_start < grid_end)) return fail(PIMC_ERR_ARG, "replica_set_vlong: bad argument");
+  RepCore& R = H->dev.core;
+  if (sa < 0 || sb < 0 || sa >= R.ns || sb >= R.ns) return fail(PIMC_ERR_ARG, "replica_set_vlong: bad species");
+  if (R.nk > 0 && !vlong_k) return fail(PIMC_ERR_ARG, "replica_set_vlong: vlong_k is NULL");
+  const double delta = (grid_end - grid_start) / (double)(num_points - 1);
+  if (R.ulong_n != 0 && (R.ulong_n != num_points || R.ulong_start != grid_start || R.ulong_delta != delta))
+    return fail(PIMC_ERR_ARG, "replica_set_vlong: the grid must match the replica's Ulong grid");
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  const int npair = R.ns * R.ns;
+  const size_t per_pair = (size_t)num_points * 6;
+  int rc;
+  if (!R.vlong) {
+    if ((rc = H->vlong.ensure((size_t)npair * per_pair * sizeof(double))) ||
+        (rc = H->vlong_k.ensure((size_t)npair * std::max(1, R.nk) * sizeof(double))) ||
+        (rc = H->vlr_const.ensure((size_t)npair * 3 * sizeof(double))))
+      return rc;
+    HIP_TRY(hipMemset(H->vlong.p, 0, (size_t)npair * per_pair * sizeof(double)));
+    HIP_TRY(hipMemset(H->vlong_k.p, 0, (size_t)npair * std::max(1, R.nk) * sizeof(double)));
+    HIP_TRY(hipMemset(H->vlr_const.p, 0, (size_t)npair * 3 * sizeof(double)));
+    R.vlong = H->vlong.as<double>();
+    R.vlong_k = H->vlong_k.as<double>();
+    R.vlr_const = H->vlr_const.as<double>();
+  }
+  if (R.ulong_n == 0) {
+    R.ulong_n = num_points;
+    R.ulong_start = grid_start;
+    R.ulong_delta = delta;
+    R.ulong_deltainv = 1.0 / delta;
+  }
+  const double cst[3] = {vlong_r0, vshort_k0, vlong_k0};
+  for (int pr : {sa * R.ns + sb, sb * R.ns + sa}) {
+    HIP_TRY(hipMemcpy(H->vlong.as<double>() + pr * per_pair, coefs, per_pair * sizeof(double), hipMemcpyHostToDevice));
+    if (R.nk > 0)
+      HIP_TRY(hipMemcpy(H->vlong_k.as<double>() + (size_t)pr * R.nk, vlong_k, (size_t)R.nk * sizeof(double),
+                        hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(H->vlr_const.as<double>() + (size_t)pr * 3, cst, sizeof cst, hipMemcpyHostToDevice));
+    R.vlong_mask |= 1u << pr;
   }
   ctx->reps_dirty = true;
   return PIMC_OK;
@@ -1633,14 +1681,15 @@ int pimc_estimator_energy(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, 
     HostReplica* H = ctx->rep(replicas[r]);
     const RepCore& R = H->dev.core;
     int nt = 0;
-    bool all_david = true;
+    bool v_known = true;   // every long-ranged pair has its Vlong tables (when long range is used)
     for (int f = 0; f < R.ns * R.ns; f++) {
       const HostFit& F = *ctx->fits[H->dev.sp.fit[f]];
       if (F.dev.kind == 0)
         nt = std::max(nt, F.num_terms);
-      else
-        all_david = false;
+      else if (R.use_long_range && !((R.vlong_mask >> f) & 1u))
+        v_known = false;
     }
+    const bool vlong_known = !R.lr_on || !(R.lr_mask & ~R.vlong_mask);
     for (int f = 0; f < R.ns * R.ns; f++) {
       const HostFit& F = *ctx->fits[H->dev.sp.fit[f]];
       if (F.dev.kind == 0 && F.num_terms != nt)
@@ -1651,7 +1700,7 @@ int pimc_estimator_energy(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, 
     const int pair_blocks = (int)std::max(1LL, std::min(2048LL, (npairs + kBlock - 1) / kBlock));
     const int kin_blocks = (int)std::max(1LL, std::min(1024LL, ((long long)R.N * M + kBlock - 1) / kBlock));
     const int lr_slices = R.lr_on ? R.nslices : 0;
-    const size_t n_pp = 2 * (size_t)pair_blocks, n_kp = kin_blocks, n_lp = 3 * (size_t)std::max(1, lr_slices);
+    const size_t n_pp = 2 * (size_t)pair_blocks, n_kp = kin_blocks, n_lp = 6 * (size_t)std::max(1, lr_slices);
     if ((rc = ctx->d_out.ensure((n_pp + n_kp + n_lp) * sizeof(double)))) return rc;
     double* dp = ctx->d_out.as<double>();
     hipError_t e = launch_energy(nt, ctx->d_reps.as<ReplicaDev>(), ctx->d_fits.as<FitDev>(), replicas[r], mode,
@@ -1668,15 +1717,22 @@ int pimc_estimator_energy(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, 
     }
     for (int b = 0; b < kin_blocks; b++) en.kinetic += h[n_pp + b];
     if (lr_slices) {
-      double hh = 0.0, bg = 0.0, k0 = 0.0;
+      double hh = 0.0, bg = 0.0, k0 = 0.0, vh = 0.0, vbg = 0.0, vk0 = 0.0;
       for (int sl = 0; sl < lr_slices; sl++) {
-        hh += h[n_pp + n_kp + 3 * sl];
-        bg += h[n_pp + n_kp + 3 * sl + 1];
-        k0 += h[n_pp + n_kp + 3 * sl + 2];
+        const double* q = h.data() + n_pp + n_kp + 6 * sl;
+        hh += q[0];
+        bg += q[1];
+        k0 += q[2];
+        vh += q[3];
+        vbg += q[4];
+        vk0 += q[5];
       }
       en.du_long = hh + (R.use_background ? bg : k0);
+      en.v_long = vh + (R.use_background ? vbg : vk0);
     }
-    en.v_short = all_david ? v : std::numeric_limits<double>::quiet_NaN();
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    en.v_short = v_known ? v : nan;
+    if (!vlong_known) en.v_long = nan;
     en.total = en.kinetic + en.du_short + en.du_long;
     out[r] = en;
   }

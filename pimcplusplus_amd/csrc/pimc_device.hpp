@@ -137,6 +137,12 @@ struct RepCore {
   const double* dulong;    // dUlong, same layout, or nullptr
   const double* dulong_k;  // [ns*ns][lr_levels][nk] dUlong_k, or nullptr
   const double* dlr_const; // [ns*ns][3][lr_levels]: dUlong_r0, dUshort_k0, dUlong_k0
+  // long-range potential (potential-energy estimator), on the Ulong grid: Vlong(r)
+  // [ns*ns][ulong_n][6], Vlong_k [ns*ns][nk], {Vlong_r0, Vshort_k0, Vlong_k0} [ns*ns][3]
+  const double* vlong;
+  const double* vlong_k;
+  const double* vlr_const;
+  unsigned vlong_mask;     // bit sa*ns+sb set where the pair has Vlong tables
   __device__ __host__ inline double* path(int mode) const { return mode ? path1 : path0; }
   __device__ __host__ inline int* perm(int mode) const { return mode ? perm1 : perm0; }
   __device__ __host__ inline d2v* rho(int mode) const { return mode ? rho1 : rho0; }

@@ -134,10 +134,20 @@ __global__ void __launch_bounds__(kBlock) k_energy_pairs(const ReplicaDev* reps,
     const double* dul = (F.kind == 1 && R.use_long_range && R.dulong)
                             ? R.dulong + (size_t)pr * R.ulong_levels * R.ulong_n * 6
                             : nullptr;
+    // ShortRangePotClass::V: V(r) (column 0 of a DavidPA fit, Z1Z2/r of a Coulomb fit), minus
+    // Vlong(r) for a long-ranged fit when long range is used
+    const double* vl = (F.kind == 1 && R.use_long_range && ((R.vlong_mask >> pr) & 1u))
+                           ? R.vlong + (size_t)pr * R.ulong_n * 6
+                           : nullptr;
+    auto vpair = [&](double r) {
+      double val = F.kind == 0 ? david_v(F, r) : F.z1z2 / r;
+      if (vl) val -= ulong_eval(R, vl, r);
+      return val;
+    };
     double a1[3], a2[3];
     load_bead(R, mode, 0, p1, a1);
     load_bead(R, mode, 0, p2, a2);
-    if (F.kind == 0) v += 0.5 * david_v(F, slice_dist(R, a1, a2));
+    v += 0.5 * vpair(slice_dist(R, a1, a2));
     for (int slice = 0; slice < M; slice++) {
       double b1[3], b2[3];
       load_bead(R, mode, slice + 1, p1, b1);
@@ -155,7 +165,7 @@ __global__ void __launch_bounds__(kBlock) k_energy_pairs(const ReplicaDev* reps,
       const double q = 0.5 * (rmag + rpmag), z = rmag - rpmag;
       du += F.kind == 1 ? bc_du(F, 0, q, s2v) : david_du<NT>(F, 0, q, z, s2v);
       if (dul) du -= 0.5 * (ulong_eval(R, dul, rmag) + ulong_eval(R, dul, rpmag));
-      if (F.kind == 0) v += (slice + 1 == M ? 0.5 : 1.0) * david_v(F, slice_dist(R, b1, b2));
+      v += (slice + 1 == M ? 0.5 : 1.0) * vpair(slice_dist(R, b1, b2));
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         a1[d] = b1[d];
@@ -225,7 +235,8 @@ __global__ void __launch_bounds__(kBlock) k_energy_kinetic(const ReplicaDev* rep
 }
 
 // LongRangeClass::d_dBeta: one block per slice of [0, M], rho_k of copy `mode`.
-// partials[slice] = {homo + hetero, background, k0} with the slice's factor.
+// partials[slice] = {homo + hetero, background, k0} with the slice's factor, then the same
+// three sums of the long-range potential.
 __global__ void __launch_bounds__(kBlock) k_energy_long_range(const ReplicaDev* reps, int rep, int mode,
                                                               double* partials) {
   __shared__ double scratch_raw[kWaves];
@@ -238,23 +249,26 @@ __global__ void __launch_bounds__(kBlock) k_energy_long_range(const ReplicaDev* 
   const double factor = (slice == 0 || slice == M) ? 0.5 : 1.0;
   const d2v* rho = R.rho(mode) + (size_t)slice * R.ns * R.nk;
   const int L = R.lr_levels;
-  double part = 0.0;
+  double part = 0.0, vpart = 0.0;   // d_dBeta and LongRangePotClass::V (LongRangePotClass.cc:32-85)
   for (int k = threadIdx.x; k < R.nk; k += blockDim.x) {
     for (int s = 0; s < R.ns; s++) {
       const int pr = s * R.ns + s;
-      if (!((R.lr_mask >> pr) & 1u)) continue;
       const d2v a = rho[(size_t)s * R.nk + k];
-      part += factor * 0.5 * 2.0 * (a.x * a.x + a.y * a.y) * R.dulong_k[((size_t)pr * L + 0) * R.nk + k];
+      if ((R.lr_mask >> pr) & 1u)
+        part += factor * 0.5 * 2.0 * (a.x * a.x + a.y * a.y) * R.dulong_k[((size_t)pr * L + 0) * R.nk + k];
+      if ((R.vlong_mask >> pr) & 1u) vpart += factor * 0.5 * 2.0 * (a.x * a.x + a.y * a.y) * R.vlong_k[(size_t)pr * R.nk + k];
     }
     for (int s1 = 0; s1 < R.ns; s1++)
       for (int s2 = s1 + 1; s2 < R.ns; s2++) {
         const int pr = s1 * R.ns + s2;
-        if (!((R.lr_mask >> pr) & 1u)) continue;
         const d2v a = rho[(size_t)s1 * R.nk + k], b = rho[(size_t)s2 * R.nk + k];
-        part += factor * 2.0 * (a.x * b.x + a.y * b.y) * R.dulong_k[((size_t)pr * L + 0) * R.nk + k];
+        if ((R.lr_mask >> pr) & 1u)
+          part += factor * 2.0 * (a.x * b.x + a.y * b.y) * R.dulong_k[((size_t)pr * L + 0) * R.nk + k];
+        if ((R.vlong_mask >> pr) & 1u) vpart += factor * 2.0 * (a.x * b.x + a.y * b.y) * R.vlong_k[(size_t)pr * R.nk + k];
       }
   }
   const double t = block_sum(part, scratch);
+  const double tv = block_sum(vpart, scratch);
   if (threadIdx.x == 0) {
     double homo = 0.0, background = 0.0, k0 = 0.0;
     for (int s = 0; s < R.ns; s++) {
@@ -273,9 +287,30 @@ __global__ void __launch_bounds__(kBlock) k_energy_long_range(const ReplicaDev* 
         background -= factor * N1 * N2 * R.dlr_const[(size_t)(pr * 3 + 1) * L + 0];
         k0 += factor * N1 * N2 * R.dlr_const[(size_t)(pr * 3 + 2) * L + 0];
       }
-    partials[3 * slice] = t + homo;
-    partials[3 * slice + 1] = background;
-    partials[3 * slice + 2] = k0;
+    partials[6 * slice] = t + homo;
+    partials[6 * slice + 1] = background;
+    partials[6 * slice + 2] = k0;
+    // the potential's Madelung, background and k = 0 terms
+    double vhomo = 0.0, vbackground = 0.0, vk0 = 0.0;
+    for (int s = 0; s < R.ns; s++) {
+      const int pr = s * R.ns + s;
+      if (!((R.vlong_mask >> pr) & 1u)) continue;
+      const int N = S.count[s];
+      vhomo -= factor * 0.5 * N * R.vlr_const[(size_t)pr * 3 + 0];
+      vbackground -= factor * 0.5 * N * N * R.vlr_const[(size_t)pr * 3 + 1];
+      vk0 += factor * 0.5 * N * N * R.vlr_const[(size_t)pr * 3 + 2];
+    }
+    for (int s1 = 0; s1 < R.ns; s1++)
+      for (int s2 = s1 + 1; s2 < R.ns; s2++) {
+        const int pr = s1 * R.ns + s2;
+        if (!((R.vlong_mask >> pr) & 1u)) continue;
+        const int N1 = S.count[s1], N2 = S.count[s2];
+        vbackground -= factor * N1 * N2 * R.vlr_const[(size_t)pr * 3 + 1];
+        vk0 += factor * N1 * N2 * R.vlr_const[(size_t)pr * 3 + 2];
+      }
+    partials[6 * slice + 3] = tv + vhomo;
+    partials[6 * slice + 4] = vbackground;
+    partials[6 * slice + 5] = vk0;
   }
 }
 

@@ -31,10 +31,11 @@ def check_energy(oracle, gpu, system, moves=30):
         for a, b in zip(eg, eo):
             for f in ("kinetic", "du_short", "du_long", "total"):
                 assert close(getattr(a, f), getattr(b, f)), (f, getattr(a, f), getattr(b, f))
-            if math.isnan(b.v_short):
-                assert math.isnan(a.v_short)
-            else:
-                assert close(a.v_short, b.v_short)
+            for f in ("v_short", "v_long"):
+                if math.isnan(getattr(b, f)):
+                    assert math.isnan(getattr(a, f)), f
+                else:
+                    assert close(getattr(a, f), getattr(b, f)), (f, getattr(a, f), getattr(b, f))
     return eo
 
 
@@ -50,11 +51,19 @@ def test_energy_charged_lj_long_range(oracle, gpu):
 def test_energy_hydrogen(oracle, gpu):
     t = tables.load_npz(f"{GOLDEN}/testcoul_fit2.npz")
     e = check_energy(oracle, gpu, workloads.hydrogen_system(t, num_replicas=2, n_pairs=8, m=32, num_levels=3))
-    assert math.isnan(e[0].v_short) and e[0].du_long != 0.0
+    assert math.isfinite(e[0].v_short) and e[0].v_long != 0.0 and e[0].du_long != 0.0
+
+
+def test_energy_hydrogen_ewald_leaves_the_potential_unknown(oracle, gpu):
+    t = tables.load_npz(f"{GOLDEN}/testcoul_fit2.npz")
+    e = check_energy(oracle, gpu, workloads.hydrogen_system(t, num_replicas=1, n_pairs=4, m=16, num_levels=2,
+                                                            long_range="ewald"), moves=10)
+    assert math.isnan(e[0].v_short) and math.isnan(e[0].v_long)
 
 
 def test_energy_ocp_background(oracle, gpu):
-    check_energy(oracle, gpu, workloads.ocp_system(num_replicas=2, n=64, m=16, num_levels=3, kc_units=4.0))
+    e = check_energy(oracle, gpu, workloads.ocp_system(num_replicas=2, n=64, m=16, num_levels=3, kc_units=4.0))
+    assert math.isfinite(e[0].v_short) and e[0].v_long != 0.0
 
 
 def test_pair_correlation_matches_oracle(oracle, gpu):
