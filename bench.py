@@ -33,6 +33,8 @@ from pimcplusplus_amd import abi, native, workloads  # noqa: E402
 METRIC = "pair·slice action evals/s, mixed vs FP64 on GPU at ΔS err ≤1e-6"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+LDS_PEAK_GBS = 256 * 128 * 2.4      # 256 CUs x 128 B/clk x 2.4 GHz
+CALIB_LIB = os.path.join(ROOT, "pimcplusplus_amd", "_lib", "libpimc_calib.so")
 FLOP_PER_EVAL = 180.0      # FP64 flops of one pair·slice evaluation (DistDisp, q/z/s², splines, sum)
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_bisection_lj_n256_m64_l4_traffic.md")
 
@@ -62,16 +64,63 @@ def bytes_per_launch(st, n: int) -> float:
     return 24.0 * (n + 1) * (sum_k + st.stages) + 16.0 * st.stages
 
 
+def measured_peaks():
+    """HBM copy and LDS read bandwidth measured on this GPU by the microbenchmarks of
+    libpimc_calib.so (GB/s), or None where the library is missing or a call fails."""
+    if not os.path.exists(CALIB_LIB):
+        return None
+    lib = ctypes.CDLL(CALIB_LIB)
+    lib.pimc_calib_hbm.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    lib.pimc_calib_lds.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
+    hbm, lds, cus = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+    out = {}
+    if lib.pimc_calib_hbm(2 << 30, 10, ctypes.byref(hbm)) == 0:
+        out["hbm_copy_GBs"] = hbm.value / 1e9
+    if lib.pimc_calib_lds(8192, ctypes.byref(lds), ctypes.byref(cus)) == 0:
+        out["lds_read_GBs"] = lds.value / 1e9
+        out["cus"] = cus.value
+    out["note"] = ("HBM: grid-stride copy of 2 GiB (read + write bytes); LDS: 16-byte lane-consecutive "
+                   "reads, eight 256-thread blocks per CU")
+    return out
+
+
+def offdiag_fraction(paths, box, end: float) -> float:
+    """Fraction of pair.slice distances below the table end, over sampled paths: the share of
+    evaluations that look up the off-diagonal columns (q < End)."""
+    L = np.asarray(box, dtype=float)
+    inside = total = 0
+    for p in paths:
+        for s in range(0, p.shape[0], 4):
+            d = p[s][:, None, :] - p[s][None, :, :]
+            d -= L * np.rint(d / L)
+            r = np.sqrt(np.sum(d * d, axis=-1))
+            iu = np.triu_indices(p.shape[1], 1)
+            inside += int(np.count_nonzero(r[iu] < end))
+            total += iu[0].size
+    return inside / max(1, total)
+
+
+def lds_bytes_per_launch(st, steps: int, mixed: bool, f_off: float, nint: int, ncol: int, noff: int) -> float:
+    """LDS bytes of the launches summarised by `st` (per launch): per evaluation the moved
+    bead (24 B, one broadcast row), the u00 row of the new slice and, with probability
+    f_off, the off-diagonal rows (16 B per row in the mixed path, 32 B in FP64); per
+    executed stage the staged level table written into LDS."""
+    row = 16.0 if mixed else 32.0
+    per_eval = 24.0 + row * (1.0 + noff * f_off)
+    return (st.pair_evals * per_eval + st.stages * nint * ncol * row) / steps
+
+
 def load_system(lib, system):
     ids = workloads.load(lib, system)
     return ids
 
 
-def run_gpu(system, precision, steps, warmup, moves_per_step, tune=False, graph=False):
+def run_gpu(system, precision, steps, warmup, moves_per_step, tune=False, graph=False, paths_out=None):
     """W untimed steps, then K steps queued back to back on the context's stream;
     the timed region is bracketed by stream synchronisations (the stats take).
     tune: the autotuner picks the launch plan first (on moves the chain then
-    repeats); graph: each step is a replay of one captured sweep."""
+    repeats); graph: each step is a replay of one captured sweep.  paths_out: a list
+    that receives the paths of the first eight replicas after the run."""
     lib = native.open_context(0)
     ids = load_system(lib, system)
     L, seed = system.num_levels, system.seed
@@ -95,6 +144,8 @@ def run_gpu(system, precision, steps, warmup, moves_per_step, tune=False, graph=
     wall = time.perf_counter() - t0
     if graph:
         lib.sweep_graph_destroy(g)
+    if paths_out is not None:
+        paths_out.extend(lib.path_get(i) for i in ids[:8])
     lib.close()
     return tot, wall, tot.device_ms, {"block_threads": plan.block_threads, "link_split": plan.link_split,
                                       "persistent": bool(plan.persistent), "cluster": plan.cluster}
@@ -257,8 +308,13 @@ def main():
     runs = (("mixed", abi.MIXED), ("fp64", abi.FP64))
     if args.only:
         runs = tuple(r for r in runs if r[0] == args.only)
+    table = system.tables["lj"]
+    end, nint, noff = float(table.grid_end), table.y.shape[1] - 1, table.num_terms * (table.num_terms + 3) // 2
+    paths = []
     for name, prec in runs:
-        tot, wall, dev_ms, _ = run_gpu(system, prec, args.steps, args.warmup, args.moves_per_step)
+        tot, wall, dev_ms, _ = run_gpu(system, prec, args.steps, args.warmup, args.moves_per_step,
+                                       paths_out=paths if not paths else None)
+        f_off = offdiag_fraction(paths, system.replicas[0].box, end)
         res[name] = {
             "evals_per_s": tot.pair_evals / wall,
             "ms_per_step": 1e3 * wall / args.steps,
@@ -268,6 +324,9 @@ def main():
             "acceptance": tot.accepted / max(1, tot.moves),
             "stages_per_move": tot.stages / max(1, tot.moves),
             "pair_evals": tot.pair_evals,
+            "offdiag_fraction": f_off,
+            "lds_bytes_per_launch": lds_bytes_per_launch(tot, args.steps, prec == abi.MIXED, f_off, nint,
+                                                         1 + noff, noff),
         }
     if args.only:
         print(json.dumps({"only": args.only, **res[args.only]}))
@@ -275,6 +334,15 @@ def main():
     err, nstages = (None, 0) if args.no_dserr else delta_s_error()
     mixed = res["mixed"]
     achieved = mixed["bytes_per_launch"] / (mixed["kernel_ms_avg"] * 1e-3) / 1e9
+    peaks = measured_peaks()
+
+    lds_peak = (peaks or {}).get("lds_read_GBs") or LDS_PEAK_GBS
+
+    def lds_roof(r):
+        a = r["lds_bytes_per_launch"] / (r["kernel_ms_avg"] * 1e-3) / 1e9
+        return {"bound": "lds", "achieved": a, "peak": lds_peak, "unit": "GB/s", "frac": a / lds_peak,
+                "frac_of_128B_per_clk": a / LDS_PEAK_GBS,
+                "bytes_per_launch": r["lds_bytes_per_launch"], "offdiag_fraction": r["offdiag_fraction"]}
     flops = mixed["device_evals_per_s"] * FLOP_PER_EVAL / 1e12
     line = {
         "metric": METRIC,
@@ -310,6 +378,13 @@ def main():
                           "frac": flops / FP64_PEAK_TFLOPS,
                           "note": f"{FLOP_PER_EVAL:.0f} algorithmic FP64 flop per pair·slice evaluation "
                                   "over the kernel time; peak = MI355X FP64 vector"},
+        "roofline_lds": {**lds_roof(mixed),
+                         "fp64": {k: v for k, v in lds_roof(res["fp64"]).items() if k in ("achieved", "frac")},
+                         "note": "LDS bytes read by the evaluations (moved bead, u00 row, off-diagonal rows "
+                                 "where q < End) plus the staged tables, over the kernel time; peak = the LDS "
+                                 "read bandwidth measured on this GPU (measured_peaks), which exceeds "
+                                 "256 CUs x 128 B/clk x 2.4 GHz"},
+        "measured_peaks": peaks,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

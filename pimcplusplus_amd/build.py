@@ -21,6 +21,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 OBJDIR = os.path.join(LIBDIR, "_build")
 LIB = os.path.join(LIBDIR, "libpimc_gpu.so")
+CALIB_SRC = os.path.join(CSRC, "calib", "pimc_calib.hip")
+CALIB_LIB = os.path.join(LIBDIR, "libpimc_calib.so")
 ARCH = os.environ.get("PIMC_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -59,19 +61,28 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(OBJDIR, exist_ok=True)
-    srcs = sources()
-    with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
-    tmp = LIB + ".tmp"
+def _link(lib: str, objs: list[str]) -> None:
+    tmp = lib + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
+
+
+def build(verbose: bool = False) -> str:
+    """Builds libpimc_gpu.so (the action library) and libpimc_calib.so (the benchmark's
+    HBM and LDS microbenchmarks, csrc/calib/)."""
+    os.makedirs(OBJDIR, exist_ok=True)
+    srcs = sources()
+    with ThreadPoolExecutor(max_workers=min(8, len(srcs) + 1)) as ex:
+        calib = ex.submit(_compile, CALIB_SRC)
+        objs = list(ex.map(_compile, srcs))
+        calib_obj = calib.result()
+    _link(LIB, objs)
+    _link(CALIB_LIB, [calib_obj])
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {CALIB_LIB}")
     return LIB
 
 
