@@ -540,9 +540,10 @@ int pimc_fit_create_david(pimc_ctx* ctx, const pimc_david_fit_desc* d, int32_t* 
   F.c32 = (const f4v*)(base + off32);
   f->num_terms = nt;
   f->staged64 = staged_bytes(nint, ncol, ng, false);
-  f->staged32 = staged_bytes(nint, ncol, ng, true);
+  F.i32_lds = (size_t)F.i32 * ncol * 32 <= (size_t)kFp64StageBytes ? F.i32 : 0;
+  f->staged32 = staged_bytes(nint, ncol, ng, true, false, F.i32_lds);
   f->staged64_lin = staged_bytes(nint, ncol, ng, false, true);
-  f->staged32_lin = staged_bytes(nint, ncol, ng, true, true);
+  f->staged32_lin = staged_bytes(nint, ncol, ng, true, true, F.i32_lds);
   ctx->fits.push_back(std::move(f));
   ctx->fits_dirty = true;
   *fit_id = (int32_t)ctx->fits.size() - 1;

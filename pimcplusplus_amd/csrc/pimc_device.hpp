@@ -95,7 +95,13 @@ struct FitDev {
   // kind 0: the FP32 rows of intervals i >= i32 are valid (not NaN) at every
   // level; the mixed loops evaluate the FP64 rows of intervals below it
   int i32;
+  // intervals whose FP64 rows a mixed stage copies into LDS beside the FP32 rows:
+  // i32 when they take at most kFp64StageBytes, else 0 (read from global memory)
+  int i32_lds;
 };
+
+// LDS a mixed stage may spend on the FP64 rows of the intervals below i32.
+constexpr int kFp64StageBytes = 12288;
 
 // Largest |c0|+|c1|+|c2|+|c3| of a column the mixed path evaluates in FP32.
 constexpr double kMixedLimit = 1.0;
@@ -389,6 +395,10 @@ struct LevelTable32 {
   typename Ptr<InLds>::template c<f4v> c;            // FP32 coefficients
   typename Ptr<InLds>::template c<double> x;
   const d4v* c64;                                     // FP64 coefficients of this level (global memory)
+  // staged tables: the FP64 rows of the first n64s intervals copied into LDS (n64s is
+  // FitDev::i32_lds: every interval the mixed lookups read in FP64, or 0)
+  const PIMC_LDS d4v* c64s;
+  int n64s;
 };
 
 __device__ inline double horner64(const d4v c, double b) { return c.x + b * (c.y + b * (c.z + b * c.w)); }
@@ -455,9 +465,15 @@ __device__ inline void eval_cols_split(const LevelTable32<InLds>& T, int ncol, i
                                        double out[NC]) {
   const int at = i * ncol + c0col;
   if (use64) {
-    const d4v* c64 = T.c64 + (size_t)at;
+    if (InLds && T.n64s > 0) {
+      const PIMC_LDS d4v* c64 = T.c64s + (size_t)at;
 #pragma unroll
-    for (int k = 0; k < NC; k++) out[k] = horner64(c64[k], b);
+      for (int k = 0; k < NC; k++) out[k] = horner64(c64[k], b);
+    } else {
+      const d4v* c64 = T.c64 + (size_t)at;
+#pragma unroll
+      for (int k = 0; k < NC; k++) out[k] = horner64(c64[k], b);
+    }
   } else {
     const float bf = (float)b;
 #pragma unroll

@@ -99,6 +99,8 @@ __device__ inline LevelTable32<false> level_table32(const FitDev& F, int lev) {
   t.c = F.c32 + (size_t)lev * F.nint * F.ncol;
   t.x = F.x;
   t.c64 = F.c64 + (size_t)lev * F.nint * F.ncol;
+  t.c64s = nullptr;
+  t.n64s = 0;
   return t;
 }
 
@@ -132,9 +134,16 @@ struct IsMixed<LevelTable32<InLds>> {
 };
 
 // Bytes of LDS one staged level of fit F takes: geo + coefficients + grid, or the
-// coefficients alone when the lookups use lin_low (lin).
-__host__ __device__ inline size_t staged_bytes(int nint, int ncol, int ng, bool mixed, bool lin = false) {
-  const size_t coef = (size_t)nint * ncol * (mixed ? 16 : 32);
+// coefficients alone when the lookups use lin_low (lin); a mixed stage adds the FP64
+// rows of its first n64s intervals, 32-byte aligned.
+__host__ __device__ inline size_t staged32_fp64_offset(int nint, int ncol, int ng, bool lin) {
+  const size_t b = (size_t)nint * ncol * 16 + (lin ? 0 : (size_t)nint * 16 + (size_t)ng * 8);
+  return (b + 31) & ~(size_t)31;
+}
+__host__ __device__ inline size_t staged_bytes(int nint, int ncol, int ng, bool mixed, bool lin = false,
+                                               int n64s = 0) {
+  if (mixed) return staged32_fp64_offset(nint, ncol, ng, lin) + (size_t)n64s * ncol * 32;
+  const size_t coef = (size_t)nint * ncol * 32;
   const size_t b = lin ? coef : (size_t)nint * 16 + coef + (size_t)ng * 8;
   return (b + 15) & ~(size_t)15;
 }
@@ -179,14 +188,18 @@ __device__ inline LevelTable32<true> stage_level32(const FitDev& F, int lev, PIM
   PIMC_LDS f4v* c = (PIMC_LDS f4v*)(LIN ? base : base + (size_t)F.nint * 16);
   const int nc = F.nint * F.ncol;
   PIMC_LDS double* xs = (PIMC_LDS double*)((PIMC_LDS unsigned char*)c + (size_t)nc * 16);
+  PIMC_LDS d4v* c64s = (PIMC_LDS d4v*)(base + staged32_fp64_offset(F.nint, F.ncol, F.ng, LIN));
   if (!LIN) block_copy(geo, F.geo, F.nint);
   block_copy(c, F.c32 + (size_t)lev * nc, nc);
   if (!LIN) block_copy(xs, F.x, F.ng);
+  block_copy(c64s, F.c64 + (size_t)lev * nc, F.i32_lds * F.ncol);
   LevelTable32<true> t;
   t.geo = geo;
   t.c = c;
   t.x = xs;
   t.c64 = F.c64 + (size_t)lev * nc;
+  t.c64s = c64s;
+  t.n64s = F.i32_lds;
   return t;
 }
 
