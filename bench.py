@@ -352,6 +352,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": mixed["ms_per_step"],
+        "kernel_ms_avg": mixed["kernel_ms_avg"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
