@@ -22,6 +22,23 @@
 //   PIMC++/src/Moves/BisectionStage.cc:128-289, BisectionBlock.cc:66-104,310-451,
 //   PIMC++/src/Moves/NoPermuteClass.cc:26-68, Common/Random/Random.h:45-46,156-180
 //   PIMC++/src/Moves/PermuteTableClass.cc:122-150,172-187   ConstructHTable, CycleClass::Apply
+//   Common/Splines/BicubicSpline.h:24-30,266-300,356-413, BicubicSpline.cc:20-154
+//   Common/PairAction/PAcoulombBCFit.cc:159-192     Coulomb U and dU
+//   Common/Splines/QuinticSpline.h:172-218          Ulong / dUlong / Vlong evaluation
+//   PIMC++/src/Actions/KineticClass.cc:187-256, ShortRangeClass.cc:256-295,
+//   PIMC++/src/Actions/LongRangeClass.cc:374-442, ActionsClass.cc:813-871   energy estimator
+//   PIMC++/src/Actions/ShortRangePotClass.cc:27-47, LongRangePotClass.cc:32-85   potential
+//   PIMC++/src/Observables/ObservableCorrelation.cc:156-221                 pair correlation
+//
+// How each part is pinned (tests/): Philox by the Random123 known-answer vectors; grids,
+// cubic, bicubic and quintic splines by scipy interpolants and brute-force interval
+// searches; DavidPA U and dU by a numpy restatement over scipy splines on the shipped
+// testcoul.dm table; DistDisp/PutInBox by brute-force image search with exact ties;
+// the short-range, kinetic and long-range actions by brute-force sums, closed forms and
+// the rock-salt Madelung constant; the estimators by tau-derivative identities, the free-
+// particle 3N/(2 beta) limit and an independent Ewald sum of the potential; the
+// bookkeeping by byte equality with numpy index operations.  Parity unpinned: the
+// reference's own SPRNG2 stream (not available), so the reference's own accept sequence.
 //
 // Deliberate, documented departures:
 //   * LogGrid::ReverseMap: the reference's floor(log(r/r0)/log(spacing)) is
@@ -1656,7 +1673,8 @@ static double long_range_dbeta(Replica& R, int mode, int slice1, int slice2, int
 }
 
 // ShortRangePotClass::V(slice) (ShortRangePotClass.cc:27-47) with DistDisp(slice, p1, p2)
-// (PathClass.h:638-660); DavidPA fits only.
+// (PathClass.h:638-660): V of the pair's fit (column 0 of a DavidPA table, Z1Z2/r of a
+// Coulomb fit), minus Vlong(r) for a long-ranged fit when long range is used.
 static double short_range_v(Ctx* c, Replica& R, int mode, int slice) {
   double val = 0.0;
   for (int ptcl1 = 0; ptcl1 < R.N; ptcl1++) {
