@@ -211,6 +211,32 @@ def delta_s_error(nrep: int = 16, nmoves: int = 40):
     return err, n
 
 
+def energy_check(nrep: int = 256, blocks: int = 8, moves: int = 40):
+    """Thermodynamic energy per time slice of the benchmark system from the mixed and the FP64
+    chains (same seeds, each replica independent): block averages after two blocks of
+    equilibration, compared through the replicas' spread.  Returns the two means, the
+    standard error of their difference and whether they agree within four of it."""
+    system = workloads.config1(num_replicas=nrep)
+    m = system.replicas[0].num_time_slices
+    res = {}
+    for prec in (abi.FP64, abi.MIXED):
+        lib = native.open_context(0)
+        ids = load_system(lib, system)
+        vals = []
+        for b in range(blocks):
+            lib.bisection_run(ids, system.num_levels, system.seed, moves * b, moves, prec, records=False,
+                              wait=False)
+            vals.append([e.total / m for e in lib.energy(ids)])
+        lib.close()
+        res[prec] = np.array(vals[2:]).mean(axis=0)
+    a, b = res[abi.FP64], res[abi.MIXED]
+    err = float(np.sqrt(a.var(ddof=1) / nrep + b.var(ddof=1) / nrep))
+    d = float(b.mean() - a.mean())
+    return {"fp64_K_per_slice": float(a.mean()), "mixed_K_per_slice": float(b.mean()), "difference": d,
+            "std_error": err, "within_4_sigma": abs(d) < 4.0 * err,
+            "sample": f"{nrep} replicas, {blocks} blocks of {moves} moves, the first two blocks dropped"}
+
+
 def oracle_lib():
     path = os.path.join(ROOT, "oracle", "_build", "liboracle_pimc.so")
     if not os.path.exists(path):
@@ -284,7 +310,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=("mixed", "fp64"), default=None,
                     help="time one precision only (profiling runs); the JSON line then lacks the other")
-    ap.add_argument("--no-dserr", action="store_true", help="skip the mixed-vs-FP64 dS comparison")
+    ap.add_argument("--no-dserr", action="store_true", help="skip the mixed-vs-FP64 dS and energy comparisons")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run an elementwise copy of MIB MiB (read MIB, write MIB) for counter scaling")
     ap.add_argument("--workload", default="lj256", choices=("lj256", "lj64", "uneven", "hydrogen", "ocp"),
@@ -332,6 +358,7 @@ def main():
         print(json.dumps({"only": args.only, **res[args.only]}))
         return
     err, nstages = (None, 0) if args.no_dserr else delta_s_error()
+    energies = None if args.no_dserr else energy_check()
     mixed = res["mixed"]
     achieved = mixed["bytes_per_launch"] / (mixed["kernel_ms_avg"] * 1e-3) / 1e9
     peaks = measured_peaks()
@@ -366,6 +393,7 @@ def main():
         "mixed_over_fp64": mixed["evals_per_s"] / res["fp64"]["evals_per_s"],
         "delta_s_max_err": err,
         "delta_s_stages_compared": nstages,
+        "energy_mixed_vs_fp64": energies,
         "acceptance": mixed["acceptance"],
         "stages_per_move": mixed["stages_per_move"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
