@@ -502,11 +502,12 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
           phase_rows_chunk<LrT>(R, mnew, mold, c0, nc, step, stride, lrC);
           __syncthreads();
           for (int k = kb + tid; k < ke; k += blockDim.x) {
-           const int ki[3] = {R.kidx[4 * k], R.kidx[4 * k + 1], R.kidx[4 * k + 2]};
+           const PIMC_GLOBAL int* kx = (const PIMC_GLOBAL int*)R.kidx + 4 * k;
+           const int ki[3] = {kx[0], kx[1], kx[2]};
            for (int j = 0; j < nc; j++) {
             const int t = (c0 + j) * step;
             const double factor = (t == 0 || t == W) ? 0.5 : 1.0;
-            const d2v* rho = R.rho0 + (size_t)(slice1 + t) * R.ns * R.nk;
+            const PIMC_GLOBAL d2v* rho = (const PIMC_GLOBAL d2v*)R.rho0 + (size_t)(slice1 + t) * R.ns * R.nk;
             LrT nr, ni, orr, oi;
             kterm<LrT>(lrC + 2 * j * stride, o1, o2, ki, nr, ni);
             kterm<LrT>(lrC + (2 * j + 1) * stride, o1, o2, ki, orr, oi);
@@ -527,7 +528,7 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
 #pragma unroll
             for (int s = 0; s < kMaxSpecies; s++) {
               if (s < R.ns && ((R.lr_mask >> (s * R.ns + s)) & 1u)) {
-                const double u = R.ulong_k[((size_t)(s * R.ns + s) * R.lr_levels + level) * R.nk + k];
+                const double u = ((const PIMC_GLOBAL double*)R.ulong_k)[((size_t)(s * R.ns + s) * R.lr_levels + level) * R.nk + k];
                 part_o += factor * 0.5 * 2.0 * (ro[s].x * ro[s].x + ro[s].y * ro[s].y) * u;
                 part_n += factor * 0.5 * 2.0 * (rn[s].x * rn[s].x + rn[s].y * rn[s].y) * u;
               }
@@ -537,7 +538,8 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
 #pragma unroll
               for (int s2 = s1 + 1; s2 < kMaxSpecies; s2++) {
                 if (s2 < R.ns && ((R.lr_mask >> (s1 * R.ns + s2)) & 1u)) {
-                  const double u = R.ulong_k[((size_t)(s1 * R.ns + s2) * R.lr_levels + level) * R.nk + k];
+                  const double u =
+                      ((const PIMC_GLOBAL double*)R.ulong_k)[((size_t)(s1 * R.ns + s2) * R.lr_levels + level) * R.nk + k];
                   part_o += factor * 2.0 * (ro[s1].x * ro[s2].x + ro[s1].y * ro[s2].y) * u;
                   part_n += factor * 2.0 * (rn[s1].x * rn[s2].x + rn[s1].y * rn[s2].y) * u;
                 }

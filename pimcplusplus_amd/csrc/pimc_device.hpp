@@ -306,13 +306,14 @@ __device__ inline int grid_iv(const GridDev& G, double x, d4v& rec) {
     t = __log2f((float)(x * G.r0inv)) * G.log2inv;
   t = fminf(fmaxf(t, 0.0f), (float)(G.n - 2));   // also maps log2(0) = -inf to 0
   int i = (int)t;
-  rec = G.iv[i];
+  const PIMC_GLOBAL d4v* iv = (const PIMC_GLOBAL d4v*)G.iv;   // global memory: plain vector loads
+  rec = iv[i];
   if (x < rec.x && i > 0) {
     i -= 1;
-    rec = G.iv[i];
+    rec = iv[i];
   } else if (x >= rec.w && i < G.n - 2) {
     i += 1;
-    rec = G.iv[i];
+    rec = iv[i];
   }
   return i;
 }
@@ -334,8 +335,9 @@ __device__ inline double bicubic(const FitDev& F, const GridDev& Q, const d4v* _
   const double a2 = h * (u * (u - 1.0) * (u - 1.0)), a3 = h * (u * u * (u - 1.0));
   const double b0 = (v - 1.0) * (v - 1.0) * (1.0 + 2.0 * v), b1 = v * v * (3.0 - 2.0 * v);
   const double b2 = k * (v * (v - 1.0) * (v - 1.0)), b3 = k * (v * v * (v - 1.0));
-  const d4v A = tab[(size_t)ix * F.nt + iy], B = tab[(size_t)ix * F.nt + iy + 1];
-  const d4v C = tab[(size_t)(ix + 1) * F.nt + iy], D = tab[(size_t)(ix + 1) * F.nt + iy + 1];
+  const PIMC_GLOBAL d4v* g = (const PIMC_GLOBAL d4v*)tab;
+  const d4v A = g[(size_t)ix * F.nt + iy], B = g[(size_t)ix * F.nt + iy + 1];
+  const d4v C = g[(size_t)(ix + 1) * F.nt + iy], D = g[(size_t)(ix + 1) * F.nt + iy + 1];
   // BCpoint = {z, dzdx, dzdy, d2zdxdy}
   return a0 * (A.x * b0 + B.x * b1 + A.z * b2 + B.z * b3) + a1 * (C.x * b0 + D.x * b1 + C.z * b2 + D.z * b3) +
          a2 * (A.y * b0 + B.y * b1 + A.w * b2 + B.w * b3) + a3 * (C.y * b0 + D.y * b1 + C.w * b2 + D.w * b3);
@@ -375,7 +377,7 @@ __device__ inline double ulong_eval(const RepCore& R, const double* __restrict__
     X = R.ulong_start + (double)I * R.ulong_delta;
   }
   const double P = (x - X);
-  const double* k = tab + (size_t)I * 6;
+  const PIMC_GLOBAL double* k = (const PIMC_GLOBAL double*)tab + (size_t)I * 6;
   return ((((k[5] * P + k[4]) * P + k[3]) * P + k[2]) * P + k[1]) * P + k[0];
 }
 
