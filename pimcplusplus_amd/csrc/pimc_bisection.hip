@@ -174,7 +174,13 @@ __device__ inline void sr_partners(const RepCore& R, const FitDev& F, const Tabl
   }
 }
 
-// Waves per SIMD the register allocation targets: four blocks of four waves per CU.
+// Waves per SIMD the register allocation of the lean kernel targets: four blocks of four
+// waves per CU.
+// The generic kernel (Coulomb fits, log grids, long range) holds more state per lane: at
+// two waves per SIMD (256 VGPRs) it runs without spills, and faster than at three or four.
+#ifndef PIMC_GENERIC_WAVES
+#define PIMC_GENERIC_WAVES 2
+#endif
 #ifndef PIMC_BISECTION_WAVES
 #define PIMC_BISECTION_WAVES 4
 #endif
@@ -698,7 +704,8 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
 enum { kOneBlock = 0, kPersistent = 1, kCluster = 2 };
 
 template <int NT, bool MIXED, int VAR, int BLOCK, int MODE>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PIMC_BISECTION_WAVES)))
+__global__ void __launch_bounds__(BLOCK)
+__attribute__((amdgpu_waves_per_eu(VAR == kLeanDavid ? PIMC_BISECTION_WAVES : PIMC_GENERIC_WAVES)))
 k_bisection(const LaunchBisection a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   PIMC_LDS unsigned char* smem = (PIMC_LDS unsigned char*)smem_raw;

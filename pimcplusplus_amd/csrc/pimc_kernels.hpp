@@ -406,9 +406,14 @@ __device__ inline double kinetic_link(const RepCore& R, P a, P b, double FourLam
 // C_0(kx) C_1(ky) C_2(kz).  T = double for FP64, float for the mixed phase sums.
 // --------------------------------------------------------------------------
 __device__ inline int kstride(const RepCore& R) { return 2 * (R.maxk0 + R.maxk1 + R.maxk2) + 3; }
-__device__ inline int kmax_of(const RepCore& R, int d) { return d == 0 ? R.maxk0 : (d == 1 ? R.maxk1 : R.maxk2); }
+// The fields of dimension d (0, 1, 2) are combined arithmetically rather than selected:
+// the compiler turns a select between fields of the register-resident R into a load at a
+// selected offset, which moves the whole of R into scratch memory when d varies by lane.
+__device__ inline int kmax_of(const RepCore& R, int d) {
+  return R.maxk0 * (d == 0) + R.maxk1 * (d == 1) + R.maxk2 * (d == 2);
+}
 __device__ inline int koff_of(const RepCore& R, int d) {
-  return d == 0 ? 0 : (d == 1 ? 2 * R.maxk0 + 1 : 2 * (R.maxk0 + R.maxk1) + 2);
+  return (2 * R.maxk0 + 1) * (d >= 1) + (2 * R.maxk1 + 1) * (d == 2);
 }
 
 // cos and sin of phi = r k_box are taken as cospi/sinpi of phi/pi: |phi| stays
@@ -436,12 +441,12 @@ __device__ inline void phase_row(double phi, int maxk, PIMC_LDS d2v* C) {
 }
 
 // The three rows of one position r (dimension d at koff_of(d)); lanes 0..2 of the caller.
-// d is a per-lane value: the dimension's fields are selected, not indexed (a
-// run-time index into the register-resident R would move it to scratch).
+// d is a per-lane value: the dimension's fields are combined arithmetically, not indexed
+// or selected (either would move the register-resident R to scratch).
 template <typename T>
 __device__ inline void phase_rows_lane(const RepCore& R, const double r[3], int d, PIMC_LDS d2v* C) {
   const double x = d == 0 ? r[0] : (d == 1 ? r[1] : r[2]);
-  const double kb = d == 0 ? R.kbox[0] : (d == 1 ? R.kbox[1] : R.kbox[2]);
+  const double kb = R.kbox[0] * (double)(d == 0) + R.kbox[1] * (double)(d == 1) + R.kbox[2] * (double)(d == 2);
   phase_row<T>(x * kb, kmax_of(R, d), C + koff_of(R, d));
 }
 

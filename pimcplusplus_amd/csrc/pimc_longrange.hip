@@ -36,7 +36,8 @@ __global__ void __launch_bounds__(kBlock) k_rho_all(const ReplicaDev* __restrict
     for (int t = threadIdx.x; t < np * 3; t += blockDim.x) {
       const int p = p0 + t / 3, d = t % 3;
       const double x = R.path0[((size_t)slice * 3 + d) * R.N + p];
-      phase_row<double>(x * R.kbox[d], kmax_of(R, d), C + (size_t)(t / 3) * stride + koff_of(R, d));
+      const double kb = R.kbox[0] * (double)(d == 0) + R.kbox[1] * (double)(d == 1) + R.kbox[2] * (double)(d == 2);
+      phase_row<double>(x * kb, kmax_of(R, d), C + (size_t)(t / 3) * stride + koff_of(R, d));
     }
     __syncthreads();
     if (k < R.nk)
