@@ -58,6 +58,7 @@ __global__ void __launch_bounds__(kBlock) k_short_range(const ReplicaDev* __rest
   __shared__ double scratch_raw[kWaves];
   PIMC_LDS double* scratch = (PIMC_LDS double*)scratch_raw;
   const ReqDev Q = reqs[blockIdx.y];
+  const PIMC_GLOBAL int* qp = (const PIMC_GLOBAL int*)reqs[blockIdx.y].ptcls;   // indexed at run time: not from Q
   const ReplicaDev* RD = reps + Q.replica;
   const RepCore R = load_uniform(&RD->core);
   const RepSpecies& S = RD->sp;
@@ -71,10 +72,10 @@ __global__ void __launch_bounds__(kBlock) k_short_range(const ReplicaDev* __rest
       return s;
     }();
     for (int i1 = 0; i1 < Q.nptcls; i1++) {
-      const int p1 = Q.ptcls[i1];
+      const int p1 = qp[i1];
       // DoPtcl: partner j is skipped if it is p1 or a mover handled before p1
       bool skipj = false;
-      for (int t = 0; t <= i1; t++) skipj |= (Q.ptcls[t] == j);
+      for (int t = 0; t <= i1; t++) skipj |= (qp[t] == j);
       if (skipj) continue;
       int sp1 = 0;
       while (sp1 + 1 < R.ns && p1 >= S.first[sp1 + 1]) sp1++;
@@ -145,13 +146,14 @@ __global__ void k_kinetic(const ReplicaDev* __restrict__ reps, const ReqDev* __r
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nreq) return;
   const ReqDev Q = reqs[r];
+  const PIMC_GLOBAL int* qp = (const PIMC_GLOBAL int*)reqs[r].ptcls;   // indexed at run time: not from Q
   const RepCore& R = reps[Q.replica].core;
   const RepSpecies& S = reps[Q.replica].sp;
   const int skip = 1 << Q.level;
   const double levelTau = R.tau * (1 << Q.level);
   double TotalK = 0.0;
   for (int pi = 0; pi < Q.nptcls; pi++) {
-    int ptcl = Q.ptcls[pi];
+    int ptcl = qp[pi];
     int sp = 0;
     while (sp + 1 < R.ns && ptcl >= S.first[sp + 1]) sp++;
     double lambda = S.lambda[sp];

@@ -89,6 +89,7 @@ __global__ void __launch_bounds__(kBlock) k_lr_req(const ReplicaDev* __restrict_
   extern __shared__ __align__(16) unsigned char smem_raw[];
   PIMC_LDS d2v* Cn = (PIMC_LDS d2v*)smem_raw;
   const ReqDev Q = reqs[r];
+  const PIMC_GLOBAL int* qp = (const PIMC_GLOBAL int*)reqs[r].ptcls;   // indexed at run time: not from Q
   const ReplicaDev* RD = reps + Q.replica;
   const RepCore R = load_uniform(&RD->core);
   const RepSpecies& S = RD->sp;
@@ -105,7 +106,7 @@ __global__ void __launch_bounds__(kBlock) k_lr_req(const ReplicaDev* __restrict_
       for (int i = threadIdx.x; i < R.ns * R.nk; i += blockDim.x)
         R.rho1[(size_t)slice * R.ns * R.nk + i] = R.rho0[(size_t)slice * R.ns * R.nk + i];
     for (int pi = 0; pi < Q.nptcls; pi++) {
-      const int p = Q.ptcls[pi];
+      const int p = qp[pi];
       int sp = 0;
       while (sp + 1 < R.ns && p >= S.first[sp + 1]) sp++;
       for (int slice = Q.slice1; slice <= Q.slice2; slice += skip) {
