@@ -186,7 +186,7 @@ def report_other(args):
     print(json.dumps(out))
 
 
-def delta_s_error(nrep: int = 16, nmoves: int = 40):
+def delta_s_error(nrep: int = 4, nmoves: int = 2500):
     """Max |dS_mixed - dS_fp64| over stages attempted on identical paths (chains run
     until their first differing decision)."""
     system = workloads.config1(num_replicas=nrep)

@@ -105,3 +105,21 @@ def test_config1_benchmark_size_launch_shapes_agree_and_sample_matches_oracle(or
         rid = workloads.load(oracle, one)
         oracle.bisection_run(rid, sysm.num_levels, sysm.seed, 0, 8)
         assert np.max(np.abs(oracle.path_get(rid[0]) - p)) < 1e-12
+
+
+def test_config1_ten_thousand_moves_against_the_oracle(oracle, gpu):
+    """The benchmark system over 10^4 seeded moves (4 replicas x 2500): the FP64 chain equals
+    the oracle's move by move, and the mixed chain's dS stays within 1e-6 of it."""
+    nrep, nmoves = 4, 2500
+    sysm = workloads.config1(num_replicas=nrep)
+    print(f"config1: {nrep} replicas x {nmoves} moves, FP64 and mixed against the oracle", flush=True)
+    io, ig = workloads.load(oracle, sysm), workloads.load(gpu, sysm)
+    ro, so = oracle.bisection_run(io, sysm.num_levels, sysm.seed, 0, nmoves)
+    rg, sg = gpu.bisection_run(ig, sysm.num_levels, sysm.seed, 0, nmoves, abi.FP64)
+    compare_chains(ro, rg)
+    assert (so.moves, so.accepted, so.pair_evals) == (sg.moves, sg.accepted, sg.pair_evals) and so.moves == nrep * nmoves
+    gm = workloads.load(gpu, sysm)
+    rm, _ = gpu.bisection_run(gm, sysm.num_levels, sysm.seed, 0, nmoves, abi.MIXED)
+    worst, n = _mixed_delta_s_error(ro, rm, nrep, nmoves)
+    print(f"mixed: max |dS error| {worst:.3g} over {n} stages", flush=True)
+    assert n > 1000 and worst <= 1e-6
