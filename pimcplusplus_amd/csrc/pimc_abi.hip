@@ -489,8 +489,13 @@ int pimc_fit_create_david(pimc_ctx* ctx, const pimc_david_fit_desc* d, int32_t* 
   }
   auto f = std::make_unique<HostFit>();
   size_t bx = (size_t)ng * 8, bgeo = (size_t)nint * 16, b64 = c64.size() * 32, b32 = c32.size() * 16;
+  // per level: the first interval above every flagged one of that level
+  std::vector<int> i32_lev(ntau, 0);
+  for (int lev = 0; lev < ntau; lev++)
+    for (int i = 0; i < nint; i++)
+      if (big[(size_t)lev * nint + i]) i32_lev[lev] = i + 1;
   size_t off_geo = 0, off_x = off_geo + bgeo, off64 = (off_x + bx + 255) & ~(size_t)255,
-         off32 = off64 + b64, total = off32 + b32;
+         off32 = off64 + b64, off_i32 = (off32 + b32 + 15) & ~(size_t)15, total = off_i32 + (size_t)ntau * 4;
   int rc = f->mem.ensure(total);
   if (rc) return rc;
   unsigned char* base = f->mem.as<unsigned char>();
@@ -498,6 +503,7 @@ int pimc_fit_create_david(pimc_ctx* ctx, const pimc_david_fit_desc* d, int32_t* 
   HIP_TRY(hipMemcpy(base + off_x, g.x.data(), bx, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(base + off64, c64.data(), b64, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(base + off32, c32.data(), b32, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(base + off_i32, i32_lev.data(), (size_t)ntau * 4, hipMemcpyHostToDevice));
   if (d->dy) {
     // energy-estimator table: V (column 0 of y), du00 and the off-diagonal dukj columns
     const int ecol = ncol + 1;
@@ -524,9 +530,8 @@ int pimc_fit_create_david(pimc_ctx* ctx, const pimc_david_fit_desc* d, int32_t* 
   F.num_terms = nt;
   // the first interval above every flagged one (all levels)
   F.i32 = 0;
-  for (int lev = 0; lev < ntau; lev++)
-    for (int i = 0; i < nint; i++)
-      if (big[(size_t)lev * nint + i]) F.i32 = std::max(F.i32, i + 1);
+  for (int lev = 0; lev < ntau; lev++) F.i32 = std::max(F.i32, i32_lev[lev]);
+  F.i32_lev = (const int*)(base + off_i32);
   F.taupos = taupos;
   F.start = g.start;
   F.end = g.end;

@@ -93,8 +93,9 @@ struct FitDev {
   double smallest_beta;
   double z1z2;
   // kind 0: the FP32 rows of intervals i >= i32 are valid (not NaN) at every
-  // level; the mixed loops evaluate the FP64 rows of intervals below it
+  // level; the mixed loops evaluate the FP64 rows of intervals below i32_lev[level]
   int i32;
+  const int* i32_lev;     // [num_tau] the same bound for each tau level alone
   // intervals whose FP64 rows a mixed stage copies into LDS beside the FP32 rows:
   // i32 when they take at most kFp64StageBytes, else 0 (read from global memory)
   int i32_lds;
@@ -106,10 +107,11 @@ struct FitDev {
 #endif
 constexpr int kFp64StageBytes = PIMC_FP64_STAGE_BYTES;
 
-// Largest |c0|+|c1|+|c2|+|c3| of a column the mixed path evaluates in FP32.  At 1.0 the
-// benchmark system's largest |dS error| over 26,000 stages was 2.3e-6; at 0.5 it is 2.6e-7.
+// Largest |c0|+|c1|+|c2|+|c3| of a column the mixed path evaluates in FP32, at each tau
+// level.  Over 26,000 stages of the benchmark system the largest |dS error| is 3.0e-7 at
+// 0.35 (5.3e-7 at 0.5, 2.3e-6 at 1.0).
 #ifndef PIMC_MIXED_LIMIT
-#define PIMC_MIXED_LIMIT 0.5
+#define PIMC_MIXED_LIMIT 0.35
 #endif
 constexpr double kMixedLimit = PIMC_MIXED_LIMIT;
 
@@ -408,6 +410,7 @@ struct LevelTable32 {
   // FitDev::i32_lds: every interval the mixed lookups read in FP64, or 0)
   const PIMC_LDS d4v* c64s;
   int n64s;
+  int i32;   // FitDev::i32_lev of this level: intervals below it are evaluated from the FP64 rows
 };
 
 __device__ inline double horner64(const d4v c, double b) { return c.x + b * (c.y + b * (c.z + b * c.w)); }

@@ -101,6 +101,7 @@ __device__ inline LevelTable32<false> level_table32(const FitDev& F, int lev) {
   t.c64 = F.c64 + (size_t)lev * F.nint * F.ncol;
   t.c64s = nullptr;
   t.n64s = 0;
+  t.i32 = F.i32_lev[lev];
   return t;
 }
 
@@ -200,6 +201,7 @@ __device__ inline LevelTable32<true> stage_level32(const FitDev& F, int lev, PIM
   t.c64 = F.c64 + (size_t)lev * nc;
   t.c64s = c64s;
   t.n64s = F.i32_lds;
+  t.i32 = F.i32_lev[lev];
   return t;
 }
 
@@ -279,7 +281,7 @@ __device__ inline double slice_u00(const FitDev& F, const LevelTable32<InLds>& T
   double b;
   const int l = table_low<LIN>(F, T, r, b);
   double u[1];
-  eval_cols_split<1>(T, F.ncol, l, 0, b, l < F.i32, u);
+  eval_cols_split<1>(T, F.ncol, l, 0, b, l < T.i32, u);
   return u[0];
 }
 
@@ -311,7 +313,7 @@ __device__ inline double link_offdiag(const FitDev& F, const Table& T, double q,
     const int lq = table_low<LIN>(F, T, q, bq);
     double cv[NOFF];
     if constexpr (IsMixed<Table>::value) {
-      eval_cols_split<NOFF>(T, F.ncol, lq, 1, bq, lq < F.i32, cv);
+      eval_cols_split<NOFF>(T, F.ncol, lq, 1, bq, lq < T.i32, cv);
     } else {
 #pragma unroll
       for (int c = 0; c < NOFF; c++) cv[c] = horner64(T.c[lq * F.ncol + 1 + c], bq);
