@@ -222,8 +222,17 @@ int pimc_accept_copy(pimc_ctx* ctx, int32_t replica, int32_t slice1, int32_t sli
                      const int32_t* ptcls, int32_t nptcls);
 int pimc_reject_copy(pimc_ctx* ctx, int32_t replica, int32_t slice1, int32_t slice2,
                      const int32_t* ptcls, int32_t nptcls);
-/* PathClass::MoveJoin(old_join, new_join). */
+/* PathClass::MoveJoin(old_join, new_join) (PIMC++/src/PathClass.cc:727-793): for new_join >
+ * old_join the slices (old_join, new_join] of copy OLD take, at particle p, the NEW bead of
+ * particle Permutation(p); for new_join < old_join the slices (new_join, old_join] of OLD take,
+ * at particle Permutation(p), the NEW bead of p; then NEW <- OLD over those slices.  The
+ * replica's join becomes new_join (PathDataClass::MoveJoin, PIMC++/src/PathDataClass.h:106-111). */
 int pimc_move_join(pimc_ctx* ctx, int32_t replica, int32_t old_join, int32_t new_join);
+/* The replica's current join (PathDataClass::Join): 0 at creation, new_join after
+ * pimc_move_join, and slice2 of the last move after pimc_bisection_run, which moves the join
+ * at the start of every move as BisectionBlockClass::MakeMove does
+ * (PIMC++/src/Moves/BisectionBlock.cc:423-424). */
+int pimc_replica_join(pimc_ctx* ctx, int32_t replica, int32_t* join);
 
 /* PermuteTableClass::ConstructHTable (PIMC++/src/Moves/PermuteTableClass.cc:122-150)
  * for the particles of `species`, read from copy `mode`:
@@ -280,6 +289,13 @@ int pimc_action_long_range(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t n
  * reference computes them (n = -nearbyint(dispA/L), m = nearbyint((dispA-dispB)/L)). */
 int pimc_dist_disp(pimc_ctx* ctx, int32_t replica, int32_t mode, int64_t n,
                    const int32_t* idx, double* out, int32_t* images);
+
+/* PathClass::PutInBox (PIMC++/src/PathClass.h:897-921): v_d += n L_d IsPeriodic_d with
+ * n = -floor(v_d / L_d + 1/2), so a displacement of exactly half a box goes to -L/2 (ties go
+ * up, unlike DistDisp's nearbyint); or, with fast = 1, PathClass::PutInBoxFast
+ * (PIMC++/src/PathClass.cc:1389-1407): subtract L where v > L/2, then add L where v < -L/2.
+ * v is [n][3], with the replica's box, replaced in place. */
+int pimc_put_in_box(pimc_ctx* ctx, int32_t replica, int32_t fast, int64_t n, double* v);
 
 /* ------------------------------------------------------------------ */
 /* Estimators (ActionsClass::Energy, PairCorrelationClass)             */
@@ -365,6 +381,10 @@ typedef struct pimc_bisection_stats {
  * With records and stats both NULL the call is asynchronous: it returns once the
  * work is queued on the context's stream, and its counters are read with
  * pimc_bisection_stats_take.  Otherwise it returns when the moves are done.
+ * Every move first moves the replica's join to its slice2 (PathDataClass::MoveJoin,
+ * BisectionBlock.cc:424); with an identity permutation that changes no bead.  A replica
+ * whose permutation is not the identity cannot run under a cluster plan
+ * (PIMC_ERR_UNSUPPORTED).
  * Replaces the per-move driver BisectionBlockClass::MakeMove
  * (PIMC++/src/Moves/BisectionBlock.cc:310-451) with NoPermuteStageClass
  * (PIMC++/src/Moves/NoPermuteClass.cc:26-68) and the stage loop of

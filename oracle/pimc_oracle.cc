@@ -1429,6 +1429,24 @@ int pimc_move_join(pimc_ctx* ctx, int32_t replica, int32_t old_join, int32_t new
   R->join = new_join;
   return PIMC_OK;
 }
+int pimc_replica_join(pimc_ctx* ctx, int32_t replica, int32_t* join) {
+  Replica* R = get_rep(&ctx->c, replica);
+  if (!R || !join) return fail(PIMC_ERR_ARG, "replica_join: bad argument");
+  *join = R->join;
+  return PIMC_OK;
+}
+// PathClass::PutInBox (PathClass.h:897-921) / PutInBoxFast (PathClass.cc:1389-1407)
+int pimc_put_in_box(pimc_ctx* ctx, int32_t replica, int32_t fast, int64_t n, double* v) {
+  Replica* R = get_rep(&ctx->c, replica);
+  if (!R || n < 0 || (n > 0 && !v) || (fast != 0 && fast != 1)) return fail(PIMC_ERR_ARG, "put_in_box: bad argument");
+  for (int64_t i = 0; i < n; i++) {
+    if (fast)
+      R->put_in_box_fast(v + 3 * i);
+    else
+      R->put_in_box(v + 3 * i);
+  }
+  return PIMC_OK;
+}
 
 // PermuteTableClass::ConstructHTable (PermuteTableClass.cc:122-150)
 int pimc_permute_htable(pimc_ctx* ctx, int32_t replica, int32_t mode, int32_t species, int32_t slice1,

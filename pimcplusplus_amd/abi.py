@@ -173,6 +173,8 @@ SIGNATURES = {
     "pimc_accept_copy": (C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32, _i32p, C.c_int32]),
     "pimc_reject_copy": (C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32, _i32p, C.c_int32]),
     "pimc_move_join": (C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32]),
+    "pimc_replica_join": (C.c_int, [_CTX, C.c_int32, _i32p]),
+    "pimc_put_in_box": (C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int64, _f64p]),
     "pimc_permute_htable": (C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                                       _f32p]),
     "pimc_cycle_apply": (C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _i32p]),
@@ -458,6 +460,19 @@ class PimcLib:
 
     def move_join(self, rep: int, old_join: int, new_join: int):
         self._check("pimc_move_join", self.lib.pimc_move_join(self.ctx, rep, old_join, new_join))
+
+    def join(self, rep: int) -> int:
+        """The replica's current join slice (PathDataClass::Join)."""
+        out = C.c_int32()
+        self._check("pimc_replica_join", self.lib.pimc_replica_join(self.ctx, rep, C.byref(out)))
+        return out.value
+
+    def put_in_box(self, rep: int, v, fast: bool = False) -> np.ndarray:
+        """PathClass::PutInBox (fast=False) or PutInBoxFast (fast=True) of displacements [n, 3]."""
+        x = np.array(_f64(v).reshape(-1, 3), dtype=np.float64, order="C", copy=True)
+        self._check("pimc_put_in_box", self.lib.pimc_put_in_box(self.ctx, rep, 1 if fast else 0, x.shape[0],
+                                                                _p(x, _f64p)))
+        return x
 
     def permute_htable(self, rep: int, species: int, slice1: int, slice2: int, epsilon: float,
                        n: int, mode: int = OLDMODE) -> np.ndarray:

@@ -52,8 +52,20 @@ def sources() -> list[str]:
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
-def _compile(src: str) -> str:
+def _up_to_date(obj: str, src: str) -> bool:
+    """The object is newer than its source and every header under csrc/ and include/."""
+    if not os.path.exists(obj):
+        return False
+    t = os.path.getmtime(obj)
+    deps = [src] + [os.path.join(d, f) for d in (CSRC, os.path.join(HERE, "..", "include"))
+                    for f in os.listdir(d) if f.endswith((".hpp", ".h"))]
+    return all(os.path.getmtime(d) < t for d in deps)
+
+
+def _compile(src: str, incremental: bool = False) -> str:
     obj = os.path.join(OBJDIR, os.path.basename(src)[:-4] + ".o")
+    if incremental and _up_to_date(obj, src):
+        return obj
     cmd = [hipcc(), *HIPCC_FLAGS, *FILE_FLAGS.get(os.path.basename(src), []), "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -70,14 +82,15 @@ def _link(lib: str, objs: list[str]) -> None:
     os.replace(tmp, lib)
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, incremental: bool = False) -> str:
     """Builds libpimc_gpu.so (the action library) and libpimc_calib.so (the benchmark's
-    HBM and LDS microbenchmarks, csrc/calib/)."""
+    HBM and LDS microbenchmarks, csrc/calib/).  incremental: objects newer than their
+    source and the headers are kept (the default recompiles everything)."""
     os.makedirs(OBJDIR, exist_ok=True)
     srcs = sources()
     with ThreadPoolExecutor(max_workers=min(8, len(srcs) + 1)) as ex:
-        calib = ex.submit(_compile, CALIB_SRC)
-        objs = list(ex.map(_compile, srcs))
+        calib = ex.submit(_compile, CALIB_SRC, incremental)
+        objs = list(ex.map(lambda f: _compile(f, incremental), srcs))
         calib_obj = calib.result()
     _link(LIB, objs)
     _link(CALIB_LIB, [calib_obj])
@@ -87,5 +100,5 @@ def build(verbose: bool = False) -> str:
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, incremental="--incremental" in sys.argv)
     sys.exit(0)

@@ -190,6 +190,7 @@ int make_host_grid(int type, double start, double end, int n, HostGrid& g) {
 struct HostReplica {
   ReplicaDev dev{};
   DevBuf path0, path1, perm0, perm1;
+  DevBuf joinbuf;                // the replica's join (one int, device-resident: the chain moves it)
   DevBuf rho0, rho1, kidx_dev, ulong_k, lr_const, ulong;
   DevBuf dulong, dulong_k, dlr_const;   // beta derivatives (energy estimator)
   DevBuf vlong, vlong_k, vlr_const;     // long-range potential (potential-energy estimator)
@@ -198,7 +199,8 @@ struct HostReplica {
   std::vector<int> kidx;         // [nk][3] index + maxk
   bool use_long_range = false;
   int M = 0;
-  int join = 0;
+  // set once a cycle has been applied: the permutation may no longer be the identity
+  bool maybe_permuted = false;
 };
 
 // PathClass::SetupkVecs3D with Include (PathClass.cc:281-292, 534-586): integer
@@ -818,8 +820,9 @@ int pimc_replica_create(pimc_ctx* ctx, const pimc_replica_desc* d, int32_t* repl
   size_t nb = (size_t)R.nslices * 3 * R.N * sizeof(double);
   int rc;
   if ((rc = H->path0.ensure(nb)) || (rc = H->path1.ensure(nb)) || (rc = H->perm0.ensure(R.N * sizeof(int))) ||
-      (rc = H->perm1.ensure(R.N * sizeof(int))))
+      (rc = H->perm1.ensure(R.N * sizeof(int))) || (rc = H->joinbuf.ensure(sizeof(int))))
     return rc;
+  HIP_TRY(hipMemset(H->joinbuf.p, 0, sizeof(int)));
   HIP_TRY(hipMemset(H->path0.p, 0, nb));
   HIP_TRY(hipMemset(H->path1.p, 0, nb));
   std::vector<int> id(R.N);
@@ -830,6 +833,7 @@ int pimc_replica_create(pimc_ctx* ctx, const pimc_replica_desc* d, int32_t* repl
   R.path1 = H->path1.as<double>();
   R.perm0 = H->perm0.as<int>();
   R.perm1 = H->perm1.as<int>();
+  R.join = H->joinbuf.as<int>();
   if (R.nk > 0) {
     size_t nr = (size_t)R.nslices * R.ns * R.nk * sizeof(d2v);
     if ((rc = H->rho0.ensure(nr)) || (rc = H->rho1.ensure(nr)) || (rc = H->kidx_dev.ensure((size_t)R.nk * 4 * sizeof(int))))
@@ -1019,8 +1023,33 @@ int pimc_move_join(pimc_ctx* ctx, int32_t replica, int32_t old_join, int32_t new
   if (rc) return rc;
   launch_move_join(ctx->d_reps.as<ReplicaDev>(), replica, H->dev.core.N, old_join, new_join, ctx->stream);
   HIP_TRY(hipGetLastError());
+  const int32_t join = new_join;   // PathDataClass::MoveJoin: Join = newJoin
+  HIP_TRY(hipMemcpyAsync(H->joinbuf.p, &join, sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
-  H->join = new_join;
+  return PIMC_OK;
+}
+int pimc_replica_join(pimc_ctx* ctx, int32_t replica, int32_t* join) {
+  HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
+  if (!H || !join) return fail(PIMC_ERR_ARG, "replica_join: bad argument");
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(hipMemcpy(join, H->joinbuf.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+  return PIMC_OK;
+}
+int pimc_put_in_box(pimc_ctx* ctx, int32_t replica, int32_t fast, int64_t n, double* v) {
+  HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
+  if (!H || n < 0 || (n > 0 && !v) || (fast != 0 && fast != 1)) return fail(PIMC_ERR_ARG, "put_in_box: bad argument");
+  if (n == 0) return PIMC_OK;
+  HIP_TRY(hipSetDevice(ctx->device));
+  int rc = ctx->sync_descriptors();
+  if (rc) return rc;
+  const size_t bytes = (size_t)n * 3 * sizeof(double);
+  if ((rc = ctx->d_in0.ensure(bytes))) return rc;
+  HIP_TRY(hipMemcpyAsync(ctx->d_in0.p, v, bytes, hipMemcpyHostToDevice, ctx->stream));
+  launch_put_in_box(ctx->d_reps.as<ReplicaDev>(), replica, fast, n, ctx->d_in0.as<double>(), ctx->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(v, ctx->d_in0.p, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
   return PIMC_OK;
 }
 
@@ -1069,6 +1098,7 @@ int pimc_cycle_apply(pimc_ctx* ctx, int32_t replica, int32_t first_ptcl, int32_t
   if (rc) return rc;
   launch_cycle_apply(ctx->d_reps.as<ReplicaDev>(), replica, slice, ptcls, length, ctx->stream);
   HIP_TRY(hipGetLastError());
+  if (length > 1) H->maybe_permuted = true;
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   return PIMC_OK;
 }
@@ -1254,6 +1284,19 @@ int prepare_bisection(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, cons
     }
     if (desc->species < 0 || desc->species >= R.ns || H->dev.sp.count[desc->species] < 1)
       return fail(PIMC_ERR_ARG, "bisection: bad species");
+    if (ctx->plan.cluster > 1 && H->maybe_permuted) {
+      // the blocks of a cluster cannot move the join of a permuted path together
+      std::vector<int> p0(R.N), p1(R.N);
+      HIP_TRY(hipSetDevice(ctx->device));
+      HIP_TRY(hipStreamSynchronize(ctx->stream));
+      HIP_TRY(hipMemcpy(p0.data(), H->perm0.p, R.N * sizeof(int), hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(p1.data(), H->perm1.p, R.N * sizeof(int), hipMemcpyDeviceToHost));
+      for (int p = 0; p < R.N; p++)
+        if (p0[p] != p || p1[p] != p)
+          return fail(PIMC_ERR_UNSUPPORTED, "bisection: a cluster plan cannot run a replica whose permutation is "
+                                            "not the identity (use a plan with cluster = 1)");
+      H->maybe_permuted = false;
+    }
     for (int f = 0; f < R.ns * R.ns; f++) {
       const HostFit& F = *ctx->fits[H->dev.sp.fit[f]];
       if (F.num_terms >= 0) {
@@ -1374,7 +1417,8 @@ int take_abort(pimc_ctx* ctx) {
   HIP_TRY(hipMemcpy(&flag, ctx->d_abort.p, sizeof(int), hipMemcpyDeviceToHost));
   if (!flag) return PIMC_OK;
   HIP_TRY(hipMemset(ctx->d_abort.p, 0, sizeof(int)));
-  return fail(PIMC_ERR_DEVICE, "bisection: a cluster barrier wait timed out; the replicas' paths are undefined");
+  return fail(PIMC_ERR_DEVICE, "bisection: a cluster launch was abandoned (a barrier wait timed out, or a replica's "
+                                "permutation is not the identity); the replicas' paths are undefined");
 }
 
 int persistent_grid(pimc_ctx* ctx, const BisectionPrep& P, const LaunchBisection& a, int precision, int nrep) {
@@ -1494,7 +1538,11 @@ int pimc_bisection_tune(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, co
       first_move + trial_moves > (int64_t)0xffffffffLL)
     return fail(PIMC_ERR_ARG, "bisection_tune: bad argument");
   BisectionPrep P0;
+  // validate with a one-block plan: the cluster trials below are skipped where they cannot run
+  const pimc_bisection_plan caller_plan = ctx->plan;
+  ctx->plan.cluster = 1;
   int rc = prepare_bisection(ctx, replicas, nrep, desc, precision, P0);
+  ctx->plan = caller_plan;
   if (rc) return rc;
   HIP_TRY(hipSetDevice(ctx->device));
   if ((rc = ctx->sync_descriptors())) return rc;
@@ -1503,7 +1551,7 @@ int pimc_bisection_tune(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, co
   // save what trial moves change: both path copies, permutations, rho_k, the counters
   auto state_of = [&](int r) {
     HostReplica* H = ctx->rep(replicas[r]);
-    return std::vector<DevBuf*>{&H->path0, &H->path1, &H->perm0, &H->perm1, &H->rho0, &H->rho1};
+    return std::vector<DevBuf*>{&H->path0, &H->path1, &H->perm0, &H->perm1, &H->joinbuf, &H->rho0, &H->rho1};
   };
   std::vector<std::unique_ptr<DevBuf>> saved;
   for (int r = 0; r < nrep; r++)
@@ -1545,7 +1593,13 @@ int pimc_bisection_tune(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, co
   for (const pimc_bisection_plan& plan : plans) {
     ctx->plan = plan;
     BisectionPrep P;
-    if ((rc = prepare_bisection(ctx, replicas, nrep, desc, precision, P))) break;
+    if ((rc = prepare_bisection(ctx, replicas, nrep, desc, precision, P))) {
+      if (rc == PIMC_ERR_UNSUPPORTED && plan.cluster > 1) {   // a permuted replica: no cluster trial
+        rc = PIMC_OK;
+        continue;
+      }
+      break;
+    }
     DevBuf ids, pos, st, q, bar, part;
     if ((rc = ids.ensure((size_t)nrep * 4)) || (rc = pos.ensure((size_t)nrep * 4)) ||
         (rc = st.ensure((size_t)nrep * 40)) || (rc = q.ensure(4)) || (rc = ensure_cluster_buffers(ctx, nrep, bar, part)))

@@ -286,6 +286,23 @@ void launch_move_join(const ReplicaDev* reps, int rep, int N, int old_join, int 
 }
 
 // --------------------------------------------------------------------------
+// PathClass::PutInBox (PathClass.h:897-921) or PutInBoxFast (PathClass.cc:1389-1407) of
+// n displacement vectors [n][3], one thread per component.
+// --------------------------------------------------------------------------
+__global__ void k_put_in_box(const ReplicaDev* __restrict__ reps, int rep, int fast, int64_t n, double* v) {
+  const RepCore& R = reps[rep].core;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * n) return;
+  const int d = (int)(t % 3);
+  v[t] = fast ? put_in_box_fast_1(v[t], R.box[d]) : put_in_box_1(v[t], R.box[d], R.boxinv[d], R.isper[d]);
+}
+void launch_put_in_box(const ReplicaDev* reps, int rep, int fast, int64_t n, double* v, hipStream_t s) {
+  const int64_t total = 3 * n;
+  if (total <= 0) return;
+  k_put_in_box<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(reps, rep, fast, n, v);
+}
+
+// --------------------------------------------------------------------------
 // PermuteTableClass::ConstructHTable (PermuteTableClass.cc:122-150): one thread
 // per (i, j) of the species' particles; displacements through PutInBox, dot
 // products summed over x, y, z in order.

@@ -295,6 +295,33 @@ __device__ inline void phase_rows_chunk(const RepCore& R, const PIMC_LDS double*
   }
 }
 
+// PathClass::MoveJoin(oldJoin, newJoin) (PathClass.cc:727-793) by one block: for
+// newJoin > oldJoin, OLD(s, p) = NEW(s, Permutation(p)) over s in (oldJoin, newJoin];
+// for newJoin < oldJoin, OLD(s, Permutation(p)) = NEW(s, p) over s in (newJoin,
+// oldJoin]; then NEW = OLD over those slices.  Outside a move the two copies and
+// their permutations are equal.  Called by every thread of the block.
+__device__ inline void move_join_block(const RepCore& R, int oldJoin, int newJoin) {
+  if (newJoin == oldJoin) return;
+  const bool fwd = newJoin > oldJoin;
+  const int lo = (fwd ? oldJoin : newJoin) + 1, hi = fwd ? newJoin : oldJoin;
+  const int total = (hi - lo + 1) * R.N;
+  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+    const int slice = lo + t / R.N, p = t % R.N;
+    const int q = R.perm1[p];
+    double v[3];
+    load_bead(R, 1, slice, fwd ? q : p, v);
+    store_bead(R, 0, slice, fwd ? p : q, v);
+  }
+  __syncthreads();   // workgroup-scope fence: the OLD writes are visible to the block
+  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+    const int slice = lo + t / R.N, p = t % R.N;
+    double v[3];
+    load_bead(R, 0, slice, p, v);
+    store_bead(R, 1, slice, p, v);
+  }
+  __syncthreads();
+}
+
 // ---- Several blocks per replica (a cluster) ----
 // The `cluster` blocks of one replica run the same moves in lock step: each
 // samples the same midpoints (same draws), takes its share of the partners and
@@ -393,6 +420,17 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
   typedef typename std::conditional<MIXED, float, double>::type LrT;   // phase-sum arithmetic
   const int numLeft = R.nslices - W;
   const double lambda = S.lambda[sp];
+  // PathDataClass::Join, moved to slice2 at the start of every move (BisectionBlock.cc:424).
+  // MoveJoin copies beads only when the permutation is not the identity; the blocks of a
+  // cluster cannot do that together (the host refuses such a launch; this is the backstop).
+  int join = __builtin_amdgcn_readfirstlane(*(const PIMC_GLOBAL int*)R.join);
+  int nonid = 0;
+  for (int p = tid; p < R.N; p += blockDim.x) nonid |= (R.perm0[p] != p) | (R.perm1[p] != p);
+  const bool permuted = __syncthreads_or(nonid) != 0;
+  if (CL && permuted) {
+    if (tid == 0) __hip_atomic_store(a.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
 
   for (int m = 0; m < a.num_moves; m++) {
     const uint32_t move = first_move + (uint32_t)m;
@@ -425,6 +463,8 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
     __syncthreads();
     const int slice1 = ctl[0];
     const int ptcl = ctl[1];
+    if (permuted) move_join_block(R, join, slice1 + W);
+    join = slice1 + W;
     for (int t = tid; t <= W; t += blockDim.x) {
       double v[3];
       load_bead(R, 0, slice1 + t, ptcl, v);
@@ -680,6 +720,7 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
     __syncthreads();
   }
   if (tid == 0 && member == 0) {
+    *R.join = join;
     int64_t* st = a.stats + (size_t)pos * 5;
     st[0] = st_moves;
     st[1] = st_acc;

@@ -130,6 +130,7 @@ struct RepCore {
   double* path1;      // NEW copy
   int* perm0;         // [N]
   int* perm1;
+  int* join;          // the join slice (PathDataClass::Join), one int
   // k space (PathClass::SetupkVecs3D): C_d(n) tables run over n in [-maxk_d, maxk_d]
   int maxk0, maxk1, maxk2;
   int lr_on;          // long-range action evaluated (UseLongRange and tables present)

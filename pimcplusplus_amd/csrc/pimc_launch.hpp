@@ -37,6 +37,7 @@ void launch_dist_disp(const ReplicaDev* reps, int rep, int mode, int64_t n, cons
 void launch_accept_reject(const ReplicaDev* reps, int rep, int slice1, int slice2, const int* ptcls, int n,
                           bool accept, hipStream_t s);
 void launch_move_join(const ReplicaDev* reps, int rep, int N, int old_join, int new_join, hipStream_t s);
+void launch_put_in_box(const ReplicaDev* reps, int rep, int fast, int64_t n, double* v, hipStream_t s);
 void launch_permute_htable(const ReplicaDev* reps, int rep, int mode, int first, int n, int slice1, int slice2,
                            double four_lambda_beta_inv, double log_eps, float* out, hipStream_t s);
 void launch_cycle_apply(const ReplicaDev* reps, int rep, int slice, const int* ptcls, int n, hipStream_t s);

@@ -65,3 +65,98 @@ def test_cycle_apply_and_copies_bit_exact(oracle, gpu):
         for mode in (0, 1):
             assert np.array_equal(oracle.path_get(io, mode), gpu.path_get(ig, mode)), (step, mode)
             assert np.array_equal(oracle.permutation(io, mode), gpu.permutation(ig, mode)), (step, mode)
+
+
+@pytest.mark.parametrize("joins", [(0, 7, 2), (3, 11, 12, 5, 0), (12, 1, 9)])
+def test_move_join_permuted_world_lines_bit_exact(oracle, gpu, joins):
+    from tests.test_oracle_join import permuted_replica, world_line_storage
+    ro, perm = permuted_replica(oracle)
+    rg, pg = permuted_replica(gpu)
+    assert np.array_equal(perm, pg)
+    W = np.random.default_rng(sum(joins)).uniform(-3.0, 3.0, (9, 13, 3))
+    for lib, r in ((oracle, ro), (gpu, rg)):
+        lib.path_set(r, world_line_storage(W, perm, joins[0]), abi.BOTHMODES)
+    for old_j, new_j in zip(joins[:-1], joins[1:]):
+        print("move_join", old_j, new_j, flush=True)
+        oracle.move_join(ro, old_j, new_j)
+        gpu.move_join(rg, old_j, new_j)
+        want = world_line_storage(W, perm, new_j)
+        for mode in (abi.OLDMODE, abi.NEWMODE):
+            assert np.array_equal(gpu.path_get(rg, mode), want), (old_j, new_j, mode)
+            assert np.array_equal(gpu.path_get(rg, mode), oracle.path_get(ro, mode))
+        assert gpu.join(rg) == oracle.join(ro) == new_j
+
+
+def test_move_join_unequal_copies_bit_exact(oracle, gpu):
+    from tests.test_oracle_join import permuted_replica
+    ro, perm = permuted_replica(oracle)
+    rg, _ = permuted_replica(gpu)
+    rng = np.random.default_rng(5)
+    for old_j, new_j in ((2, 9), (9, 1), (1, 1), (0, 12)):
+        old, new = rng.normal(size=(13, 9, 3)), rng.normal(size=(13, 9, 3))
+        for lib, r in ((oracle, ro), (gpu, rg)):
+            lib.path_set(r, old, abi.OLDMODE)
+            lib.path_set(r, new, abi.NEWMODE)
+            lib.move_join(r, old_j, new_j)
+        for mode in (abi.OLDMODE, abi.NEWMODE):
+            assert np.array_equal(gpu.path_get(rg, mode), oracle.path_get(ro, mode)), (old_j, new_j, mode)
+
+
+@pytest.mark.parametrize("L", [2.0, 0.5, 3.0, 7.1529])
+def test_put_in_box_bit_exact_with_ties(oracle, gpu, L):
+    y = np.zeros((2, 10, 7))
+    ids = []
+    for lib in (oracle, gpu):
+        fid = lib.fit_create_david(0, 0.1, L / 2, y, 2, 0.025, 0.025)
+        ids.append(lib.replica_create(1, 2, (L, L, L), 0.025, [1], [6.0597], [fid]))
+    ties = (np.arange(-4, 4) + 0.5) * L
+    rng = np.random.default_rng(int(L * 100))
+    v = np.concatenate([np.stack([ties, -ties, ties[::-1]], axis=1), rng.uniform(-6 * L, 6 * L, (400, 3)),
+                        np.array([[L / 2, -L / 2, 0.0]])])
+    for fast in (False, True):
+        a = oracle.put_in_box(ids[0], v, fast=fast)
+        b = gpu.put_in_box(ids[1], v, fast=fast)
+        assert np.array_equal(a, b), fast
+    if L in (2.0, 0.5):          # exact ties: PutInBox sends +-L/2 to -L/2
+        assert np.all(gpu.put_in_box(ids[1], v[:8]) == -L / 2)
+
+
+def permute_config0(lib, sysm, ids, cycles=((0, 5, 9), (2, 11, 7, 3))):
+    """Cycles applied at slice 8 of every replica (CycleClass::Apply) and accepted."""
+    for r in ids:
+        for c in cycles:
+            lib.cycle_apply(r, 0, 8, list(c))
+            lib.accept_copy(r, 8, 8, list(c))
+
+
+def test_chain_with_permuted_replicas_matches_oracle(oracle, gpu):
+    from tests.test_gpu_parity import compare_chains
+    sysm = workloads.config0(num_replicas=3)
+    io, ig = workloads.load(oracle, sysm), workloads.load(gpu, sysm)
+    permute_config0(oracle, sysm, io[:2])       # replica 2 keeps the identity permutation
+    permute_config0(gpu, sysm, ig[:2])
+    for first in (0, 150):
+        print("permuted chain, moves", first, flush=True)
+        ro, so = oracle.bisection_run(io, sysm.num_levels, sysm.seed, first, 150)
+        rg, sg = gpu.bisection_run(ig, sysm.num_levels, sysm.seed, first, 150, abi.FP64)
+        compare_chains(ro, rg)
+        assert (so.moves, so.accepted, so.stages) == (sg.moves, sg.accepted, sg.stages)
+    for a, b in zip(io, ig):
+        assert oracle.join(a) == gpu.join(b)
+        assert np.array_equal(oracle.permutation(a), gpu.permutation(b))
+        for mode in (abi.OLDMODE, abi.NEWMODE):
+            assert np.max(np.abs(oracle.path_get(a, mode) - gpu.path_get(b, mode))) < 1e-12
+    assert not np.array_equal(gpu.permutation(ig[0]), np.arange(sysm.replicas[0].num_particles))
+
+
+def test_cluster_plan_refuses_a_permuted_replica(gpu):
+    sysm = workloads.config0(num_replicas=2)
+    ig = workloads.load(gpu, sysm)
+    permute_config0(gpu, sysm, ig[:1])
+    gpu.set_plan(256, 1, False, 2)
+    with pytest.raises(abi.PimcError, match="permutation"):
+        gpu.bisection_run(ig, sysm.num_levels, sysm.seed, 0, 4, abi.FP64)
+    gpu.set_plan(256, 1, False, 2)
+    gpu.bisection_run(ig[1:], sysm.num_levels, sysm.seed, 0, 4, abi.FP64)   # identity: runs
+    plan = gpu.bisection_tune(ig, sysm.num_levels, sysm.seed, 0, 2, abi.FP64)
+    assert plan.cluster == 1
