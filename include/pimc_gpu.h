@@ -275,6 +275,22 @@ typedef struct pimc_action_req {
 /* ShortRangeClass::SingleAction for each request; out[i] is request i's total. */
 int pimc_action_short_range(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t nreq,
                             int32_t precision, double* out);
+/* pimc_action_short_range without any allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, hosts that forbid hipMalloc).
+ * No counterpart in the reference.  Same requests, arithmetic and results as
+ * pimc_action_short_range (the same kernels, bit-identical outputs); its request staging,
+ * per-tile partial sums and outputs live in a fixed device array of
+ * PIMC_NOALLOC_SCRATCH_BYTES per GPU, which the library holds until the process ends.
+ * Limits: nreq <= PIMC_NOALLOC_MAX_REQS and nreq * ceil(N_max / 256) * 8 + nreq * 48 bytes
+ * within the array (N_max: the largest particle count of the requested replicas); larger
+ * calls return PIMC_ERR_ARG.  Fits and replicas created or changed since the last call of
+ * another action entry point must be uploaded by such a call first (PIMC_ERR_STATE
+ * otherwise).  Every call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (two contexts on different host threads) race. */
+#define PIMC_NOALLOC_SCRATCH_BYTES (4u << 20)
+#define PIMC_NOALLOC_MAX_REQS 1024
+int pimc_action_short_range_noalloc(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t nreq,
+                                    int32_t precision, double* out);
 /* KineticClass::SingleAction. */
 int pimc_action_kinetic(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t nreq, double* out);
 /* LongRangeClass::SingleAction: a NEWMODE request first updates rho_k of copy NEW

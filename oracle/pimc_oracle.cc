@@ -1535,6 +1535,13 @@ int pimc_action_short_range(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t 
   }
   return PIMC_OK;
 }
+// The no-allocation variant of the GPU library has nothing to allocate here: the same call.
+int pimc_action_short_range_noalloc(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t nreq, int32_t precision,
+                                    double* out) {
+  if (nreq > PIMC_NOALLOC_MAX_REQS)
+    return fail(PIMC_ERR_ARG, "action_short_range_noalloc: more than PIMC_NOALLOC_MAX_REQS requests");
+  return pimc_action_short_range(ctx, reqs, nreq, precision, out);
+}
 int pimc_action_kinetic(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t nreq, double* out) {
   if (!ctx || (nreq > 0 && (!reqs || !out))) return fail(PIMC_ERR_ARG, "action_kinetic: NULL argument");
   for (int i = 0; i < nreq; i++) {

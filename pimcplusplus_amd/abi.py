@@ -181,6 +181,7 @@ SIGNATURES = {
     "pimc_rho_k_update_all": (C.c_int, [_CTX, C.c_int32]),
     "pimc_rho_k_get": (C.c_int, [_CTX, C.c_int32, C.c_int32, _f64p]),
     "pimc_action_short_range": (C.c_int, [_CTX, C.POINTER(ActionReq), C.c_int32, C.c_int32, _f64p]),
+    "pimc_action_short_range_noalloc": (C.c_int, [_CTX, C.POINTER(ActionReq), C.c_int32, C.c_int32, _f64p]),
     "pimc_action_kinetic": (C.c_int, [_CTX, C.POINTER(ActionReq), C.c_int32, _f64p]),
     "pimc_action_long_range": (C.c_int, [_CTX, C.POINTER(ActionReq), C.c_int32, C.c_int32, _f64p]),
     "pimc_dist_disp": (C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int64, _i32p, _f64p, _i32p]),
@@ -517,6 +518,14 @@ class PimcLib:
         arr, n = self.make_reqs(reqs)
         out = np.zeros(max(1, n), dtype=np.float64)
         self._check("pimc_action_short_range", self.lib.pimc_action_short_range(
+            self.ctx, arr, n, precision, _p(out, _f64p)))
+        return out[:n]
+
+    def action_short_range_noalloc(self, reqs, precision: int = FP64) -> np.ndarray:
+        """pimc_action_short_range_noalloc: the same results, no device allocation."""
+        arr, n = self.make_reqs(reqs)
+        out = np.zeros(max(1, n), dtype=np.float64)
+        self._check("pimc_action_short_range_noalloc", self.lib.pimc_action_short_range_noalloc(
             self.ctx, arr, n, precision, _p(out, _f64p)))
         return out[:n]
 

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <set>
 #include <string>
 #include <limits>
@@ -23,6 +24,15 @@ using namespace pimc;
 namespace {
 
 thread_local std::string g_err;
+
+// Device memory of pimc_action_short_range_noalloc: request staging, per-tile partial
+// sums and outputs.  One copy per GPU, at its own address on each; host code uses the
+// address hipGetSymbolAddress returns for the current device (g_noalloc_ptr), never
+// this name.
+static __device__ __align__(256) unsigned char scratch[PIMC_NOALLOC_SCRATCH_BYTES];
+constexpr int kNoallocDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kNoallocDevices] = {};
 
 int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -376,6 +386,16 @@ int pimc_ctx_create(int32_t device, pimc_ctx** out) {
     return fail(PIMC_ERR_DEVICE, std::string("ctx_create: no GPU device (") + hipGetErrorString(e) + ")");
   if (device < 0 || device >= count) return fail(PIMC_ERR_ARG, "ctx_create: bad device index");
   HIP_TRY(hipSetDevice(device));
+  if (device < kNoallocDevices) {
+    // once per device, with the device current: the device address of its scratch array
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[device]) {
+      void* ptr = nullptr;
+      HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+      if (!ptr) return fail(PIMC_ERR_DEVICE, "ctx_create: no device address for the scratch array");
+      g_noalloc_ptr[device] = ptr;
+    }
+  }
   auto c = std::make_unique<pimc_ctx>();
   c->device = device;
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -1168,6 +1188,70 @@ int pimc_action_short_range(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t 
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(out, ctx->d_out.p, (size_t)nreq * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return PIMC_OK;
+}
+
+int pimc_action_short_range_noalloc(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t nreq, int32_t precision,
+                                    double* out) {
+  if (!ctx || nreq < 0 || (nreq > 0 && (!reqs || !out)))
+    return fail(PIMC_ERR_ARG, "action_short_range_noalloc: bad argument");
+  if (precision != PIMC_FP64 && precision != PIMC_MIXED)
+    return fail(PIMC_ERR_ARG, "action_short_range_noalloc: bad precision");
+  if (nreq > PIMC_NOALLOC_MAX_REQS)
+    return fail(PIMC_ERR_ARG, "action_short_range_noalloc: more than PIMC_NOALLOC_MAX_REQS requests");
+  if (nreq == 0) return PIMC_OK;
+  unsigned char* base =
+      ctx->device < kNoallocDevices ? (unsigned char*)g_noalloc_ptr[ctx->device] : nullptr;
+  if (!base) return fail(PIMC_ERR_STATE, "action_short_range_noalloc: no scratch array on this device");
+  if ((ctx->fits_dirty && !ctx->fits.empty()) || (ctx->reps_dirty && !ctx->reps.empty()))
+    return fail(PIMC_ERR_STATE, "action_short_range_noalloc: fits or replicas changed since the last upload; "
+                                "call another action entry point once first");
+  ReqDev h[PIMC_NOALLOC_MAX_REQS];
+  int maxN = 0, nt = -1;
+  for (int i = 0; i < nreq; i++) {
+    int rc = check_req(ctx, reqs[i]);
+    if (rc) return rc;
+    const ReplicaDev& R = ctx->reps[reqs[i].replica]->dev;
+    maxN = std::max(maxN, R.core.N);
+    for (int f = 0; f < R.core.ns * R.core.ns; f++) {
+      const HostFit& F = *ctx->fits[R.sp.fit[f]];
+      if (F.num_terms >= 0) {
+        if (nt < 0) nt = F.num_terms;
+        if (F.num_terms != nt)
+          return fail(PIMC_ERR_UNSUPPORTED, "action_short_range_noalloc: fits with different num_terms in one call");
+      }
+      if (reqs[i].level >= F.num_levels())
+        return fail(PIMC_ERR_ARG, "action_short_range_noalloc: level beyond the fit's tau levels");
+      if (F.dev.kind == 1 && R.core.use_long_range &&
+          (!((R.core.ulong_mask >> f) & 1u) || reqs[i].level >= R.core.ulong_levels))
+        return fail(PIMC_ERR_STATE, "action_short_range_noalloc: long-range pair fit without Ulong tables");
+    }
+    h[i].replica = reqs[i].replica;
+    h[i].mode = reqs[i].mode;
+    h[i].slice1 = reqs[i].slice1;
+    h[i].slice2 = reqs[i].slice2;
+    h[i].level = reqs[i].level;
+    h[i].nptcls = reqs[i].nptcls;
+    for (int k = 0; k < PIMC_MAX_ACTIVE; k++) h[i].ptcls[k] = k < reqs[i].nptcls ? reqs[i].ptcls[k] : -1;
+  }
+  if (nt < 0) nt = 0;
+  // scratch layout: requests [nreq], outputs [nreq], partial sums [nreq][ntiles]
+  const int ntiles = (maxN + kBlock - 1) / kBlock;
+  const size_t req_bytes = ((size_t)nreq * sizeof(ReqDev) + 255) & ~(size_t)255;
+  const size_t out_bytes = ((size_t)nreq * sizeof(double) + 255) & ~(size_t)255;
+  const size_t part_bytes = (size_t)nreq * ntiles * sizeof(double);
+  if (req_bytes + out_bytes + part_bytes > PIMC_NOALLOC_SCRATCH_BYTES)
+    return fail(PIMC_ERR_ARG, "action_short_range_noalloc: requests x particle tiles exceed the scratch array");
+  ReqDev* d_reqs = (ReqDev*)base;
+  double* d_out = (double*)(base + req_bytes);
+  double* d_part = (double*)(base + req_bytes + out_bytes);
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipMemcpyAsync(d_reqs, h, (size_t)nreq * sizeof(ReqDev), hipMemcpyHostToDevice, ctx->stream));
+  LaunchShortRange a{ctx->d_reps.as<ReplicaDev>(), ctx->d_fits.as<FitDev>(), d_reqs, nreq, ntiles, d_part, d_out};
+  launch_short_range(nt, precision == PIMC_MIXED, a, ctx->stream);   // every tile writes its partial sum
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, d_out, (size_t)nreq * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));   // h and the caller's buffers are read or written until here
   return PIMC_OK;
 }
 
