@@ -44,8 +44,13 @@ def hipcc() -> str:
 # of loop preheaders (machine LICM would hoist them into registers that stay live
 # across the whole kernel and cap the occupancy).
 FILE_FLAGS = {
-    "pimc_bisection.hip": ["-mllvm", "-disable-machine-licm"],
+    "pimc_bisection": ["-mllvm", "-disable-machine-licm"],   # every pimc_bisection*.hip
 }
+
+
+def file_flags(src: str) -> list[str]:
+    name = os.path.basename(src)
+    return [f for prefix, flags in FILE_FLAGS.items() if name.startswith(prefix) for f in flags]
 
 
 def sources() -> list[str]:
@@ -66,7 +71,7 @@ def _compile(src: str, incremental: bool = False) -> str:
     obj = os.path.join(OBJDIR, os.path.basename(src)[:-4] + ".o")
     if incremental and _up_to_date(obj, src):
         return obj
-    cmd = [hipcc(), *HIPCC_FLAGS, *FILE_FLAGS.get(os.path.basename(src), []), "-c", src, "-o", obj]
+    cmd = [hipcc(), *HIPCC_FLAGS, *file_flags(src), "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{r.stderr}")
