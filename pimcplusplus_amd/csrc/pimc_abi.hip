@@ -5,6 +5,7 @@
 // of pimc_actions.hip and pimc_bisection.hip on the context's stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -201,7 +202,7 @@ struct HostReplica {
   ReplicaDev dev{};
   DevBuf path0, path1, perm0, perm1;
   DevBuf joinbuf;                // the replica's join (one int, device-resident: the chain moves it)
-  DevBuf rho0, rho1, kidx_dev, ulong_k, lr_const, ulong;
+  DevBuf rho0, rho1, kidx_dev, ulong_k, lr_const, ulong, ulong32;
   DevBuf dulong, dulong_k, dlr_const;   // beta derivatives (energy estimator)
   DevBuf vlong, vlong_k, vlr_const;     // long-range potential (potential-energy estimator)
   unsigned dulong_mask = 0, dlr_mask = 0;
@@ -613,6 +614,35 @@ int pimc_fit_create_coulomb_bc(pimc_ctx* ctx, const pimc_coulomb_bc_fit_desc* d,
     HIP_TRY(hipMemcpy(f->dmem.p, dbc.data(), dbc.size() * sizeof(d4v), hipMemcpyHostToDevice));
     f->dev.dbc = f->dmem.as<d4v>();
   }
+  // The mixed path: each node's derivatives {dz/dx, dz/dy, d2z/dxdy} in FP32 (it evaluates a
+  // cell anchored on its first corner's FP64 value, so FP32 rounds the variation across the
+  // cell, not U), and per level the first q interval from which every cell has sum over its
+  // corners of |z - z_first| + h|dz/dx| + k|dz/dy| + hk|d2z/dxdy| within kBcMixedLimit: the
+  // cells below it, deep in the core, are evaluated from the FP64 nodes.  The bound is loose
+  // so that the lanes of a wave rarely split between the two evaluations.
+  std::vector<f4v> bc32(bc.size());
+  for (size_t i = 0; i < bc.size(); i++) bc32[i] = f4v{(float)bc[i].y, (float)bc[i].z, (float)bc[i].w, 0.0f};
+  std::vector<int> i32lev(nb, 0);
+  for (int b = 0; b < nb; b++) {
+    int first = 0;
+    for (int ix = 0; ix + 1 < nq; ix++) {
+      const double h = qg.x[ix + 1] - qg.x[ix];
+      bool ok = true;
+      for (int iy = 0; iy + 1 < nt && ok; iy++) {
+        const double k = tg.x[iy + 1] - tg.x[iy];
+        double m = 0.0;
+        const double z0 = bc[((size_t)b * nq + ix) * nt + iy].x;
+        for (int cx = 0; cx < 2; cx++)
+          for (int cy = 0; cy < 2; cy++) {
+            const d4v& P = bc[((size_t)b * nq + ix + cx) * nt + iy + cy];
+            m += fabs(P.x - z0) + h * fabs(P.y) + k * fabs(P.z) + h * k * fabs(P.w);
+          }
+        ok = m <= kBcMixedLimit;
+      }
+      if (!ok) first = ix + 1;
+    }
+    i32lev[b] = first;
+  }
   // interval records {X_i, h_i, 1/h_i, X_{i+1}} of both grids
   auto records = [](const HostGrid& g) {
     std::vector<d4v> r(g.n - 1);
@@ -626,7 +656,9 @@ int pimc_fit_create_coulomb_bc(pimc_ctx* ctx, const pimc_coulomb_bc_fit_desc* d,
   size_t bq = (size_t)nq * 8, bt = (size_t)nt * 8, bb = bc.size() * sizeof(d4v);
   size_t bqi = qiv.size() * sizeof(d4v), bti = tiv.size() * sizeof(d4v);
   size_t off_q = 0, off_t = (bq + 255) & ~(size_t)255, off_b = (off_t + bt + 255) & ~(size_t)255;
-  size_t off_qi = (off_b + bb + 255) & ~(size_t)255, off_ti = (off_qi + bqi + 255) & ~(size_t)255, total = off_ti + bti;
+  size_t off_qi = (off_b + bb + 255) & ~(size_t)255, off_ti = (off_qi + bqi + 255) & ~(size_t)255;
+  size_t off_b32 = (off_ti + bti + 255) & ~(size_t)255, off_il = (off_b32 + bc32.size() * sizeof(f4v) + 255) & ~(size_t)255;
+  size_t total = off_il + (size_t)nb * sizeof(int);
   if ((rc = f->mem.ensure(total))) return rc;
   unsigned char* base = f->mem.as<unsigned char>();
   HIP_TRY(hipMemcpy(base + off_q, qg.x.data(), bq, hipMemcpyHostToDevice));
@@ -634,8 +666,13 @@ int pimc_fit_create_coulomb_bc(pimc_ctx* ctx, const pimc_coulomb_bc_fit_desc* d,
   HIP_TRY(hipMemcpy(base + off_b, bc.data(), bb, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(base + off_qi, qiv.data(), bqi, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(base + off_ti, tiv.data(), bti, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(base + off_b32, bc32.data(), bc32.size() * sizeof(f4v), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(base + off_il, i32lev.data(), (size_t)nb * sizeof(int), hipMemcpyHostToDevice));
   FitDev& F = f->dev;
   F.kind = 1;
+  F.c32 = (const f4v*)(base + off_b32);
+  F.i32_lev = (const int*)(base + off_il);
+  F.i32 = *std::max_element(i32lev.begin(), i32lev.end());
   F.grid_type = qg.type;
   F.ng = nq;
   F.nint = nq - 1;
@@ -692,8 +729,24 @@ int pimc_replica_set_ulong(pimc_ctx* ctx, int32_t replica, int32_t sa, int32_t s
     R.ulong_deltainv = 1.0 / delta;
     R.ulong = H->ulong.as<double>();
   }
+  if (!R.ulong32) {
+    const size_t n32 = (size_t)R.ns * R.ns * num_levels * num_points;
+    int rc = H->ulong32.ensure(n32 * sizeof(UlongKnot32));
+    if (rc) return rc;
+    HIP_TRY(hipMemset(H->ulong32.p, 0, n32 * sizeof(UlongKnot32)));
+    R.ulong32 = H->ulong32.as<UlongKnot32>();
+  }
+  // the mixed path's knots: Y in FP64, B..F in FP32
+  std::vector<UlongKnot32> c32((size_t)num_levels * num_points);
+  for (size_t k = 0; k < c32.size(); k++) {
+    c32[k].y = coefs[6 * k];
+    for (int c = 0; c < 5; c++) c32[k].c[c] = (float)coefs[6 * k + 1 + c];
+    c32[k].pad = 0.0f;
+  }
   for (int pr : {sa * R.ns + sb, sb * R.ns + sa}) {
     HIP_TRY(hipMemcpy(H->ulong.as<double>() + pr * per_pair, coefs, per_pair * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(H->ulong32.as<UlongKnot32>() + (size_t)pr * num_levels * num_points, c32.data(),
+                      c32.size() * sizeof(UlongKnot32), hipMemcpyHostToDevice));
     R.ulong_mask |= 1u << pr;
   }
   if (dcoefs) {
@@ -1363,7 +1416,9 @@ int prepare_bisection(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, cons
     if ((1 << desc->num_levels) >= R.nslices) return fail(PIMC_ERR_ARG, "bisection: too few slices for num_levels");
     if (R.lr_on) {
       if (desc->lowest_level >= R.lr_levels) return fail(PIMC_ERR_ARG, "bisection: long-range tables lack the lowest level");
-      lr_bytes = std::max(lr_bytes, (size_t)2 * (2 * (R.maxk0 + R.maxk1 + R.maxk2) + 3) * sizeof(d2v));   // per slice
+      // per slice: the NEW and OLD phase rows, FP64 (16 B) or FP32 (8 B) entries
+      lr_bytes = std::max(lr_bytes, (size_t)2 * (2 * (R.maxk0 + R.maxk1 + R.maxk2) + 3) *
+                                        (precision == PIMC_MIXED ? sizeof(f2v) : sizeof(d2v)));
       has_lr = true;
     }
     if (desc->species < 0 || desc->species >= R.ns || H->dev.sp.count[desc->species] < 1)

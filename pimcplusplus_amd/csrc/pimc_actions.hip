@@ -18,7 +18,8 @@ __global__ void __launch_bounds__(kBlock) k_fit_eval_u(const FitDev* __restrict_
   if (i >= n) return;
   const int lev = level + F.taupos;
   if (F.kind == 1)
-    out[i] = pair_u_bc(F, qgrid_of(F), level, q[i], s2[i]);
+    out[i] = MIXED ? pair_u_bc_mixed(F, qgrid_of(F), level, F.i32_lev[level], q[i], s2[i])
+                   : pair_u_bc(F, qgrid_of(F), level, q[i], s2[i]);
   else if (MIXED)
     out[i] = pair_u<NT>(F, level_table32(F, lev), q[i], z[i], s2[i]);
   else
@@ -81,7 +82,7 @@ __global__ void __launch_bounds__(kBlock) k_short_range(const ReplicaDev* __rest
       while (sp1 + 1 < R.ns && p1 >= S.first[sp1 + 1]) sp1++;
       const FitDev F = fits[S.fit[sp1 * R.ns + sp2]];
       const int lev = Q.level + F.taupos;
-      const double* ul = ulong_for(R, F, sp1 * R.ns + sp2, Q.level);
+      const UlongView ul = ulong_for(R, F, sp1 * R.ns + sp2, Q.level);
       double ma[3], mb[3], pa[3], pb[3];
       load_bead(R, Q.mode, Q.slice1, p1, ma);
       load_bead(R, Q.mode, Q.slice1, j, pa);
@@ -89,8 +90,10 @@ __global__ void __launch_bounds__(kBlock) k_short_range(const ReplicaDev* __rest
         load_bead(R, Q.mode, slice + skip, p1, mb);
         load_bead(R, Q.mode, slice + skip, j, pb);
         double u;
-        if (F.kind == 1)
-          u = pair_link_u<NT>(R, F, BcView{qgrid_of(F), Q.level}, ma, mb, pa, pb, ul);
+        if (F.kind == 1 && MIXED)
+          u = pair_link_u<NT>(R, F, BcView32{{qgrid_of(F), Q.level, F.i32_lev[Q.level]}}, ma, mb, pa, pb, ul);
+        else if (F.kind == 1)
+          u = pair_link_u<NT>(R, F, BcView{qgrid_of(F), Q.level, 0}, ma, mb, pa, pb, ul);
         else if (MIXED)
           u = pair_link_u<NT>(R, F, level_table32(F, lev), ma, mb, pa, pb, ul);
         else

@@ -30,7 +30,7 @@ namespace pimc {
 // slice's terms are evaluated again for this link only.
 template <int NT, bool MIXED, bool BC, bool LIN, typename Table>
 __device__ inline double sr_link(const RepCore& R, const FitDev& F, const Table& T, const BcView& Q,
-                                 const double* __restrict__ ul, const double pbox[3], const double hsafe[3],
+                                 const UlongView& ul, const double pbox[3], const double hsafe[3],
                                  double urmin, const PIMC_LDS double* m, const double p[3], double dA[3], double& rA,
                                  double& uA, double& lA) {
   double dn[3], raw[3], e[3];
@@ -52,7 +52,7 @@ __device__ inline double sr_link(const RepCore& R, const FitDev& F, const Table&
   const double rn = MIXED ? sqrt_seeded(n2) : sqrt_dist(n2);
   double un = 0.0, ln = 0.0;
   if constexpr (!BC) un = slice_u00<LIN>(F, T, rn);
-  if (ul) ln = ulong_eval(R, ul, rn);
+  if (ul.t64) ln = ulong_at<MIXED>(R, ul, rn);
   double db[3] = {dn[0], dn[1], dn[2]};
   double rb = rn, ub = un, lb = ln;
   if (!same) {
@@ -66,19 +66,19 @@ __device__ inline double sr_link(const RepCore& R, const FitDev& F, const Table&
     const double b2 = db[0] * db[0] + db[1] * db[1] + db[2] * db[2];
     rb = MIXED ? sqrt_seeded(b2) : sqrt_dist(b2);
     if constexpr (!BC) ub = slice_u00<LIN>(F, T, rb);
-    if (ul) lb = ulong_eval(R, ul, rb);
+    if (ul.t64) lb = ulong_at<MIXED>(R, ul, rb);
   }
   const double s2 = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
   const double q = 0.5 * (rA + rb);
   const double z = rA - rb;
   double U;
   if constexpr (BC) {
-    U = pair_u_bc(F, Q.q, Q.level, q, s2);
+    U = MIXED ? pair_u_bc_mixed(F, Q.q, Q.level, Q.i32, q, s2) : pair_u_bc(F, Q.q, Q.level, q, s2);
   } else {
     const double rmin = F.start;
     U = (rA < rmin || rb < rmin) ? urmin : 0.5 * (uA + ub) + link_offdiag<NT, LIN>(F, T, q, z, s2);
   }
-  if (ul) U -= 0.5 * (lA + lb);
+  if (ul.t64) U -= 0.5 * (lA + lb);
 #pragma unroll
   for (int d = 0; d < 3; d++) dA[d] = dn[d];
   rA = rn;
@@ -96,7 +96,7 @@ __device__ inline double sr_link(const RepCore& R, const FitDev& F, const Table&
 // consecutive partners, so bead loads coalesce.
 template <int NT, bool MIXED, bool BC, bool LIN, typename Table>
 __device__ inline void sr_partners(const RepCore& R, const FitDev& F, const Table& T, const BcView& Q,
-                                   const double* ul, int j0, int j1, int ptcl, int slice1, int K, int step, int ls,
+                                   const UlongView ul, int j0, int j1, int ptcl, int slice1, int K, int step, int ls,
                                    const PIMC_LDS double* mold, const PIMC_LDS double* mnew, Kahan& acc_old,
                                    Kahan& acc_new) {
   const size_t row = (size_t)3 * R.N * step;     // distance between consecutive link slices
@@ -155,9 +155,9 @@ __device__ inline void sr_partners(const RepCore& R, const FitDev& F, const Tabl
       uO = slice_u00<LIN>(F, T, rO);
       uN = slice_u00<LIN>(F, T, rN);
     }
-    if (ul) {
-      lO = ulong_eval(R, ul, rO);
-      lN = ulong_eval(R, ul, rN);
+    if (ul.t64) {
+      lO = ulong_at<MIXED>(R, ul, rO);
+      lN = ulong_at<MIXED>(R, ul, rN);
     }
     pj += 2 * row;
     for (int k = kbeg; k < kend; k += 2) {
@@ -288,7 +288,7 @@ __host__ __device__ inline size_t bisection_ctl_offset(const LaunchBisection& a)
 // stride.  One lane per (slice, copy, dimension).
 template <typename LrT>
 __device__ inline void phase_rows_chunk(const RepCore& R, const PIMC_LDS double* mnew, const PIMC_LDS double* mold,
-                                        int c0, int nc, int step, int stride, PIMC_LDS d2v* lrC) {
+                                        int c0, int nc, int step, int stride, PIMC_LDS PhaseRow<LrT>* lrC) {
   for (int i = threadIdx.x; i < nc * 6; i += blockDim.x) {
     const int j = i / 6, c = (i % 6) / 3, d = i % 3;
     const PIMC_LDS double* src = (c == 0 ? mnew : mold) + 3 * (c0 + j) * step;
@@ -412,14 +412,14 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
   PIMC_LDS int* ctl = (PIMC_LDS int*)(scratch + 2 * kMaxWaves);           // slice1, ptcl, accept, item
   PIMC_LDS double* gauss = (PIMC_LDS double*)(ctl + 4);                   // [gauss_offset(W, L)]
   PIMC_LDS double* logu_tab = gauss + gauss_offset(W, L);                 // [PIMC_MAX_LEVELS]
-  PIMC_LDS d2v* lrC = (PIMC_LDS d2v*)(logu_tab + PIMC_MAX_LEVELS);        // [lr_chunk][2][kstride]: NEW, OLD rows
+  typedef typename std::conditional<MIXED, float, double>::type LrT;   // phase-sum arithmetic
+  PIMC_LDS PhaseRow<LrT>* lrC = (PIMC_LDS PhaseRow<LrT>*)(logu_tab + PIMC_MAX_LEVELS);   // [lr_chunk][2][kstride]: NEW, OLD rows
 
   int64_t st_moves = 0, st_acc = 0, st_stages = 0, st_evals = 0, st_kevals = 0;
   int nbar = 0;   // cluster barriers passed (the same count on every member)
   // this member's k-vectors [kb, ke) of the long-range sums and updates
   int kb = 0, ke = R.nk;
   if constexpr (CL) cluster_share(0, R.nk, member, a.cluster, kb, ke);
-  typedef typename std::conditional<MIXED, float, double>::type LrT;   // phase-sum arithmetic
   const int numLeft = R.nslices - W;
   const double lambda = S.lambda[sp];
   // PathDataClass::Join, moved to slice2 at the start of every move (BisectionBlock.cc:424).
@@ -510,12 +510,12 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
       Kahan acc_old, acc_new;
       for (int s2 = 0; s2 < R.ns; s2++) {
         const FitDev F = load_uniform(&a.fits[S.fit[sp * R.ns + s2]]);
-        const double* ul = ulong_for(R, F, sp * R.ns + s2, level);
+        const UlongView ul = ulong_for(R, F, sp * R.ns + s2, level);
         int j0 = S.first[s2], j1 = S.first[s2] + S.count[s2];
         if constexpr (CL) cluster_share(j0, j1, member, a.cluster, j0, j1);
         if constexpr (HAS_BC) {
           if (F.kind == 1) {
-            const BcView Q{qgrid_of(F), level};
+            const BcView Q{qgrid_of(F), level, MIXED ? F.i32_lev[level] : 0};
             sr_partners<NT, MIXED, true, false>(R, F, LevelTable64<true>{}, Q, ul, j0, j1, ptcl, slice1, K, step, a.link_split, mold,
                                          mnew, acc_old, acc_new);
             continue;
@@ -529,7 +529,7 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
             return stage_level64<LIN>(F, level + F.taupos, tbl);
         }();
         __syncthreads();
-        sr_partners<NT, MIXED, false, LIN>(R, F, T, BcView{}, nullptr, j0, j1, ptcl, slice1, K, step, a.link_split, mold, mnew,
+        sr_partners<NT, MIXED, false, LIN>(R, F, T, BcView{}, UlongView{nullptr, nullptr}, j0, j1, ptcl, slice1, K, step, a.link_split, mold, mnew,
                                       acc_old, acc_new);
       }
       double sr_old, sr_new;

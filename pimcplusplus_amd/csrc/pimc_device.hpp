@@ -20,6 +20,7 @@ constexpr int kMaxActive = 4;
 // Plain vector types for table data (they load from LDS as single ds_read_b128s).
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef double d2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
 typedef double d4v __attribute__((ext_vector_type(4)));
 
 #define PIMC_LDS __attribute__((address_space(3)))
@@ -94,6 +95,9 @@ struct FitDev {
   double z1z2;
   // kind 0: the FP32 rows of intervals i >= i32 are valid (not NaN) at every
   // level; the mixed loops evaluate the FP64 rows of intervals below i32_lev[level]
+  // kind 1: c32 holds each node's {dz/dx, dz/dy, d2z/dxdy, 0} in FP32 ([num_tau][ng][nt]
+  // f4v), and the mixed path evaluates the cells of q intervals below i32_lev[level] from
+  // the FP64 nodes
   int i32;
   const int* i32_lev;     // [num_tau] the same bound for each tau level alone
   // intervals whose FP64 rows a mixed stage copies into LDS beside the FP32 rows:
@@ -114,6 +118,21 @@ constexpr int kFp64StageBytes = PIMC_FP64_STAGE_BYTES;
 #define PIMC_MIXED_LIMIT 0.35
 #endif
 constexpr double kMixedLimit = PIMC_MIXED_LIMIT;
+
+// Largest variation across a Coulomb bicubic cell (sum over its corners of |z - z_first| +
+// h|dz/dx| + k|dz/dy| + hk|d2z/dxdy|) the mixed path evaluates in FP32, anchored on the
+// cell's first FP64 node value.
+#ifndef PIMC_BC_MIXED_LIMIT
+#define PIMC_BC_MIXED_LIMIT 4.0
+#endif
+constexpr double kBcMixedLimit = PIMC_BC_MIXED_LIMIT;
+
+// A knot of a quintic Ulong table for the mixed path: Y in FP64, then B, C, D, E, F in FP32.
+struct __attribute__((aligned(32))) UlongKnot32 {
+  double y;
+  float c[5];
+  float pad;
+};
 
 // Scalars of a replica the inner loops use; kernels hold a block-uniform copy
 // (scalar registers).  Arrays are only indexed with compile-time constants.
@@ -150,6 +169,7 @@ struct RepCore {
   unsigned ulong_mask;     // bit sa*ns+sb set where the pair has Ulong tables
   double ulong_start, ulong_delta, ulong_deltainv;
   const double* ulong;     // [ns*ns][ulong_levels][ulong_n][6]
+  const struct UlongKnot32* ulong32;   // the same knots for the mixed path, [ns*ns][ulong_levels][ulong_n]
   const double* dulong;    // dUlong, same layout, or nullptr
   const double* dulong_k;  // [ns*ns][lr_levels][nk] dUlong_k, or nullptr
   const double* dlr_const; // [ns*ns][3][lr_levels]: dUlong_r0, dUshort_k0, dUlong_k0
@@ -333,14 +353,24 @@ __device__ inline int grid_iv(const GridDev& G, double x, d4v& rec) {
 // evaluation (BicubicSpline.h:266-300, 356-413); the table is read from global
 // memory.  The local coordinates are (x - X_i) times the stored 1/h_i.
 // --------------------------------------------------------------------------
-__device__ inline double bicubic(const FitDev& F, const GridDev& Q, const d4v* __restrict__ tab, double x,
-                                 double y) {
+// The cell of (x, y): interval indices and records on the q and t grids.
+struct BcCell {
+  int ix, iy;
   d4v qr, tr;
-  const int ix = grid_iv(Q, x, qr);
-  const int iy = grid_iv(F.tgrid, y, tr);
-  const double h = qr.y, k = tr.y;
-  const double u = (x - qr.x) * qr.z;
-  const double v = (y - tr.x) * tr.z;
+};
+__device__ inline BcCell bc_cell(const FitDev& F, const GridDev& Q, double x, double y) {
+  BcCell c;
+  c.ix = grid_iv(Q, x, c.qr);
+  c.iy = grid_iv(F.tgrid, y, c.tr);
+  return c;
+}
+
+__device__ inline double bicubic_at(const FitDev& F, const d4v* __restrict__ tab, const BcCell& c, double x,
+                                    double y) {
+  const int ix = c.ix, iy = c.iy;
+  const double h = c.qr.y, k = c.tr.y;
+  const double u = (x - c.qr.x) * c.qr.z;
+  const double v = (y - c.tr.x) * c.tr.z;
   const double a0 = (u - 1.0) * (u - 1.0) * (1.0 + 2.0 * u), a1 = u * u * (3.0 - 2.0 * u);
   const double a2 = h * (u * (u - 1.0) * (u - 1.0)), a3 = h * (u * u * (u - 1.0));
   const double b0 = (v - 1.0) * (v - 1.0) * (1.0 + 2.0 * v), b1 = v * v * (3.0 - 2.0 * v);
@@ -351,6 +381,38 @@ __device__ inline double bicubic(const FitDev& F, const GridDev& Q, const d4v* _
   // BCpoint = {z, dzdx, dzdy, d2zdxdy}
   return a0 * (A.x * b0 + B.x * b1 + A.z * b2 + B.z * b3) + a1 * (C.x * b0 + D.x * b1 + C.z * b2 + D.z * b3) +
          a2 * (A.y * b0 + B.y * b1 + A.w * b2 + B.w * b3) + a3 * (C.y * b0 + D.y * b1 + C.w * b2 + D.w * b3);
+}
+
+__device__ inline double bicubic(const FitDev& F, const GridDev& Q, const d4v* __restrict__ tab, double x,
+                                 double y) {
+  return bicubic_at(F, tab, bc_cell(F, Q, x, y), x, y);
+}
+
+// The same cell in the mixed path, anchored on its first corner: with A the node value
+// z at (ix, iy), U = A + sum_ij a_i(u) b_j(v) (z_ij - A) + (the derivative terms), which is
+// the same interpolant because a0 + a1 = b0 + b1 = 1.  The corner differences are formed
+// in FP64 from the FP64 nodes; the basis and the sum run in FP32 on the differences and on
+// the FP32 node derivatives tab32 = {dz/dx, dz/dy, d2z/dxdy, 0}, so the FP32 rounding is
+// relative to the variation across the cell, not to U.
+__device__ inline double bicubic32_at(const FitDev& F, const d4v* __restrict__ tab, const f4v* __restrict__ tab32,
+                                      const BcCell& c, double x, double y) {
+  const int ix = c.ix, iy = c.iy;
+  const float h = (float)c.qr.y, k = (float)c.tr.y;
+  const float u = (float)((x - c.qr.x) * c.qr.z);
+  const float v = (float)((y - c.tr.x) * c.tr.z);
+  const float a0 = (u - 1.0f) * (u - 1.0f) * (1.0f + 2.0f * u), a1 = u * u * (3.0f - 2.0f * u);
+  const float a2 = h * (u * (u - 1.0f) * (u - 1.0f)), a3 = h * (u * u * (u - 1.0f));
+  const float b0 = (v - 1.0f) * (v - 1.0f) * (1.0f + 2.0f * v), b1 = v * v * (3.0f - 2.0f * v);
+  const float b2 = k * (v * (v - 1.0f) * (v - 1.0f)), b3 = k * (v * v * (v - 1.0f));
+  const size_t iA = (size_t)ix * F.nt + iy, iC = (size_t)(ix + 1) * F.nt + iy;
+  const PIMC_GLOBAL double* z = (const PIMC_GLOBAL double*)tab;   // z is the first field of a BCpoint
+  const double zA = z[4 * iA], zB = z[4 * (iA + 1)], zC = z[4 * iC], zD = z[4 * (iC + 1)];
+  const float dB = (float)(zB - zA), dC = (float)(zC - zA), dD = (float)(zD - zA);
+  const PIMC_GLOBAL f4v* g = (const PIMC_GLOBAL f4v*)tab32;
+  const f4v A = g[iA], B = g[iA + 1], C = g[iC], D = g[iC + 1];
+  const float s = a0 * (dB * b1 + A.y * b2 + B.y * b3) + a1 * (dC * b0 + dD * b1 + C.y * b2 + D.y * b3) +
+                  a2 * (A.x * b0 + B.x * b1 + A.z * b2 + B.z * b3) + a3 * (C.x * b0 + D.x * b1 + C.z * b2 + D.z * b3);
+  return zA + (double)s;
 }
 
 __device__ inline double pair_u_bc(const FitDev& F, const GridDev& Q, int level, double q, double s2) {
@@ -365,8 +427,27 @@ __device__ inline double pair_u_bc(const FitDev& F, const GridDev& Q, int level,
   return beta * (F.z1z2 / q);
 }
 
+// Mixed-precision PAcoulombBCFitClass::U: the cells of q intervals at and above i32 (the
+// level's bound) in FP32; in FP64 the cells below it (near the core), q below the first
+// knot (an extrapolation, where the Hermite basis grows large and cancels) and the
+// Coulomb tail.
+__device__ inline double pair_u_bc_mixed(const FitDev& F, const GridDev& Q, int level, int i32, double q,
+                                         double s2) {
+  if (q <= (F.end * 1.0000001)) {
+    const double x = q, y = q == 0 ? 0.0 : sqrt(s2) / (2.0 * q);
+    const BcCell c = bc_cell(F, Q, x, y);
+    const size_t lev = (size_t)level * F.ng * F.nt;
+    return (c.ix < i32 || x < c.qr.x) ? bicubic_at(F, F.bc + lev, c, x, y)
+                                      : bicubic32_at(F, F.bc + lev, (const f4v*)F.c32 + lev, c, x, y);
+  }
+  double beta = F.smallest_beta;
+  for (int i = 0; i < level; i++) beta *= 2.0;
+  return beta * (F.z1z2 / q);
+}
+
 // QuinticSpline::operator() (QuinticSpline.h:172-218) of the replica's Ulong table.
-__device__ inline double ulong_eval(const RepCore& R, const double* __restrict__ tab, double x) {
+// The knot I of x and P = x - X_I.
+__device__ inline int ulong_knot(const RepCore& R, double x, double& P) {
   int J;
   {
 #pragma clang fp contract(off)
@@ -386,9 +467,24 @@ __device__ inline double ulong_eval(const RepCore& R, const double* __restrict__
 #pragma clang fp contract(off)
     X = R.ulong_start + (double)I * R.ulong_delta;
   }
-  const double P = (x - X);
+  P = (x - X);
+  return I;
+}
+__device__ inline double ulong_eval(const RepCore& R, const double* __restrict__ tab, double x) {
+  double P;
+  const int I = ulong_knot(R, x, P);
   const PIMC_GLOBAL double* k = (const PIMC_GLOBAL double*)tab + (size_t)I * 6;
   return ((((k[5] * P + k[4]) * P + k[3]) * P + k[2]) * P + k[1]) * P + k[0];
+}
+// Mixed path: the knot value Y in FP64 plus the rest of the same Horner form,
+// (((F P + E) P + D) P + C) P + B) P, in FP32 (P is at most one grid spacing, so this part
+// is small next to Y); one 32-byte record per knot.
+__device__ inline double ulong_eval32(const RepCore& R, const UlongKnot32* __restrict__ tab32, double x) {
+  double P;
+  const int I = ulong_knot(R, x, P);
+  const PIMC_GLOBAL UlongKnot32* k = (const PIMC_GLOBAL UlongKnot32*)tab32 + I;
+  const float p = (float)P;
+  return k->y + (double)(((((k->c[4] * p + k->c[3]) * p + k->c[2]) * p + k->c[1]) * p + k->c[0]) * p);
 }
 
 // --------------------------------------------------------------------------

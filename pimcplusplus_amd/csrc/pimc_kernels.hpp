@@ -114,15 +114,23 @@ __device__ inline double pair_u(const FitDev& F, const LevelTable32<InLds>& T, d
   return pair_u_mixed<NT>(F, T, q, z, s2);
 }
 
-// A level of a PAcoulombBCFitClass fit (table in global memory).
+// A level of a PAcoulombBCFitClass fit (tables in global memory); i32 is the level's bound
+// below which the mixed path evaluates cells from the FP64 nodes.
 struct BcView {
   GridDev q;
   int level;
+  int i32;
 };
+// The same level evaluated by the mixed path.
+struct BcView32 : BcView {};
 __device__ inline GridDev qgrid_of(const FitDev& F) { return F.qgrid; }
 template <int NT>
 __device__ inline double pair_u(const FitDev& F, const BcView& T, double q, double z, double s2) {
   return pair_u_bc(F, T.q, T.level, q, s2);
+}
+template <int NT>
+__device__ inline double pair_u(const FitDev& F, const BcView32& T, double q, double z, double s2) {
+  return pair_u_bc_mixed(F, T.q, T.level, T.i32, q, s2);
 }
 
 template <typename Table>
@@ -133,6 +141,24 @@ template <bool InLds>
 struct IsMixed<LevelTable32<InLds>> {
   static constexpr bool value = true;
 };
+template <>
+struct IsMixed<BcView32> {
+  static constexpr bool value = true;
+};
+
+// The Ulong tables of one species pair and level: the FP64 knots and their B..F in FP32
+// (mixed path); t64 is nullptr where ShortRange subtracts no Ulong.
+struct UlongView {
+  const double* t64;
+  const UlongKnot32* t32;
+};
+template <bool MIXED>
+__device__ inline double ulong_at(const RepCore& R, const UlongView& u, double x) {
+  if constexpr (MIXED)
+    return ulong_eval32(R, u.t32, x);
+  else
+    return ulong_eval(R, u.t64, x);
+}
 
 // Bytes of LDS one staged level of fit F takes: geo + coefficients + grid, or the
 // coefficients alone when the lookups use lin_low (lin); a mixed stage adds the FP64
@@ -212,7 +238,7 @@ __device__ inline LevelTable32<true> stage_level32(const FitDev& F, int lev, PIM
 template <int NT, typename Table>
 __device__ inline double pair_link_u(const RepCore& R, const FitDev& F, const Table& T, const double ma[3],
                                      const double mb[3], const double pa[3], const double pb[3],
-                                     const double* __restrict__ ulong_tab = nullptr) {
+                                     const UlongView ul = UlongView{nullptr, nullptr}) {
   constexpr bool kMixed = IsMixed<Table>::value;
   double da[3], db[3];
 #pragma unroll
@@ -230,7 +256,7 @@ __device__ inline double pair_link_u(const RepCore& R, const FitDev& F, const Ta
   double z = ra - rb;
   double U = pair_u<NT>(F, T, q, z, s2);
   // long-ranged fits: subtract the part the k-space action carries (ShortRangeClass.cc:165-166)
-  if (ulong_tab) U -= 0.5 * (ulong_eval(R, ulong_tab, ra) + ulong_eval(R, ulong_tab, rb));
+  if (ul.t64) U -= 0.5 * (ulong_at<kMixed>(R, ul, ra) + ulong_at<kMixed>(R, ul, rb));
   return U;
 }
 
@@ -364,10 +390,11 @@ __device__ inline double cons_image_1(double da, double b, double box, double bo
   return b + m * isper * box;
 }
 
-// Ulong table of pair `pair` at `level` when ShortRange must subtract it, else nullptr.
-__device__ inline const double* ulong_for(const RepCore& R, const FitDev& F, int pair, int level) {
-  if (F.kind != 1 || !R.use_long_range) return nullptr;
-  return R.ulong + ((size_t)pair * R.ulong_levels + level) * R.ulong_n * 6;
+// Ulong tables of pair `pair` at `level` when ShortRange must subtract them, else {nullptr, nullptr}.
+__device__ inline UlongView ulong_for(const RepCore& R, const FitDev& F, int pair, int level) {
+  if (F.kind != 1 || !R.use_long_range) return UlongView{nullptr, nullptr};
+  const size_t knot = ((size_t)pair * R.ulong_levels + level) * R.ulong_n;
+  return UlongView{R.ulong + knot * 6, R.ulong32 + knot};
 }
 
 __device__ inline void load_bead(const RepCore& R, int mode, int slice, int p, double out[3]) {
@@ -418,11 +445,15 @@ __device__ inline int koff_of(const RepCore& R, int d) {
   return (2 * R.maxk0 + 1) * (d >= 1) + (2 * R.maxk1 + 1) * (d == 2);
 }
 
+// A phase-table entry (re, im) in the arithmetic of the phase sums: FP64 (d2v) or FP32 (f2v).
+template <typename T>
+using PhaseRow = typename std::conditional<sizeof(T) == 8, d2v, f2v>::type;
+
 // cos and sin of phi = r k_box are taken as cospi/sinpi of phi/pi: |phi| stays
 // within a few pi here, and the pi-scaled forms need no long argument reduction
 // (whose tables and registers would otherwise be live inside the chain kernel).
 template <typename T>
-__device__ inline void phase_row(double phi, int maxk, PIMC_LDS d2v* C) {
+__device__ inline void phase_row(double phi, int maxk, PIMC_LDS PhaseRow<T>* C) {
   T c, s;
   constexpr double kInvPi = 0.31830988618379067154;
   if constexpr (sizeof(T) == 8) {
@@ -431,14 +462,14 @@ __device__ inline void phase_row(double phi, int maxk, PIMC_LDS d2v* C) {
     sincospif((float)(phi * kInvPi), &s, &c);
   }
   T re = 1, im = 0;
-  C[maxk] = d2v{1.0, 0.0};
+  C[maxk] = PhaseRow<T>{1, 0};
   for (int n = 1; n <= maxk; n++) {
     T nre = c * re - s * im;   // tempC * C(n-1)
     T nim = c * im + s * re;
     re = nre;
     im = nim;
-    C[maxk + n] = d2v{(double)re, (double)im};
-    C[maxk - n] = d2v{(double)re, -(double)im};
+    C[maxk + n] = PhaseRow<T>{re, im};
+    C[maxk - n] = PhaseRow<T>{re, -im};
   }
 }
 
@@ -446,19 +477,18 @@ __device__ inline void phase_row(double phi, int maxk, PIMC_LDS d2v* C) {
 // d is a per-lane value: the dimension's fields are combined arithmetically, not indexed
 // or selected (either would move the register-resident R to scratch).
 template <typename T>
-__device__ inline void phase_rows_lane(const RepCore& R, const double r[3], int d, PIMC_LDS d2v* C) {
+__device__ inline void phase_rows_lane(const RepCore& R, const double r[3], int d, PIMC_LDS PhaseRow<T>* C) {
   const double x = d == 0 ? r[0] : (d == 1 ? r[1] : r[2]);
   const double kb = R.kbox[0] * (double)(d == 0) + R.kbox[1] * (double)(d == 1) + R.kbox[2] * (double)(d == 2);
   phase_row<T>(x * kb, kmax_of(R, d), C + koff_of(R, d));
 }
 
 template <typename T>
-__device__ inline void kterm(const PIMC_LDS d2v* C, int o1, int o2, const int* ki, T& re, T& im) {
-  const d2v a = C[ki[0]], b = C[o1 + ki[1]], c = C[o2 + ki[2]];
-  const T ar = (T)a.x, ai = (T)a.y, br = (T)b.x, bi = (T)b.y, cr = (T)c.x, ci = (T)c.y;
-  const T abr = ar * br - ai * bi, abi = ar * bi + ai * br;   // C0 * C1
-  re = abr * cr - abi * ci;                                   // (C0 * C1) * C2
-  im = abr * ci + abi * cr;
+__device__ inline void kterm(const PIMC_LDS PhaseRow<T>* C, int o1, int o2, const int* ki, T& re, T& im) {
+  const PhaseRow<T> a = C[ki[0]], b = C[o1 + ki[1]], c = C[o2 + ki[2]];
+  const T abr = a.x * b.x - a.y * b.y, abi = a.x * b.y + a.y * b.x;   // C0 * C1
+  re = abr * c.x - abi * c.y;                                         // (C0 * C1) * C2
+  im = abr * c.y + abi * c.x;
 }
 
 // The k-independent terms (Madelung, background, k=0) of one slice, added to

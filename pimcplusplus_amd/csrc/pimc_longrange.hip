@@ -87,18 +87,18 @@ template <bool MIXED>
 __global__ void __launch_bounds__(kBlock) k_lr_req(const ReplicaDev* __restrict__ reps, const ReqDev* __restrict__ reqs,
                                                    int r, double* __restrict__ out) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  PIMC_LDS d2v* Cn = (PIMC_LDS d2v*)smem_raw;
+  typedef typename std::conditional<MIXED, float, double>::type PT;
+  PIMC_LDS PhaseRow<PT>* Cn = (PIMC_LDS PhaseRow<PT>*)smem_raw;
   const ReqDev Q = reqs[r];
   const PIMC_GLOBAL int* qp = (const PIMC_GLOBAL int*)reqs[r].ptcls;   // indexed at run time: not from Q
   const ReplicaDev* RD = reps + Q.replica;
   const RepCore R = load_uniform(&RD->core);
   const RepSpecies& S = RD->sp;
   const int stride = kstride(R);
-  PIMC_LDS d2v* Co = Cn + stride;
-  PIMC_LDS double* scratch = (PIMC_LDS double*)(Co + stride);
+  PIMC_LDS PhaseRow<PT>* Co = Cn + stride;
+  PIMC_LDS double* scratch = (PIMC_LDS double*)(Co + stride);   // 8-byte aligned for either entry size
   const int o1 = koff_of(R, 1), o2 = koff_of(R, 2);
   const int skip = 1 << Q.level;
-  typedef typename std::conditional<MIXED, float, double>::type PT;
 
   if (Q.mode == 1) {
     // NEW <- OLD over the range, then add the moved particles' new phases and remove the old ones

@@ -1,6 +1,11 @@
 """GPU parity of the Coulomb path: PAcoulombBCFitClass::U (bicubic), the Ulong subtraction of
 ShortRangeClass::SingleAction, and bisection chains of the hydrogen plasma and the
-one-component plasma (short range + Ewald long range) against the oracle."""
+one-component plasma (short range + Ewald long range) against the oracle.
+
+FP64 path: 1e-12 relative.  Mixed path (FP32 bicubic cells away from the core, FP32
+Horner tail of the quintic Ulong): the contract is |dS| <= 1e-6 per stage; single
+evaluations are checked to 2e-6 (1 + |U|) and must differ from FP64 somewhere (the FP32
+path ran)."""
 
 import math
 
@@ -36,10 +41,22 @@ def test_fit_eval_u_coulomb_bc(oracle, gpu, qtype):
     z = rng.uniform(-1, 1, n)
     for lev in range(3):
         want = oracle.fit_eval_u(fo, lev, q, z, s2)
-        for prec in (FP64, MIXED):          # the bicubic is FP64 on both paths
-            got = gpu.fit_eval_u(fg, lev, q, z, s2, prec)
-            # contracted (FMA) Hermite sums: ~1e-13 where q=0 extrapolates below the grid
-            assert np.max(np.abs(got - want) / (1 + np.abs(want))) < 1e-12
+        got = gpu.fit_eval_u(fg, lev, q, z, s2, FP64)
+        # contracted (FMA) Hermite sums: ~1e-13 where q=0 extrapolates below the grid
+        assert np.max(np.abs(got - want) / (1 + np.abs(want))) < 1e-12
+        mixed = gpu.fit_eval_u(fg, lev, q, z, s2, MIXED)
+        assert np.max(np.abs(mixed - want) / (1 + np.abs(want))) < 2e-6
+    # a table small enough for FP32 cells: the mixed path evaluates them in FP32
+    small = 0.02 * umat
+    fo = oracle.fit_create_coulomb_bc(qg, tg, small, 0.125, -1.0)
+    fg = gpu.fit_create_coulomb_bc(qg, tg, small, 0.125, -1.0)
+    for lev in range(3):
+        want = oracle.fit_eval_u(fo, lev, q, z, s2)
+        got = gpu.fit_eval_u(fg, lev, q, z, s2, FP64)
+        assert np.max(np.abs(got - want) / (1 + np.abs(want))) < 1e-12
+        mixed = gpu.fit_eval_u(fg, lev, q, z, s2, MIXED)
+        assert np.max(np.abs(mixed - want)) < 2e-7
+        assert np.mean(mixed != got) > 0.5
 
 
 def test_fit_eval_u_testcoul_resampled(oracle, gpu):
@@ -80,7 +97,8 @@ def test_short_range_with_ulong_requests(oracle, gpu):
     got = gpu.action_short_range([(ig[x[0]],) + x[1:] for x in reqs], FP64)
     assert np.max(np.abs(got - want) / (1 + np.abs(want))) < 1e-12
     mixed = gpu.action_short_range([(ig[x[0]],) + x[1:] for x in reqs], MIXED)
-    assert np.max(np.abs(mixed - want) / (1 + np.abs(want))) < 1e-12
+    assert np.max(np.abs(mixed - want)) < 1e-6
+    assert np.any(mixed != got)
 
 
 def test_bisection_hydrogen_fp64(oracle, gpu):
@@ -93,6 +111,28 @@ def test_bisection_hydrogen_fp64(oracle, gpu):
     for a, b in zip(io, ig):
         assert np.max(np.abs(oracle.rho_k(a) - gpu.rho_k(b))) < 1e-10
         assert np.max(np.abs(oracle.path_get(a) - gpu.path_get(b))) < 1e-12
+
+
+def test_bisection_hydrogen_mixed_delta_s(oracle, gpu):
+    sysm = workloads.hydrogen_system(_testcoul(), num_replicas=2, n_pairs=16, m=64, num_levels=4)
+    io, ig = load_both(oracle, gpu, sysm)
+    ro, _ = oracle.bisection_run(io, sysm.num_levels, sysm.seed, 0, 300)
+    rm, _ = gpu.bisection_run(ig, sysm.num_levels, sysm.seed, 0, 300, MIXED)
+    rg = workloads.load(gpu, sysm)
+    r64, _ = gpu.bisection_run(rg, sysm.num_levels, sysm.seed, 0, 300, FP64)
+    worst, n = 0.0, 0
+    for r in range(2):
+        for m in range(300):
+            ns = ro["num_stages"][r, m]
+            if ns != rm["num_stages"][r, m] or ro["accepted"][r, m] != rm["accepted"][r, m]:
+                break
+            d = (ro["new_action"][r, m, :ns] - ro["old_action"][r, m, :ns]) - \
+                (rm["new_action"][r, m, :ns] - rm["old_action"][r, m, :ns])
+            worst = max(worst, float(np.max(np.abs(d))))
+            n += ns
+    print(f"hydrogen mixed: max |dS error| {worst:.3g} over {n} stages", flush=True)
+    assert n > 300 and worst <= 1e-6
+    assert not np.array_equal(rm["new_action"], r64["new_action"])
 
 
 def test_bisection_ocp_background_fp64_and_mixed(oracle, gpu):

@@ -47,6 +47,7 @@ def test_config2_hydrogen_full_size(oracle, gpu):
     gm = workloads.load(gpu, sysm)
     rm, _ = gpu.bisection_run(gm, sysm.num_levels, sysm.seed, 0, 150, abi.MIXED)
     worst, n = _mixed_delta_s_error(ro, rm, 2, 150)
+    print(f"config2 mixed: max |dS error| {worst:.3g} over {n} stages", flush=True)
     assert n > 200 and worst <= 1e-6
 
 
@@ -57,13 +58,20 @@ def test_config4_ocp_full_size(oracle, gpu, cluster):
     io = workloads.load(oracle, sysm)
     gpu.set_plan(256, 4 if cluster > 1 else 1, False, cluster)
     ig = workloads.load(gpu, sysm)
-    ro, so = oracle.bisection_run(io, sysm.num_levels, sysm.seed, 0, 12)
-    rg, sg = gpu.bisection_run(ig, sysm.num_levels, sysm.seed, 0, 12, abi.FP64)
+    nmoves = 150
+    ro, so = oracle.bisection_run(io, sysm.num_levels, sysm.seed, 0, nmoves)
+    rg, sg = gpu.bisection_run(ig, sysm.num_levels, sysm.seed, 0, nmoves, abi.FP64)
     compare_chains(ro, rg)
     assert (so.accepted, so.k_evals) == (sg.accepted, sg.k_evals)
     for a, b in zip(io, ig):
         assert np.max(np.abs(oracle.path_get(a) - gpu.path_get(b))) < 1e-12
         assert np.max(np.abs(oracle.rho_k(a) - gpu.rho_k(b))) < 1e-9
+    # the mixed path at this size (FP32 bicubic, Ulong tail and k-space phase sums)
+    gm = workloads.load(gpu, sysm)
+    rm, _ = gpu.bisection_run(gm, sysm.num_levels, sysm.seed, 0, nmoves, abi.MIXED)
+    worst, n = _mixed_delta_s_error(ro, rm, 2, nmoves)
+    print(f"config4 mixed: max |dS error| {worst:.3g} over {n} stages", flush=True)
+    assert n > 200 and worst <= 1e-6
 
 
 def test_config3_uneven_full_size(oracle, gpu):
