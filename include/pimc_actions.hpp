@@ -17,7 +17,8 @@
 //     global, so two paths can be evaluated side by side;
 //   - errors throw pimc::Error with the library's message where the reference
 //     asserts or aborts;
-//   - d_dBeta is not provided by this library yet: it throws.
+//   - d_dBeta(slice1, slice2, level) evaluates on the path's active copy, as the
+//     reference's reads do.
 #pragma once
 
 #include <chrono>
@@ -112,9 +113,9 @@ class ActionBase {
   virtual double Action(int slice1, int slice2, const std::vector<int>& activeParticles, int level) {
     return SingleAction(slice1, slice2, activeParticles, level);
   }
-  virtual double d_dBeta(int, int, int) {
-    throw Error(GetName() + "::d_dBeta is not provided by this library", PIMC_ERR_UNSUPPORTED);
-  }
+  // ActionBaseClass::d_dBeta (ActionBase.h:57), the action's beta derivative over links
+  // [slice1, slice2) at `level`.
+  virtual double d_dBeta(int slice1, int slice2, int level) = 0;
   virtual std::string GetName() = 0;
   // The actions keep no per-move state outside the path (rho_k lives with the path).
   virtual void AcceptCopy(int, int) {}
@@ -135,6 +136,12 @@ class ActionBase {
     r.nptcls = (int)ptcls.size();
     for (size_t i = 0; i < ptcls.size(); i++) r.ptcls[i] = ptcls[i];
     return r;
+  }
+  double DBeta(int32_t action, int slice1, int slice2, int level) const {
+    const pimc_range_req r{Path_.Replica(), Path_.GetMode(), slice1, slice2, level, 0};
+    double out = 0.0;
+    check(pimc_action_d_dbeta(Path_.Ctx().get(), action, &r, 1, &out), "pimc_action_d_dbeta");
+    return out;
   }
   template <typename F>
   double Timed(F&& f) {
@@ -157,6 +164,10 @@ class ShortRange : public ActionBase {
       return out;
     });
   }
+  // ShortRangeClass::d_dBeta (ShortRangeClass.cc:256-295)
+  double d_dBeta(int slice1, int slice2, int level) override {
+    return DBeta(PIMC_ACTION_SHORT_RANGE, slice1, slice2, level);
+  }
   std::string GetName() override { return "ShortRange"; }
 
  private:
@@ -175,6 +186,8 @@ class Kinetic : public ActionBase {
       return out;
     });
   }
+  // KineticClass::d_dBeta (KineticClass.cc:187-256)
+  double d_dBeta(int slice1, int slice2, int level) override { return DBeta(PIMC_ACTION_KINETIC, slice1, slice2, level); }
   std::string GetName() override { return "Kinetic"; }
 };
 
@@ -190,6 +203,10 @@ class LongRange : public ActionBase {
       check(pimc_action_long_range(Path_.Ctx().get(), &r, 1, precision_, &out), "pimc_action_long_range");
       return out;
     });
+  }
+  // LongRangeClass::d_dBeta (LongRangeClass.cc:374-442)
+  double d_dBeta(int slice1, int slice2, int level) override {
+    return DBeta(PIMC_ACTION_LONG_RANGE, slice1, slice2, level);
   }
   std::string GetName() override { return "LongRange"; }
 

@@ -344,6 +344,28 @@ typedef struct pimc_energy {
 
 int pimc_estimator_energy(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, int32_t mode, pimc_energy* out);
 
+/* ActionBaseClass::d_dBeta(slice1, slice2, level) (PIMC++/src/Actions/ActionBase.h:57) of one
+ * action on copy `mode` of a replica, for each request:
+ *   PIMC_ACTION_SHORT_RANGE  ShortRangeClass::d_dBeta (PIMC++/src/Actions/ShortRangeClass.cc:256-295):
+ *                            every pair over links [slice1, slice2) at `level`, the fits' dU minus
+ *                            dUlong of long-ranged pairs when long range is used
+ *   PIMC_ACTION_KINETIC      KineticClass::d_dBeta (PIMC++/src/Actions/KineticClass.cc:187-256)
+ *   PIMC_ACTION_LONG_RANGE   LongRangeClass::d_dBeta (PIMC++/src/Actions/LongRangeClass.cc:374-442):
+ *                            slices slice1 .. slice2 step 2^level (ends weighted 1/2), with rho_k of
+ *                            those slices recomputed from the positions as CalcRho_ks_Fast does (the
+ *                            stored rho_k is left as it is)
+ * The short- and long-range parts need the beta-derivative tables (pimc_estimator_energy). */
+enum pimc_action_kind { PIMC_ACTION_SHORT_RANGE = 0, PIMC_ACTION_KINETIC = 1, PIMC_ACTION_LONG_RANGE = 2 };
+typedef struct pimc_range_req {
+  int32_t replica;
+  int32_t mode;              /* PIMC_OLDMODE or PIMC_NEWMODE */
+  int32_t slice1;
+  int32_t slice2;
+  int32_t level;
+  int32_t reserved;
+} pimc_range_req;
+int pimc_action_d_dbeta(pimc_ctx* ctx, int32_t action, const pimc_range_req* reqs, int32_t nreq, double* out);
+
 /* PairCorrelationClass::Accumulate (PIMC++/src/Observables/ObservableCorrelation.cc:156-221):
  * adds to hist[num_points-1] the count of minimum-image distances between
  * particles of species s1 and s2 (pairs counted once when s1 == s2) on slices

@@ -1758,6 +1758,44 @@ static double long_range_v(Replica& R, int mode, int slice) {
   return Vlong;
 }
 
+// ActionBaseClass::d_dBeta(slice1, slice2, level) of one action (ActionBase.h:57).
+int pimc_action_d_dbeta(pimc_ctx* ctx, int32_t action, const pimc_range_req* reqs, int32_t nreq, double* out) {
+  if (!ctx || nreq < 0 || (nreq > 0 && (!reqs || !out))) return fail(PIMC_ERR_ARG, "action_d_dbeta: bad argument");
+  if (action != PIMC_ACTION_SHORT_RANGE && action != PIMC_ACTION_KINETIC && action != PIMC_ACTION_LONG_RANGE)
+    return fail(PIMC_ERR_ARG, "action_d_dbeta: bad action");
+  for (int i = 0; i < nreq; i++) {
+    const pimc_range_req& q = reqs[i];
+    Replica* R = get_rep(&ctx->c, q.replica);
+    if (!R) return fail(PIMC_ERR_ARG, "action_d_dbeta: bad replica id");
+    if (q.mode != PIMC_OLDMODE && q.mode != PIMC_NEWMODE) return fail(PIMC_ERR_ARG, "action_d_dbeta: bad mode");
+    if (q.level < 0 || q.level >= PIMC_MAX_LEVELS) return fail(PIMC_ERR_ARG, "action_d_dbeta: bad level");
+    const int skip = 1 << q.level;
+    if (q.slice1 < 0 || q.slice1 > q.slice2 || q.slice1 + ((q.slice2 - q.slice1 + skip - 1) / skip) * skip > R->M)
+      return fail(PIMC_ERR_ARG, "action_d_dbeta: bad slice range");
+    if (action == PIMC_ACTION_LONG_RANGE && R->nk == 0) return fail(PIMC_ERR_STATE, "action_d_dbeta: replica has no k set");
+    if (action != PIMC_ACTION_KINETIC)
+      for (int f = 0; f < R->ns * R->ns; f++) {
+        const PairFit& pa = *ctx->c.fits[R->pair_fit[f]];
+        if (!pa.has_dU()) return fail(PIMC_ERR_STATE, "action_d_dbeta: a pair fit has no beta-derivative table");
+        if (pa.IsLongRange() && R->use_long_range && (!R->dulong[f].set || R->dulong[f].levels <= q.level))
+          return fail(PIMC_ERR_STATE, "action_d_dbeta: long-range pair fit without dUlong tables");
+        if (R->long_range && R->use_long_range && R->lr[f].set && !R->lr[f].dset)
+          return fail(PIMC_ERR_STATE, "action_d_dbeta: long-range tables without beta derivatives");
+      }
+  }
+  for (int i = 0; i < nreq; i++) {
+    const pimc_range_req& q = reqs[i];
+    Replica& R = *get_rep(&ctx->c, q.replica);
+    if (action == PIMC_ACTION_KINETIC)
+      out[i] = kinetic_dbeta(R, q.mode, q.slice1, q.slice2, q.level);
+    else if (action == PIMC_ACTION_SHORT_RANGE)
+      out[i] = short_range_dbeta(&ctx->c, R, q.mode, q.slice1, q.slice2, q.level);
+    else
+      out[i] = (R.long_range && R.use_long_range) ? long_range_dbeta(R, q.mode, q.slice1, q.slice2, q.level) : 0.0;
+  }
+  return PIMC_OK;
+}
+
 int pimc_estimator_energy(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, int32_t mode, pimc_energy* out) {
   if (!ctx || nrep < 0 || (nrep > 0 && (!replicas || !out)) || mode < 0 || mode > 1)
     return fail(PIMC_ERR_ARG, "estimator_energy: bad argument");

@@ -95,6 +95,20 @@ class ActionReq(C.Structure):
     ]
 
 
+class RangeReq(C.Structure):
+    _fields_ = [
+        ("replica", C.c_int32),
+        ("mode", C.c_int32),
+        ("slice1", C.c_int32),
+        ("slice2", C.c_int32),
+        ("level", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+ACTION_SHORT_RANGE, ACTION_KINETIC, ACTION_LONG_RANGE = 0, 1, 2
+
+
 class BisectionDesc(C.Structure):
     _fields_ = [
         ("num_levels", C.c_int32),
@@ -163,6 +177,7 @@ SIGNATURES = {
         C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _f64p, _f64p, _f64p, _f64p,
                   _f64p, _f64p, _f64p, _f64p]),
     "pimc_estimator_energy": (C.c_int, [_CTX, _i32p, C.c_int32, C.c_int32, C.POINTER(Energy)]),
+    "pimc_action_d_dbeta": (C.c_int, [_CTX, C.c_int32, C.POINTER(RangeReq), C.c_int32, _f64p]),
     "pimc_estimator_pair_correlation": (
         C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32,
                   C.POINTER(C.c_int64)]),
@@ -417,6 +432,18 @@ class PimcLib:
         self._check("pimc_estimator_energy", self.lib.pimc_estimator_energy(
             self.ctx, _p(reps, _i32p), reps.size, mode, out))
         return [out[i] for i in range(reps.size)]
+
+    def d_dbeta(self, action: int, reqs) -> np.ndarray:
+        """ActionBaseClass::d_dBeta of one action (ACTION_SHORT_RANGE, ACTION_KINETIC or
+        ACTION_LONG_RANGE) for requests (replica, mode, slice1, slice2, level)."""
+        reqs = list(reqs)
+        arr = (RangeReq * max(1, len(reqs)))()
+        for i, (rep, mode, s1, s2, level) in enumerate(reqs):
+            arr[i].replica, arr[i].mode, arr[i].slice1, arr[i].slice2, arr[i].level = rep, mode, s1, s2, level
+        out = np.zeros(max(1, len(reqs)), dtype=np.float64)
+        self._check("pimc_action_d_dbeta", self.lib.pimc_action_d_dbeta(self.ctx, action, arr, len(reqs),
+                                                                        _p(out, _f64p)))
+        return out[:len(reqs)]
 
     def pair_correlation(self, rep: int, s1: int, s2: int, start: float, end: float, num_points: int,
                          mode: int = OLDMODE, hist=None) -> np.ndarray:

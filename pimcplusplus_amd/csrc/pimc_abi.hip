@@ -1856,65 +1856,90 @@ int pimc_sweep_graph_destroy(pimc_ctx* ctx, int32_t graph_id) {
   return PIMC_OK;
 }
 
-int pimc_estimator_energy(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, int32_t mode, pimc_energy* out) {
-  if (!ctx || nrep < 0 || (nrep > 0 && (!replicas || !out)) || mode < 0 || mode > 1)
-    return fail(PIMC_ERR_ARG, "estimator_energy: bad argument");
-  for (int r = 0; r < nrep; r++) {
-    HostReplica* H = ctx->rep(replicas[r]);
-    if (!H) return fail(PIMC_ERR_ARG, "estimator_energy: bad replica");
-    const RepCore& R = H->dev.core;
-    for (int f = 0; f < R.ns * R.ns; f++) {
-      const FitDev& F = ctx->fits[H->dev.sp.fit[f]]->dev;
-      if ((F.kind == 0 && !F.e64) || (F.kind == 1 && !F.dbc))
-        return fail(PIMC_ERR_STATE, "estimator_energy: a pair fit has no beta-derivative table");
-      if (F.kind == 1 && R.use_long_range && !((H->dulong_mask >> f) & 1u))
-        return fail(PIMC_ERR_STATE, "estimator_energy: long-range pair fit without dUlong tables");
-    }
-    if (R.lr_on && (R.lr_mask & ~H->dlr_mask))
-      return fail(PIMC_ERR_STATE, "estimator_energy: long-range tables without beta derivatives");
+}  // extern "C"
+
+namespace {
+
+// What ActionsClass::Energy and the actions' d_dBeta need from the replica's tables (the
+// fits' beta derivatives, dUlong, the long-range derivative tables) at `level`.
+int check_dbeta_tables(pimc_ctx* ctx, HostReplica* H, int level, const char* who) {
+  const RepCore& R = H->dev.core;
+  for (int f = 0; f < R.ns * R.ns; f++) {
+    const HostFit& HF = *ctx->fits[H->dev.sp.fit[f]];
+    const FitDev& F = HF.dev;
+    if ((F.kind == 0 && !F.e64) || (F.kind == 1 && !F.dbc))
+      return fail(PIMC_ERR_STATE, std::string(who) + ": a pair fit has no beta-derivative table");
+    if (level >= HF.num_levels()) return fail(PIMC_ERR_ARG, std::string(who) + ": level beyond the fit's tau levels");
+    if (F.kind == 1 && R.use_long_range && (!((H->dulong_mask >> f) & 1u) || level >= R.ulong_levels))
+      return fail(PIMC_ERR_STATE, std::string(who) + ": long-range pair fit without dUlong tables");
   }
-  HIP_TRY(hipSetDevice(ctx->device));
-  int rc = ctx->sync_descriptors();
-  if (rc) return rc;
-  for (int r = 0; r < nrep; r++) {
-    HostReplica* H = ctx->rep(replicas[r]);
-    const RepCore& R = H->dev.core;
-    int nt = 0;
-    bool v_known = true;   // every long-ranged pair has its Vlong tables (when long range is used)
-    for (int f = 0; f < R.ns * R.ns; f++) {
-      const HostFit& F = *ctx->fits[H->dev.sp.fit[f]];
-      if (F.dev.kind == 0)
-        nt = std::max(nt, F.num_terms);
-      else if (R.use_long_range && !((R.vlong_mask >> f) & 1u))
-        v_known = false;
-    }
-    const bool vlong_known = !R.lr_on || !(R.lr_mask & ~R.vlong_mask);
-    for (int f = 0; f < R.ns * R.ns; f++) {
-      const HostFit& F = *ctx->fits[H->dev.sp.fit[f]];
-      if (F.dev.kind == 0 && F.num_terms != nt)
-        return fail(PIMC_ERR_UNSUPPORTED, "estimator_energy: fits with different num_terms in one replica");
-    }
-    const long long npairs = (long long)R.N * (R.N - 1) / 2;
-    const int M = R.nslices - 1;
-    const int pair_blocks = (int)std::max(1LL, std::min(2048LL, (npairs + kBlock - 1) / kBlock));
-    const int kin_blocks = (int)std::max(1LL, std::min(1024LL, ((long long)R.N * M + kBlock - 1) / kBlock));
-    const int lr_slices = R.lr_on ? R.nslices : 0;
-    const size_t n_pp = 2 * (size_t)pair_blocks, n_kp = kin_blocks, n_lp = 6 * (size_t)std::max(1, lr_slices);
-    if ((rc = ctx->d_out.ensure((n_pp + n_kp + n_lp) * sizeof(double)))) return rc;
-    double* dp = ctx->d_out.as<double>();
-    hipError_t e = launch_energy(nt, ctx->d_reps.as<ReplicaDev>(), ctx->d_fits.as<FitDev>(), replicas[r], mode,
-                                 pair_blocks, kin_blocks, lr_slices, dp, dp + n_pp, dp + n_pp + n_kp, ctx->stream);
-    if (e != hipSuccess) return fail(PIMC_ERR_DEVICE, std::string("estimator_energy launch: ") + hipGetErrorString(e));
-    std::vector<double> h(n_pp + n_kp + n_lp);
-    HIP_TRY(hipMemcpyAsync(h.data(), dp, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    pimc_energy en{};
+  if (R.lr_on && (R.lr_mask & ~H->dlr_mask))
+    return fail(PIMC_ERR_STATE, std::string(who) + ": long-range tables without beta derivatives");
+  if (R.lr_on && level >= R.lr_levels) return fail(PIMC_ERR_ARG, std::string(who) + ": level beyond the long-range tables");
+  return PIMC_OK;
+}
+
+// The actions' d_dBeta over links [slice1, slice2) at `level` of copy `mode` (the
+// long-range sum over slices slice1 .. slice2), and the potential over the same slices:
+// which = 1 short range, 2 kinetic, 4 long range (or together).  Fills en.du_short,
+// en.kinetic, en.du_long, en.v_short and en.v_long of the parts asked for.
+int dbeta_parts(pimc_ctx* ctx, int replica, int mode, int slice1, int slice2, int level, int which, pimc_energy& en) {
+  HostReplica* H = ctx->rep(replica);
+  const RepCore& R = H->dev.core;
+  int nt = 0;
+  bool v_known = true;   // every long-ranged pair has its Vlong tables (when long range is used)
+  for (int f = 0; f < R.ns * R.ns; f++) {
+    const HostFit& F = *ctx->fits[H->dev.sp.fit[f]];
+    if (F.dev.kind == 0)
+      nt = std::max(nt, F.num_terms);
+    else if (R.use_long_range && !((R.vlong_mask >> f) & 1u))
+      v_known = false;
+  }
+  for (int f = 0; f < R.ns * R.ns; f++) {
+    const HostFit& F = *ctx->fits[H->dev.sp.fit[f]];
+    if (F.dev.kind == 0 && F.num_terms != nt)
+      return fail(PIMC_ERR_UNSUPPORTED, "d_dBeta: fits with different num_terms in one replica");
+  }
+  const bool vlong_known = !R.lr_on || !(R.lr_mask & ~R.vlong_mask);
+  const int skip = 1 << level;
+  const long long nlinks = (slice2 - slice1 + skip - 1) / skip;
+  const long long npairs = (long long)R.N * (R.N - 1) / 2;
+  const int pair_blocks = (which & 1) ? (int)std::max(1LL, std::min(2048LL, (npairs + kBlock - 1) / kBlock)) : 0;
+  const int kin_blocks =
+      (which & 2) ? (int)std::max(1LL, std::min(1024LL, ((long long)R.N * nlinks + kBlock - 1) / kBlock)) : 0;
+  const int lr_slices = ((which & 4) && R.lr_on) ? (slice2 - slice1) / skip + 1 : 0;
+  const size_t n_pp = 2 * (size_t)pair_blocks, n_kp = kin_blocks, n_lp = 6 * (size_t)lr_slices;
+  int rc;
+  if ((rc = ctx->d_out.ensure(std::max<size_t>(1, n_pp + n_kp + n_lp) * sizeof(double)))) return rc;
+  if (lr_slices && (rc = ctx->d_in1.ensure((size_t)lr_slices * R.ns * R.nk * sizeof(d2v)))) return rc;
+  double* dp = ctx->d_out.as<double>();
+  if (lr_slices)   // LongRangeClass::d_dBeta recomputes rho_k from the positions (CalcRho_ks_Fast)
+    launch_rho_slices(ctx->d_reps.as<ReplicaDev>(), replica, mode, slice1, skip, lr_slices, R.ns, R.nk,
+                      2 * (R.maxk0 + R.maxk1 + R.maxk2) + 3, ctx->d_in1.as<d2v>(), ctx->stream);
+  hipError_t e = launch_energy(nt, ctx->d_reps.as<ReplicaDev>(), ctx->d_fits.as<FitDev>(), replica, mode, slice1,
+                               slice2, level, pair_blocks, kin_blocks, lr_slices, ctx->d_in1.as<d2v>(), dp, dp + n_pp,
+                               dp + n_pp + n_kp, ctx->stream);
+  if (e != hipSuccess) return fail(PIMC_ERR_DEVICE, std::string("d_dBeta launch: ") + hipGetErrorString(e));
+  std::vector<double> h(n_pp + n_kp + n_lp);
+  if (!h.empty()) HIP_TRY(hipMemcpyAsync(h.data(), dp, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  if (which & 1) {
     double v = 0.0;
+    en.du_short = 0.0;
     for (int b = 0; b < pair_blocks; b++) {
       en.du_short += h[2 * b];
       v += h[2 * b + 1];
     }
+    en.v_short = v_known ? v : nan;
+  }
+  if (which & 2) {
+    en.kinetic = 0.0;
     for (int b = 0; b < kin_blocks; b++) en.kinetic += h[n_pp + b];
+  }
+  if (which & 4) {
+    en.du_long = 0.0;
+    en.v_long = 0.0;
     if (lr_slices) {
       double hh = 0.0, bg = 0.0, k0 = 0.0, vh = 0.0, vbg = 0.0, vk0 = 0.0;
       for (int sl = 0; sl < lr_slices; sl++) {
@@ -1929,11 +1954,67 @@ int pimc_estimator_energy(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, 
       en.du_long = hh + (R.use_background ? bg : k0);
       en.v_long = vh + (R.use_background ? vbg : vk0);
     }
-    const double nan = std::numeric_limits<double>::quiet_NaN();
-    en.v_short = v_known ? v : nan;
     if (!vlong_known) en.v_long = nan;
+  }
+  return PIMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pimc_estimator_energy(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, int32_t mode, pimc_energy* out) {
+  if (!ctx || nrep < 0 || (nrep > 0 && (!replicas || !out)) || mode < 0 || mode > 1)
+    return fail(PIMC_ERR_ARG, "estimator_energy: bad argument");
+  for (int r = 0; r < nrep; r++) {
+    HostReplica* H = ctx->rep(replicas[r]);
+    if (!H) return fail(PIMC_ERR_ARG, "estimator_energy: bad replica");
+    int rc = check_dbeta_tables(ctx, H, 0, "estimator_energy");
+    if (rc) return rc;
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  int rc = ctx->sync_descriptors();
+  if (rc) return rc;
+  for (int r = 0; r < nrep; r++) {
+    pimc_energy en{};
+    if ((rc = dbeta_parts(ctx, replicas[r], mode, 0, ctx->rep(replicas[r])->M, 0, 7, en))) return rc;
     en.total = en.kinetic + en.du_short + en.du_long;
     out[r] = en;
+  }
+  return PIMC_OK;
+}
+
+int pimc_action_d_dbeta(pimc_ctx* ctx, int32_t action, const pimc_range_req* reqs, int32_t nreq, double* out) {
+  if (!ctx || nreq < 0 || (nreq > 0 && (!reqs || !out))) return fail(PIMC_ERR_ARG, "action_d_dbeta: bad argument");
+  if (action != PIMC_ACTION_SHORT_RANGE && action != PIMC_ACTION_KINETIC && action != PIMC_ACTION_LONG_RANGE)
+    return fail(PIMC_ERR_ARG, "action_d_dbeta: bad action");
+  for (int i = 0; i < nreq; i++) {
+    const pimc_range_req& q = reqs[i];
+    HostReplica* H = ctx->rep(q.replica);
+    if (!H) return fail(PIMC_ERR_ARG, "action_d_dbeta: bad replica id");
+    if (q.mode != PIMC_OLDMODE && q.mode != PIMC_NEWMODE) return fail(PIMC_ERR_ARG, "action_d_dbeta: bad mode");
+    if (q.level < 0 || q.level >= PIMC_MAX_LEVELS) return fail(PIMC_ERR_ARG, "action_d_dbeta: bad level");
+    const int skip = 1 << q.level;
+    if (q.slice1 < 0 || q.slice1 > q.slice2 || q.slice1 + ((q.slice2 - q.slice1 + skip - 1) / skip) * skip > H->M)
+      return fail(PIMC_ERR_ARG, "action_d_dbeta: bad slice range");
+    if (action == PIMC_ACTION_LONG_RANGE && H->dev.core.nk == 0)
+      return fail(PIMC_ERR_STATE, "action_d_dbeta: replica has no k set");
+    if (action != PIMC_ACTION_KINETIC) {
+      int rc = check_dbeta_tables(ctx, H, q.level, "action_d_dbeta");
+      if (rc) return rc;
+    }
+  }
+  if (nreq == 0) return PIMC_OK;
+  HIP_TRY(hipSetDevice(ctx->device));
+  int rc = ctx->sync_descriptors();
+  if (rc) return rc;
+  const int which = action == PIMC_ACTION_SHORT_RANGE ? 1 : action == PIMC_ACTION_KINETIC ? 2 : 4;
+  for (int i = 0; i < nreq; i++) {
+    pimc_energy en{};
+    if ((rc = dbeta_parts(ctx, reqs[i].replica, reqs[i].mode, reqs[i].slice1, reqs[i].slice2, reqs[i].level, which,
+                          en)))
+      return rc;
+    out[i] = which == 1 ? en.du_short : which == 2 ? en.kinetic : en.du_long;
   }
   return PIMC_OK;
 }

@@ -107,17 +107,18 @@ __device__ inline double slice_dist(const RepCore& R, const double a[3], const d
   return sqrt(d2);
 }
 
-// dU and V of every pair over the whole path of copy `mode` (slices [0, M], level 0).
+// dU of every pair over the links [slice1, slice2) at `level`, and V over the slices
+// slice1, slice1 + 2^level, .., slice2 (ends weighted 1/2), of copy `mode`.
 // partials[block] = {du, v}.
 template <int NT>
 __global__ void __launch_bounds__(kBlock) k_energy_pairs(const ReplicaDev* reps, const FitDev* fits, int rep, int mode,
-                                                         double* partials) {
+                                                         int slice1, int slice2, int level, double* partials) {
   __shared__ double scratch_raw[kWaves];
   PIMC_LDS double* scratch = (PIMC_LDS double*)scratch_raw;
   const ReplicaDev* RD = reps + rep;
   const RepCore R = load_uniform(&RD->core);
   const RepSpecies& S = RD->sp;
-  const int M = R.nslices - 1;
+  const int skip = 1 << level;
   const long long npairs = (long long)R.N * (R.N - 1) / 2;
   double du = 0.0, v = 0.0;
   for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npairs;
@@ -132,7 +133,7 @@ __global__ void __launch_bounds__(kBlock) k_energy_pairs(const ReplicaDev* reps,
     const int pr = s1 * R.ns + s2;
     const FitDev& F = fits[S.fit[pr]];
     const double* dul = (F.kind == 1 && R.use_long_range && R.dulong)
-                            ? R.dulong + (size_t)pr * R.ulong_levels * R.ulong_n * 6
+                            ? R.dulong + ((size_t)pr * R.ulong_levels + level) * R.ulong_n * 6
                             : nullptr;
     // ShortRangePotClass::V: V(r) (column 0 of a DavidPA fit, Z1Z2/r of a Coulomb fit), minus
     // Vlong(r) for a long-ranged fit when long range is used
@@ -145,13 +146,13 @@ __global__ void __launch_bounds__(kBlock) k_energy_pairs(const ReplicaDev* reps,
       return val;
     };
     double a1[3], a2[3];
-    load_bead(R, mode, 0, p1, a1);
-    load_bead(R, mode, 0, p2, a2);
+    load_bead(R, mode, slice1, p1, a1);
+    load_bead(R, mode, slice1, p2, a2);
     v += 0.5 * vpair(slice_dist(R, a1, a2));
-    for (int slice = 0; slice < M; slice++) {
+    for (int slice = slice1; slice < slice2; slice += skip) {
       double b1[3], b2[3];
-      load_bead(R, mode, slice + 1, p1, b1);
-      load_bead(R, mode, slice + 1, p2, b2);
+      load_bead(R, mode, slice + skip, p1, b1);
+      load_bead(R, mode, slice + skip, p2, b2);
       double da[3], db[3];
 #pragma unroll
       for (int d = 0; d < 3; d++) {
@@ -163,9 +164,9 @@ __global__ void __launch_bounds__(kBlock) k_energy_pairs(const ReplicaDev* reps,
       const double e0 = da[0] - db[0], e1 = da[1] - db[1], e2 = da[2] - db[2];
       const double s2v = e0 * e0 + e1 * e1 + e2 * e2;
       const double q = 0.5 * (rmag + rpmag), z = rmag - rpmag;
-      du += F.kind == 1 ? bc_du(F, 0, q, s2v) : david_du<NT>(F, 0, q, z, s2v);
+      du += F.kind == 1 ? bc_du(F, level, q, s2v) : david_du<NT>(F, level, q, z, s2v);
       if (dul) du -= 0.5 * (ulong_eval(R, dul, rmag) + ulong_eval(R, dul, rpmag));
-      v += (slice + 1 == M ? 0.5 : 1.0) * vpair(slice_dist(R, b1, b2));
+      v += (slice + skip >= slice2 ? 0.5 : 1.0) * vpair(slice_dist(R, b1, b2));
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         a1[d] = b1[d];
@@ -181,20 +182,22 @@ __global__ void __launch_bounds__(kBlock) k_energy_pairs(const ReplicaDev* reps,
   }
 }
 
-// KineticClass::d_dBeta over every particle and link of copy `mode`.
-__global__ void __launch_bounds__(kBlock) k_energy_kinetic(const ReplicaDev* reps, int rep, int mode, double* partials) {
+// KineticClass::d_dBeta over every particle and the links [slice1, slice2) at `level` of copy `mode`.
+__global__ void __launch_bounds__(kBlock) k_energy_kinetic(const ReplicaDev* reps, int rep, int mode, int slice1,
+                                                           int slice2, int level, double* partials) {
   __shared__ double scratch_raw[kWaves];
   PIMC_LDS double* scratch = (PIMC_LDS double*)scratch_raw;
   const ReplicaDev* RD = reps + rep;
   const RepCore R = load_uniform(&RD->core);
   const RepSpecies& S = RD->sp;
-  const int M = R.nslices - 1;
-  const double levelTau = R.tau;
-  const long long nitems = (long long)R.N * M;
+  const int skip = 1 << level;
+  const int nlinks = (slice2 - slice1 + skip - 1) / skip;
+  const double levelTau = ldexp(R.tau, level);
+  const long long nitems = (long long)R.N * nlinks;
   double spring = 0.0;
   for (long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x; it < nitems;
        it += (long long)gridDim.x * blockDim.x) {
-    const int ptcl = (int)(it / M), slice = (int)(it % M);
+    const int ptcl = (int)(it / nlinks), slice = slice1 + (int)(it % nlinks) * skip;
     int sp = 0;
     for (int s = 1; s < R.ns; s++)
       if (ptcl >= S.first[s]) sp = s;
@@ -204,7 +207,7 @@ __global__ void __launch_bounds__(kBlock) k_energy_kinetic(const ReplicaDev* rep
     spring += (0.5 * 3) / levelTau;
     double a[3], b[3];
     load_bead(R, mode, slice, ptcl, a);
-    load_bead(R, mode, slice + 1, ptcl, b);
+    load_bead(R, mode, slice + skip, ptcl, b);
     double GaussSum[3], numSum[3], Z = 1.0;
 #pragma unroll
     for (int dim = 0; dim < 3; dim++) {
@@ -234,20 +237,22 @@ __global__ void __launch_bounds__(kBlock) k_energy_kinetic(const ReplicaDev* rep
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-// LongRangeClass::d_dBeta: one block per slice of [0, M], rho_k of copy `mode`.
-// partials[slice] = {homo + hetero, background, k0} with the slice's factor, then the same
-// three sums of the long-range potential.
-__global__ void __launch_bounds__(kBlock) k_energy_long_range(const ReplicaDev* reps, int rep, int mode,
+// LongRangeClass::d_dBeta: one block per slice slice1 + b 2^level (b = blockIdx.x) up to
+// slice2, rho_k of that slice from rho ([b][species][k]).  partials[b] = {homo + hetero,
+// background, k0} with the slice's factor, then the same three sums of the long-range
+// potential.
+__global__ void __launch_bounds__(kBlock) k_energy_long_range(const ReplicaDev* reps, int rep, int slice1, int slice2,
+                                                              int level, const d2v* __restrict__ rho_in,
                                                               double* partials) {
   __shared__ double scratch_raw[kWaves];
   PIMC_LDS double* scratch = (PIMC_LDS double*)scratch_raw;
   const ReplicaDev* RD = reps + rep;
   const RepCore R = load_uniform(&RD->core);
   const RepSpecies& S = RD->sp;
-  const int M = R.nslices - 1;
-  const int slice = blockIdx.x;
-  const double factor = (slice == 0 || slice == M) ? 0.5 : 1.0;
-  const d2v* rho = R.rho(mode) + (size_t)slice * R.ns * R.nk;
+  const int b = blockIdx.x;
+  const int slice = slice1 + b * (1 << level);
+  const double factor = (slice == slice1 || slice == slice2) ? 0.5 : 1.0;
+  const d2v* rho = rho_in + (size_t)b * R.ns * R.nk;
   const int L = R.lr_levels;
   double part = 0.0, vpart = 0.0;   // d_dBeta and LongRangePotClass::V (LongRangePotClass.cc:32-85)
   for (int k = threadIdx.x; k < R.nk; k += blockDim.x) {
@@ -255,16 +260,16 @@ __global__ void __launch_bounds__(kBlock) k_energy_long_range(const ReplicaDev* 
       const int pr = s * R.ns + s;
       const d2v a = rho[(size_t)s * R.nk + k];
       if ((R.lr_mask >> pr) & 1u)
-        part += factor * 0.5 * 2.0 * (a.x * a.x + a.y * a.y) * R.dulong_k[((size_t)pr * L + 0) * R.nk + k];
+        part += factor * 0.5 * 2.0 * (a.x * a.x + a.y * a.y) * R.dulong_k[((size_t)pr * L + level) * R.nk + k];
       if ((R.vlong_mask >> pr) & 1u) vpart += factor * 0.5 * 2.0 * (a.x * a.x + a.y * a.y) * R.vlong_k[(size_t)pr * R.nk + k];
     }
     for (int s1 = 0; s1 < R.ns; s1++)
       for (int s2 = s1 + 1; s2 < R.ns; s2++) {
         const int pr = s1 * R.ns + s2;
-        const d2v a = rho[(size_t)s1 * R.nk + k], b = rho[(size_t)s2 * R.nk + k];
+        const d2v a = rho[(size_t)s1 * R.nk + k], c = rho[(size_t)s2 * R.nk + k];
         if ((R.lr_mask >> pr) & 1u)
-          part += factor * 2.0 * (a.x * b.x + a.y * b.y) * R.dulong_k[((size_t)pr * L + 0) * R.nk + k];
-        if ((R.vlong_mask >> pr) & 1u) vpart += factor * 2.0 * (a.x * b.x + a.y * b.y) * R.vlong_k[(size_t)pr * R.nk + k];
+          part += factor * 2.0 * (a.x * c.x + a.y * c.y) * R.dulong_k[((size_t)pr * L + level) * R.nk + k];
+        if ((R.vlong_mask >> pr) & 1u) vpart += factor * 2.0 * (a.x * c.x + a.y * c.y) * R.vlong_k[(size_t)pr * R.nk + k];
       }
   }
   const double t = block_sum(part, scratch);
@@ -275,21 +280,21 @@ __global__ void __launch_bounds__(kBlock) k_energy_long_range(const ReplicaDev* 
       const int pr = s * R.ns + s;
       if (!((R.lr_mask >> pr) & 1u)) continue;
       const int N = S.count[s];
-      homo -= factor * 0.5 * N * R.dlr_const[(size_t)(pr * 3 + 0) * L + 0];
-      background -= factor * 0.5 * N * N * R.dlr_const[(size_t)(pr * 3 + 1) * L + 0];
-      k0 += factor * 0.5 * N * N * R.dlr_const[(size_t)(pr * 3 + 2) * L + 0];
+      homo -= factor * 0.5 * N * R.dlr_const[(size_t)(pr * 3 + 0) * L + level];
+      background -= factor * 0.5 * N * N * R.dlr_const[(size_t)(pr * 3 + 1) * L + level];
+      k0 += factor * 0.5 * N * N * R.dlr_const[(size_t)(pr * 3 + 2) * L + level];
     }
     for (int s1 = 0; s1 < R.ns; s1++)
       for (int s2 = s1 + 1; s2 < R.ns; s2++) {
         const int pr = s1 * R.ns + s2;
         if (!((R.lr_mask >> pr) & 1u)) continue;
         const int N1 = S.count[s1], N2 = S.count[s2];
-        background -= factor * N1 * N2 * R.dlr_const[(size_t)(pr * 3 + 1) * L + 0];
-        k0 += factor * N1 * N2 * R.dlr_const[(size_t)(pr * 3 + 2) * L + 0];
+        background -= factor * N1 * N2 * R.dlr_const[(size_t)(pr * 3 + 1) * L + level];
+        k0 += factor * N1 * N2 * R.dlr_const[(size_t)(pr * 3 + 2) * L + level];
       }
-    partials[6 * slice] = t + homo;
-    partials[6 * slice + 1] = background;
-    partials[6 * slice + 2] = k0;
+    partials[6 * b] = t + homo;
+    partials[6 * b + 1] = background;
+    partials[6 * b + 2] = k0;
     // the potential's Madelung, background and k = 0 terms
     double vhomo = 0.0, vbackground = 0.0, vk0 = 0.0;
     for (int s = 0; s < R.ns; s++) {
@@ -308,9 +313,9 @@ __global__ void __launch_bounds__(kBlock) k_energy_long_range(const ReplicaDev* 
         vbackground -= factor * N1 * N2 * R.vlr_const[(size_t)pr * 3 + 1];
         vk0 += factor * N1 * N2 * R.vlr_const[(size_t)pr * 3 + 2];
       }
-    partials[6 * slice + 3] = tv + vhomo;
-    partials[6 * slice + 4] = vbackground;
-    partials[6 * slice + 5] = vk0;
+    partials[6 * b + 3] = tv + vhomo;
+    partials[6 * b + 4] = vbackground;
+    partials[6 * b + 5] = vk0;
   }
 }
 
@@ -353,17 +358,22 @@ __global__ void __launch_bounds__(kBlock) k_pair_correlation(const ReplicaDev* r
   }
 }
 
-hipError_t launch_energy(int num_terms, const ReplicaDev* reps, const FitDev* fits, int rep, int mode,
-                         int pair_blocks, int kin_blocks, int lr_slices, double* pair_partials,
-                         double* kin_partials, double* lr_partials, hipStream_t s) {
-  switch (num_terms) {
-    case 0: k_energy_pairs<0><<<pair_blocks, kBlock, 0, s>>>(reps, fits, rep, mode, pair_partials); break;
-    case 1: k_energy_pairs<1><<<pair_blocks, kBlock, 0, s>>>(reps, fits, rep, mode, pair_partials); break;
-    case 2: k_energy_pairs<2><<<pair_blocks, kBlock, 0, s>>>(reps, fits, rep, mode, pair_partials); break;
-    default: k_energy_pairs<3><<<pair_blocks, kBlock, 0, s>>>(reps, fits, rep, mode, pair_partials); break;
+hipError_t launch_energy(int num_terms, const ReplicaDev* reps, const FitDev* fits, int rep, int mode, int slice1,
+                         int slice2, int level, int pair_blocks, int kin_blocks, int lr_slices, const d2v* lr_rho,
+                         double* pair_partials, double* kin_partials, double* lr_partials, hipStream_t s) {
+  if (pair_blocks > 0) {
+#define PIMC_PAIRS(NT_) k_energy_pairs<NT_><<<pair_blocks, kBlock, 0, s>>>(reps, fits, rep, mode, slice1, slice2, level, pair_partials)
+    switch (num_terms) {
+      case 0: PIMC_PAIRS(0); break;
+      case 1: PIMC_PAIRS(1); break;
+      case 2: PIMC_PAIRS(2); break;
+      default: PIMC_PAIRS(3); break;
+    }
+#undef PIMC_PAIRS
   }
-  k_energy_kinetic<<<kin_blocks, kBlock, 0, s>>>(reps, rep, mode, kin_partials);
-  if (lr_slices > 0) k_energy_long_range<<<lr_slices, kBlock, 0, s>>>(reps, rep, mode, lr_partials);
+  if (kin_blocks > 0) k_energy_kinetic<<<kin_blocks, kBlock, 0, s>>>(reps, rep, mode, slice1, slice2, level, kin_partials);
+  if (lr_slices > 0)
+    k_energy_long_range<<<lr_slices, kBlock, 0, s>>>(reps, rep, slice1, slice2, level, lr_rho, lr_partials);
   return hipGetLastError();
 }
 

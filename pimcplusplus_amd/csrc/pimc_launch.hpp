@@ -42,14 +42,22 @@ void launch_permute_htable(const ReplicaDev* reps, int rep, int mode, int first,
                            double four_lambda_beta_inv, double log_eps, float* out, hipStream_t s);
 void launch_cycle_apply(const ReplicaDev* reps, int rep, int slice, const int* ptcls, int n, hipStream_t s);
 void launch_rho_all(const ReplicaDev* reps, int rep, int nslices, int ns, int nk, int stride, hipStream_t s);
+// rho_k of copy `mode` at slices slice1 + i skip (i < nsl), recomputed from the positions,
+// into out[nsl][ns][nk] (PathClass::CalcRho_ks_Fast, PathClass.cc:616-647).
+void launch_rho_slices(const ReplicaDev* reps, int rep, int mode, int slice1, int skip, int nsl, int ns, int nk,
+                       int stride, d2v* out, hipStream_t s);
 void launch_long_range(bool mixed, const ReplicaDev* reps, const ReqDev* reqs, int nreq, const int* strides,
                        double* out, hipStream_t s);
 
-// Energy estimator: partial sums per block (pairs: {du, v}; kinetic; long range per
-// slice: {homo+hetero, background, k0}); lr_slices = 0 skips the long-range kernel.
-hipError_t launch_energy(int num_terms, const ReplicaDev* reps, const FitDev* fits, int rep, int mode,
-                         int pair_blocks, int kin_blocks, int lr_slices, double* pair_partials,
-                         double* kin_partials, double* lr_partials, hipStream_t s);
+// d_dBeta of the actions over links [slice1, slice2) at `level` (slices slice1, slice1 +
+// 2^level, .., slice2) of copy `mode`, as partial sums per block: pairs {du, v} (the
+// potential over the same slices, ends weighted 1/2), kinetic, long range per slice
+// {homo+hetero, background, k0, and the potential's three}.  A zero block count skips
+// that kernel.  The long-range kernel reads rho_k of those slices from lr_rho
+// ([slice][species][k], recomputed from the positions).
+hipError_t launch_energy(int num_terms, const ReplicaDev* reps, const FitDev* fits, int rep, int mode, int slice1,
+                         int slice2, int level, int pair_blocks, int kin_blocks, int lr_slices, const d2v* lr_rho,
+                         double* pair_partials, double* kin_partials, double* lr_partials, hipStream_t s);
 hipError_t launch_pair_correlation(const ReplicaDev* reps, int rep, int mode, int f1, int n1, int f2, int n2,
                                    int same, double start, double deltainv, double end, int nbins,
                                    unsigned long long* hist, int blocks, hipStream_t s);

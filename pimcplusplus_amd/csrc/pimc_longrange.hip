@@ -1,6 +1,7 @@
 // pimc_longrange.hip -- rho_k kernels and LongRangeClass::SingleAction.
 //
-//   k_rho_all   PathClass::UpdateRho_ks()        PathClass.cc:709-723 (+ CalcRho_ks_Fast :616-647)
+//   k_rho_all   PathClass::UpdateRho_ks()        PathClass.cc:709-723 (+ CalcRho_ks_Fast :616-647);
+//               CalcRho_ks_Fast of chosen slices of one copy into a scratch array (d_dBeta)
 //   k_lr_req    LongRangeClass::SingleAction     LongRangeClass.cc:273-371, with
 //               PathClass::UpdateRho_ks(s1, s2, changed, level) PathClass.cc:649-707 in NEWMODE
 // rho_k is [slice][species][k] complex double; a thread owns one k and adds
@@ -13,12 +14,16 @@ namespace pimc {
 
 constexpr int kRhoChunk = 16;   // particles whose phase tables are staged together
 
-__global__ void __launch_bounds__(kBlock) k_rho_all(const ReplicaDev* __restrict__ reps, int rep) {
+// rho_k of slice slice1 + blockIdx.x * skip and species blockIdx.y from copy `mode`: into
+// out[blockIdx.x][species][k] when out is given, else into both mirrored copies.
+__global__ void __launch_bounds__(kBlock) k_rho_all(const ReplicaDev* __restrict__ reps, int rep, int mode, int slice1,
+                                                    int skip, d2v* __restrict__ out) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   PIMC_LDS d2v* C = (PIMC_LDS d2v*)smem_raw;
   const ReplicaDev* RD = reps + rep;
   const RepCore R = load_uniform(&RD->core);
-  const int slice = blockIdx.x, s = blockIdx.y;
+  const int slice = slice1 + (int)blockIdx.x * skip, s = blockIdx.y;
+  const double* path = R.path(mode);
   const int k = blockIdx.z * blockDim.x + threadIdx.x;
   const int first = RD->sp.first[s], end = first + RD->sp.count[s];
   const int stride = kstride(R);
@@ -35,7 +40,7 @@ __global__ void __launch_bounds__(kBlock) k_rho_all(const ReplicaDev* __restrict
     __syncthreads();
     for (int t = threadIdx.x; t < np * 3; t += blockDim.x) {
       const int p = p0 + t / 3, d = t % 3;
-      const double x = R.path0[((size_t)slice * 3 + d) * R.N + p];
+      const double x = path[((size_t)slice * 3 + d) * R.N + p];
       const double kb = R.kbox[0] * (double)(d == 0) + R.kbox[1] * (double)(d == 1) + R.kbox[2] * (double)(d == 2);
       phase_row<double>(x * kb, kmax_of(R, d), C + (size_t)(t / 3) * stride + koff_of(R, d));
     }
@@ -49,15 +54,25 @@ __global__ void __launch_bounds__(kBlock) k_rho_all(const ReplicaDev* __restrict
       }
   }
   if (k < R.nk) {
-    const size_t at = ((size_t)slice * R.ns + s) * R.nk + k;
-    R.rho0[at] = d2v{re, im};
-    R.rho1[at] = d2v{re, im};
+    if (out) {
+      out[((size_t)blockIdx.x * R.ns + s) * R.nk + k] = d2v{re, im};
+    } else {
+      const size_t at = ((size_t)slice * R.ns + s) * R.nk + k;
+      R.rho0[at] = d2v{re, im};
+      R.rho1[at] = d2v{re, im};
+    }
   }
 }
 
 void launch_rho_all(const ReplicaDev* reps, int rep, int nslices, int ns, int nk, int stride, hipStream_t s) {
   dim3 grid(nslices, ns, (nk + kBlock - 1) / kBlock);
-  k_rho_all<<<grid, kBlock, (size_t)kRhoChunk * stride * 16, s>>>(reps, rep);
+  k_rho_all<<<grid, kBlock, (size_t)kRhoChunk * stride * 16, s>>>(reps, rep, 0, 0, 1, nullptr);
+}
+
+void launch_rho_slices(const ReplicaDev* reps, int rep, int mode, int slice1, int skip, int nsl, int ns, int nk,
+                       int stride, d2v* out, hipStream_t s) {
+  dim3 grid(nsl, ns, (nk + kBlock - 1) / kBlock);
+  k_rho_all<<<grid, kBlock, (size_t)kRhoChunk * stride * 16, s>>>(reps, rep, mode, slice1, skip, out);
 }
 
 // Sum over k of this thread's k-dependent terms of LongRangeClass::SingleAction on copy `mode`

@@ -88,8 +88,20 @@ int main() {
   } catch (const Error& e) {
     threw = e.code == PIMC_ERR_ARG;
   }
+  // d_dBeta through the ActionBase interface: the kinetic one needs no tables; the
+  // short-range one needs the fit's beta-derivative table, which this fit lacks
+  ActionBase* as_base = &kin;
+  const double kin_dbeta = as_base->d_dBeta(0, M, 0);
+  bool dbeta_needs_tables = false;
+  try {
+    sr.d_dBeta(0, M, 0);
+  } catch (const Error& e) {
+    dbeta_needs_tables = e.code == PIMC_ERR_STATE;
+  }
   std::printf("{\"s_old\": %.17g, \"s_new\": %.17g, \"s_new_after_reject\": %.17g, "
-              "\"s_old_after_accept\": %.17g, \"empty_request_throws\": %s, \"time_spent\": %.6g}\n",
-              s_old, s_new, s_new_after_reject, s_old_after_accept, threw ? "true" : "false", sr.TimeSpent);
+              "\"s_old_after_accept\": %.17g, \"empty_request_throws\": %s, \"time_spent\": %.6g, "
+              "\"kin_dbeta\": %.17g, \"dbeta_needs_tables\": %s}\n",
+              s_old, s_new, s_new_after_reject, s_old_after_accept, threw ? "true" : "false", sr.TimeSpent,
+              kin_dbeta, dbeta_needs_tables ? "true" : "false");
   return 0;
 }

@@ -118,3 +118,28 @@ def test_mixed_and_fp64_energies_agree_within_error_bars(gpu):
     d = res[abi.MIXED] - res[abi.FP64]
     err = math.sqrt(res[abi.FP64].var(ddof=1) / len(d) + res[abi.MIXED].var(ddof=1) / len(d))
     assert abs(d.mean()) < 4.0 * err + 1e-9
+
+
+@pytest.mark.parametrize("name", ["lj", "charged_lj", "hydrogen"])
+def test_d_dbeta_per_range_matches_oracle(oracle, gpu, name):
+    from tests.test_oracle_dbeta import LR, systems
+    system = dict(systems())[name]
+    io, ig = both(oracle, gpu, system)
+    oracle.bisection_run(io, system.num_levels, system.seed, 0, 25)
+    gpu.bisection_run(ig, system.num_levels, system.seed, 0, 25, abi.FP64)
+    M = system.replicas[0].num_time_slices
+    rng = np.random.default_rng(len(name))
+    reqs = []
+    for _ in range(24):
+        level = int(rng.integers(0, 3))
+        skip = 1 << level
+        n = int(rng.integers(1, M // skip + 1))
+        s1 = int(rng.integers(0, M - n * skip + 1))
+        reqs.append((int(rng.integers(0, 2)), int(rng.integers(0, 2)), s1, s1 + n * skip, level))
+    for action in (abi.ACTION_SHORT_RANGE, abi.ACTION_KINETIC, abi.ACTION_LONG_RANGE):
+        if action == LR and name == "lj":
+            continue
+        print("d_dBeta", name, action, flush=True)
+        want = oracle.d_dbeta(action, [(io[x[0]],) + x[1:] for x in reqs])
+        got = gpu.d_dbeta(action, [(ig[x[0]],) + x[1:] for x in reqs])
+        assert np.max(np.abs(got - want) / (1 + np.abs(want))) < 1e-12, action
