@@ -186,6 +186,60 @@ def report_other(args):
     print(json.dumps(out))
 
 
+def report_dropin(args):
+    """The drop-in mode: PIMC++ keeps its move loop and calls the library once per
+    SingleAction, as StageClass::StageAction does (StageClass.cc:124-136).  Times the calls a
+    level-4 bisection stage issues on one replica of the benchmark system (Kinetic and
+    ShortRange, OLD then NEW, each with one moved particle) and, for the hydrogen system, the
+    lowest stage's LongRange calls; reports calls per second and the short-range pair·slice
+    evaluations per second of this mode.  Inputs are resident; each call returns its result
+    to the host (one synchronisation per call)."""
+    from pimcplusplus_amd.tables import load_npz
+    out = {"metric": METRIC, "unit": "pair·slice evals/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "higher_is_better": True, "data": "synthetic",
+           "config": {"workload": "drop-in SingleAction calls, one request per call", "driver": "host loop"}}
+    lib = native.open_context(0)
+    for name, system, lr in (("lj256", workloads.config1(num_replicas=1), False),
+                             ("hydrogen", workloads.config2(load_npz(os.path.join(ROOT, "tests", "golden",
+                                                                                  "testcoul_fit2.npz")),
+                                                            num_replicas=1), True)):
+        ids = workloads.load(lib, system)
+        n = system.replicas[0].num_particles
+        W = 1 << system.num_levels
+        rng = np.random.default_rng(5)
+        calls, evals, t_sr, t_kin, t_lr = 0, 0, 0.0, 0.0, 0.0
+        for it in range(args.warmup + args.steps):
+            s1 = int(rng.integers(0, system.replicas[0].num_time_slices - W))
+            p = int(rng.integers(0, system.replicas[0].species_count[0]))
+            for level in range(system.num_levels - 1, -1, -1):
+                for mode in (abi.OLDMODE, abi.NEWMODE):
+                    req = [(ids[0], mode, s1, s1 + W, level, [p])]
+                    t0 = time.perf_counter()
+                    lib.action_kinetic(req)
+                    t1 = time.perf_counter()
+                    lib.action_short_range(req, abi.FP64)
+                    t2 = time.perf_counter()
+                    if lr and level == 0:
+                        lib.action_long_range(req, abi.FP64)
+                    t3 = time.perf_counter()
+                    if it >= args.warmup:
+                        t_kin += t1 - t0
+                        t_sr += t2 - t1
+                        t_lr += t3 - t2
+                        calls += 2 + (1 if lr and level == 0 else 0)
+                        evals += (n - 1) * (W >> level)
+        nsr = 2 * system.num_levels * args.steps
+        out[name] = {"short_range_calls_per_s": nsr / t_sr,
+                     "short_range_evals_per_s": evals / t_sr,
+                     "kinetic_us_per_call": 1e6 * t_kin / nsr,
+                     "short_range_us_per_call": 1e6 * t_sr / nsr,
+                     "long_range_us_per_call": (1e6 * t_lr / (2 * args.steps)) if lr else None,
+                     "calls": calls}
+    lib.close()
+    out["value"] = out["lj256"]["short_range_evals_per_s"]
+    print(json.dumps(out))
+
+
 def delta_s_error(nrep: int = 4, nmoves: int = 2500):
     """Max |dS_mixed - dS_fp64| over stages attempted on identical paths (chains run
     until their first differing decision)."""
@@ -313,8 +367,9 @@ def main():
     ap.add_argument("--no-dserr", action="store_true", help="skip the mixed-vs-FP64 dS and energy comparisons")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run an elementwise copy of MIB MiB (read MIB, write MIB) for counter scaling")
-    ap.add_argument("--workload", default="lj256", choices=("lj256", "lj64", "uneven", "hydrogen", "ocp"),
-                    help="lj256 (default) is the headline configuration; the others print a report line")
+    ap.add_argument("--workload", default="lj256", choices=("lj256", "lj64", "uneven", "hydrogen", "ocp", "dropin"),
+                    help="lj256 (default) is the headline configuration; the others print a report line "
+                         "(dropin: per-SingleAction calls of the drop-in mode)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
@@ -325,6 +380,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "dropin":
+        report_dropin(args)
+        return
     if args.workload != "lj256":
         report_other(args)
         return

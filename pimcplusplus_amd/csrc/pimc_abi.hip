@@ -1034,17 +1034,16 @@ int pimc_path_set_beads(pimc_ctx* ctx, int32_t replica, int32_t mode, int32_t pt
   if (!H || !xyz || mode < 0 || mode > 2 || ptcl < 0 || ptcl >= H->dev.core.N || slice1 < 0 || slice2 > H->M ||
       slice1 > slice2)
     return fail(PIMC_ERR_ARG, "path_set_beads: bad argument");
-  const RepCore& R = H->dev.core;
   HIP_TRY(hipSetDevice(ctx->device));
+  int rc = ctx->sync_descriptors();
+  if (rc) return rc;
+  const size_t bytes = (size_t)(slice2 - slice1 + 1) * 3 * sizeof(double);
+  if ((rc = ctx->d_in2.ensure(bytes))) return rc;
+  HIP_TRY(hipMemcpyAsync(ctx->d_in2.p, xyz, bytes, hipMemcpyHostToDevice, ctx->stream));
+  launch_set_beads(ctx->d_reps.as<ReplicaDev>(), replica, mode, ptcl, slice1, slice2, ctx->d_in2.as<double>(),
+                   ctx->stream);
+  HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(ctx->stream));
-  for (int m = 0; m < 2; m++) {
-    if (!(mode == 2 || mode == m)) continue;
-    double* base = (double*)(m ? H->path1.p : H->path0.p);
-    for (int s = slice1; s <= slice2; s++)
-      for (int d = 0; d < 3; d++)
-        HIP_TRY(hipMemcpy(base + ((size_t)s * 3 + d) * R.N + ptcl, xyz + 3 * (s - slice1) + d, sizeof(double),
-                          hipMemcpyHostToDevice));
-  }
   return PIMC_OK;
 }
 int pimc_permutation_get(pimc_ctx* ctx, int32_t replica, int32_t mode, int32_t* perm) {
@@ -1329,7 +1328,8 @@ int pimc_action_long_range(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t n
   if (!ctx || nreq < 0 || (nreq > 0 && (!reqs || !out))) return fail(PIMC_ERR_ARG, "action_long_range: bad argument");
   if (precision != PIMC_FP64 && precision != PIMC_MIXED) return fail(PIMC_ERR_ARG, "action_long_range: bad precision");
   if (nreq == 0) return PIMC_OK;
-  std::vector<int> strides(nreq);
+  std::vector<int> strides(nreq), rids(nreq);
+  int max_nk = 0;
   for (int i = 0; i < nreq; i++) {
     int rc = check_req(ctx, reqs[i]);
     if (rc) return rc;
@@ -1338,15 +1338,21 @@ int pimc_action_long_range(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t n
     if (R.lr_mask && reqs[i].level >= R.lr_levels)
       return fail(PIMC_ERR_ARG, "action_long_range: level beyond the long-range tables");
     strides[i] = 2 * (R.maxk0 + R.maxk1 + R.maxk2) + 3;
+    rids[i] = reqs[i].replica;
+    max_nk = std::max(max_nk, R.nk);
   }
+  const int kblocks = long_range_kblocks(max_nk);
   HIP_TRY(hipSetDevice(ctx->device));
   int rc = ctx->sync_descriptors();
   if (rc) return rc;
   rc = upload_reqs(ctx, reqs, nreq);
   if (rc) return rc;
-  if ((rc = ctx->d_out.ensure((size_t)nreq * sizeof(double)))) return rc;
+  if ((rc = ctx->d_out.ensure((size_t)nreq * sizeof(double))) ||
+      (rc = ctx->d_partials.ensure((size_t)nreq * kblocks * sizeof(double))))
+    return rc;
   launch_long_range(precision == PIMC_MIXED, ctx->d_reps.as<ReplicaDev>(), ctx->d_reqs.as<ReqDev>(), nreq,
-                    strides.data(), ctx->d_out.as<double>(), ctx->stream);
+                    strides.data(), rids.data(), kblocks, ctx->d_partials.as<double>(), ctx->d_out.as<double>(),
+                    ctx->stream);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(out, ctx->d_out.p, (size_t)nreq * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));

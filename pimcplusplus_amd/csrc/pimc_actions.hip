@@ -289,6 +289,25 @@ void launch_move_join(const ReplicaDev* reps, int rep, int N, int old_join, int 
 }
 
 // --------------------------------------------------------------------------
+// Path.SetPos of one particle over a slice range, one thread per (slice, dimension).
+// --------------------------------------------------------------------------
+__global__ void k_set_beads(const ReplicaDev* __restrict__ reps, int rep, int mode, int ptcl, int slice1, int nsl,
+                            const double* __restrict__ xyz) {
+  const RepCore& R = reps[rep].core;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * nsl) return;
+  const int slice = slice1 + t / 3, d = t % 3;
+  const size_t at = ((size_t)slice * 3 + d) * R.N + ptcl;
+  if (mode != 1) R.path0[at] = xyz[t];
+  if (mode != 0) R.path1[at] = xyz[t];
+}
+void launch_set_beads(const ReplicaDev* reps, int rep, int mode, int ptcl, int slice1, int slice2, const double* xyz,
+                      hipStream_t s) {
+  const int nsl = slice2 - slice1 + 1;
+  k_set_beads<<<(3 * nsl + 255) / 256, 256, 0, s>>>(reps, rep, mode, ptcl, slice1, nsl, xyz);
+}
+
+// --------------------------------------------------------------------------
 // PathClass::PutInBox (PathClass.h:897-921) or PutInBoxFast (PathClass.cc:1389-1407) of
 // n displacement vectors [n][3], one thread per component.
 // --------------------------------------------------------------------------

@@ -37,6 +37,10 @@ void launch_dist_disp(const ReplicaDev* reps, int rep, int mode, int64_t n, cons
 void launch_accept_reject(const ReplicaDev* reps, int rep, int slice1, int slice2, const int* ptcls, int n,
                           bool accept, hipStream_t s);
 void launch_move_join(const ReplicaDev* reps, int rep, int N, int old_join, int new_join, hipStream_t s);
+// Path.SetPos of one particle over slices [slice1, slice2] from xyz [(slice2-slice1+1)][3]
+// (device memory), into copy `mode` (2: both).
+void launch_set_beads(const ReplicaDev* reps, int rep, int mode, int ptcl, int slice1, int slice2, const double* xyz,
+                      hipStream_t s);
 void launch_put_in_box(const ReplicaDev* reps, int rep, int fast, int64_t n, double* v, hipStream_t s);
 void launch_permute_htable(const ReplicaDev* reps, int rep, int mode, int first, int n, int slice1, int slice2,
                            double four_lambda_beta_inv, double log_eps, float* out, hipStream_t s);
@@ -46,8 +50,12 @@ void launch_rho_all(const ReplicaDev* reps, int rep, int nslices, int ns, int nk
 // into out[nsl][ns][nk] (PathClass::CalcRho_ks_Fast, PathClass.cc:616-647).
 void launch_rho_slices(const ReplicaDev* reps, int rep, int mode, int slice1, int skip, int nsl, int ns, int nk,
                        int stride, d2v* out, hipStream_t s);
+// LongRangeClass::SingleAction requests: k blocks of each request in parallel, requests of
+// distinct replicas in one launch (in order otherwise); partials is [nreq][kblocks],
+// kblocks = long_range_kblocks(the largest nk of the requests' replicas).
+int long_range_kblocks(int max_nk);
 void launch_long_range(bool mixed, const ReplicaDev* reps, const ReqDev* reqs, int nreq, const int* strides,
-                       double* out, hipStream_t s);
+                       const int* replicas, int kblocks, double* partials, double* out, hipStream_t s);
 
 // d_dBeta of the actions over links [slice1, slice2) at `level` (slices slice1, slice1 +
 // 2^level, .., slice2) of copy `mode`, as partial sums per block: pairs {du, v} (the

@@ -77,3 +77,23 @@ def test_bisection_with_long_range_mixed(oracle, gpu):
             n += ns
     assert n > 100
     assert worst <= 1e-6
+
+
+def test_long_range_requests_batched_over_replicas_keep_their_order(oracle, gpu):
+    """Requests on distinct replicas share a launch, requests on one replica run in order:
+    a NEWMODE request updates rho_k NEW, and a later request on that replica reads it."""
+    sysm = workloads.charged_lj_system(n_plus=24, n_minus=24, m=32, kc_units=4.0, num_replicas=3)
+    io, ig = both(oracle, gpu, sysm)
+    rng = np.random.default_rng(41)
+    for k, (a, b) in enumerate(zip(io, ig)):
+        new = sysm.replicas[k].beads.copy()
+        new[3:12, [2, 30]] += rng.normal(scale=0.1, size=(9, 2, 3))
+        oracle.path_set(a, new, abi.NEWMODE)
+        gpu.path_set(b, new, abi.NEWMODE)
+    pattern = [(0, 1, 2, 10, 0, [2]), (1, 1, 2, 10, 1, [30]), (0, 0, 2, 10, 0, [2]), (2, 1, 0, 16, 0, [2, 30]),
+               (0, 1, 4, 12, 2, [30]), (1, 0, 0, 8, 0, [2]), (2, 1, 8, 16, 1, [30]), (0, 1, 2, 10, 0, [2])]
+    want = oracle.action_long_range([(io[x[0]],) + x[1:] for x in pattern])
+    # the requests write rho_k NEW only from OLD, so the same pattern runs again from the same state
+    for prec, tol in ((abi.FP64, 1e-11), (abi.MIXED, 1e-6)):
+        got = gpu.action_long_range([(ig[x[0]],) + x[1:] for x in pattern], prec)
+        assert np.max(np.abs(got - want) / (1 + np.abs(want))) < tol, prec
