@@ -39,11 +39,17 @@ FLOP_PER_EVAL = 180.0      # FP64 flops of one pair·slice evaluation (DistDisp,
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_bisection_lj_n256_m64_l4_traffic.md")
 
 
-def measured_traffic(replicas: int, moves_per_step: int):
-    """HBM bytes per launch from the committed counter summary (FETCH_SIZE and WRITE_SIZE of
-    the same workload, scaled by a copy kernel of known size), when it covers this workload."""
+def traffic_profile(workload: str) -> str:
+    """Committed FETCH_SIZE/WRITE_SIZE summary of the generic kernel on `workload`."""
+    return os.path.join(ROOT, "profiles", f"k_bisection_generic_{workload}_traffic.md")
+
+
+def measured_traffic(want: str, path: str = TRAFFIC_PROFILE):
+    """HBM bytes per launch from a committed counter summary (FETCH_SIZE and WRITE_SIZE of
+    the same workload, scaled by a copy kernel of known size), when its workload string
+    holds `want` (replica count, moves per launch and, for a fixed plan, the plan)."""
     try:
-        text = open(TRAFFIC_PROFILE).read()
+        text = open(path).read()
     except OSError:
         return None
     block = text.split("```json", 1)[1].split("```", 1)[0] if "```json" in text else ""
@@ -51,7 +57,6 @@ def measured_traffic(replicas: int, moves_per_step: int):
         d = json.loads(block)
     except ValueError:
         return None
-    want = f"{replicas} replicas, {moves_per_step} moves/launch"
     return d.get("traffic_bytes_per_launch") if want in d.get("workload", "") else None
 
 
@@ -115,7 +120,7 @@ def load_system(lib, system):
     return ids
 
 
-def run_gpu(system, precision, steps, warmup, moves_per_step, tune=False, graph=False, paths_out=None):
+def run_gpu(system, precision, steps, warmup, moves_per_step, tune=False, graph=False, paths_out=None, plan=None):
     """W untimed steps, then K steps queued back to back on the context's stream;
     the timed region is bracketed by stream synchronisations (the stats take).
     tune: the autotuner picks the launch plan first (on moves the chain then
@@ -124,6 +129,8 @@ def run_gpu(system, precision, steps, warmup, moves_per_step, tune=False, graph=
     lib = native.open_context(0)
     ids = load_system(lib, system)
     L, seed = system.num_levels, system.seed
+    if plan is not None:
+        lib.set_plan(*plan)
     plan = lib.bisection_tune(ids, L, seed, 0, moves_per_step, precision) if tune else lib.get_plan()
     move = 0
     if graph:
@@ -175,15 +182,94 @@ def report_other(args):
            "config": {"workload": system.name, "replicas": len(system.replicas),
                       "moves_per_step": args.moves_per_step, "num_levels": system.num_levels,
                       "driver": "tuned plan, captured sweep graph"}}
-    for name, prec in (("mixed", abi.MIXED), ("fp64", abi.FP64)):
+    nmean = float(np.mean([r.num_particles for r in system.replicas]))
+    fixed = None
+    if args.plan:
+        b, ls, pers, cl = (int(x) for x in args.plan.split(","))
+        fixed = (b, ls, bool(pers), cl)
+        out["config"]["driver"] = f"fixed plan {args.plan}, eager launches"
+    runs = (("mixed", abi.MIXED), ("fp64", abi.FP64))
+    if args.only:
+        runs = tuple(r for r in runs if r[0] == args.only)
+    for name, prec in runs:
         tot, wall, dev_ms, plan = run_gpu(system, prec, args.steps, args.warmup, args.moves_per_step,
-                                          tune=True, graph=True)
+                                          tune=fixed is None, graph=fixed is None, plan=fixed)
+        sr_bytes, lr_bytes = workload_bytes(tot, nmean)
+        kms = dev_ms / args.steps
         out[name] = {"evals_per_s": tot.pair_evals / wall, "ms_per_step": 1e3 * wall / args.steps,
-                     "kernel_ms_avg": dev_ms / args.steps, "k_evals_per_s": tot.k_evals / wall,
-                     "acceptance": tot.accepted / max(1, tot.moves), "plan": plan}
+                     "kernel_ms_avg": kms, "k_evals_per_s": tot.k_evals / wall,
+                     "acceptance": tot.accepted / max(1, tot.moves), "plan": plan,
+                     "bytes_per_launch": (sr_bytes + lr_bytes) / args.steps,
+                     "long_range_bytes_per_launch": lr_bytes / args.steps}
+    if args.only:
+        print(json.dumps({"only": args.only, **out[args.only]}))
+        return
     out["value"] = out["mixed"]["evals_per_s"]
     out["mixed_over_fp64"] = out["mixed"]["evals_per_s"] / out["fp64"]["evals_per_s"]
+    achieved = {k: out[k]["bytes_per_launch"] / (out[k]["kernel_ms_avg"] * 1e-3) / 1e9 for k in ("mixed", "fp64")}
+    out["roofline"] = {"bound": "hbm", "achieved": achieved["mixed"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": achieved["mixed"] / HBM_PEAK_GBS,
+                       "traffic": measured_traffic(f"{len(system.replicas)} replicas, {args.moves_per_step} "
+                                                   f"moves/launch, plan {args.plan}", traffic_profile(args.workload))
+                       if args.plan else None,
+                       "fp64": {"achieved": achieved["fp64"], "frac": achieved["fp64"] / HBM_PEAK_GBS},
+                       "kernel": "k_bisection (the workload's variant and tuned plan)",
+                       "note": "algorithmic bytes: 24 (K+1)(N+1) + 16 per executed stage (short range; N the mean "
+                               "particle count) plus 16 B per counted k-term (32 B per k.slice.species update of "
+                               "the lowest stage, counted once per OLD/NEW pair), over the kernel's HIP-event time"}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds / 2, system=system, moves_per_call=4)
+    if args.workload == "ocp" and not args.no_dserr:
+        out["estimators_mixed_vs_fp64"] = estimators_check(system)
     print(json.dumps(out))
+
+
+def estimators_check(system, blocks: int = 10, moves: int = 1024, bins: int = 40, drop: int = 2):
+    """The OCP's acceptance check: the thermodynamic energy and the pair correlation g(r) of
+    mixed chains against FP64 chains that share no random numbers (the mixed chains use
+    another seed).  Each replica contributes its mean over the blocks after `drop` blocks of
+    equilibration.  Replica r starts from the same configuration in both precisions, so the
+    comparison is paired: the mean over replicas of (mixed - FP64) against the standard error
+    of those differences, for the energy and for every g(r) bin (largest |z|, mean z^2)."""
+    M = system.replicas[0].num_time_slices
+    L = system.replicas[0].box[0]
+    res = {}
+    for prec, seed in ((abi.FP64, system.seed), (abi.MIXED, system.seed + 7919)):
+        lib = native.open_context(0)
+        ids = load_system(lib, system)
+        lib.bisection_tune(ids, system.num_levels, seed, 0, 8, prec)
+        e, g = [], []
+        for b in range(blocks):
+            lib.bisection_run(ids, system.num_levels, seed, moves * b, moves, prec, records=False, wait=False)
+            e.append([x.total / M for x in lib.energy(ids)])
+            g.append([lib.pair_correlation(r, 0, 0, 0.0, L / 2, bins + 1) for r in ids])
+        lib.close()
+        res[prec] = (np.array(e[drop:]).mean(axis=0), np.array(g[drop:], dtype=float).mean(axis=0))
+    (ea, ga), (eb, gb) = res[abi.FP64], res[abi.MIXED]
+    n = len(ea)
+    de, dg = eb - ea, gb - ga
+    e_err = float(de.std(ddof=1) / np.sqrt(n))
+    g_err = dg.std(axis=0, ddof=1) / np.sqrt(n)
+    ok = g_err > 0
+    z = dg.mean(axis=0)[ok] / g_err[ok]
+    return {"energy": {"fp64_per_slice": float(ea.mean()), "mixed_per_slice": float(eb.mean()),
+                       "difference": float(de.mean()), "std_error": e_err,
+                       "within_4_sigma": abs(float(de.mean())) < 4 * e_err},
+            "g_of_r": {"bins": bins, "range": [0.0, L / 2], "max_abs_z": float(np.max(np.abs(z))),
+                       "mean_z2": float(np.mean(z * z)), "bins_compared": int(ok.sum()),
+                       "within_error_bars": bool(np.max(np.abs(z)) < 4.5)},
+            "comparison": "paired per replica (same start, independent random numbers)",
+            "sample": f"{n} replicas per precision, {blocks} blocks of {moves} moves, the first {drop} dropped; "
+                      "mixed chains on another seed"}
+
+
+def workload_bytes(tot, nmean: float):
+    """Algorithmic HBM bytes of the summarised launches: the short-range stage bytes of
+    bytes_per_launch with the mean N, and 32 B per k.slice.species term of the long-range
+    stage (rho_k read and update), which the counters count once for OLD and once for NEW."""
+    sr = bytes_per_launch(tot, max(2.0, nmean))
+    lr = 16.0 * tot.k_evals
+    return sr, lr
 
 
 def report_dropin(args):
@@ -240,10 +326,13 @@ def report_dropin(args):
     print(json.dumps(out))
 
 
-def delta_s_error(nrep: int = 4, nmoves: int = 2500):
+def delta_s_error(nrep: int = 4, nmoves: int = 2500, seed_offset: int = 0, replica_offset: int = 0):
     """Max |dS_mixed - dS_fp64| over stages attempted on identical paths (chains run
-    until their first differing decision)."""
-    system = workloads.config1(num_replicas=nrep)
+    until their first differing decision).  seed_offset / replica_offset select a sample
+    other than the one the FP32 limit was chosen on."""
+    system = workloads.config1(num_replicas=nrep + replica_offset)
+    system = workloads.System(system.name, system.tables, system.desired_tau, system.replicas[replica_offset:],
+                              system.num_levels, system.seed + seed_offset)
     out = {}
     for prec in (abi.FP64, abi.MIXED):
         lib = native.open_context(0)
@@ -265,21 +354,21 @@ def delta_s_error(nrep: int = 4, nmoves: int = 2500):
     return err, n
 
 
-def energy_check(nrep: int = 256, blocks: int = 8, moves: int = 40):
-    """Thermodynamic energy per time slice of the benchmark system from the mixed and the FP64
-    chains (same seeds, each replica independent): block averages after two blocks of
-    equilibration, compared through the replicas' spread.  Returns the two means, the
-    standard error of their difference and whether they agree within four of it."""
+def energy_check(nrep: int = 256, blocks: int = 8, moves: int = 400):
+    """Thermodynamic energy per time slice of the benchmark system from mixed and FP64 chains
+    that share no random numbers (the mixed chains use another seed): each replica's mean over
+    the blocks after two blocks of equilibration, the two precisions compared through the
+    spread over their independent replicas.  Returns the two means, the standard error of
+    their difference and whether they agree within four of it."""
     system = workloads.config1(num_replicas=nrep)
     m = system.replicas[0].num_time_slices
     res = {}
-    for prec in (abi.FP64, abi.MIXED):
+    for prec, seed in ((abi.FP64, system.seed), (abi.MIXED, system.seed + 7919)):
         lib = native.open_context(0)
         ids = load_system(lib, system)
         vals = []
         for b in range(blocks):
-            lib.bisection_run(ids, system.num_levels, system.seed, moves * b, moves, prec, records=False,
-                              wait=False)
+            lib.bisection_run(ids, system.num_levels, seed, moves * b, moves, prec, records=False, wait=False)
             vals.append([e.total / m for e in lib.energy(ids)])
         lib.close()
         res[prec] = np.array(vals[2:]).mean(axis=0)
@@ -288,7 +377,8 @@ def energy_check(nrep: int = 256, blocks: int = 8, moves: int = 40):
     d = float(b.mean() - a.mean())
     return {"fp64_K_per_slice": float(a.mean()), "mixed_K_per_slice": float(b.mean()), "difference": d,
             "std_error": err, "within_4_sigma": abs(d) < 4.0 * err,
-            "sample": f"{nrep} replicas, {blocks} blocks of {moves} moves, the first two blocks dropped"}
+            "comparison": "independent chains: mixed with another seed, error from the spread over replicas",
+            "sample": f"{nrep} replicas per precision, {blocks} blocks of {moves} moves, the first two blocks dropped"}
 
 
 def oracle_lib():
@@ -298,27 +388,54 @@ def oracle_lib():
     return ctypes.CDLL(path)
 
 
-def cpu_baseline(seconds: float = 8.0, threads: int | None = None):
+def host_cores():
+    """Cores this process may use: its CPU affinity, capped by the cgroup CPU quota."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()
+        if quota != "max":
+            n = min(n, max(1, int(float(quota) / float(period))))
+    except (OSError, ValueError):
+        pass
+    return n
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(seconds: float = 8.0, threads: int | None = None, system=None, moves_per_call: int = 20):
     """The oracle (the C++ restatement of the reference's CPU path) on host cores:
-    one replica per thread, the reference's clone-parallel model."""
+    one replica per thread, the reference's clone-parallel model; every core the process
+    may use (affinity and cgroup quota).  system: the workload whose replicas the threads
+    run (default: the benchmark's N=256, M=64 system)."""
     lib = oracle_lib()
-    threads = threads or max(1, min(16, len(os.sched_getaffinity(0))))
-    system = workloads.config1(num_replicas=threads)
+    threads = threads or host_cores()
+    if system is None:
+        system = workloads.config1(num_replicas=threads)
+    reps = [system.replicas[r % len(system.replicas)] for r in range(threads)]
     ctxs = []
     for r in range(threads):
         o = abi.PimcLib(lib)
-        sub = workloads.System(system.name, system.tables, system.desired_tau, [system.replicas[r]],
+        sub = workloads.System(system.name, system.tables, system.desired_tau, [reps[r]],
                                system.num_levels, system.seed)
         ids = workloads.load(o, sub)
         ctxs.append((o, ids))
 
     def chunk(o, ids, res, k):
-        evals, t0, move = 0, time.perf_counter(), 0
+        evals, kev, t0, move = 0, 0, time.perf_counter(), 0
         while time.perf_counter() - t0 < seconds:
-            _, st = o.bisection_run(ids, system.num_levels, system.seed, move, 20, records=False)
+            _, st = o.bisection_run(ids, system.num_levels, system.seed, move, moves_per_call, records=False)
             evals += st.pair_evals
-            move += 20
-        res[k] = (evals, time.perf_counter() - t0)
+            kev += st.k_evals
+            move += moves_per_call
+        res[k] = (evals, time.perf_counter() - t0, kev)
 
     # one thread alone first (the reference's per-rank model), then all threads together
     single = [None]
@@ -331,13 +448,19 @@ def cpu_baseline(seconds: float = 8.0, threads: int | None = None):
     for t in ths:
         t.join()
     wall = time.perf_counter() - t0
-    total = sum(e for e, _ in res)
+    total = sum(r[0] for r in res)
+    ktotal = sum(r[2] for r in res)
     for o, _ in ctxs:
         o.close()
-    return {"value": total / wall, "unit": "pair·slice evals/s", "cores": threads, "kind": "port",
-            "sample": f"config N=256 M=64 level-4, one replica per thread, {seconds:.0f} s per thread "
-                      f"({total} evaluations); single thread {single[0][0] / single[0][1]:.4g} evals/s",
-            "single_core_value": single[0][0] / single[0][1]}
+    out = {"value": total / wall, "unit": "pair·slice evals/s", "cores": threads, "kind": "port",
+           "sample": f"{system.name}, one replica per thread, {seconds:.0f} s per thread "
+                     f"({total} evaluations); single thread {single[0][0] / single[0][1]:.4g} evals/s",
+           "single_core_value": single[0][0] / single[0][1],
+           "cpu_model": cpu_model(), "host_cores": host_cores(),
+           "build": "g++ -O3 -march=x86-64-v3 -ffp-contract=off (oracle/Makefile)"}
+    if ktotal:
+        out["k_evals_per_s"] = ktotal / wall
+    return out
 
 
 def calibrate_copy(mib: int):
@@ -367,6 +490,9 @@ def main():
     ap.add_argument("--no-dserr", action="store_true", help="skip the mixed-vs-FP64 dS and energy comparisons")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run an elementwise copy of MIB MiB (read MIB, write MIB) for counter scaling")
+    ap.add_argument("--plan", default=None, metavar="BLOCK,SPLIT,PERSISTENT,CLUSTER",
+                    help="non-headline workloads: this launch plan with eager launches instead of the tuner "
+                         "and a captured sweep graph (profiling runs)")
     ap.add_argument("--workload", default="lj256", choices=("lj256", "lj64", "uneven", "hydrogen", "ocp", "dropin"),
                     help="lj256 (default) is the headline configuration; the others print a report line "
                          "(dropin: per-SingleAction calls of the drop-in mode)")
@@ -416,6 +542,8 @@ def main():
         print(json.dumps({"only": args.only, **res[args.only]}))
         return
     err, nstages = (None, 0) if args.no_dserr else delta_s_error()
+    # the same check on a sample the FP32 limit was not chosen on (another seed and replicas)
+    err2, nstages2 = (None, 0) if args.no_dserr else delta_s_error(seed_offset=104729, replica_offset=16)
     energies = None if args.no_dserr else energy_check()
     mixed = res["mixed"]
     achieved = mixed["bytes_per_launch"] / (mixed["kernel_ms_avg"] * 1e-3) / 1e9
@@ -451,12 +579,14 @@ def main():
         "mixed_over_fp64": mixed["evals_per_s"] / res["fp64"]["evals_per_s"],
         "delta_s_max_err": err,
         "delta_s_stages_compared": nstages,
+        "delta_s_fresh_sample": {"max_err": err2, "stages": nstages2,
+                                 "sample": "4 replicas x 2500 moves, seed + 104729, replicas 16-19"},
         "energy_mixed_vs_fp64": energies,
         "acceptance": mixed["acceptance"],
         "stages_per_move": mixed["stages_per_move"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.replicas, args.moves_per_step),
+                     "traffic": measured_traffic(f"{args.replicas} replicas, {args.moves_per_step} moves/launch"),
                      "kernel": "k_bisection<NT=2,MIXED>",
                      "note": "algorithmic bytes = 24 (K+1)(N+1) + 16 per executed stage, over the "
                              "kernel's HIP-event time; traffic from the committed FETCH_SIZE/WRITE_SIZE "
