@@ -162,10 +162,10 @@ def other_workload(name: str, replicas: int):
     """The configurations besides the headline one, for reports."""
     if name == "uneven":
         return workloads.config3(num_replicas=128)
-    if name == "hydrogen":
+    if name in ("hydrogen", "hydrogen15k"):
         from pimcplusplus_amd.tables import load_npz
         return workloads.config2(load_npz(os.path.join(ROOT, "tests", "golden", "testcoul_fit2.npz")),
-                                 num_replicas=replicas)
+                                 num_replicas=replicas, m=168 if name == "hydrogen15k" else 128)
     if name == "ocp":
         return workloads.config4(num_replicas=16)
     if name == "lj64":
@@ -181,6 +181,7 @@ def report_other(args):
            "warmup": args.warmup, "higher_is_better": True, "data": "synthetic",
            "config": {"workload": system.name, "replicas": len(system.replicas),
                       "moves_per_step": args.moves_per_step, "num_levels": system.num_levels,
+                      "tau": system.replicas[0].tau, "temperature_k": workloads.temperature_k(system),
                       "driver": "tuned plan, captured sweep graph"}}
     nmean = float(np.mean([r.num_particles for r in system.replicas]))
     fixed = None
@@ -493,9 +494,10 @@ def main():
     ap.add_argument("--plan", default=None, metavar="BLOCK,SPLIT,PERSISTENT,CLUSTER",
                     help="non-headline workloads: this launch plan with eager launches instead of the tuner "
                          "and a captured sweep graph (profiling runs)")
-    ap.add_argument("--workload", default="lj256", choices=("lj256", "lj64", "uneven", "hydrogen", "ocp", "dropin"),
+    ap.add_argument("--workload", default="lj256", choices=("lj256", "lj64", "uneven", "hydrogen", "hydrogen15k", "ocp", "dropin"),
                     help="lj256 (default) is the headline configuration; the others print a report line "
-                         "(dropin: per-SingleAction calls of the drop-in mode)")
+                         "(hydrogen15k: the hydrogen system at M=168, T=15,037 K; dropin: per-SingleAction "
+                         "calls of the drop-in mode)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)

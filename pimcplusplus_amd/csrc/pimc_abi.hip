@@ -110,41 +110,58 @@ void make_log(HostGrid& g, double s, double e, int n) {
   for (int i = 0; i < n; i++) g.x[i] = s * pow(g.spacing, (double)i);
 }
 
-// Second derivatives of one column (Numerical Recipes tridiagonal, the
-// reference's MultiCubicSpline::Update); deriv > 0.99e30 means natural.
+// Solves the tridiagonal system lo[i] u[i-1] + di[i] u[i] + up[i] u[i+1] = rhs[i]
+// (lo[0] and up[n-1] unused) by forward elimination and back substitution; the
+// systems below are diagonally dominant, so no pivoting is needed.  rhs is overwritten
+// with u.
+void solve_tridiagonal(const std::vector<double>& lo, const std::vector<double>& di, const std::vector<double>& up,
+                       std::vector<double>& rhs) {
+  const size_t n = di.size();
+  std::vector<double> upn(n);   // the super-diagonal after elimination, with a unit diagonal
+  double piv = di[0];
+  upn[0] = up[0] / piv;
+  rhs[0] /= piv;
+  for (size_t i = 1; i < n; i++) {
+    piv = di[i] - lo[i] * upn[i - 1];
+    upn[i] = i + 1 < n ? up[i] / piv : 0.0;
+    rhs[i] = (rhs[i] - lo[i] * rhs[i - 1]) / piv;
+  }
+  for (size_t i = n - 1; i-- > 0;) rhs[i] -= upn[i] * rhs[i + 1];
+}
+
+// Knot second derivatives m_j of the C2 cubic spline through (x_j, y_j), y_j = y[j * stride]
+// (the setup of MultiCubicSpline::Update, CubicSpline.cc:77-128).  With h_j = x_{j+1} - x_j
+// and secant slopes s_j = (y_{j+1} - y_j) / h_j, continuity of the first derivative at an
+// interior knot reads
+//   h_{j-1} m_{j-1} + 2 (h_{j-1} + h_j) m_j + h_j m_{j+1} = 6 (s_j - s_{j-1}).
+// An end with a given slope f adds 2 h_0 m_0 + h_0 m_1 = 6 (s_0 - f) (first knot) or
+// h m_{n-2} + 2 h m_{n-1} = 6 (f - s_{n-2}) (last knot); a slope > 0.99e30 means a
+// natural end, m = 0.
 void spline_d2(const std::vector<double>& x, const double* y, int stride, double start_deriv, double end_deriv,
                std::vector<double>& d2) {
-  const int N = (int)x.size();
-  d2.assign(N, 0.0);
-  std::vector<double> U(N, 0.0);
-  auto Y = [&](int j) { return y[(size_t)j * stride]; };
-  if (start_deriv > 0.99e30) {
-    d2[0] = 0.0;
-    U[0] = 0.0;
-  } else {
-    d2[0] = -0.5;
-    U[0] = (3.0 / (x[1] - x[0])) * ((Y(1) - Y(0)) / (x[1] - x[0]) - start_deriv);
+  const int n = (int)x.size();
+  std::vector<double> lo(n, 0.0), di(n, 1.0), up(n, 0.0);
+  d2.assign(n, 0.0);
+  auto secant = [&](int j) { return (y[(size_t)(j + 1) * stride] - y[(size_t)j * stride]) / (x[j + 1] - x[j]); };
+  for (int j = 1; j < n - 1; j++) {
+    lo[j] = x[j] - x[j - 1];
+    up[j] = x[j + 1] - x[j];
+    di[j] = 2.0 * (lo[j] + up[j]);
+    d2[j] = 6.0 * (secant(j) - secant(j - 1));
   }
-  d2[N - 1] = 0.0;
-  for (int j = 1; j < N - 1; j++) {
-    double sig = (x[j] - x[j - 1]) / (x[j + 1] - x[j - 1]);
-    double p = sig * d2[j - 1] + 2.0;
-    d2[j] = (sig - 1.0) / p;
-    U[j] = (6.0 * ((Y(j + 1) - Y(j)) / (x[j + 1] - x[j]) - (Y(j) - Y(j - 1)) / (x[j] - x[j - 1])) /
-                (x[j + 1] - x[j - 1]) -
-            sig * U[j - 1]) /
-           p;
+  if (start_deriv <= 0.99e30) {
+    const double h = x[1] - x[0];
+    di[0] = 2.0 * h;
+    up[0] = h;
+    d2[0] = 6.0 * (secant(0) - start_deriv);
   }
-  double Un, Qn;
-  if (end_deriv > 0.99e30) {
-    Un = 0.0;
-    Qn = 0.0;
-  } else {
-    Qn = 0.5;
-    Un = (3.0 / (x[N - 1] - x[N - 2])) * (end_deriv - (Y(N - 1) - Y(N - 2)) / (x[N - 1] - x[N - 2]));
+  if (end_deriv <= 0.99e30) {
+    const double h = x[n - 1] - x[n - 2];
+    lo[n - 1] = h;
+    di[n - 1] = 2.0 * h;
+    d2[n - 1] = 6.0 * (end_deriv - secant(n - 2));
   }
-  d2[N - 1] = (Un - Qn * U[N - 2]) / (Qn * d2[N - 2] + 1.0);
-  for (int k = N - 2; k >= 0; k--) d2[k] = d2[k] * d2[k + 1] + U[k];
+  solve_tridiagonal(lo, di, up, d2);
 }
 
 struct HostFit {
@@ -156,35 +173,33 @@ struct HostFit {
   int num_levels() const { return dev.num_tau - dev.taupos; }
 };
 
-// Slopes of one line of a bicubic table: the first-derivative tridiagonal solve of
-// BicubicSpline::XUpdate / YUpdate / BiUpdate (BicubicSpline.cc:20-154), natural ends.
-// v[i * stride] are the values along the line, d[i * stride] receives the slopes.
+// Knot slopes d_j of the C2 cubic through the values v_j = v[j * stride] along one line
+// of a bicubic table, written to d[j * stride] (the slopes BicubicSpline::XUpdate /
+// YUpdate / BiUpdate set up, BicubicSpline.cc:20-154).  On [x_j, x_{j+1}] the cubic
+// Hermite piece with end slopes d_j, d_{j+1} has second derivative
+// (6 s_j - 4 d_j - 2 d_{j+1}) / h_j at its left end and (4 d_{j+1} + 2 d_j - 6 s_j) / h_j
+// at its right end (h_j = x_{j+1} - x_j, s_j the secant slope).  Equal second
+// derivatives at an interior knot give
+//   h_j d_{j-1} + 2 (h_{j-1} + h_j) d_j + h_{j-1} d_{j+1} = 3 (h_j s_{j-1} + h_{j-1} s_j),
+// and zero second derivatives at the ends (natural) give 2 d_0 + d_1 = 3 s_0 and
+// d_{n-2} + 2 d_{n-1} = 3 s_{n-2}.
 void bc_slopes(const std::vector<double>& x, const double* v, double* d, size_t stride) {
-  const int N = (int)x.size();
-  std::vector<double> mu(N);
-  auto V = [&](int i) { return v[(size_t)i * stride]; };
-  auto Dd = [&](int i) -> double& { return d[(size_t)i * stride]; };
-  Dd(0) = 1.5 * (V(1) - V(0)) / (x[1] - x[0]);
-  Dd(N - 1) = 1.5 * (V(N - 1) - V(N - 2)) / (x[N - 1] - x[N - 2]);
-  mu[0] = 0.5;
-  for (int j = 1; j < (N - 1); j++) {
-    double lambda = 0.5 * (x[j + 1] - x[j]) / (x[j + 1] - x[j - 1]);
-    mu[j] = 0.5 - lambda;
-    Dd(j) = 3.0 * (lambda * (V(j) - V(j - 1)) / (x[j] - x[j - 1]) + mu[j] * (V(j + 1) - V(j)) / (x[j + 1] - x[j]));
-    double cj = 1.0 - lambda * mu[j - 1];
-    Dd(j) -= lambda * Dd(j - 1);
-    mu[j] /= cj;
-    Dd(j) /= cj;
+  const int n = (int)x.size();
+  std::vector<double> lo(n, 0.0), di(n, 2.0), up(n, 0.0), rhs(n);
+  auto secant = [&](int j) { return (v[(size_t)(j + 1) * stride] - v[(size_t)j * stride]) / (x[j + 1] - x[j]); };
+  up[0] = 1.0;
+  rhs[0] = 3.0 * secant(0);
+  for (int j = 1; j < n - 1; j++) {
+    const double hl = x[j] - x[j - 1], hr = x[j + 1] - x[j];
+    lo[j] = hr;
+    di[j] = 2.0 * (hl + hr);
+    up[j] = hl;
+    rhs[j] = 3.0 * (hr * secant(j - 1) + hl * secant(j));
   }
-  int j = N - 1;
-  double lambda = 0.5;
-  mu[j] = 0.5 - lambda;
-  Dd(j) = 3.0 * (lambda * (V(j) - V(j - 1)) / (x[j] - x[j - 1]));
-  double cj = 1.0 - lambda * mu[j - 1];
-  Dd(j) -= lambda * Dd(j - 1);
-  mu[j] /= cj;
-  Dd(j) /= cj;
-  for (j = N - 2; j >= 0; j--) Dd(j) -= mu[j] * Dd(j + 1);
+  lo[n - 1] = 1.0;
+  rhs[n - 1] = 3.0 * secant(n - 2);
+  solve_tridiagonal(lo, di, up, rhs);
+  for (int j = 0; j < n; j++) d[(size_t)j * stride] = rhs[j];
 }
 
 int make_host_grid(int type, double start, double end, int n, HostGrid& g) {

@@ -300,9 +300,22 @@ def hydrogen_system(dm_table: DavidTable, num_replicas: int = 1, n_pairs: int = 
                   meta={"rs": rs, "box": box, "kc": kc, "alpha": alpha, "tau": tau})
 
 
-def config2(dm_table: DavidTable, num_replicas: int = 1) -> System:
-    """32 e + 32 p, rs=1.75 (L = 8.9561 bohr), M=128, tau=0.125 Ha^-1, kc = 5 2pi/L (242 k)."""
-    return hydrogen_system(dm_table, num_replicas=num_replicas)
+def config2(dm_table: DavidTable, num_replicas: int = 1, m: int = 128) -> System:
+    """32 e + 32 p, rs=1.75 (L = 8.9561 bohr), tau=0.125 Ha^-1, kc = 5 2pi/L (242 k).
+
+    The e-p table tabulates tau = 2^i / 64 Ha^-1 only, so tau stays 0.125 and the temperature
+    follows from M: T = 1 / (M tau) Ha is 19,737 K at M = 128 and 15,037 K at M = 168."""
+    return hydrogen_system(dm_table, num_replicas=num_replicas, m=m)
+
+
+HARTREE_K = 315775.02   # 1 Ha / k_B in kelvin
+
+
+def temperature_k(system: System) -> float:
+    """T = 1 / (M tau) of the first replica in kelvin: tau is in Ha^-1 for the Coulomb systems
+    (those with an rs) and in K^-1 for the Lennard-Jones ones."""
+    r = system.replicas[0]
+    return (HARTREE_K if "rs" in system.meta else 1.0) / (r.num_time_slices * r.tau)
 
 
 def ocp_system(num_replicas: int = 16, n: int = 1024, rs: float = 1.0, m: int = 128, tau: float = 0.05,

@@ -34,9 +34,12 @@ def _mixed_delta_s_error(ro, rm, nrep, nmoves):
     return worst, n
 
 
-def test_config2_hydrogen_full_size(oracle, gpu):
-    sysm = workloads.config2(load_npz(os.path.join(os.path.dirname(__file__), "golden", "testcoul_fit2.npz")), num_replicas=2)
-    print("config2: 32 e + 32 p, M=128, level 4, breakup tables", flush=True)
+@pytest.mark.parametrize("m", [128, 168])
+def test_config2_hydrogen_full_size(oracle, gpu, m):
+    # M = 128 as stated (T = 19,737 K at tau = 0.125) and M = 168 (T = 15,037 K, the stated temperature)
+    sysm = workloads.config2(load_npz(os.path.join(os.path.dirname(__file__), "golden", "testcoul_fit2.npz")),
+                             num_replicas=2, m=m)
+    print(f"config2: 32 e + 32 p, M={m}, T={workloads.temperature_k(sysm):.0f} K, level 4, breakup tables", flush=True)
     io, ig = workloads.load(oracle, sysm), workloads.load(gpu, sysm)
     ro, so = oracle.bisection_run(io, sysm.num_levels, sysm.seed, 0, 150)
     rg, sg = gpu.bisection_run(ig, sysm.num_levels, sysm.seed, 0, 150, abi.FP64)
