@@ -227,6 +227,9 @@ struct HostReplica {
   int M = 0;
   // set once a cycle has been applied: the permutation may no longer be the identity
   bool maybe_permuted = false;
+  // set by the resident chain, which updates rho_k OLD only: copy NEW is refreshed from OLD
+  // (refresh_rho1) before any call that reads or writes it
+  bool rho1_stale = false;
 };
 
 // PathClass::SetupkVecs3D with Include (PathClass.cc:281-292, 534-586): integer
@@ -265,6 +268,7 @@ void setup_kvecs(HostReplica& H, double kcut) {
 // A captured sweep and the device state its replays use.
 struct SweepGraph {
   DevBuf ids, pos, stats, queue, first, bar, part;
+  std::vector<int> reps;   // the replicas a replay moves
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int num_moves = 0;
@@ -336,6 +340,16 @@ struct pimc_ctx {
 };
 
 namespace {
+
+// Copy NEW of rho_k <- copy OLD where a chain run left NEW behind (stream-ordered after it).
+int refresh_rho1(pimc_ctx* ctx, HostReplica* H) {
+  if (!H->rho1_stale) return PIMC_OK;
+  const RepCore& R = H->dev.core;
+  HIP_TRY(hipMemcpyAsync(H->rho1.p, H->rho0.p, (size_t)R.nslices * R.ns * R.nk * sizeof(d2v),
+                         hipMemcpyDeviceToDevice, ctx->stream));
+  H->rho1_stale = false;
+  return PIMC_OK;
+}
 
 
 int check_req(pimc_ctx* ctx, const pimc_action_req& q) {
@@ -1084,6 +1098,7 @@ static int accept_reject(pimc_ctx* ctx, int32_t replica, int32_t slice1, int32_t
   HIP_TRY(hipGetLastError());
   const RepCore& R = H->dev.core;
   if (R.nk > 0) {
+    if ((rc = refresh_rho1(ctx, H))) return rc;
     // Rho_k[OLD] <- Rho_k[NEW] (accept) or the reverse over the slice range, all species
     const size_t off = (size_t)slice1 * R.ns * R.nk, cnt = (size_t)(slice2 - slice1 + 1) * R.ns * R.nk;
     d2v* dst = accept ? H->rho0.as<d2v>() : H->rho1.as<d2v>();
@@ -1201,6 +1216,7 @@ int pimc_rho_k_update_all(pimc_ctx* ctx, int32_t replica) {
   launch_rho_all(ctx->d_reps.as<ReplicaDev>(), replica, R.nslices, R.ns, R.nk,
                  2 * (R.maxk0 + R.maxk1 + R.maxk2) + 3, ctx->stream);
   HIP_TRY(hipGetLastError());
+  H->rho1_stale = false;   // both copies written
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   return PIMC_OK;
 }
@@ -1210,6 +1226,10 @@ int pimc_rho_k_get(pimc_ctx* ctx, int32_t replica, int32_t mode, double* rho) {
   const RepCore& R = H->dev.core;
   if (R.nk == 0) return fail(PIMC_ERR_STATE, "rho_k_get: replica has no k set");
   HIP_TRY(hipSetDevice(ctx->device));
+  if (mode == 1) {
+    int rc = refresh_rho1(ctx, H);
+    if (rc) return rc;
+  }
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   HIP_TRY(hipMemcpy(rho, mode ? H->rho1.p : H->rho0.p, (size_t)R.nslices * R.ns * R.nk * sizeof(d2v),
                     hipMemcpyDeviceToHost));
@@ -1360,6 +1380,8 @@ int pimc_action_long_range(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t n
   HIP_TRY(hipSetDevice(ctx->device));
   int rc = ctx->sync_descriptors();
   if (rc) return rc;
+  for (int i = 0; i < nreq; i++)
+    if ((rc = refresh_rho1(ctx, ctx->reps[reqs[i].replica].get()))) return rc;
   rc = upload_reqs(ctx, reqs, nreq);
   if (rc) return rc;
   if ((rc = ctx->d_out.ensure((size_t)nreq * sizeof(double))) ||
@@ -1437,6 +1459,7 @@ int prepare_bisection(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, cons
     if ((1 << desc->num_levels) >= R.nslices) return fail(PIMC_ERR_ARG, "bisection: too few slices for num_levels");
     if (R.lr_on) {
       if (desc->lowest_level >= R.lr_levels) return fail(PIMC_ERR_ARG, "bisection: long-range tables lack the lowest level");
+      H->rho1_stale = true;   // the chain updates rho_k OLD only
       // per slice: the NEW and OLD phase rows, FP64 (16 B) or FP32 (8 B) entries
       lr_bytes = std::max(lr_bytes, (size_t)2 * (2 * (R.maxk0 + R.maxk1 + R.maxk2) + 3) *
                                         (precision == PIMC_MIXED ? sizeof(f2v) : sizeof(d2v)));
@@ -1841,6 +1864,7 @@ int pimc_sweep_graph_create(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep
   HIP_TRY(hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0));
   g->num_moves = num_moves;
   g->first_move = first_move;
+  g->reps = P.ids;
   ctx->graphs.push_back(std::move(g));
   *graph_id = (int32_t)ctx->graphs.size() - 1;
   return PIMC_OK;
@@ -1863,6 +1887,8 @@ int pimc_sweep_graph_launch(pimc_ctx* ctx, int32_t graph_id, int32_t num_sweeps)
   }
   ctx->timed.push_back(ev);
   HIP_TRY(hipEventRecord(ev.first, ctx->stream));
+  for (int r : g.reps)
+    if (ctx->rep(r)->dev.core.lr_on) ctx->rep(r)->rho1_stale = true;   // replays update rho_k OLD only
   for (int i = 0; i < num_sweeps; i++) HIP_TRY(hipGraphLaunch(g.exec, ctx->stream));
   HIP_TRY(hipEventRecord(ev.second, ctx->stream));
   g.sweeps += num_sweeps;

@@ -536,14 +536,23 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
       block_sum2(acc_old.value(), acc_new.value(), scratch, sr_old, sr_new);
 
       // ---- Long-range action at the lowest level (LongRangeClass.cc:273-371) ----
-      // rho_k NEW = OLD + e^{ik r_new} - e^{ik r_old} over the window, computed on the
-      // fly; copy OLD is read, nothing is written until the move is accepted.
-      double lr_kold = 0.0, lr_knew = 0.0;
+      // Only the moved species' rho_k changes: NEW = OLD + delta over the window, with
+      // delta = e^{ik r_new} - e^{ik r_old} of the moved bead.  The action change is formed
+      // from delta directly: u |rho + delta|^2 - u |rho|^2 = u (2 Re(rho* delta) + |delta|^2)
+      // for the moved species' own term and 2 u Re(rho_s* delta) for each pair (s, sp), so
+      // no species' |rho|^2 is formed twice.  The OLD total, a sum over every species pair,
+      // is formed only for the move records.  Copy OLD is read; nothing is written until
+      // the move is accepted.
+      double lr_kold = 0.0, lr_delta = 0.0;
       const bool with_lr = HAS_LR && R.lr_on && level == a.lowest_level;
       if (HAS_LR && with_lr) {
+        const bool want_old = a.records != nullptr;
         const int stride = kstride(R), o1 = koff_of(R, 1), o2 = koff_of(R, 2);
         const int nslc = W / step + 1;   // slices t = 0, step, .., W; phase rows staged lr_chunk at a time
-        double part_o = 0.0, part_n = 0.0;
+        const PIMC_GLOBAL double* ukt = (const PIMC_GLOBAL double*)R.ulong_k + (size_t)level * R.nk;
+        const size_t upair = (size_t)R.lr_levels * R.nk;   // distance between species pairs in ulong_k
+        const bool own = (R.lr_mask >> (sp * R.ns + sp)) & 1u;
+        double part_o = 0.0, part_d = 0.0;
         for (int c0 = 0; c0 < nslc; c0 += a.lr_chunk) {
           const int nc = min(a.lr_chunk, nslc - c0);
           __syncthreads();
@@ -552,6 +561,14 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
           for (int k = kb + tid; k < ke; k += blockDim.x) {
            const PIMC_GLOBAL int* kx = (const PIMC_GLOBAL int*)R.kidx + 4 * k;
            const int ki[3] = {kx[0], kx[1], kx[2]};
+           // u_k of the pairs that involve the moved species (zero where a pair has no table)
+           const double u_own = own ? ukt[(sp * R.ns + sp) * upair + k] : 0.0;
+           double u_x[kMaxSpecies];
+#pragma unroll
+           for (int s = 0; s < kMaxSpecies; s++) {
+             const int pr = s < sp ? s * R.ns + sp : sp * R.ns + s;
+             u_x[s] = (s < R.ns && s != sp && ((R.lr_mask >> pr) & 1u)) ? ukt[pr * upair + k] : 0.0;
+           }
            for (int j = 0; j < nc; j++) {
             const int t = (c0 + j) * step;
             const double factor = (t == 0 || t == W) ? 0.5 : 1.0;
@@ -559,55 +576,45 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
             LrT nr, ni, orr, oi;
             kterm<LrT>(lrC + 2 * j * stride, o1, o2, ki, nr, ni);
             kterm<LrT>(lrC + (2 * j + 1) * stride, o1, o2, ki, orr, oi);
-            d2v ro[kMaxSpecies], rn[kMaxSpecies];
+            const double dx = (double)nr - (double)orr, dy = (double)ni - (double)oi;
+            // the species' rho_k this term needs (a species without a pair to sp only for the record)
+            d2v ro[kMaxSpecies], rs = d2v{0.0, 0.0};
+            double d = 0.0;
 #pragma unroll
             for (int s = 0; s < kMaxSpecies; s++) {
-              if (s < R.ns) {
-                ro[s] = rho[(size_t)s * R.nk + k];
-                rn[s] = ro[s];
-                if (s == sp) {
-                  rn[s].x += (double)nr;
-                  rn[s].y += (double)ni;
-                  rn[s].x -= (double)orr;
-                  rn[s].y -= (double)oi;
-                }
-              }
+              ro[s] = (s < R.ns && (s == sp || want_old || u_x[s] != 0.0)) ? rho[(size_t)s * R.nk + k] : d2v{0.0, 0.0};
+              if (s == sp) rs = ro[s];   // selected, not indexed: sp is a run-time value
+              else d += u_x[s] * (ro[s].x * dx + ro[s].y * dy);
             }
+            part_d += factor * (u_own * (2.0 * (rs.x * dx + rs.y * dy) + (dx * dx + dy * dy)) + 2.0 * d);
+            if (want_old) {
 #pragma unroll
-            for (int s = 0; s < kMaxSpecies; s++) {
-              if (s < R.ns && ((R.lr_mask >> (s * R.ns + s)) & 1u)) {
-                const double u = ((const PIMC_GLOBAL double*)R.ulong_k)[((size_t)(s * R.ns + s) * R.lr_levels + level) * R.nk + k];
-                part_o += factor * 0.5 * 2.0 * (ro[s].x * ro[s].x + ro[s].y * ro[s].y) * u;
-                part_n += factor * 0.5 * 2.0 * (rn[s].x * rn[s].x + rn[s].y * rn[s].y) * u;
-              }
+              for (int s = 0; s < kMaxSpecies; s++)
+                if (s < R.ns && ((R.lr_mask >> (s * R.ns + s)) & 1u))
+                  part_o += factor * 0.5 * 2.0 * (ro[s].x * ro[s].x + ro[s].y * ro[s].y) * ukt[(s * R.ns + s) * upair + k];
+#pragma unroll
+              for (int s1 = 0; s1 < kMaxSpecies; s1++)
+#pragma unroll
+                for (int s2 = s1 + 1; s2 < kMaxSpecies; s2++)
+                  if (s2 < R.ns && ((R.lr_mask >> (s1 * R.ns + s2)) & 1u))
+                    part_o += factor * 2.0 * (ro[s1].x * ro[s2].x + ro[s1].y * ro[s2].y) * ukt[(s1 * R.ns + s2) * upair + k];
             }
-#pragma unroll
-            for (int s1 = 0; s1 < kMaxSpecies; s1++)
-#pragma unroll
-              for (int s2 = s1 + 1; s2 < kMaxSpecies; s2++) {
-                if (s2 < R.ns && ((R.lr_mask >> (s1 * R.ns + s2)) & 1u)) {
-                  const double u =
-                      ((const PIMC_GLOBAL double*)R.ulong_k)[((size_t)(s1 * R.ns + s2) * R.lr_levels + level) * R.nk + k];
-                  part_o += factor * 2.0 * (ro[s1].x * ro[s2].x + ro[s1].y * ro[s2].y) * u;
-                  part_n += factor * 2.0 * (rn[s1].x * rn[s2].x + rn[s1].y * rn[s2].y) * u;
-                }
-              }
            }
           }
         }
-        block_sum2(part_o, part_n, scratch, lr_kold, lr_knew);
+        block_sum2(part_o, part_d, scratch, lr_kold, lr_delta);
       }
 
       // ---- Metropolis test (StageClass.cc:69-72) ----
       bool cl_ok = true;
       if constexpr (CL) {
         if (tid == 0) {
-          double v[4] = {sr_old, sr_new, lr_kold, lr_knew};
+          double v[4] = {sr_old, sr_new, lr_kold, lr_delta};
           cl_ok = cluster_sum(a, item, member, nbar, v);
           sr_old = v[0];
           sr_new = v[1];
           lr_kold = v[2];
-          lr_knew = v[3];
+          lr_delta = v[3];
         }
         nbar++;
       }
@@ -629,17 +636,19 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
         oldAction += sr_old;
         newAction += kin_n;
         newAction += sr_new;
+        double currActionChange = newAction - oldAction;
         if (with_lr) {
-          // the k-independent terms are the same on both copies
+          // the k-independent terms are the same on both copies and cancel in the change;
+          // the records carry the totals
+          currActionChange += lr_delta;
           double homo = 0.0, background = 0.0, k0 = 0.0;
           for (int t = 0; t <= W; t += step)
             lr_const_terms(R, S, level, (t == 0 || t == W) ? 0.5 : 1.0, homo, background, k0);
           const double cst = R.use_background ? background : k0;
           oldAction += (lr_kold + homo) + cst;
-          newAction += (lr_knew + homo) + cst;
+          newAction += ((lr_kold + lr_delta) + homo) + cst;
           st_kevals += 2 * (int64_t)(W / step + 1) * R.nk * R.lr_terms;
         }
-        const double currActionChange = newAction - oldAction;
         const double logAcceptProb = logSampleRatio - currActionChange + prev;
         const double logu = logu_tab[level];
         const int ok = logAcceptProb >= logu;
@@ -665,7 +674,9 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
     }
     if (CL && aborted) return;
 
-    // ---- AcceptCopy: rho_k OLD, NEW <- the updated rho_k; PutInBoxFast on the NEW beads, OLD <- NEW ----
+    // ---- AcceptCopy: rho_k OLD <- the updated rho_k; PutInBoxFast on the NEW beads, OLD <- NEW ----
+    // The chain reads rho_k OLD only and leaves copy NEW behind; the host refreshes NEW from
+    // OLD before any call that uses it.
     if (HAS_LR && accepted && R.lr_on) {
       const int step = 1 << a.lowest_level;
       const int stride = kstride(R), o1 = koff_of(R, 1), o2 = koff_of(R, 2);
@@ -688,7 +699,6 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
             v.x -= (double)orr;
             v.y -= (double)oi;
             R.rho0[base + k] = v;
-            R.rho1[base + k] = v;
           }
         }
       }
