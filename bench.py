@@ -327,6 +327,117 @@ def report_dropin(args):
     print(json.dumps(out))
 
 
+def longrange_requests(system, ids, lib, seed: int = 7):
+    """One lowest-level bisection window per replica (slices s1 .. s1 + 2^L, level 0) with one
+    moved particle of species 0, whose NEW beads inside the window are displaced (set once on
+    the NEW copy); returns the NEWMODE and OLDMODE request lists and the full-path OLDMODE list
+    (every slice, level 0)."""
+    W = 1 << system.num_levels
+    rng = np.random.default_rng(seed)
+    new, old, full = [], [], []
+    for i, r in zip(ids, system.replicas):
+        M = r.num_time_slices
+        s1 = int(rng.integers(0, M - W))
+        p = int(rng.integers(0, r.species_count[0]))
+        x = np.asarray(r.beads, dtype=float)[s1 + 1:s1 + W, p] + rng.normal(scale=0.05, size=(W - 1, 3))
+        lib.path_set_beads(i, abi.NEWMODE, p, s1 + 1, x)
+        new.append((i, abi.NEWMODE, s1, s1 + W, 0, [p]))
+        old.append((i, abi.OLDMODE, s1, s1 + W, 0, [p]))
+        full.append((i, abi.OLDMODE, 0, M, 0, [p]))
+    return new, old, full
+
+
+def report_longrange(args):
+    """The long-range entry point pimc_action_long_range (LongRangeClass::SingleAction with
+    UpdateRho_ks) at batch size: one call holds a request for every replica of the hydrogen
+    workload (32 e + 32 p, 242 k), as a clone-parallel driver would issue them.  Per step: a
+    NEWMODE call over each replica's lowest-level window (rho_k NEW written from OLD plus the
+    moved bead's phases), an OLDMODE call over the same windows, and an OLDMODE call over every
+    slice.  Algorithmic HBM bytes: 16 B read per (k, slice, species), plus 16 B written in
+    NEWMODE.  Device time from the library's HIP events around each call."""
+    from pimcplusplus_amd.tables import load_npz
+    system = workloads.config2(load_npz(os.path.join(ROOT, "tests", "golden", "testcoul_fit2.npz")),
+                               num_replicas=args.replicas)
+    r0 = system.replicas[0]
+    out = {"metric": "k.slice.species long-range terms/s (drop-in LongRange SingleAction, batched)",
+           "unit": "k.slice.species terms/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": True, "data": "synthetic",
+           "config": {"workload": f"{system.name}, pimc_action_long_range, one request per replica per call",
+                      "replicas": len(system.replicas)}}
+    lib = native.open_context(0)
+    ids = load_system(lib, system)
+    new, old, full = longrange_requests(system, ids, lib)
+    nk, ns = lib.replicas[ids[0]].nk, len(r0.species_count)
+    out["config"]["nk"], out["config"]["species"] = nk, ns
+    W = (1 << system.num_levels) + 1
+    calls = {"new_window": (new, 32.0 * W), "old_window": (old, 16.0 * W),
+             "old_full_path": (full, 16.0 * (r0.num_time_slices + 1))}
+    if args.lr_call:
+        calls = {args.lr_call: calls[args.lr_call]}
+    precs = (("fp64", abi.FP64), ("mixed", abi.MIXED))
+    if args.only:
+        precs = tuple(p for p in precs if p[0] == args.only)
+    for prec_name, prec in precs:
+        res = {}
+        for name, (reqs, bytes_per_k) in calls.items():
+            for _ in range(args.warmup):
+                lib.action_long_range(reqs, prec)
+            lib.bisection_stats_take(ids)
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                lib.action_long_range(reqs, prec)
+            wall = time.perf_counter() - t0
+            dev_ms = lib.bisection_stats_take(ids).device_ms / args.steps
+            terms = len(reqs) * (bytes_per_k / (32.0 if name == "new_window" else 16.0)) * nk * ns
+            nbytes = len(reqs) * bytes_per_k * nk * ns
+            res[name] = {"kernel_ms_avg": dev_ms, "wall_ms_per_call": 1e3 * wall / args.steps,
+                         "terms_per_s": terms / (dev_ms * 1e-3), "bytes_per_call": nbytes,
+                         "achieved_GBs": nbytes / (dev_ms * 1e-3) / 1e9,
+                         "frac": nbytes / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        out[prec_name] = res
+    lib.close()
+    if args.only or args.lr_call:
+        print(json.dumps(out))
+        return
+    main_call = out["fp64"]["new_window"]
+    out["value"] = main_call["terms_per_s"]
+    out["dtype"] = "f64"
+    out["roofline"] = {"bound": "hbm", "achieved": main_call["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": main_call["frac"],
+                       "traffic": measured_traffic(f"{len(system.replicas)} NEWMODE window requests",
+                                                   os.path.join(ROOT, "profiles", "k_lr_req_hydrogen_traffic.md")),
+                       "kernel": "k_lr_req<FP64> (NEWMODE window requests)",
+                       "note": "algorithmic bytes: 32 B per (k, slice, species) of a NEWMODE request (rho_k OLD "
+                               "read, NEW written), 16 B of an OLDMODE one, over the call's HIP-event time"}
+    if not args.no_cpu:
+        out["cpu_baseline"] = longrange_cpu_baseline(system, new, old, min(args.cpu_seconds, 10.0))
+    print(json.dumps(out))
+
+
+def longrange_cpu_baseline(system, new, old, seconds: float):
+    """The oracle's LongRange SingleAction on one host core: NEWMODE then OLDMODE requests of
+    the same windows, replica by replica, for about `seconds`."""
+    o = abi.PimcLib(oracle_lib())
+    nrep = 64
+    sub = workloads.System(system.name, system.tables, system.desired_tau, system.replicas[:nrep],
+                           system.num_levels, system.seed)
+    ids = workloads.load(o, sub)
+    nq, oq, _ = longrange_requests(sub, ids, o)
+    nk, ns = o.replicas[ids[0]].nk, len(sub.replicas[0].species_count)
+    W = (1 << system.num_levels) + 1
+    terms, t0, i = 0, time.perf_counter(), 0
+    while time.perf_counter() - t0 < seconds:
+        o.action_long_range([nq[i % nrep]])
+        o.action_long_range([oq[i % nrep]])
+        terms += 2 * W * nk * ns
+        i += 1
+    wall = time.perf_counter() - t0
+    o.close()
+    return {"value": terms / wall, "unit": "k.slice.species terms/s", "cores": 1, "kind": "port",
+            "sample": f"{i} NEWMODE + OLDMODE window request pairs over {nrep} hydrogen replicas, one core",
+            "cpu_model": cpu_model(), "build": "g++ -O3 -march=x86-64-v3 -ffp-contract=off (oracle/Makefile)"}
+
+
 def delta_s_error(nrep: int = 4, nmoves: int = 2500, seed_offset: int = 0, replica_offset: int = 0):
     """Max |dS_mixed - dS_fp64| over stages attempted on identical paths (chains run
     until their first differing decision).  seed_offset / replica_offset select a sample
@@ -494,10 +605,13 @@ def main():
     ap.add_argument("--plan", default=None, metavar="BLOCK,SPLIT,PERSISTENT,CLUSTER",
                     help="non-headline workloads: this launch plan with eager launches instead of the tuner "
                          "and a captured sweep graph (profiling runs)")
-    ap.add_argument("--workload", default="lj256", choices=("lj256", "lj64", "uneven", "hydrogen", "hydrogen15k", "ocp", "dropin"),
+    ap.add_argument("--lr-call", default=None, choices=("new_window", "old_window", "old_full_path"),
+                    help="--workload longrange: time this call only (profiling runs)")
+    ap.add_argument("--workload", default="lj256",
+                    choices=("lj256", "lj64", "uneven", "hydrogen", "hydrogen15k", "ocp", "dropin", "longrange"),
                     help="lj256 (default) is the headline configuration; the others print a report line "
                          "(hydrogen15k: the hydrogen system at M=168, T=15,037 K; dropin: per-SingleAction "
-                         "calls of the drop-in mode)")
+                         "calls of the drop-in mode; longrange: batched LongRange SingleAction requests)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
@@ -510,6 +624,9 @@ def main():
 
     if args.workload == "dropin":
         report_dropin(args)
+        return
+    if args.workload == "longrange":
+        report_longrange(args)
         return
     if args.workload != "lj256":
         report_other(args)

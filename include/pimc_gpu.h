@@ -293,9 +293,12 @@ int pimc_action_short_range_noalloc(pimc_ctx* ctx, const pimc_action_req* reqs, 
                                     int32_t precision, double* out);
 /* KineticClass::SingleAction. */
 int pimc_action_kinetic(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t nreq, double* out);
-/* LongRangeClass::SingleAction: a NEWMODE request first updates rho_k of copy NEW
- * over [slice1, slice2] step 2^level from OLD (PathClass::UpdateRho_ks).  Requests
- * are processed in order. */
+/* LongRangeClass::SingleAction (PIMC++/src/Actions/LongRangeClass.cc:273-371): a NEWMODE
+ * request first updates rho_k of copy NEW over [slice1, slice2] step 2^level from OLD
+ * (PathClass::UpdateRho_ks, PIMC++/src/PathClass.cc:649-707).  Requests are processed in
+ * order; requests on distinct replicas share a launch.  A NEWMODE request whose k set is too
+ * large for its moved particles' phase rows (2 nptcls rows of 2(maxk0+maxk1+maxk2)+3
+ * entries of 16 B beyond 24 KiB) returns PIMC_ERR_UNSUPPORTED. */
 int pimc_action_long_range(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t nreq,
                            int32_t precision, double* out);
 
@@ -474,7 +477,8 @@ int pimc_sweep_graph_destroy(pimc_ctx* ctx, int32_t graph_id);
 /* Waits for the context's stream, then stores in *stats the counters every
  * pimc_bisection_run call since the previous take added for the listed
  * replicas, and clears those replicas' counters.  device_ms is the compute time
- * of all pimc_bisection_run calls on the context since the previous take.
+ * (device events) of all pimc_bisection_run calls, sweep-graph replays and
+ * pimc_action_long_range calls on the context since the previous take.
  * (The reference keeps such counters per move object, MoveBaseClass::NumAccepted
  * and TimesCalled, PIMC++/src/Moves/MoveBase.h:70-77, and the time per action,
  * ActionBaseClass::TimeSpent, PIMC++/src/Actions/ActionBase.h:37.) */

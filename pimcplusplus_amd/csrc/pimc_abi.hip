@@ -301,8 +301,10 @@ struct pimc_ctx {
   uint64_t wait_ticks = 0;
   pimc_bisection_plan plan{256, 1, 0, 1, 0.0};   // launch shape of pimc_bisection_run
   std::vector<std::unique_ptr<SweepGraph>> graphs;
-  // event pairs around bisection launches since the last take, and a pool of spare ones
+  // event pairs around timed launches since the last take, and a pool of spare ones; past
+  // kMaxTimed pairs the finished ones are folded into timed_ms
   std::vector<std::pair<hipEvent_t, hipEvent_t>> timed, spare;
+  double timed_ms = 0.0;
 
   // Host staging of the request upload; kept alive until the call that uses it synchronizes.
   std::vector<ReqDev> h_reqs;
@@ -340,6 +342,32 @@ struct pimc_ctx {
 };
 
 namespace {
+
+// An event pair for a timed launch (bisection runs, sweep replays, long-range action calls),
+// queued for pimc_bisection_stats_take.  The list stays bounded for callers that never take:
+// past kMaxTimed pairs the stream is drained and the pairs are folded into ctx->timed_ms.
+constexpr size_t kMaxTimed = 1024;
+int timed_pair(pimc_ctx* ctx, std::pair<hipEvent_t, hipEvent_t>& ev) {
+  if (ctx->timed.size() >= kMaxTimed) {
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (auto& e : ctx->timed) {
+      float ms = 0.0f;
+      HIP_TRY(hipEventElapsedTime(&ms, e.first, e.second));
+      ctx->timed_ms += ms;
+      ctx->spare.push_back(e);
+    }
+    ctx->timed.clear();
+  }
+  if (!ctx->spare.empty()) {
+    ev = ctx->spare.back();
+    ctx->spare.pop_back();
+  } else {
+    HIP_TRY(hipEventCreate(&ev.first));
+    HIP_TRY(hipEventCreate(&ev.second));
+  }
+  ctx->timed.push_back(ev);
+  return PIMC_OK;
+}
 
 // Copy NEW of rho_k <- copy OLD where a chain run left NEW behind (stream-ordered after it).
 int refresh_rho1(pimc_ctx* ctx, HostReplica* H) {
@@ -1363,7 +1391,7 @@ int pimc_action_long_range(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t n
   if (!ctx || nreq < 0 || (nreq > 0 && (!reqs || !out))) return fail(PIMC_ERR_ARG, "action_long_range: bad argument");
   if (precision != PIMC_FP64 && precision != PIMC_MIXED) return fail(PIMC_ERR_ARG, "action_long_range: bad precision");
   if (nreq == 0) return PIMC_OK;
-  std::vector<int> strides(nreq), rids(nreq);
+  std::vector<int> rids(nreq), nss(nreq), modes(nreq);
   int max_nk = 0;
   for (int i = 0; i < nreq; i++) {
     int rc = check_req(ctx, reqs[i]);
@@ -1372,8 +1400,11 @@ int pimc_action_long_range(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t n
     if (R.nk == 0) return fail(PIMC_ERR_STATE, "action_long_range: replica has no k set");
     if (R.lr_mask && reqs[i].level >= R.lr_levels)
       return fail(PIMC_ERR_ARG, "action_long_range: level beyond the long-range tables");
-    strides[i] = 2 * (R.maxk0 + R.maxk1 + R.maxk2) + 3;
+    if (reqs[i].mode == 1 && !long_range_phase_fits(2 * (R.maxk0 + R.maxk1 + R.maxk2) + 3, reqs[i].nptcls))
+      return fail(PIMC_ERR_UNSUPPORTED, "action_long_range: the k set is too large for the moved particles' phase rows");
     rids[i] = reqs[i].replica;
+    nss[i] = R.ns;
+    modes[i] = reqs[i].mode;
     max_nk = std::max(max_nk, R.nk);
   }
   const int kblocks = long_range_kblocks(max_nk);
@@ -1387,10 +1418,14 @@ int pimc_action_long_range(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t n
   if ((rc = ctx->d_out.ensure((size_t)nreq * sizeof(double))) ||
       (rc = ctx->d_partials.ensure((size_t)nreq * kblocks * sizeof(double))))
     return rc;
+  const std::vector<LrGroup> groups = long_range_groups(nreq, rids.data(), nss.data(), modes.data());
+  std::pair<hipEvent_t, hipEvent_t> ev;   // device time, read by pimc_bisection_stats_take
+  if (int rc_ev = timed_pair(ctx, ev)) return rc_ev;
+  HIP_TRY(hipEventRecord(ev.first, ctx->stream));
   launch_long_range(precision == PIMC_MIXED, ctx->d_reps.as<ReplicaDev>(), ctx->d_reqs.as<ReqDev>(), nreq,
-                    strides.data(), rids.data(), kblocks, ctx->d_partials.as<double>(), ctx->d_out.as<double>(),
-                    ctx->stream);
+                    groups, kblocks, ctx->d_partials.as<double>(), ctx->d_out.as<double>(), ctx->stream);
   HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(ev.second, ctx->stream));
   HIP_TRY(hipMemcpyAsync(out, ctx->d_out.p, (size_t)nreq * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   return PIMC_OK;
@@ -1614,14 +1649,7 @@ int persistent_grid(pimc_ctx* ctx, const BisectionPrep& P, const LaunchBisection
 
 int launch_timed(pimc_ctx* ctx, const BisectionPrep& P, const LaunchBisection& a, int precision, int nrep) {
   std::pair<hipEvent_t, hipEvent_t> ev;
-  if (!ctx->spare.empty()) {
-    ev = ctx->spare.back();
-    ctx->spare.pop_back();
-  } else {
-    HIP_TRY(hipEventCreate(&ev.first));
-    HIP_TRY(hipEventCreate(&ev.second));
-  }
-  ctx->timed.push_back(ev);
+  if (int rc_ev = timed_pair(ctx, ev)) return rc_ev;
   const int grid = ctx->plan.persistent ? persistent_grid(ctx, P, a, precision, nrep) : nrep;
   int rc = check_cluster_fits(ctx, P, a, precision, nrep);
   if (rc) return rc;
@@ -1878,14 +1906,7 @@ int pimc_sweep_graph_launch(pimc_ctx* ctx, int32_t graph_id, int32_t num_sweeps)
     return fail(PIMC_ERR_ARG, "sweep_graph_launch: move numbers beyond 2^32");
   HIP_TRY(hipSetDevice(ctx->device));
   std::pair<hipEvent_t, hipEvent_t> ev;
-  if (!ctx->spare.empty()) {
-    ev = ctx->spare.back();
-    ctx->spare.pop_back();
-  } else {
-    HIP_TRY(hipEventCreate(&ev.first));
-    HIP_TRY(hipEventCreate(&ev.second));
-  }
-  ctx->timed.push_back(ev);
+  if (int rc_ev = timed_pair(ctx, ev)) return rc_ev;
   HIP_TRY(hipEventRecord(ev.first, ctx->stream));
   for (int r : g.reps)
     if (ctx->rep(r)->dev.core.lr_on) ctx->rep(r)->rho1_stale = true;   // replays update rho_k OLD only
@@ -2106,6 +2127,8 @@ int pimc_bisection_stats_take(pimc_ctx* ctx, const int32_t* replicas, int32_t nr
   memset(stats, 0, sizeof *stats);
   int rc = take_abort(ctx);
   if (rc) return rc;
+  stats->device_ms = ctx->timed_ms;
+  ctx->timed_ms = 0.0;
   for (auto& e : ctx->timed) {
     float ms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&ms, e.first, e.second));

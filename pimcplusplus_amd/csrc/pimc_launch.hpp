@@ -1,6 +1,8 @@
 // pimc_launch.hpp -- host-side launch wrappers of the kernels.
 #pragma once
 
+#include <vector>
+
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -54,8 +56,20 @@ void launch_rho_slices(const ReplicaDev* reps, int rep, int mode, int slice1, in
 // distinct replicas in one launch (in order otherwise); partials is [nreq][kblocks],
 // kblocks = long_range_kblocks(the largest nk of the requests' replicas).
 int long_range_kblocks(int max_nk);
-void launch_long_range(bool mixed, const ReplicaDev* reps, const ReqDev* reqs, int nreq, const int* strides,
-                       const int* replicas, int kblocks, double* partials, double* out, hipStream_t s);
+// The launches of a long-range call: requests [r0, r1) on distinct replicas, the largest
+// species count among them and whether any is a NEWMODE request.  nspecies[r] and modes[r]:
+// the species count of request r's replica and its mode.
+struct LrGroup {
+  int r0, r1, nsp;
+  bool upd;
+};
+std::vector<LrGroup> long_range_groups(int nreq, const int* replicas, const int* nspecies, const int* modes);
+void launch_long_range(bool mixed, const ReplicaDev* reps, const ReqDev* reqs, int nreq,
+                       const std::vector<LrGroup>& groups, int kblocks, double* partials, double* out,
+                       hipStream_t s);
+// Whether the phase rows of one slice's moved beads (NEW and OLD copies, FP64) fit the
+// request kernel's LDS for a k set of `stride` entries per row.
+bool long_range_phase_fits(int stride, int nptcls);
 
 // d_dBeta of the actions over links [slice1, slice2) at `level` (slices slice1, slice1 +
 // 2^level, .., slice2) of copy `mode`, as partial sums per block: pairs {du, v} (the

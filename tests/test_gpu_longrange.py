@@ -97,3 +97,55 @@ def test_long_range_requests_batched_over_replicas_keep_their_order(oracle, gpu)
     for prec, tol in ((abi.FP64, 1e-11), (abi.MIXED, 1e-6)):
         got = gpu.action_long_range([(ig[x[0]],) + x[1:] for x in pattern], prec)
         assert np.max(np.abs(got - want) / (1 + np.abs(want))) < tol, prec
+
+
+def test_chain_without_records_then_requests_match_the_oracle(oracle, gpu):
+    """The chain without move records (the benchmark's path: no OLD long-range totals) takes the
+    oracle's decisions, and updates rho_k OLD only; copy NEW is refreshed from OLD before the
+    calls that use it, so rho_k NEW, a NEWMODE request and its accept copy after the chain
+    match the oracle."""
+    sysm = workloads.charged_lj_system(n_plus=16, n_minus=16, m=32, kc_units=4.0, num_replicas=2)
+    io, ig = both(oracle, gpu, sysm)
+    _, so = oracle.bisection_run(io, sysm.num_levels, sysm.seed, 0, 150)
+    _, sg = gpu.bisection_run(ig, sysm.num_levels, sysm.seed, 0, 150, abi.FP64, records=False)
+    assert so.accepted == sg.accepted > 0 and so.stages == sg.stages
+    for a, b in zip(io, ig):
+        assert np.max(np.abs(oracle.path_get(a) - gpu.path_get(b))) < 1e-12
+        for mode in (0, 1):
+            assert np.max(np.abs(oracle.rho_k(a, mode) - gpu.rho_k(b, mode))) < 1e-10
+    gpu.bisection_stats_take(ig)
+    new = oracle.path_get(io[1])
+    new[5:12, 3] += 0.1
+    oracle.path_set(io[1], new, abi.NEWMODE)
+    gpu.path_set(ig[1], new, abi.NEWMODE)
+    want = oracle.action_long_range([(io[1], abi.NEWMODE, 4, 12, 0, [3])])
+    got = gpu.action_long_range([(ig[1], abi.NEWMODE, 4, 12, 0, [3])])
+    assert abs(got[0] - want[0]) < 1e-12 * (1 + abs(want[0]))
+    assert gpu.bisection_stats_take(ig).device_ms > 0.0   # the call's device time
+    oracle.accept_copy(io[1], 4, 12, [3])
+    gpu.accept_copy(ig[1], 4, 12, [3])
+    for mode in (0, 1):
+        assert np.max(np.abs(oracle.rho_k(io[1], mode) - gpu.rho_k(ig[1], mode))) < 1e-10
+
+
+def test_long_range_requests_over_whole_paths_and_several_particles(oracle, gpu):
+    """Requests over every slice (more slices than one staged chunk of phase rows) and with
+    four moved particles of both species, at levels 0 and 1, in both precisions."""
+    sysm = workloads.charged_lj_system(n_plus=20, n_minus=20, m=64, kc_units=6.0, num_replicas=2)
+    io, ig = both(oracle, gpu, sysm)
+    rng = np.random.default_rng(77)
+    moved = [1, 7, 25, 33]
+    for k, (a, b) in enumerate(zip(io, ig)):
+        new = sysm.replicas[k].beads.copy()
+        new[1:64, moved] += rng.normal(scale=0.1, size=(63, 4, 3))
+        oracle.path_set(a, new, abi.NEWMODE)
+        gpu.path_set(b, new, abi.NEWMODE)
+    pattern = [(0, 1, 0, 64, 0, moved), (1, 1, 0, 64, 1, moved[:3]), (0, 0, 0, 64, 0, moved[:1]),
+               (1, 0, 2, 62, 1, moved[1:2])]
+    want = oracle.action_long_range([(io[x[0]],) + x[1:] for x in pattern])
+    for prec, tol in ((abi.FP64, 1e-11), (abi.MIXED, 1e-6)):
+        got = gpu.action_long_range([(ig[x[0]],) + x[1:] for x in pattern], prec)
+        assert np.max(np.abs(got - want) / (1 + np.abs(want))) < tol, prec
+        if prec == abi.FP64:   # the mixed call writes rho_k NEW from FP32 phases
+            for a, b in zip(io, ig):
+                assert np.max(np.abs(oracle.rho_k(a, 1) - gpu.rho_k(b, 1))) < 1e-10
