@@ -43,7 +43,7 @@
 extern "C" {
 #endif
 
-#define PIMC_ABI_VERSION 1
+#define PIMC_ABI_VERSION 2
 #define PIMC_MAX_ACTIVE 4   /* particles moved together (PermuteType TABLE moves up to 4) */
 #define PIMC_MAX_LEVELS 8   /* bisection levels */
 #define PIMC_MAX_SPECIES 4
@@ -249,6 +249,25 @@ int pimc_permute_htable(pimc_ctx* ctx, int32_t replica, int32_t mode, int32_t sp
  * length 1..4, entries distinct. */
 int pimc_cycle_apply(pimc_ctx* ctx, int32_t replica, int32_t first_ptcl, int32_t slice,
                      int32_t length, const int32_t* cycle);
+/* PermuteType TABLE for the bisection moves of `species` on this replica, with the cycle
+ * weights gamma[0..3] (Gamma of 1- to 4-cycles) and the cut-off epsilon
+ * (TablePermuteStageClass::Read, PIMC++/src/Moves/TablePermuteStage.cc:142-166).  Each
+ * pimc_bisection_run move of that species then runs the stage list of
+ * BisectionBlockClass with TABLE (PIMC++/src/Moves/BisectionBlock.cc:51-64,126): the
+ * HTable of the window (PermuteTableClass::ConstructHTable), the boson cycle table
+ * (ConstructBosonCycleTable, PermuteTableClass.cc:339-449, no excluded particle, no z
+ * focus), a cycle drawn from it and applied at slice2 (AttemptPermutation), the bisection
+ * stages moving every particle of the cycle, and the final permutation test with the
+ * reverse table (TablePermuteStageClass::Attempt, TablePermuteStage.cc:200-260,
+ * CalcReverseProb, UpdateHTable).  gamma == NULL switches the replica back to no
+ * permutations.  Replicas with the long-range action, cluster plans, num_levels above 5, a
+ * moved species whose n x n float HTable and the staged spline level exceed a block's 64 KB
+ * of LDS (about 100 particles), the autotuner and sweep graphs refuse TABLE moves
+ * (PIMC_ERR_UNSUPPORTED); every replica of one call moves with TABLE or none does.  The GPU
+ * walks the cycle table in the reference's order without storing it, so its size is not
+ * bounded. */
+int pimc_replica_set_permute(pimc_ctx* ctx, int32_t replica, int32_t species, const double* gamma,
+                             double epsilon);
 
 /* PathClass::UpdateRho_ks(): recompute rho_k of every slice and species from
  * the OLD copy, then NEW <- OLD (PathClass.cc:709-723). */
@@ -395,13 +414,25 @@ typedef struct pimc_bisection_desc {
 /* What one move did. Stage s is bisection level num_levels-1-s. */
 typedef struct pimc_move_record {
   int32_t slice1;
-  int32_t ptcl;
+  int32_t ptcl;              /* the moved particle (with TABLE permutations: the cycle's first) */
   int32_t accepted;          /* 1 if every stage accepted and the move was kept */
-  int32_t num_stages;        /* stages attempted (the first rejection ends the move) */
+  int32_t num_stages;        /* bisection stages attempted (the first rejection ends the move) */
   double old_action[PIMC_MAX_LEVELS];
   double new_action[PIMC_MAX_LEVELS];
   double log_sample_ratio[PIMC_MAX_LEVELS];
   double log_u[PIMC_MAX_LEVELS];
+  /* TABLE permutation moves (pimc_replica_set_permute); zero otherwise.  cycle holds the
+   * chosen cycle's particles (absolute indices, cycle_length of them); forward_t is the
+   * probability of choosing it (AttemptPermutation), reverse_t that of the reverse choice
+   * (CalcReverseProb), accept_prob and psi the final permutation test (0 where the move
+   * ended at a bisection stage). */
+  int32_t cycle_length;
+  int32_t cycle[PIMC_MAX_ACTIVE];
+  int32_t num_cycles;        /* entries of the forward cycle table */
+  double forward_t;
+  double reverse_t;
+  double accept_prob;
+  double psi;
 } pimc_move_record;
 
 /* Counters of a pimc_bisection_run call, summed over replicas and moves. */

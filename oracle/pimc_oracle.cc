@@ -63,6 +63,7 @@
 #include <string>
 #include <limits>
 #include <vector>
+#include <algorithm>
 
 namespace {
 
@@ -105,7 +106,7 @@ double uniform53(uint32_t w0, uint32_t w1) {
   return (double)(v >> 11) * 0x1.0p-53;
 }
 
-enum { TAG_SLICE = 1, TAG_PTCL = 2, TAG_GAUSS = 3, TAG_ACCEPT = 4 };
+enum { TAG_SLICE = 1, TAG_PTCL = 2, TAG_GAUSS = 3, TAG_ACCEPT = 4, TAG_PERMUTE = 5, TAG_PERMUTE_ACCEPT = 6 };
 
 // Counter layout: c0 = move, c1 = tag<<24 | level<<16 | attempt, c2 = index,
 // c3 = replica stream; key = seed (low word, high word).
@@ -638,6 +639,11 @@ struct Replica {
   std::vector<VlongTables> vlong;
   uint32_t stream = 0;
   int join = 0;
+  // PermuteType TABLE for moves of `permute_species` (TablePermuteStageClass::Read)
+  bool permute_on = false;
+  int permute_species = 0;
+  double permute_gamma[4] = {0, 0, 0, 0};
+  double permute_epsilon = 0;
 
   double* P(int mode, int slice, int ptcl) { return &path[mode][((size_t)slice * N + ptcl) * 3]; }
   cplx& R(int mode, int slice, int s, int k) { return rho[mode][((size_t)slice * ns + s) * nk + k]; }
@@ -1040,8 +1046,15 @@ double stage_action(Ctx* c, Replica& R, int mode, int slice1, int slice2, const 
   return TotalAction;
 }
 
+void table_move(Ctx* c, Replica& R, const pimc_bisection_desc& bd, uint32_t move, pimc_move_record* rec,
+                pimc_bisection_stats* st);
+
 void bisection_move(Ctx* c, Replica& R, const pimc_bisection_desc& bd, uint32_t move,
                     pimc_move_record* rec, pimc_bisection_stats* st) {
+  if (R.permute_on && R.permute_species == bd.species) {
+    table_move(c, R, bd, move, rec, st);
+    return;
+  }
   const int L = bd.num_levels;
   // BisectionBlockClass::ChooseTimeSlices (BisectionBlock.cc:363-374)
   int sliceSep = 1 << L;
@@ -1088,6 +1101,226 @@ void bisection_move(Ctx* c, Replica& R, const pimc_bisection_desc& bd, uint32_t 
     accept_copy(R, slice1, slice2, ptcls, 1);
   else
     reject_copy(R, slice1, slice2, ptcls, 1);
+  if (rec) rec->accepted = toAccept ? 1 : 0;
+  if (st) {
+    st->moves++;
+    if (toAccept) st->accepted++;
+  }
+}
+
+// ------------------------------------------------------------------------
+// Bisection move with PermuteType TABLE (BisectionBlock.cc:51-64,126: the table permute
+// stage first and last in the stage list)
+// ------------------------------------------------------------------------
+
+// One entry of PermuteTableClass::ConstructHTable / UpdateHTable (PermuteTableClass.cc:88-150)
+// for species-relative particles i, j from copy `mode`: exp(-(|r_j(slice2) - r_i(slice1)|^2 -
+// |r_i(slice2) - r_i(slice1)|^2) / (4 lambda beta)), stored as float, 0 below log(epsilon).
+float htable_entry(Replica& R, int mode, int first, int i, int j, int slice1, int slice2, double logEps,
+                   double fourLambdaBetaInv) {
+  double disp_ii[3], disp_ij[3];
+  for (int d = 0; d < 3; d++) disp_ii[d] = R.P(mode, slice2, i + first)[d] - R.P(mode, slice1, i + first)[d];
+  R.put_in_box(disp_ii);
+  const double dist_ii = disp_ii[0] * disp_ii[0] + disp_ii[1] * disp_ii[1] + disp_ii[2] * disp_ii[2];
+  for (int d = 0; d < 3; d++) disp_ij[d] = R.P(mode, slice2, j + first)[d] - R.P(mode, slice1, i + first)[d];
+  R.put_in_box(disp_ij);
+  const double dist_ij = disp_ij[0] * disp_ij[0] + disp_ij[1] * disp_ij[1] + disp_ij[2] * disp_ij[2];
+  const double toExp = (-dist_ij + dist_ii) * fourLambdaBetaInv;
+  return toExp < logEps ? 0.0f : (float)exp(toExp);
+}
+
+struct CycleEntry {
+  int len;
+  int rep[4];
+  float P, C;   // weight and running sum, in float as CycleClass holds them
+};
+
+// PermuteTableClass::ConstructBosonCycleTable (PermuteTableClass.cc:339-449) with no excluded
+// particle and no z focus: the cycles in the reference's loop order, each weight
+// Gamma[len-1] prod H along the cycle rounded to float, kept when above epsilon, with the
+// running sum C formed in float.  Returns Norm (the last C); table may be null.
+float boson_cycle_table(const float* H, int n, const double* Gamma, double epsilon, std::vector<CycleEntry>* table) {
+  int num = 0;
+  float last_c = 0.0f;
+  auto add = [&](const CycleEntry& e) {
+    if (table) table->push_back(e);
+    last_c = e.C;
+    num++;
+  };
+  for (int i = 0; i < n; i++) {
+    CycleEntry t{};
+    t.len = 1;
+    t.rep[0] = i;
+    t.P = (float)Gamma[0];
+    t.C = t.P;
+    if (num != 0) t.C += last_c;
+    add(t);
+    for (int j = i + 1; j < n; j++) {   // 2 and higher cycles
+      t.rep[1] = j;
+      const double hprod = H[(size_t)i * n + j];
+      t.len = 2;
+      t.P = (float)(Gamma[1] * hprod * H[(size_t)j * n + i]);
+      t.C = t.P + last_c;
+      if (t.P > epsilon) add(t);
+      if (hprod * Gamma[2] * Gamma[3] > epsilon) {
+        for (int k = i + 1; k < n; k++) {   // 3 and higher cycles
+          if (k == j) continue;
+          t.rep[2] = k;
+          const double hprod2 = hprod * H[(size_t)j * n + k];
+          t.len = 3;
+          t.P = (float)(Gamma[2] * hprod2 * H[(size_t)k * n + i]);
+          t.C = t.P + last_c;
+          if (t.P > epsilon) add(t);
+          if (hprod2 * Gamma[3] > epsilon) {
+            for (int l = i + 1; l < n; l++) {
+              if (l == j || l == k) continue;
+              const double hprod3 = hprod2 * H[(size_t)k * n + l];
+              t.rep[3] = l;
+              t.len = 4;
+              t.P = (float)(Gamma[3] * hprod3 * H[(size_t)l * n + i]);
+              t.C = t.P + last_c;
+              if (t.P > epsilon) add(t);
+            }
+          }
+        }
+      }
+    }
+  }
+  return last_c;
+}
+
+// PermuteTableClass::FindEntry (PermuteTableClass.h:166-190): binary search of xi Norm in C.
+int find_entry(const std::vector<CycleEntry>& table, double Norm, double xi) {
+  xi *= Norm;
+  int hi = (int)table.size() - 1, lo = 0;
+  if (xi < table[0].C) return 0;
+  while (hi - lo > 1) {
+    const int attempt = (hi + lo) >> 1;
+    if (xi < table[attempt].C)
+      hi = attempt;
+    else
+      lo = attempt;
+  }
+  return hi;
+}
+
+// CycleClass::Apply (PermuteTableClass.cc:172-187) on the NEW copy at one slice.
+void cycle_apply_new(Replica& R, int first, int slice, int len, const int* rep) {
+  double tempPos[3];
+  memcpy(tempPos, R.P(1, slice, rep[0] + first), sizeof tempPos);
+  const int tempPtcl = R.perm[1][rep[0] + first];
+  for (int i = 0; i < len - 1; i++) {
+    memcpy(R.P(1, slice, rep[i] + first), R.P(1, slice, rep[i + 1] + first), 3 * sizeof(double));
+    R.perm[1][rep[i] + first] = R.perm[1][rep[i + 1] + first];
+  }
+  memcpy(R.P(1, slice, rep[len - 1] + first), tempPos, sizeof tempPos);
+  R.perm[1][rep[len - 1] + first] = tempPtcl;
+}
+
+void table_move(Ctx* c, Replica& R, const pimc_bisection_desc& bd, uint32_t move, pimc_move_record* rec,
+                pimc_bisection_stats* st) {
+  const int L = bd.num_levels;
+  // BisectionBlockClass::ChooseTimeSlices, MoveJoin (BisectionBlock.cc:363-374, 424)
+  const int sliceSep = 1 << L;
+  const int numLeft = R.nslices - sliceSep;
+  const int slice1 = (int)floor(rng_uniform(bd.seed, R.stream, move, TAG_SLICE, 0) * numLeft);
+  const int slice2 = slice1 + sliceSep;
+  move_join(R, R.join, slice2);
+  R.join = slice2;
+  const int sp = bd.species, first = R.first[sp], n = R.count[sp];
+  const double* Gamma = R.permute_gamma;
+  const double epsilon = R.permute_epsilon;
+  // TablePermuteStageClass::InitBlock: Forw->ConstructHTable (TablePermuteStage.cc:127-140)
+  const double logEps = log(epsilon);
+  const double beta = R.tau * (double)(slice2 - slice1);
+  const double fourLambdaBetaInv = 1.0 / (4.0 * R.lambda[sp] * beta);
+  std::vector<float> H((size_t)n * n);
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++)
+      H[(size_t)i * n + j] = htable_entry(R, 1, first, i, j, slice1, slice2, logEps, fourLambdaBetaInv);
+  // first Attempt: the cycle table and AttemptPermutation (TablePermuteStage.cc:204-230,
+  // PermuteTableClass.cc:189-203)
+  std::vector<CycleEntry> table;
+  const double Norm = boson_cycle_table(H.data(), n, Gamma, epsilon, &table);
+  const double NormInv = 1.0 / Norm;
+  const double xi = rng_uniform(bd.seed, R.stream, move, TAG_PERMUTE, 0);
+  const CycleEntry cyc = table[find_entry(table, Norm, xi)];
+  cycle_apply_new(R, first, slice2, cyc.len, cyc.rep);
+  const double forwT = cyc.P * NormInv;
+  const int len = cyc.len;
+  int ptcls[4];
+  for (int i = 0; i < len; i++) ptcls[i] = cyc.rep[i] + first;
+  double prevActionChange = -log(cyc.P / Gamma[len - 1]);
+  if (rec) {
+    memset(rec, 0, sizeof *rec);
+    rec->slice1 = slice1;
+    rec->ptcl = ptcls[0];
+    rec->cycle_length = len;
+    for (int i = 0; i < len; i++) rec->cycle[i] = ptcls[i];
+    rec->num_cycles = (int)table.size();
+    rec->forward_t = forwT;
+  }
+  // the bisection stages, moving every particle of the cycle (LocalStageClass::Attempt)
+  const bool lr = has_long_range(R);
+  bool toAccept = true;
+  int stage = 0;
+  for (int level = L - 1; level >= bd.lowest_level && toAccept; level--, stage++) {
+    const double logSampleRatio = bisection_sample(R, slice1, L, level, ptcls, len, bd.seed, move);
+    const bool with_lr = lr && (level == bd.lowest_level);
+    const double oldAction = stage_action(c, R, PIMC_OLDMODE, slice1, slice2, ptcls, len, level, with_lr, st);
+    const double newAction = stage_action(c, R, PIMC_NEWMODE, slice1, slice2, ptcls, len, level, with_lr, st);
+    const double currActionChange = newAction - oldAction;
+    const double logAcceptProb = logSampleRatio - currActionChange + prevActionChange;
+    const double logu = log(rng_uniform(bd.seed, R.stream, move, TAG_ACCEPT, level));
+    toAccept = logAcceptProb >= logu;
+    prevActionChange = currActionChange;
+    if (st) st->stages++;
+    if (rec) {
+      rec->old_action[stage] = oldAction;
+      rec->new_action[stage] = newAction;
+      rec->log_sample_ratio[stage] = logSampleRatio;
+      rec->log_u[stage] = logu;
+      rec->num_stages = stage + 1;
+    }
+  }
+  if (toAccept) {
+    // last Attempt (TablePermuteStage.cc:232-260): Rev->UpdateHTable on the NEW copy, then
+    // CalcReverseProb (PermuteTableClass.cc:205-214) and the test
+    double revT;
+    if (len != 1) {
+      std::vector<float> Hr = H;   // Rev's table is Forw's from InitBlock
+      for (int i = 0; i < n; i++)
+        for (int jC = 0; jC < len; jC++) {
+          const int j = cyc.rep[jC];
+          Hr[(size_t)i * n + j] = htable_entry(R, 1, first, i, j, slice1, slice2, logEps, fourLambdaBetaInv);
+        }
+      for (int iC = 0; iC < len; iC++) {
+        const int i = cyc.rep[iC];
+        for (int j = 0; j < n; j++)
+          Hr[(size_t)i * n + j] = htable_entry(R, 1, first, i, j, slice1, slice2, logEps, fourLambdaBetaInv);
+      }
+      const double revNorm = boson_cycle_table(Hr.data(), n, Gamma, epsilon, nullptr);
+      const double revNormInv = 1.0 / revNorm;
+      revT = (Gamma[len - 1] * Gamma[len - 1]) / (cyc.P) * revNormInv;
+    } else {
+      revT = cyc.P * NormInv;
+    }
+    const double actionChange = -log(cyc.P / Gamma[len - 1]);
+    const double pi_ratio = exp(-actionChange);
+    const double Tratio = forwT / revT;
+    const double acceptProb = std::min(1.0, pi_ratio / Tratio);
+    const double psi = rng_uniform(bd.seed, R.stream, move, TAG_PERMUTE_ACCEPT, 0);
+    toAccept = acceptProb > psi;
+    if (rec) {
+      rec->reverse_t = revT;
+      rec->accept_prob = acceptProb;
+      rec->psi = psi;
+    }
+  }
+  if (toAccept)
+    accept_copy(R, slice1, slice2, ptcls, len);
+  else
+    reject_copy(R, slice1, slice2, ptcls, len);
   if (rec) rec->accepted = toAccept ? 1 : 0;
   if (st) {
     st->moves++;
@@ -1462,20 +1695,29 @@ int pimc_permute_htable(pimc_ctx* ctx, int32_t replica, int32_t mode, int32_t sp
   const double beta = R->tau * (double)(slice2 - slice1);
   const double fourLambdaBetaInv = 1.0 / (4.0 * lambda * beta);
   const int N = R->count[species];
-  for (int i = 0; i < N; i++) {
-    double disp_ii[3];
-    for (int d = 0; d < 3; d++) disp_ii[d] = R->P(mode, slice2, i + firstPtcl)[d] - R->P(mode, slice1, i + firstPtcl)[d];
-    R->put_in_box(disp_ii);
-    const double dist_ii = disp_ii[0] * disp_ii[0] + disp_ii[1] * disp_ii[1] + disp_ii[2] * disp_ii[2];
-    for (int j = 0; j < N; j++) {
-      double disp_ij[3];
-      for (int d = 0; d < 3; d++) disp_ij[d] = R->P(mode, slice2, j + firstPtcl)[d] - R->P(mode, slice1, i + firstPtcl)[d];
-      R->put_in_box(disp_ij);
-      const double dist_ij = disp_ij[0] * disp_ij[0] + disp_ij[1] * disp_ij[1] + disp_ij[2] * disp_ij[2];
-      const double toExp = (-dist_ij + dist_ii) * fourLambdaBetaInv;
-      htable[(size_t)i * N + j] = toExp < logEps ? 0.0f : (float)exp(toExp);
-    }
+  for (int i = 0; i < N; i++)
+    for (int j = 0; j < N; j++)
+      htable[(size_t)i * N + j] = htable_entry(*R, mode, firstPtcl, i, j, slice1, slice2, logEps, fourLambdaBetaInv);
+  return PIMC_OK;
+}
+
+// TablePermuteStageClass::Read (TablePermuteStage.cc:142-166): PermuteType TABLE for `species`
+int pimc_replica_set_permute(pimc_ctx* ctx, int32_t replica, int32_t species, const double* gamma,
+                             double epsilon) {
+  Replica* R = ctx ? get_rep(&ctx->c, replica) : nullptr;
+  if (!R) return fail(PIMC_ERR_ARG, "replica_set_permute: bad replica");
+  if (!gamma) {
+    R->permute_on = false;
+    return PIMC_OK;
   }
+  if (species < 0 || species >= R->ns) return fail(PIMC_ERR_ARG, "replica_set_permute: bad species");
+  if (!(epsilon > 0.0)) return fail(PIMC_ERR_ARG, "replica_set_permute: epsilon must be positive");
+  for (int i = 0; i < 4; i++)
+    if (!(gamma[i] > 0.0)) return fail(PIMC_ERR_ARG, "replica_set_permute: gamma must be positive");
+  R->permute_on = true;
+  R->permute_species = species;
+  for (int i = 0; i < 4; i++) R->permute_gamma[i] = gamma[i];
+  R->permute_epsilon = epsilon;
   return PIMC_OK;
 }
 
@@ -1878,6 +2120,22 @@ int pimc_estimator_pair_correlation(pimc_ctx* ctx, int32_t replica, int32_t mode
   return PIMC_OK;
 }
 
+// The GPU library's launch rules for TABLE permutation moves, mirrored so both backends refuse
+// the same calls: every listed replica moves `species` with TABLE or none does, and a TABLE
+// replica has no long-range action.
+static int check_table_launch(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, int species, int r,
+                              const char* who) {
+  const Replica* R = get_rep(&ctx->c, replicas[r]);
+  const bool table = R->permute_on && R->permute_species == species;
+  const Replica* R0 = get_rep(&ctx->c, replicas[0]);
+  if (R0 && table != (R0->permute_on && R0->permute_species == species))
+    return fail(PIMC_ERR_UNSUPPORTED, std::string(who) + ": replicas with and without TABLE permutations in one call");
+  if (table && has_long_range(*R))
+    return fail(PIMC_ERR_UNSUPPORTED, std::string(who) + ": TABLE permutation moves with the long-range action");
+  (void)nrep;
+  return PIMC_OK;
+}
+
 int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, const pimc_bisection_desc* desc,
                        int64_t first_move, int32_t num_moves, int32_t precision, pimc_move_record* records,
                        pimc_bisection_stats* stats) {
@@ -1893,6 +2151,8 @@ int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, con
     if (!R) return fail(PIMC_ERR_ARG, "bisection_run: bad replica");
     if ((1 << desc->num_levels) >= R->nslices) return fail(PIMC_ERR_ARG, "bisection_run: too few slices for num_levels");
     if (desc->species < 0 || desc->species >= R->ns) return fail(PIMC_ERR_ARG, "bisection_run: bad species");
+    int rc = check_table_launch(ctx, replicas, nrep, desc->species, r, "bisection_run");
+    if (rc) return rc;
     for (int s = 0; s < R->ns; s++)
       for (int t = 0; t < R->ns; t++) {
         const PairFit& f = *ctx->c.fits[R->pair_fit[s * R->ns + t]];
@@ -1959,8 +2219,12 @@ int pimc_bisection_tune(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, co
       first_move + trial_moves > (int64_t)0xffffffffLL)
     return fail(PIMC_ERR_ARG, "bisection_tune: bad argument");
   if (precision != PIMC_FP64 && precision != PIMC_MIXED) return fail(PIMC_ERR_ARG, "bisection_tune: bad precision");
-  for (int r = 0; r < nrep; r++)
-    if (!get_rep(&ctx->c, replicas[r])) return fail(PIMC_ERR_ARG, "bisection_tune: bad replica");
+  for (int r = 0; r < nrep; r++) {
+    Replica* R = get_rep(&ctx->c, replicas[r]);
+    if (!R) return fail(PIMC_ERR_ARG, "bisection_tune: bad replica");
+    if (R->permute_on && R->permute_species == desc->species)
+      return fail(PIMC_ERR_UNSUPPORTED, "bisection_tune: TABLE permutation moves run with the one-block plan only");
+  }
   ctx->plan = pimc_bisection_plan{256, 1, 0, 1, 0.0};
   if (out) *out = ctx->plan;
   return PIMC_OK;
@@ -1974,6 +2238,11 @@ int pimc_sweep_graph_create(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep
   // the same checks as a run, without moving anything
   int rc = pimc_bisection_run(ctx, replicas, nrep, desc, first_move, 0, precision, nullptr, nullptr);
   if (rc) return rc;
+  for (int r = 0; r < nrep; r++) {
+    const Replica* R = get_rep(&ctx->c, replicas[r]);
+    if (R->permute_on && R->permute_species == desc->species)
+      return fail(PIMC_ERR_UNSUPPORTED, "sweep_graph_create: TABLE permutation moves are not captured in graphs");
+  }
   OracleSweep g;
   g.replicas.assign(replicas, replicas + nrep);
   g.desc = *desc;
@@ -2076,6 +2345,20 @@ int oracle_multicubic(int32_t type, double start, double end, int32_t npoints, i
   }
   return PIMC_OK;
 }
+int oracle_cycle_table(const float* htable, int32_t n, const double* gamma, double epsilon, int32_t cap,
+                       int32_t* len, int32_t* rep, float* P, float* C, float* norm, double xi, int32_t* found) {
+  std::vector<CycleEntry> table;
+  *norm = boson_cycle_table(htable, n, gamma, epsilon, &table);
+  for (int e = 0; e < (int)table.size() && e < cap; e++) {
+    len[e] = table[e].len;
+    for (int q = 0; q < 4; q++) rep[4 * e + q] = q < table[e].len ? table[e].rep[q] : -1;
+    P[e] = table[e].P;
+    C[e] = table[e].C;
+  }
+  if (xi >= 0.0 && found) *found = find_entry(table, (double)*norm, xi);
+  return (int)table.size();
+}
+
 int oracle_bisection_sample(void* ctx, int32_t replica, int32_t ptcl, int32_t slice1, int32_t num_levels,
                             int32_t level, uint64_t seed, uint32_t move, double* log_sample_ratio) {
   pimc_ctx* c = (pimc_ctx*)ctx;

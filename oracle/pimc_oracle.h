@@ -47,6 +47,13 @@ int oracle_bisection_sample(void* ctx, int32_t replica, int32_t ptcl, int32_t sl
                             int32_t num_levels, int32_t level, uint64_t seed, uint32_t move,
                             double* log_sample_ratio);
 
+/* PermuteTableClass::ConstructBosonCycleTable over an n x n float HTable (row-major), no
+ * excluded particle: writes up to cap entries (len[e], rep[e][4] species-relative, P[e], C[e])
+ * in the table's order and returns the entry count (which may exceed cap); *norm = the last C.
+ * With xi >= 0, *found = FindEntry(xi) over the whole table. */
+int oracle_cycle_table(const float* htable, int32_t n, const double* gamma, double epsilon, int32_t cap,
+                       int32_t* len, int32_t* rep, float* P, float* C, float* norm, double xi, int32_t* found);
+
 #ifdef __cplusplus
 }
 #endif

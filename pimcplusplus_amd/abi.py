@@ -12,7 +12,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-PIMC_ABI_VERSION = 1
+PIMC_ABI_VERSION = 2
 MAX_ACTIVE = 4
 MAX_LEVELS = 8
 MAX_SPECIES = 4
@@ -129,6 +129,13 @@ class MoveRecord(C.Structure):
         ("new_action", C.c_double * MAX_LEVELS),
         ("log_sample_ratio", C.c_double * MAX_LEVELS),
         ("log_u", C.c_double * MAX_LEVELS),
+        ("cycle_length", C.c_int32),
+        ("cycle", C.c_int32 * MAX_ACTIVE),
+        ("num_cycles", C.c_int32),
+        ("forward_t", C.c_double),
+        ("reverse_t", C.c_double),
+        ("accept_prob", C.c_double),
+        ("psi", C.c_double),
     ]
 
 
@@ -193,6 +200,7 @@ SIGNATURES = {
     "pimc_permute_htable": (C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                                       _f32p]),
     "pimc_cycle_apply": (C.c_int, [_CTX, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _i32p]),
+    "pimc_replica_set_permute": (C.c_int, [_CTX, C.c_int32, C.c_int32, _f64p, C.c_double]),
     "pimc_rho_k_update_all": (C.c_int, [_CTX, C.c_int32]),
     "pimc_rho_k_get": (C.c_int, [_CTX, C.c_int32, C.c_int32, _f64p]),
     "pimc_action_short_range": (C.c_int, [_CTX, C.POINTER(ActionReq), C.c_int32, C.c_int32, _f64p]),
@@ -516,6 +524,15 @@ class PimcLib:
         self._check("pimc_cycle_apply", self.lib.pimc_cycle_apply(self.ctx, rep, first_ptcl, slice_, c.size,
                                                                   _p(c, _i32p)))
 
+    def set_permute(self, rep: int, species: int, gamma=None, epsilon: float = 0.0):
+        """PermuteType TABLE for the bisection moves of `species` (gamma: the 4 cycle weights),
+        or no permutations (gamma None)."""
+        g = None if gamma is None else _f64(gamma)
+        if g is not None and g.size != 4:
+            raise ValueError("gamma holds the weights of 1- to 4-cycles")
+        self._check("pimc_replica_set_permute", self.lib.pimc_replica_set_permute(
+            self.ctx, rep, species, _p(g, _f64p) if g is not None else None, epsilon))
+
     def rho_k_update_all(self, rep: int):
         self._check("pimc_rho_k_update_all", self.lib.pimc_rho_k_update_all(self.ctx, rep))
 
@@ -644,7 +661,9 @@ def records_to_numpy(rec, nrep: int, nmoves: int) -> dict:
     raw = np.ctypeslib.as_array(C.cast(rec, C.POINTER(C.c_uint8)), shape=(C.sizeof(rec),)).copy()
     dt = np.dtype([("slice1", "<i4"), ("ptcl", "<i4"), ("accepted", "<i4"), ("num_stages", "<i4"),
                    ("old_action", "<f8", (MAX_LEVELS,)), ("new_action", "<f8", (MAX_LEVELS,)),
-                   ("log_sample_ratio", "<f8", (MAX_LEVELS,)), ("log_u", "<f8", (MAX_LEVELS,))])
+                   ("log_sample_ratio", "<f8", (MAX_LEVELS,)), ("log_u", "<f8", (MAX_LEVELS,)),
+                   ("cycle_length", "<i4"), ("cycle", "<i4", (MAX_ACTIVE,)), ("num_cycles", "<i4"),
+                   ("forward_t", "<f8"), ("reverse_t", "<f8"), ("accept_prob", "<f8"), ("psi", "<f8")])
     assert dt.itemsize == C.sizeof(MoveRecord)
     arr = raw.view(dt).reshape(nrep, nmoves)
     return {name: np.array(arr[name]) for name in dt.names}

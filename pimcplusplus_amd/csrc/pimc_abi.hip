@@ -230,6 +230,10 @@ struct HostReplica {
   // set by the resident chain, which updates rho_k OLD only: copy NEW is refreshed from OLD
   // (refresh_rho1) before any call that reads or writes it
   bool rho1_stale = false;
+  // PermuteType TABLE for moves of permute_species (pimc_replica_set_permute)
+  bool permute_on = false;
+  int permute_species = 0;
+  PermuteDev permute{};
 };
 
 // PathClass::SetupkVecs3D with Include (PathClass.cc:281-292, 534-586): integer
@@ -298,6 +302,7 @@ struct pimc_ctx {
   // cluster launches: barrier counters, partial sums, the sticky abort flag, and
   // the bound of one barrier wait in device wall-clock ticks
   DevBuf d_bar, d_part, d_abort;
+  DevBuf d_permute;   // per list entry of a TABLE launch: its cycle weights and cut-off
   uint64_t wait_ticks = 0;
   pimc_bisection_plan plan{256, 1, 0, 1, 0.0};   // launch shape of pimc_bisection_run
   std::vector<std::unique_ptr<SweepGraph>> graphs;
@@ -1233,6 +1238,25 @@ int pimc_cycle_apply(pimc_ctx* ctx, int32_t replica, int32_t first_ptcl, int32_t
   return PIMC_OK;
 }
 
+int pimc_replica_set_permute(pimc_ctx* ctx, int32_t replica, int32_t species, const double* gamma,
+                             double epsilon) {
+  HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
+  if (!H) return fail(PIMC_ERR_ARG, "replica_set_permute: bad replica");
+  if (!gamma) {
+    H->permute_on = false;
+    return PIMC_OK;
+  }
+  if (species < 0 || species >= H->dev.core.ns) return fail(PIMC_ERR_ARG, "replica_set_permute: bad species");
+  if (!(epsilon > 0.0)) return fail(PIMC_ERR_ARG, "replica_set_permute: epsilon must be positive");
+  for (int i = 0; i < 4; i++)
+    if (!(gamma[i] > 0.0)) return fail(PIMC_ERR_ARG, "replica_set_permute: gamma must be positive");
+  H->permute_on = true;
+  H->permute_species = species;
+  for (int i = 0; i < 4; i++) H->permute.gamma[i] = gamma[i];
+  H->permute.epsilon = epsilon;
+  return PIMC_OK;
+}
+
 int pimc_rho_k_update_all(pimc_ctx* ctx, int32_t replica) {
   HostReplica* H = ctx ? ctx->rep(replica) : nullptr;
   if (!H) return fail(PIMC_ERR_ARG, "rho_k_update_all: bad replica");
@@ -1470,6 +1494,8 @@ constexpr size_t kLrPhaseBytes = 24 * 1024;
 struct BisectionPrep {
   int nt = 0;
   bool lean = false;
+  bool table = false;        // PermuteType TABLE moves (k_bisection_table)
+  size_t htable_bytes = 0;   // the largest moved species' HTable
   size_t table_bytes = 0, lr_bytes = 0;
   int lr_chunk = 1;
   std::vector<int> order;    // list positions, longest replica first (persistent launches)
@@ -1502,6 +1528,17 @@ int prepare_bisection(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, cons
     }
     if (desc->species < 0 || desc->species >= R.ns || H->dev.sp.count[desc->species] < 1)
       return fail(PIMC_ERR_ARG, "bisection: bad species");
+    const bool table = H->permute_on && H->permute_species == desc->species;
+    if (r > 0 && table != P.table)
+      return fail(PIMC_ERR_UNSUPPORTED, "bisection: replicas with and without TABLE permutations in one call");
+    P.table = table;
+    if (table) {
+      if (R.lr_on) return fail(PIMC_ERR_UNSUPPORTED, "bisection: TABLE permutation moves with the long-range action");
+      if (ctx->plan.cluster > 1) return fail(PIMC_ERR_UNSUPPORTED, "bisection: TABLE permutation moves need cluster = 1");
+      if (desc->num_levels > 5) return fail(PIMC_ERR_UNSUPPORTED, "bisection: TABLE permutation moves need num_levels <= 5");
+      const size_t n = (size_t)H->dev.sp.count[desc->species];
+      P.htable_bytes = std::max(P.htable_bytes, n * n * sizeof(float));
+    }
     if (ctx->plan.cluster > 1 && H->maybe_permuted) {
       // the blocks of a cluster cannot move the join of a permuted path together
       std::vector<int> p0(R.N), p1(R.N);
@@ -1650,6 +1687,13 @@ int persistent_grid(pimc_ctx* ctx, const BisectionPrep& P, const LaunchBisection
 int launch_timed(pimc_ctx* ctx, const BisectionPrep& P, const LaunchBisection& a, int precision, int nrep) {
   std::pair<hipEvent_t, hipEvent_t> ev;
   if (int rc_ev = timed_pair(ctx, ev)) return rc_ev;
+  if (P.table) {   // one block of 256 threads per replica
+    HIP_TRY(hipEventRecord(ev.first, ctx->stream));
+    hipError_t e = launch_bisection_table(P.nt, precision == PIMC_MIXED, P.lean, a, ctx->stream);
+    if (e != hipSuccess) return fail(PIMC_ERR_DEVICE, std::string("bisection launch: ") + hipGetErrorString(e));
+    HIP_TRY(hipEventRecord(ev.second, ctx->stream));
+    return PIMC_OK;
+  }
   const int grid = ctx->plan.persistent ? persistent_grid(ctx, P, a, precision, nrep) : nrep;
   int rc = check_cluster_fits(ctx, P, a, precision, nrep);
   if (rc) return rc;
@@ -1692,10 +1736,24 @@ int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, con
     HIP_TRY(hipMemcpy(ctx->d_repids.p, P.ids.data(), (size_t)nrep * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(ctx->d_reppos.p, P.order.data(), (size_t)nrep * sizeof(int), hipMemcpyHostToDevice));
   }
-  const LaunchBisection a =
+  LaunchBisection a =
       make_launch(ctx, P, desc, nrep, (uint32_t)first_move, num_moves, ctx->d_repids.as<int>(), ctx->d_reppos.as<int>(),
                   ctx->d_stats.as<int64_t>(), records ? ctx->d_records.as<pimc_move_record>() : nullptr,
                   ctx->d_queue.as<int>(), nullptr, ctx->d_bar.as<int>(), ctx->d_part.as<double>());
+  if (P.table) {
+    // each list entry's cycle weights and cut-off; one block per replica, no work queue
+    std::vector<PermuteDev> pd(nrep);
+    for (int i = 0; i < nrep; i++) pd[i] = ctx->rep(P.ids[i])->permute;
+    if ((rc = ctx->d_permute.ensure((size_t)nrep * sizeof(PermuteDev)))) return rc;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));   // earlier launches may still read the old entries
+    HIP_TRY(hipMemcpy(ctx->d_permute.p, pd.data(), (size_t)nrep * sizeof(PermuteDev), hipMemcpyHostToDevice));
+    a.permute = ctx->d_permute.as<PermuteDev>();
+    a.lds_htable_bytes = P.htable_bytes;
+    a.queue = nullptr;
+    if (bisection_table_lds_bytes(a) > 64 * 1024)
+      return fail(PIMC_ERR_UNSUPPORTED, "bisection: the moved species' HTable and the staged spline level exceed "
+                                        "the 64 KB of LDS of a TABLE permutation block (too many particles)");
+  }
   if ((rc = launch_timed(ctx, P, a, precision, nrep))) return rc;
   if (!records && !stats) return PIMC_OK;   // asynchronous: counters via pimc_bisection_stats_take
   if (records)
@@ -1755,6 +1813,7 @@ int pimc_bisection_tune(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, co
   int rc = prepare_bisection(ctx, replicas, nrep, desc, precision, P0);
   ctx->plan = caller_plan;
   if (rc) return rc;
+  if (P0.table) return fail(PIMC_ERR_UNSUPPORTED, "bisection_tune: TABLE permutation moves run with one block per replica");
   HIP_TRY(hipSetDevice(ctx->device));
   if ((rc = ctx->sync_descriptors())) return rc;
   if ((rc = ensure_acc(ctx))) return rc;
@@ -1860,6 +1919,7 @@ int pimc_sweep_graph_create(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep
   BisectionPrep P;
   int rc = prepare_bisection(ctx, replicas, nrep, desc, precision, P);
   if (rc) return rc;
+  if (P.table) return fail(PIMC_ERR_UNSUPPORTED, "sweep_graph_create: TABLE permutation moves are not captured in graphs");
   HIP_TRY(hipSetDevice(ctx->device));
   if ((rc = ctx->sync_descriptors())) return rc;
   if ((rc = ensure_acc(ctx))) return rc;

@@ -87,16 +87,33 @@ __device__ inline double sr_link(const RepCore& R, const FitDev& F, const Table&
   return U;
 }
 
+// The moved particles a partner loop leaves out: one (the chain without permutations) or the
+// particles of a permutation cycle.
+struct SkipOne {
+  int p;
+  __device__ inline bool has(int j) const { return j == p; }
+};
+struct SkipSet {
+  int p[kMaxActive];
+  int n;
+  __device__ inline bool has(int j) const {
+    bool in = false;
+#pragma unroll
+    for (int i = 0; i < kMaxActive; i++) in = in || (i < n && j == p[i]);
+    return in;
+  }
+};
+
 // ShortRange partner loop of one stage (ShortRangeClass.cc:149-169) for partners
-// [j0, j1): each thread walks one partner's links in order, OLD and NEW copies
+// [j0, j1) other than the moved ones: each thread walks one partner's links in order, OLD and NEW copies
 // of the moved particle side by side, two links per iteration with the
 // partner's beads loaded two links ahead.  `ls` threads share a partner, each
 // taking a contiguous range of K/ls links (at least two; K is a power of two):
 // the block covers blockDim/ls partners per pass.  Consecutive lanes take
 // consecutive partners, so bead loads coalesce.
-template <int NT, bool MIXED, bool BC, bool LIN, typename Table>
+template <int NT, bool MIXED, bool BC, bool LIN, typename Table, typename Skip = SkipOne>
 __device__ inline void sr_partners(const RepCore& R, const FitDev& F, const Table& T, const BcView& Q,
-                                   const UlongView ul, int j0, int j1, int ptcl, int slice1, int K, int step, int ls,
+                                   const UlongView ul, int j0, int j1, Skip skip, int slice1, int K, int step, int ls,
                                    const PIMC_LDS double* mold, const PIMC_LDS double* mnew, Kahan& acc_old,
                                    Kahan& acc_new) {
   const size_t row = (size_t)3 * R.N * step;     // distance between consecutive link slices
@@ -130,7 +147,7 @@ __device__ inline void sr_partners(const RepCore& R, const FitDev& F, const Tabl
   const int kper = K / split;
   const int kbeg = sub * kper, kend = kbeg + kper;
   for (int j = j0 + (int)threadIdx.x % ppass; j < j1; j += ppass) {
-    if (j == ptcl) continue;
+    if (skip.has(j)) continue;
     const PIMC_GLOBAL double* pj = (const PIMC_GLOBAL double*)R.path0 + (size_t)(slice1 + kbeg * step) * 3 * R.N + j;
     double p0[3], p1[3], p2[3];
     load(pj, p0);
@@ -516,7 +533,7 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
         if constexpr (HAS_BC) {
           if (F.kind == 1) {
             const BcView Q{qgrid_of(F), level, MIXED ? F.i32_lev[level] : 0};
-            sr_partners<NT, MIXED, true, false>(R, F, LevelTable64<true>{}, Q, ul, j0, j1, ptcl, slice1, K, step, a.link_split, mold,
+            sr_partners<NT, MIXED, true, false>(R, F, LevelTable64<true>{}, Q, ul, j0, j1, SkipOne{ptcl}, slice1, K, step, a.link_split, mold,
                                          mnew, acc_old, acc_new);
             continue;
           }
@@ -529,7 +546,7 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
             return stage_level64<LIN>(F, level + F.taupos, tbl);
         }();
         __syncthreads();
-        sr_partners<NT, MIXED, false, LIN>(R, F, T, BcView{}, UlongView{nullptr, nullptr}, j0, j1, ptcl, slice1, K, step, a.link_split, mold, mnew,
+        sr_partners<NT, MIXED, false, LIN>(R, F, T, BcView{}, UlongView{nullptr, nullptr}, j0, j1, SkipOne{ptcl}, slice1, K, step, a.link_split, mold, mnew,
                                       acc_old, acc_new);
       }
       double sr_old, sr_new;
@@ -727,6 +744,10 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
           rec->log_sample_ratio[s] = 0.0;
           rec->log_u[s] = 0.0;
         }
+        rec->cycle_length = 0;   // no permutation stage
+        for (int q = 0; q < kMaxActive; q++) rec->cycle[q] = 0;
+        rec->num_cycles = 0;
+        rec->forward_t = rec->reverse_t = rec->accept_prob = rec->psi = 0.0;
       }
     }
     __syncthreads();

@@ -200,7 +200,7 @@ struct ReplicaDev {
 // Philox4x32-10 and the random-stream layout
 //   key = seed; counter = {move, tag<<24 | level<<16 | attempt, index, stream}
 // --------------------------------------------------------------------------
-enum { kTagSlice = 1, kTagPtcl = 2, kTagGauss = 3, kTagAccept = 4 };
+enum { kTagSlice = 1, kTagPtcl = 2, kTagGauss = 3, kTagAccept = 4, kTagPermute = 5, kTagPermuteAccept = 6 };
 
 __host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 #pragma unroll

@@ -110,6 +110,15 @@ struct LaunchBisection {
   double* part;               // [nrep][2][cluster][4] partial stage sums
   int* abort_flag;            // set when a barrier wait timed out (sticky until the host clears it)
   uint64_t wait_ticks;        // bound of one barrier wait, in device wall-clock ticks
+  // TABLE permutation moves (k_bisection_table): per list entry, the cycle weights and cut-off
+  const struct PermuteDev* permute;
+  size_t lds_htable_bytes;    // the HTable of the largest moved species (n^2 floats)
+};
+
+// TablePermuteStageClass settings of one replica of a TABLE launch.
+struct PermuteDev {
+  double gamma[4];
+  double epsilon;
 };
 constexpr int kMaxCluster = 32;
 // Returns hipSuccess or the launch error.  lean: every pair fit is a DavidPA
@@ -119,6 +128,9 @@ constexpr int kMaxCluster = 32;
 hipError_t launch_bisection(int num_terms, bool mixed, bool lean, const LaunchBisection& a, int block, int grid,
                             hipStream_t s);
 hipError_t launch_sweep_advance(uint32_t* first_move, int num_moves, hipStream_t s);
+// The chain with PermuteType TABLE (a.permute set): one block of 256 threads per replica.
+hipError_t launch_bisection_table(int num_terms, bool mixed, bool lean, const LaunchBisection& a, hipStream_t s);
+size_t bisection_table_lds_bytes(const LaunchBisection& a);
 size_t bisection_lds_bytes_for(const LaunchBisection& a);
 // mode: 0 one block per replica, 1 persistent, 2 cluster.
 int bisection_blocks_per_cu(int num_terms, bool mixed, bool lean, int block, size_t lds, int mode);

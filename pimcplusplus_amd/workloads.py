@@ -108,6 +108,22 @@ def lj_system(name: str, n: int, m: int, tau: float, num_levels: int, seed: int,
                   num_levels=num_levels, seed=seed)
 
 
+def dilute_boson_system(num_replicas: int = 2, n: int = 16, m: int = 32, tau: float = 0.05, num_levels: int = 3,
+                        density: float = 0.005, seed: int = 77) -> System:
+    """Helium-4 atoms (mass 4 amu, LJ pair action) at a low density, for bisection moves with
+    PermuteType TABLE: nearest neighbours sit ~6 A apart, so over a window of 2^L slices
+    (4 lambda beta ~ 10 A^2 at the defaults) exchanges are drawn and accepted regularly."""
+    box = (n / density) ** (1.0 / 3.0)
+    table = lj_table_for(box, tau, num_levels)
+    rng = np.random.default_rng(seed)
+    reps = [ReplicaSpec(num_particles=n, num_time_slices=m, box=(box, box, box), tau=tau, species_count=[n],
+                        lambdas=[LAMBDA_AMU / 4.0], fit_keys=["lj"], beads=random_start(n, box, m, rng),
+                        rng_stream=r)
+            for r in range(num_replicas)]
+    return System(name=f"bose-n{n}-m{m}-l{num_levels}", tables={"lj": table}, desired_tau={"lj": tau},
+                  replicas=reps, num_levels=num_levels, seed=seed)
+
+
 def config0(num_replicas: int = 1) -> System:
     """64 LJ particles, M=32, tau=0.025, level-3 bisection, seed 1000."""
     return lj_system("lj-n64-m32-l3", 64, 32, 0.025, 3, 1000, num_replicas)

@@ -41,6 +41,9 @@ def oracle_cdll() -> ctypes.CDLL:
         lib.oracle_multicubic.restype = ctypes.c_int
         lib.oracle_bisection_sample.argtypes = [ctypes.c_void_p, i32, i32, i32, i32, i32, u64, u32, P(f64)]
         lib.oracle_bisection_sample.restype = ctypes.c_int
+        lib.oracle_cycle_table.argtypes = [P(ctypes.c_float), i32, P(f64), f64, i32, P(i32), P(i32),
+                                           P(ctypes.c_float), P(ctypes.c_float), P(ctypes.c_float), f64, P(i32)]
+        lib.oracle_cycle_table.restype = ctypes.c_int
         _oracle = lib
     return _oracle
 
@@ -94,3 +97,23 @@ def small_random_table(rng, num_points=40, num_tau=3, num_terms=2, grid_type=0, 
             y[lev, :, c] = (a[0] * np.exp(-x * (0.5 + abs(a[1]))) + 0.1 * a[2] * np.sin(2 * x + a[3])) / (1 + c)
     y[:, -1, :] = 0.0
     return y
+
+
+def cycle_table(htable, gamma, epsilon, xi=-1.0, cap=200000):
+    """The oracle's ConstructBosonCycleTable over a float HTable: (len, rep [e, 4], P, C, norm,
+    FindEntry(xi) or -1)."""
+    h = np.ascontiguousarray(htable, dtype=np.float32)
+    n = h.shape[0]
+    g = np.ascontiguousarray(gamma, dtype=np.float64)
+    ln = np.zeros(cap, np.int32)
+    rep = np.zeros((cap, 4), np.int32)
+    P = np.zeros(cap, np.float32)
+    C = np.zeros(cap, np.float32)
+    norm = ctypes.c_float()
+    found = ctypes.c_int32(-1)
+    fp = ctypes.POINTER(ctypes.c_float)
+    cnt = oracle_cdll().oracle_cycle_table(h.ctypes.data_as(fp), n, f64p(g), epsilon, cap, i32p(ln), i32p(rep),
+                                           P.ctypes.data_as(fp), C.ctypes.data_as(fp), ctypes.byref(norm), xi,
+                                           ctypes.byref(found))
+    assert 0 <= cnt <= cap
+    return ln[:cnt], rep[:cnt], P[:cnt], C[:cnt], float(norm.value), int(found.value)
