@@ -513,13 +513,27 @@ def host_cores():
 
 
 def cpu_model():
+    """The host CPU: lscpu's model name (the /proc/cpuinfo one where lscpu is missing), with
+    the x86 family/model numbers, which name the core generation when the name is generic."""
+    name, fam = None, {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                name = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            key = line.split(":", 1)[0].strip()
+            if key in ("model name", "cpu family", "model") and key not in fam:
+                fam[key] = line.split(":", 1)[1].strip()
     except OSError:
         pass
-    return "unknown"
+    name = name or fam.get("model name", "unknown")
+    if "cpu family" in fam and "model" in fam:
+        name += f" (family {fam['cpu family']}, model {fam['model']})"
+    return name
 
 
 def cpu_baseline(seconds: float = 8.0, threads: int | None = None, system=None, moves_per_call: int = 20):
