@@ -120,7 +120,8 @@ def load_system(lib, system):
     return ids
 
 
-def run_gpu(system, precision, steps, warmup, moves_per_step, tune=False, graph=False, paths_out=None, plan=None):
+def run_gpu(system, precision, steps, warmup, moves_per_step, tune=False, graph=False, paths_out=None, plan=None,
+            setup=None):
     """W untimed steps, then K steps queued back to back on the context's stream;
     the timed region is bracketed by stream synchronisations (the stats take).
     tune: the autotuner picks the launch plan first (on moves the chain then
@@ -128,6 +129,8 @@ def run_gpu(system, precision, steps, warmup, moves_per_step, tune=False, graph=
     that receives the paths of the first eight replicas after the run."""
     lib = native.open_context(0)
     ids = load_system(lib, system)
+    if setup:
+        setup(lib, ids)
     L, seed = system.num_levels, system.seed
     if plan is not None:
         lib.set_plan(*plan)
@@ -170,7 +173,23 @@ def other_workload(name: str, replicas: int):
         return workloads.config4(num_replicas=16)
     if name == "lj64":
         return workloads.config0(num_replicas=replicas)
+    if name == "bose":
+        return workloads.dilute_boson_system(num_replicas=replicas)
     raise ValueError(name)
+
+
+# PermuteType TABLE settings of the boson workload
+BOSE_GAMMA, BOSE_EPSILON = (1.0, 1.0, 1.0, 1.0), 1e-4
+
+
+def permute_setup(workload: str):
+    """The replicas' set-up beyond loading: TABLE permutations for the boson workload."""
+    if workload != "bose":
+        return None
+    def setup(lib, ids):
+        for r in ids:
+            lib.set_permute(r, 0, BOSE_GAMMA, BOSE_EPSILON)
+    return setup
 
 
 def report_other(args):
@@ -185,6 +204,10 @@ def report_other(args):
                       "driver": "tuned plan, captured sweep graph"}}
     nmean = float(np.mean([r.num_particles for r in system.replicas]))
     fixed = None
+    setup = permute_setup(args.workload)
+    if setup:   # TABLE moves run with one block per replica, eager launches
+        fixed = (256, 1, False, 1)
+        out["config"]["driver"] = "PermuteType TABLE, one block per replica, eager launches"
     if args.plan:
         b, ls, pers, cl = (int(x) for x in args.plan.split(","))
         fixed = (b, ls, bool(pers), cl)
@@ -194,7 +217,7 @@ def report_other(args):
         runs = tuple(r for r in runs if r[0] == args.only)
     for name, prec in runs:
         tot, wall, dev_ms, plan = run_gpu(system, prec, args.steps, args.warmup, args.moves_per_step,
-                                          tune=fixed is None, graph=fixed is None, plan=fixed)
+                                          tune=fixed is None, graph=fixed is None, plan=fixed, setup=setup)
         sr_bytes, lr_bytes = workload_bytes(tot, nmean)
         kms = dev_ms / args.steps
         out[name] = {"evals_per_s": tot.pair_evals / wall, "ms_per_step": 1e3 * wall / args.steps,
@@ -219,7 +242,7 @@ def report_other(args):
                                "particle count) plus 16 B per counted k-term (32 B per k.slice.species update of "
                                "the lowest stage, counted once per OLD/NEW pair), over the kernel's HIP-event time"}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds / 2, system=system, moves_per_call=4)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds / 2, system=system, moves_per_call=4, setup=setup)
     if args.workload == "ocp" and not args.no_dserr:
         out["estimators_mixed_vs_fp64"] = estimators_check(system)
     print(json.dumps(out))
@@ -536,7 +559,8 @@ def cpu_model():
     return name
 
 
-def cpu_baseline(seconds: float = 8.0, threads: int | None = None, system=None, moves_per_call: int = 20):
+def cpu_baseline(seconds: float = 8.0, threads: int | None = None, system=None, moves_per_call: int = 20,
+                 setup=None):
     """The oracle (the C++ restatement of the reference's CPU path) on host cores:
     one replica per thread, the reference's clone-parallel model; every core the process
     may use (affinity and cgroup quota).  system: the workload whose replicas the threads
@@ -552,6 +576,8 @@ def cpu_baseline(seconds: float = 8.0, threads: int | None = None, system=None, 
         sub = workloads.System(system.name, system.tables, system.desired_tau, [reps[r]],
                                system.num_levels, system.seed)
         ids = workloads.load(o, sub)
+        if setup:
+            setup(o, ids)
         ctxs.append((o, ids))
 
     def chunk(o, ids, res, k):
@@ -622,10 +648,12 @@ def main():
     ap.add_argument("--lr-call", default=None, choices=("new_window", "old_window", "old_full_path"),
                     help="--workload longrange: time this call only (profiling runs)")
     ap.add_argument("--workload", default="lj256",
-                    choices=("lj256", "lj64", "uneven", "hydrogen", "hydrogen15k", "ocp", "dropin", "longrange"),
+                    choices=("lj256", "lj64", "uneven", "hydrogen", "hydrogen15k", "ocp", "dropin", "longrange",
+                             "bose"),
                     help="lj256 (default) is the headline configuration; the others print a report line "
                          "(hydrogen15k: the hydrogen system at M=168, T=15,037 K; dropin: per-SingleAction "
-                         "calls of the drop-in mode; longrange: batched LongRange SingleAction requests)")
+                         "calls of the drop-in mode; longrange: batched LongRange SingleAction requests; bose: "
+                         "dilute helium bosons with PermuteType TABLE moves)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)

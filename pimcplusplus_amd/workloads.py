@@ -124,6 +124,22 @@ def dilute_boson_system(num_replicas: int = 2, n: int = 16, m: int = 32, tau: fl
                   replicas=reps, num_levels=num_levels, seed=seed)
 
 
+def coulomb_boson_system(num_replicas: int = 2, n: int = 12, m: int = 32, tau: float = 0.125, num_levels: int = 3,
+                         density: float = 0.05, z1z2: float = 0.5, seed: int = 5) -> System:
+    """Bosons of mass 1 (lambda = 1/2 in Hartree units) with a weak repulsive Coulomb pair
+    action (a PAcoulombBCFitClass table on a log q grid) and no long-range action: the generic
+    chain kernel with PermuteType TABLE, where 2-, 3- and 4-cycles are all accepted."""
+    from .tables import GRID_LOG, coulomb_chord_table
+    box = (n / density) ** (1.0 / 3.0)
+    tab = coulomb_chord_table(z1z2, tau, num_levels, q_grid=(GRID_LOG, 0.005, 0.75 * math.sqrt(3.0) * box))
+    rng = np.random.default_rng(seed)
+    reps = [ReplicaSpec(num_particles=n, num_time_slices=m, box=(box, box, box), tau=tau, species_count=[n],
+                        lambdas=[0.5], fit_keys=["bb"], beads=random_start(n, box, m, rng), rng_stream=r)
+            for r in range(num_replicas)]
+    return System(name=f"coulomb-bosons-n{n}-m{m}-l{num_levels}", tables={"bb": tab}, desired_tau={},
+                  replicas=reps, num_levels=num_levels, seed=seed)
+
+
 def config0(num_replicas: int = 1) -> System:
     """64 LJ particles, M=32, tau=0.025, level-3 bisection, seed 1000."""
     return lj_system("lj-n64-m32-l3", 64, 32, 0.025, 3, 1000, num_replicas)

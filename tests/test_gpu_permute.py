@@ -199,6 +199,41 @@ def test_table_permutation_chain_matches_the_oracle(oracle, gpu):
         assert oracle.join(a) == gpu.join(b)
 
 
+@pytest.mark.parametrize("precision", [abi.FP64, abi.MIXED])
+def test_table_permutation_chain_generic_kernel_matches_the_oracle(oracle, gpu, precision):
+    """The generic kernel's TABLE moves (a bicubic Coulomb pair action on a log grid), where
+    2-, 3- and 4-cycles are drawn and accepted: FP64 chains equal the oracle's, the mixed
+    chain's stage changes stay within 1e-6 of them."""
+    sysm = workloads.coulomb_boson_system(num_replicas=2)
+    print("Coulomb bosons, TABLE, generic kernel, 2 x 250 moves, precision", precision, flush=True)
+    io, ro, so = table_chains(oracle, sysm, 250)
+    ig, rg, sg = table_chains(gpu, sysm, 250, precision)
+    cl, acc = ro["cycle_length"], ro["accepted"].astype(bool)
+    assert all(np.sum(acc & (cl == k)) >= 2 for k in (2, 3, 4))
+    if precision == abi.FP64:
+        assert np.array_equal(ro["cycle"], rg["cycle"])
+        compare_chains(ro, rg)
+        for key in ("forward_t", "reverse_t", "accept_prob"):
+            assert np.max(np.abs(ro[key] - rg[key]) / (1e-30 + np.abs(ro[key]))) < 1e-6, key
+        for a, b in zip(io, ig):
+            assert np.max(np.abs(oracle.path_get(a) - gpu.path_get(b, 0))) < 1e-12
+            assert np.array_equal(oracle.permutation(a, 0), gpu.permutation(b, 0))
+        return
+    worst, n = 0.0, 0
+    for r in range(2):
+        for m in range(250):
+            ns = ro["num_stages"][r, m]
+            if ns != rg["num_stages"][r, m] or ro["accepted"][r, m] != rg["accepted"][r, m] or \
+                    ro["cycle_length"][r, m] != rg["cycle_length"][r, m]:
+                break
+            d = (ro["new_action"][r, m, :ns] - ro["old_action"][r, m, :ns]) - \
+                (rg["new_action"][r, m, :ns] - rg["old_action"][r, m, :ns])
+            worst = max(worst, float(np.max(np.abs(d))))
+            n += ns
+    assert n > 100
+    assert worst <= 1e-6
+
+
 def test_table_permutation_chain_mixed_within_the_bound(oracle, gpu):
     sysm = workloads.dilute_boson_system(num_replicas=2)
     print("dilute bosons, TABLE, mixed, 2 x 200 moves", flush=True)
