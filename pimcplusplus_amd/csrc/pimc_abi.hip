@@ -303,6 +303,7 @@ struct pimc_ctx {
   // the bound of one barrier wait in device wall-clock ticks
   DevBuf d_bar, d_part, d_abort;
   DevBuf d_permute;   // per list entry of a TABLE launch: its cycle weights and cut-off
+  DevBuf d_cycles;    // cycle-table scratch of TABLE launches
   uint64_t wait_ticks = 0;
   pimc_bisection_plan plan{256, 1, 0, 1, 0.0};   // launch shape of pimc_bisection_run
   std::vector<std::unique_ptr<SweepGraph>> graphs;
@@ -1489,6 +1490,9 @@ namespace {
 
 // LDS for the staged long-range phase tables of a bisection launch.
 constexpr size_t kLrPhaseBytes = 24 * 1024;
+// Cycle-table entries one block of a TABLE launch stores (8 B each); larger tables are walked
+// by one thread instead.
+constexpr int kCycleScratchEntries = 1 << 16;
 
 // What a bisection launch over a replica list needs, validated.
 struct BisectionPrep {
@@ -1750,6 +1754,16 @@ int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, con
     a.permute = ctx->d_permute.as<PermuteDev>();
     a.lds_htable_bytes = P.htable_bytes;
     a.queue = nullptr;
+    // cycle-table scratch: one region per block, blocks = two per CU (the kernel's occupancy)
+    int cus = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    a.permute_grid = std::max(1, std::min(nrep, 2 * std::max(1, cus)));
+    a.permute_nmax = 0;
+    for (int i = 0; i < nrep; i++) a.permute_nmax = std::max(a.permute_nmax, ctx->rep(P.ids[i])->dev.sp.count[desc->species]);
+    a.permute_cap = kCycleScratchEntries;
+    if ((rc = ctx->d_cycles.ensure(bisection_table_scratch_bytes(a.permute_nmax, a.permute_cap, a.permute_grid))))
+      return rc;
+    a.permute_scratch = ctx->d_cycles.p;
     if (bisection_table_lds_bytes(a) > 64 * 1024)
       return fail(PIMC_ERR_UNSUPPORTED, "bisection: the moved species' HTable and the staged spline level exceed "
                                         "the 64 KB of LDS of a TABLE permutation block (too many particles)");

@@ -14,12 +14,15 @@
 // The HTable updates of TablePermuteStageClass::Accept/Reject only serve later steps of the
 // same block, and StepsPerBlock = 1 rebuilds the table every move, so they change nothing.
 //
-// One block (4 waves) per replica.  The moved species' HTable lives in LDS.  The cycle table
-// is walked by one thread in the reference's loop order (its weights and running sums are
-// float, and the order of the float additions is part of the result), twice: once for Norm
-// and the entry count, once to find the drawn entry -- the first whose running sum exceeds
-// xi Norm, which is what the reference's binary search over the non-decreasing sums returns
-// -- so no table is stored and its size is unbounded.
+// One block (4 waves) per replica at a time; the grid's blocks take the replicas in turn.  The
+// moved species' HTable lives in LDS.  The cycle table is built by the block: the cycles that
+// begin (i) or (i, j) form one unit each; the units count their kept entries in parallel, one
+// thread turns the counts into offsets, the units write their float weights in the
+// reference's order, and one thread forms the float running sums in that order (the order of
+// the float additions is part of the result).  The drawn entry is the first whose running sum
+// exceeds xi Norm, which is what the reference's binary search over the non-decreasing sums
+// returns.  A table larger than the block's scratch falls back to walking the loops with one
+// thread, twice (once for Norm, once for the search), so its size is unbounded.
 #include "pimc_bisection.hpp"
 
 namespace pimc {
@@ -156,6 +159,141 @@ done:
   return last;
 }
 
+// The block's region of the cycle-table scratch: unit counts (then offsets), and the kept
+// entries' weights and particles (one byte each, 0xff beyond the cycle's length).
+struct CycleScratch {
+  int* unit;
+  float* P;
+  uint32_t* rep;
+  int cap;
+};
+__host__ __device__ inline int cycle_units(int n) { return n + n * (n - 1) / 2; }
+__host__ __device__ inline size_t cycle_region_bytes(int n_max, int cap) {
+  return (((size_t)cycle_units(n_max) * 4 + 255) & ~(size_t)255) + (size_t)cap * 8;
+}
+__device__ inline CycleScratch cycle_scratch(const LaunchBisection& a, int block) {
+  unsigned char* base = (unsigned char*)a.permute_scratch + (size_t)block * cycle_region_bytes(a.permute_nmax, a.permute_cap);
+  CycleScratch S;
+  S.unit = (int*)base;
+  S.P = (float*)(base + (((size_t)cycle_units(a.permute_nmax) * 4 + 255) & ~(size_t)255));
+  S.rep = (uint32_t*)(S.P + a.permute_cap);
+  S.cap = a.permute_cap;
+  return S;
+}
+
+// Unit u of the cycle table: per i, the 1-cycle (i) (j = -1), then the cycles that begin (i, j)
+// for j = i+1 .. n-1, in the reference's order.
+__device__ inline void unit_ij(int u, int n, int& i, int& j) {
+  int base = 0;
+  for (i = 0; i < n; i++) {
+    if (u < base + (n - i)) {
+      const int r = u - base;
+      j = r == 0 ? -1 : i + r;
+      return;
+    }
+    base += n - i;
+  }
+  i = n - 1;
+  j = -1;
+}
+
+// The kept entries of unit (i, j) in the reference's order (the loop body of
+// ConstructBosonCycleTable, PermuteTableClass.cc:365-440): f(len, particles, P).
+template <typename Fn>
+__device__ inline void unit_entries(const PIMC_LDS float* H, int n, const double* G, double eps, int i, int j, Fn f) {
+#pragma clang fp contract(off)
+  if (j < 0) {
+    f(1, (uint32_t)i | 0xffffff00u, (float)G[0]);
+    return;
+  }
+  const uint32_t r01 = (uint32_t)i | ((uint32_t)j << 8);
+  const double hprod = (double)H[i * n + j];
+  const float P2 = (float)(G[1] * hprod * (double)H[j * n + i]);
+  if ((double)P2 > eps) f(2, r01 | 0xffff0000u, P2);
+  if (!(hprod * G[2] * G[3] > eps)) return;
+  for (int k = i + 1; k < n; k++) {
+    if (k == j) continue;
+    const double hprod2 = hprod * (double)H[j * n + k];
+    const float P3 = (float)(G[2] * hprod2 * (double)H[k * n + i]);
+    const uint32_t r012 = r01 | ((uint32_t)k << 16);
+    if ((double)P3 > eps) f(3, r012 | 0xff000000u, P3);
+    if (!(hprod2 * G[3] > eps)) continue;
+    for (int l = i + 1; l < n; l++) {
+      if (l == j || l == k) continue;
+      const double hprod3 = hprod2 * (double)H[k * n + l];
+      const float P4 = (float)(G[3] * hprod3 * (double)H[l * n + i]);
+      if ((double)P4 > eps) f(4, r012 | ((uint32_t)l << 24), P4);
+    }
+  }
+}
+
+// The block builds the cycle table of H into S: returns the entry count (every thread), the
+// entries written when the count is within S.cap.  Called by every thread.
+__device__ inline int cycle_table_block(const PIMC_LDS float* H, int n, const double* G, double eps,
+                                        const CycleScratch& S, PIMC_LDS int* word) {
+  const int U = cycle_units(n);
+  for (int u = threadIdx.x; u < U; u += blockDim.x) {
+    int i, j, c = 0;
+    unit_ij(u, n, i, j);
+    unit_entries(H, n, G, eps, i, j, [&](int, uint32_t, float) { c++; });
+    S.unit[u] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int off = 0;
+    for (int u = 0; u < U; u++) {
+      const int c = S.unit[u];
+      S.unit[u] = off;
+      off += c;
+    }
+    *word = off;
+  }
+  __syncthreads();
+  const int E = *word;
+  if (E <= S.cap)
+    for (int u = threadIdx.x; u < U; u += blockDim.x) {
+      int i, j;
+      unit_ij(u, n, i, j);
+      int e = S.unit[u];
+      unit_entries(H, n, G, eps, i, j, [&](int, uint32_t r, float P) {
+        S.P[e] = P;
+        S.rep[e] = r;
+        e++;
+      });
+    }
+  __syncthreads();
+  return E;
+}
+
+// Norm of a stored table: the float running sum of its weights in order (one thread).
+__device__ inline float stored_norm(const CycleScratch& S, int E) {
+  float last = 0.0f;
+  for (int e = 0; e < E; e++) last = S.P[e] + last;
+  return last;
+}
+// The drawn entry of a stored table: the first whose running sum exceeds target, else the
+// last (one thread).
+__device__ inline CycleHit stored_find(const CycleScratch& S, int E, double target) {
+  float last = 0.0f;
+  int e = 0;
+  for (; e < E; e++) {
+    last = S.P[e] + last;
+    if ((double)last > target) break;
+  }
+  if (e == E) e = E - 1;
+  CycleHit hit{};
+  const uint32_t r = S.rep[e];
+  hit.len = 0;
+#pragma unroll
+  for (int q = 0; q < kMaxActive; q++) {
+    const uint32_t b = (r >> (8 * q)) & 0xffu;
+    hit.rep[q] = b == 0xffu ? 0 : (int)b;
+    hit.len += b == 0xffu ? 0 : 1;
+  }
+  hit.P = S.P[e];
+  return hit;
+}
+
 // ShortRange terms of the pairs among the moved particles (ShortRangeClass.cc:138-169 with
 // DoPtcl: a pair of moved particles counted once), OLD and NEW, links of this stage.
 // Thread t < npair * K takes pair t / K at link t % K; the pairs go through the full
@@ -225,6 +363,7 @@ __device__ void bisection_replica_table(const LaunchBisection& a, int item, PIMC
   PIMC_LDS int* cp = ctl + 2;                              // the cycle's particles (absolute)
   PIMC_LDS double* dctl = (PIMC_LDS double*)(ctl + 16);   // forwT, P, prevActionChange
   const int rowlen = 3 * (W + 1);
+  const CycleScratch CS = cycle_scratch(a, blockIdx.x);
 
   int64_t st_moves = 0, st_acc = 0, st_stages = 0, st_evals = 0;
   const int first = S.first[sp], n = S.count[sp];
@@ -256,13 +395,20 @@ __device__ void bisection_replica_table(const LaunchBisection& a, int item, PIMC
     }
     __syncthreads();
     // first Attempt of the permute stage: the cycle table, the drawn cycle
+    const int E = cycle_table_block(H, n, Pm.gamma, Pm.epsilon, CS, ctl + 8);
     if (tid == 0) {
-      int count = 0;
+      int count = E;
       CycleHit hit{};
-      const double Norm = cycle_walk(H, n, Pm.gamma, Pm.epsilon, -1.0, &count, &hit);
       const double xi = rng_uniform(a.seed, R.stream, move, kTagPermute, 0);
-      int count2 = 0;
-      cycle_walk(H, n, Pm.gamma, Pm.epsilon, xi * Norm, &count2, &hit);
+      double Norm;
+      if (E <= CS.cap) {
+        Norm = stored_norm(CS, E);
+        hit = stored_find(CS, E, xi * Norm);
+      } else {   // beyond the scratch: walk the loops twice
+        Norm = cycle_walk(H, n, Pm.gamma, Pm.epsilon, -1.0, &count, &hit);
+        int count2 = 0;
+        cycle_walk(H, n, Pm.gamma, Pm.epsilon, xi * Norm, &count2, &hit);
+      }
       ctl[1] = hit.len;
       for (int q = 0; q < kMaxActive; q++) ctl[2 + q] = hit.rep[q] + first;
       ctl[6] = count;
@@ -451,13 +597,15 @@ __device__ void bisection_replica_table(const LaunchBisection& a, int item, PIMC
         }
         __syncthreads();
       }
+      const int Er = np != 1 ? cycle_table_block(H, n, Pm.gamma, Pm.epsilon, CS, ctl + 8) : 0;
       if (tid == 0) {
         const double P = dctl[1];
         const double forwT = dctl[0];
         if (np != 1) {
           int count = 0;
           CycleHit unused{};
-          const double revNorm = cycle_walk(H, n, Pm.gamma, Pm.epsilon, -1.0, &count, &unused);
+          const double revNorm = Er <= CS.cap ? (double)stored_norm(CS, Er)
+                                              : (double)cycle_walk(H, n, Pm.gamma, Pm.epsilon, -1.0, &count, &unused);
           revT = (Pm.gamma[np - 1] * Pm.gamma[np - 1]) / (P) * (1.0 / revNorm);
         } else {
           revT = forwT;
@@ -536,18 +684,24 @@ __device__ void bisection_replica_table(const LaunchBisection& a, int item, PIMC
 template <int NT, bool MIXED, int VAR>
 __global__ void __launch_bounds__(kBlock) k_bisection_table(const LaunchBisection a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  bisection_replica_table<NT, MIXED, VAR>(a, blockIdx.x, (PIMC_LDS unsigned char*)smem_raw);
+  for (int item = blockIdx.x; item < a.nrep; item += gridDim.x) {   // uniform per block
+    bisection_replica_table<NT, MIXED, VAR>(a, item, (PIMC_LDS unsigned char*)smem_raw);
+    __syncthreads();
+  }
 }
+
+size_t bisection_table_scratch_bytes(int n_max, int cap, int grid) { return cycle_region_bytes(n_max, cap) * grid; }
 
 size_t bisection_table_lds_bytes(const LaunchBisection& a) { return table_lds(a).total; }
 
 template <bool MIXED, int VAR>
 static hipError_t launch_table_t(int nt, const LaunchBisection& a, size_t lds, hipStream_t s) {
+  const int grid = a.permute_grid;
   switch (nt) {
-    case 0: k_bisection_table<0, MIXED, VAR><<<a.nrep, kBlock, lds, s>>>(a); break;
-    case 1: k_bisection_table<1, MIXED, VAR><<<a.nrep, kBlock, lds, s>>>(a); break;
-    case 2: k_bisection_table<2, MIXED, VAR><<<a.nrep, kBlock, lds, s>>>(a); break;
-    default: k_bisection_table<3, MIXED, VAR><<<a.nrep, kBlock, lds, s>>>(a); break;
+    case 0: k_bisection_table<0, MIXED, VAR><<<grid, kBlock, lds, s>>>(a); break;
+    case 1: k_bisection_table<1, MIXED, VAR><<<grid, kBlock, lds, s>>>(a); break;
+    case 2: k_bisection_table<2, MIXED, VAR><<<grid, kBlock, lds, s>>>(a); break;
+    default: k_bisection_table<3, MIXED, VAR><<<grid, kBlock, lds, s>>>(a); break;
   }
   return hipGetLastError();
 }
@@ -556,6 +710,7 @@ hipError_t launch_bisection_table(int num_terms, bool mixed, bool lean, const La
   if (a.nrep <= 0 || a.num_moves <= 0) return hipSuccess;
   const size_t lds = table_lds(a).total;
   if (lds > 64 * 1024 || !a.permute || a.num_levels > 5) return hipErrorInvalidValue;
+  if (!a.permute_scratch || a.permute_grid < 1 || a.permute_grid > a.nrep || a.permute_cap < 1) return hipErrorInvalidValue;
   if (a.link_split != 1 && a.link_split != 2 && a.link_split != 4) return hipErrorInvalidValue;
   const int nt = num_terms > 3 ? 3 : num_terms;
   if (lean) return mixed ? launch_table_t<true, kLeanDavid>(nt, a, lds, s) : launch_table_t<false, kLeanDavid>(nt, a, lds, s);

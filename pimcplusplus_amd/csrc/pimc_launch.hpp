@@ -113,6 +113,10 @@ struct LaunchBisection {
   // TABLE permutation moves (k_bisection_table): per list entry, the cycle weights and cut-off
   const struct PermuteDev* permute;
   size_t lds_htable_bytes;    // the HTable of the largest moved species (n^2 floats)
+  void* permute_scratch;      // permute_grid regions of cycle-table scratch
+  int permute_nmax;           // the largest moved species' particle count
+  int permute_cap;            // table entries a region holds
+  int permute_grid;           // blocks of a TABLE launch (each takes replicas in turn)
 };
 
 // TablePermuteStageClass settings of one replica of a TABLE launch.
@@ -131,6 +135,8 @@ hipError_t launch_sweep_advance(uint32_t* first_move, int num_moves, hipStream_t
 // The chain with PermuteType TABLE (a.permute set): one block of 256 threads per replica.
 hipError_t launch_bisection_table(int num_terms, bool mixed, bool lean, const LaunchBisection& a, hipStream_t s);
 size_t bisection_table_lds_bytes(const LaunchBisection& a);
+// Device bytes of the cycle-table scratch of a TABLE launch of `grid` blocks.
+size_t bisection_table_scratch_bytes(int n_max, int cap, int grid);
 size_t bisection_lds_bytes_for(const LaunchBisection& a);
 // mode: 0 one block per replica, 1 persistent, 2 cluster.
 int bisection_blocks_per_cu(int num_terms, bool mixed, bool lean, int block, size_t lds, int mode);
