@@ -275,3 +275,22 @@ def test_table_permutation_moves_refuse_unsupported_launches(gpu):
     gpu.set_permute(ids[0], 0)                    # back to no permutations
     gpu.set_permute(ids[1], 0)
     gpu.bisection_run(ids, sysm.num_levels, sysm.seed, 0, 2, abi.FP64)
+
+
+def test_table_permutation_large_cycle_tables_match_the_oracle(oracle, gpu):
+    """32 helium atoms at the liquid density with 16-slice windows: the cycle tables hold about
+    2e5 entries, beyond a block's scratch, so the kernel walks the reference's loops with one
+    thread; the chains still equal the oracle's."""
+    sysm = workloads.lj_system("dense-bosons", 32, 32, 0.05, 4, 1234, num_replicas=2)
+    print("dense bosons, TABLE, 2 x 12 moves (cycle tables beyond the scratch)", flush=True)
+    io, ro, so = table_chains(oracle, sysm, 12)
+    ig, rg, sg = table_chains(gpu, sysm, 12)
+    assert ro["num_cycles"].min() > 65536
+    assert np.array_equal(ro["num_cycles"], rg["num_cycles"])
+    assert np.array_equal(ro["cycle"], rg["cycle"])
+    compare_chains(ro, rg)
+    for key in ("forward_t", "reverse_t", "accept_prob"):
+        assert np.max(np.abs(ro[key] - rg[key]) / (1e-30 + np.abs(ro[key]))) < 1e-6, key
+    for a, b in zip(io, ig):
+        assert np.max(np.abs(oracle.path_get(a) - gpu.path_get(b, 0))) < 1e-12
+        assert np.array_equal(oracle.permutation(a, 0), gpu.permutation(b, 0))
