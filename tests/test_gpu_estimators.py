@@ -106,18 +106,31 @@ def test_free_particle_energy_is_three_n_over_two_beta(gpu):
 
 
 def test_mixed_and_fp64_energies_agree_within_error_bars(gpu):
+    """The thermodynamic energy and g(r) of mixed and FP64 chains that share no random numbers
+    (the mixed chains run on another seed): each replica starts from the same configuration in
+    both precisions, so the replicas' (mixed - FP64) differences are compared with their own
+    standard error -- the energy within 4 of it, every g(r) bin within 4.5."""
     sysm = workloads.config0(num_replicas=256)
+    L = sysm.replicas[0].box[0]
+    bins, blocks, moves, drop = 30, 8, 100, 2
     res = {}
-    for prec in (abi.FP64, abi.MIXED):
+    for prec, seed in ((abi.FP64, sysm.seed), (abi.MIXED, sysm.seed + 7919)):
         ids = workloads.load(gpu, sysm)
-        vals = []
-        for block in range(8):
-            gpu.bisection_run(ids, sysm.num_levels, sysm.seed, 50 * block, 50, prec, records=False, wait=False)
-            vals.append([e.total for e in gpu.energy(ids)])
-        res[prec] = np.array(vals[2:]).mean(axis=0)
-    d = res[abi.MIXED] - res[abi.FP64]
-    err = math.sqrt(res[abi.FP64].var(ddof=1) / len(d) + res[abi.MIXED].var(ddof=1) / len(d))
-    assert abs(d.mean()) < 4.0 * err + 1e-9
+        e, g = [], []
+        for block in range(blocks):
+            gpu.bisection_run(ids, sysm.num_levels, seed, moves * block, moves, prec, records=False, wait=False)
+            e.append([x.total for x in gpu.energy(ids)])
+            g.append([gpu.pair_correlation(r, 0, 0, 0.0, L / 2, bins + 1) for r in ids])
+        res[prec] = (np.array(e[drop:]).mean(axis=0), np.array(g[drop:], dtype=float).mean(axis=0))
+    (ea, ga), (eb, gb) = res[abi.FP64], res[abi.MIXED]
+    assert not np.array_equal(ea, eb)          # the chains differ
+    de, dg = eb - ea, gb - ga
+    n = len(de)
+    assert abs(de.mean()) < 4.0 * de.std(ddof=1) / math.sqrt(n)
+    gerr = dg.std(axis=0, ddof=1) / math.sqrt(n)
+    ok = gerr > 0
+    assert ok.sum() >= bins // 2
+    assert np.max(np.abs(dg.mean(axis=0)[ok]) / gerr[ok]) < 4.5
 
 
 @pytest.mark.parametrize("name", ["lj", "charged_lj", "hydrogen"])
