@@ -1754,14 +1754,16 @@ int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, con
     a.permute = ctx->d_permute.as<PermuteDev>();
     a.lds_htable_bytes = P.htable_bytes;
     a.queue = nullptr;
-    // cycle-table scratch: one region per block, blocks = two per CU (the kernel's occupancy)
-    int cus = 0;
+    // LDS per block: the CU's LDS shared by the blocks the kernel's registers allow; grid: the
+    // blocks resident at once; cycle-table scratch: one region per block
+    int cus = 0, lds_cu = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-    a.permute_grid = std::max(1, std::min(nrep, 2 * std::max(1, cus)));
-    a.permute_nmax = 0;
-    for (int i = 0; i < nrep; i++) a.permute_nmax = std::max(a.permute_nmax, ctx->rep(P.ids[i])->dev.sp.count[desc->species]);
+    HIP_TRY(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, ctx->device));
+    a.permute_lds_limit = bisection_table_lds_limit((size_t)lds_cu);
+    const int per_cu = bisection_table_blocks_per_cu(P.nt, precision == PIMC_MIXED, P.lean, bisection_table_lds_bytes(a));
+    a.permute_grid = std::max(1, std::min(nrep, std::max(1, per_cu) * std::max(1, cus)));
     a.permute_cap = kCycleScratchEntries;
-    if ((rc = ctx->d_cycles.ensure(bisection_table_scratch_bytes(a.permute_nmax, a.permute_cap, a.permute_grid))))
+    if ((rc = ctx->d_cycles.ensure(bisection_table_scratch_bytes(a.permute_cap, a.permute_grid))))
       return rc;
     a.permute_scratch = ctx->d_cycles.p;
     if (bisection_table_lds_bytes(a) > 64 * 1024)

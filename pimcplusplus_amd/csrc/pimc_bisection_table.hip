@@ -23,6 +23,8 @@
 // exceeds xi Norm, which is what the reference's binary search over the non-decreasing sums
 // returns.  A table larger than the block's scratch falls back to walking the loops with one
 // thread, twice (once for Norm, once for the search), so its size is unbounded.
+#include <algorithm>
+
 #include "pimc_bisection.hpp"
 
 namespace pimc {
@@ -39,19 +41,27 @@ __host__ __device__ inline int tbl_gauss_offset(int W, int level) {
   return off;
 }
 
-// LDS of the table chain (byte offsets): the staged spline level, the HTable, the moved
+// LDS of the table chain (byte offsets): the staged spline level, the HTable and its 4-cycle
+// bounds, the moved
 // particles' beads (OLD and NEW, kMaxActive x (W+1) x 3), the per-midpoint log sample
 // probabilities and per-link kinetic actions of every moved particle, reduction scratch,
-// control words, the move's Gaussian pairs and Metropolis draws.
+// control words, the move's Gaussian pairs and Metropolis draws, then cycle-table storage up to
+// the block's share of the CU's LDS (a.permute_lds_limit).
 struct TableLds {
-  size_t h, mold, mnew, lnew, lold, kold, knew, scratch, ctl, gauss, logu, total;
+  size_t h, mx, mold, mnew, lnew, lold, kold, knew, scratch, ctl, gauss, logu, cyc, total;
+  int cyc_cap;   // cycle-table entries of the LDS area (12 bytes each)
 };
+// Waves per SIMD the kernel is compiled for (its register budget): with one wave of each block
+// on each SIMD, also the blocks a CU holds at once.
+constexpr int kTableWaves = 4;
 __host__ __device__ inline TableLds table_lds(const LaunchBisection& a) {
   const int W = 1 << a.num_levels;
   auto up = [](size_t x) { return (x + 15) & ~(size_t)15; };
   TableLds t;
   size_t o = up(a.lds_table_bytes);
   t.h = o;
+  o += up(a.lds_htable_bytes);
+  t.mx = o;
   o += up(a.lds_htable_bytes);
   t.mold = o;
   o += sizeof(double) * kMaxActive * (W + 1) * 3;
@@ -73,7 +83,10 @@ __host__ __device__ inline TableLds table_lds(const LaunchBisection& a) {
   o += sizeof(double) * tbl_gauss_offset(W, a.num_levels);
   t.logu = o;
   o += sizeof(double) * PIMC_MAX_LEVELS;
-  t.total = o;
+  // the rest of the block's LDS holds cycle tables (P, C, particles)
+  t.cyc = up(o);
+  t.cyc_cap = t.cyc < a.permute_lds_limit ? (int)((a.permute_lds_limit - t.cyc) / 12) : 0;
+  t.total = t.cyc + (size_t)12 * t.cyc_cap;
   return t;
 }
 
@@ -102,14 +115,22 @@ struct CycleHit {
   float P;
 };
 
+// Gamma[0..3] (the weights of 1- to 4-cycles) and the cut-off of a TABLE stage.  Fields, not an
+// array, so that they stay in registers (no index is computed at run time).
+struct Gammas {
+  double g0, g1, g2, g3, eps;
+  __device__ inline double of(int len) const { return len == 1 ? g0 : len == 2 ? g1 : len == 3 ? g2 : g3; }
+};
+
 // One walk of ConstructBosonCycleTable (PermuteTableClass.cc:339-449) with no excluded particle
 // and no z focus over the n x n table H: every cycle i < j, k, l in the reference's loop order,
 // weight Gamma[len-1] times the H products along the cycle, rounded to float, kept above
 // epsilon, its running sum C = P + (the last kept C) in float.  Returns Norm (the last C) and
 // the number of kept entries.  With target >= 0 the walk stops at the first kept entry whose
-// C exceeds target and returns it in *hit (the last kept entry when none does).  One thread.
-__device__ float cycle_walk(const PIMC_LDS float* H, int n, const double* G, double eps, double target, int* count,
-                            CycleHit* hit) {
+// C exceeds target and returns it in *hit (the last kept entry when none does).  The 4-cycle
+// loops that the bounds Mx rule out (see unit_entries) are skipped.  One thread.
+__device__ float cycle_walk(const PIMC_LDS float* H, const PIMC_LDS float* Mx, int n, const Gammas& G, double target,
+                            int* count, CycleHit* hit) {
 #pragma clang fp contract(off)
   int num = 0;
   float last = 0.0f;
@@ -127,27 +148,27 @@ __device__ float cycle_walk(const PIMC_LDS float* H, int n, const double* G, dou
   };
   for (int i = 0; i < n; i++) {
     rep[0] = i;
-    const float P1 = (float)G[0];
+    const float P1 = (float)G.g0;
     if (keep(1, P1, num != 0 ? P1 + last : P1)) goto done;
     for (int j = i + 1; j < n; j++) {   // 2 and higher cycles
       rep[1] = j;
       const double hprod = (double)H[i * n + j];
-      const float P2 = (float)(G[1] * hprod * (double)H[j * n + i]);
-      if ((double)P2 > eps && keep(2, P2, P2 + last)) goto done;
-      if (hprod * G[2] * G[3] > eps) {
+      const float P2 = (float)(G.g1 * hprod * (double)H[j * n + i]);
+      if ((double)P2 > G.eps && keep(2, P2, P2 + last)) goto done;
+      if (hprod * G.g2 * G.g3 > G.eps) {
         for (int k = i + 1; k < n; k++) {   // 3 and higher cycles
           if (k == j) continue;
           rep[2] = k;
           const double hprod2 = hprod * (double)H[j * n + k];
-          const float P3 = (float)(G[2] * hprod2 * (double)H[k * n + i]);
-          if ((double)P3 > eps && keep(3, P3, P3 + last)) goto done;
-          if (hprod2 * G[3] > eps) {
+          const float P3 = (float)(G.g2 * hprod2 * (double)H[k * n + i]);
+          if ((double)P3 > G.eps && keep(3, P3, P3 + last)) goto done;
+          if (hprod2 * G.g3 > G.eps && G.g3 * hprod2 * (double)Mx[k * n + i] * (1.0 + 1e-6) > G.eps) {
             for (int l = i + 1; l < n; l++) {
               if (l == j || l == k) continue;
               rep[3] = l;
               const double hprod3 = hprod2 * (double)H[k * n + l];
-              const float P4 = (float)(G[3] * hprod3 * (double)H[l * n + i]);
-              if ((double)P4 > eps && keep(4, P4, P4 + last)) goto done;
+              const float P4 = (float)(G.g3 * hprod3 * (double)H[l * n + i]);
+              if ((double)P4 > G.eps && keep(4, P4, P4 + last)) goto done;
             }
           }
         }
@@ -159,130 +180,234 @@ done:
   return last;
 }
 
-// The block's region of the cycle-table scratch: unit counts (then offsets), and the kept
-// entries' weights and particles (one byte each, 0xff beyond the cycle's length).
+// Storage of a built cycle table: the kept entries' float weights P, their float running sums C
+// and their particles (one byte each, 0xff beyond the cycle's length).  In LDS when the table
+// fits there, else in the block's region of global scratch (cap entries).
 struct CycleScratch {
-  int* unit;
   float* P;
+  float* C;
   uint32_t* rep;
   int cap;
 };
-__host__ __device__ inline int cycle_units(int n) { return n + n * (n - 1) / 2; }
-__host__ __device__ inline size_t cycle_region_bytes(int n_max, int cap) {
-  return (((size_t)cycle_units(n_max) * 4 + 255) & ~(size_t)255) + (size_t)cap * 8;
-}
+__host__ __device__ inline size_t cycle_region_bytes(int cap) { return (size_t)cap * 12; }
 __device__ inline CycleScratch cycle_scratch(const LaunchBisection& a, int block) {
-  unsigned char* base = (unsigned char*)a.permute_scratch + (size_t)block * cycle_region_bytes(a.permute_nmax, a.permute_cap);
+  unsigned char* base = (unsigned char*)a.permute_scratch + (size_t)block * cycle_region_bytes(a.permute_cap);
   CycleScratch S;
-  S.unit = (int*)base;
-  S.P = (float*)(base + (((size_t)cycle_units(a.permute_nmax) * 4 + 255) & ~(size_t)255));
-  S.rep = (uint32_t*)(S.P + a.permute_cap);
+  S.P = (float*)base;
+  S.C = S.P + a.permute_cap;
+  S.rep = (uint32_t*)(S.C + a.permute_cap);
   S.cap = a.permute_cap;
   return S;
 }
 
-// Unit u of the cycle table: per i, the 1-cycle (i) (j = -1), then the cycles that begin (i, j)
-// for j = i+1 .. n-1, in the reference's order.
-__device__ inline void unit_ij(int u, int n, int& i, int& j) {
-  int base = 0;
-  for (i = 0; i < n; i++) {
-    if (u < base + (n - i)) {
-      const int r = u - base;
-      j = r == 0 ? -1 : i + r;
-      return;
-    }
-    base += n - i;
+// The table in units, in the reference's order: per i, the 1-cycle (i), then per j > i the
+// 2-cycle (i, j) followed by one unit per k in (i, n) minus {j}: the 3-cycle (i, j, k) and
+// the 4-cycles that extend it.  Particle i owns 1 + m^2 units (m = n - i - 1), n + sum of
+// m^2 over m < n in all.
+struct UnitPos {
+  int i, j, k;   // j < 0: the 1-cycle (i); k < 0: the 2-cycle (i, j)
+};
+__host__ __device__ inline int cycle_units(int n) { return n + (n - 1) * n * (2 * n - 1) / 6; }
+__host__ __device__ inline UnitPos unit_at(int u, int n) {
+  int i = 0;
+  for (; i < n - 1; i++) {
+    const int m = n - i - 1;
+    if (u < 1 + m * m) break;
+    u -= 1 + m * m;
   }
-  i = n - 1;
-  j = -1;
+  if (u == 0) return UnitPos{i, -1, -1};
+  const int m = n - i - 1, r = u - 1, j = i + 1 + r / m, kk = r % m;
+  if (kk == 0) return UnitPos{i, j, -1};
+  const int k = i + kk;
+  return UnitPos{i, j, k >= j ? k + 1 : k};
+}
+__host__ __device__ inline void unit_next(UnitPos& p, int n) {
+  if (p.j < 0) {   // after the 1-cycle of i: the 2-cycle (i, i+1), or the next i
+    if (p.i + 1 < n) {
+      p.j = p.i + 1;
+    } else {
+      p.i++;
+    }
+    return;
+  }
+  int k = p.k < 0 ? p.i + 1 : p.k + 1;
+  if (k == p.j) k++;
+  if (k < n) {
+    p.k = k;
+    return;
+  }
+  p.k = -1;
+  if (p.j + 1 < n) {
+    p.j++;
+  } else {
+    p.i++;
+    p.j = -1;
+  }
 }
 
-// The kept entries of unit (i, j) in the reference's order (the loop body of
-// ConstructBosonCycleTable, PermuteTableClass.cc:365-440): f(len, particles, P).
+// Bounds that let a unit skip its 4-cycle loop: Mx(k, i) >= max over l > i of H(k,l) H(l,i)
+// (the products of two floats are exact in FP64; the maximum is rounded up to float).  Called
+// by every thread.
+__device__ inline void cycle_bounds(const PIMC_LDS float* H, PIMC_LDS float* Mx, int n) {
+  for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+    const int k = t / n, i = t % n;
+    double m = 0.0;
+    for (int l = i + 1; l < n; l++) m = fmax(m, (double)H[k * n + l] * (double)H[l * n + i]);
+    float f = (float)m;
+    if ((double)f < m) f = __uint_as_float(__float_as_uint(f) + 1u);
+    Mx[t] = f;
+  }
+  __syncthreads();
+}
+
+// The kept entries of one unit in the reference's order (the loop body of
+// ConstructBosonCycleTable, PermuteTableClass.cc:365-440): f(particles, P).  The 4-cycle loop
+// is skipped when G3 hprod2 Mx(k, i) (1 + 1e-6) <= eps: every P4 of the loop is at most
+// G3 hprod2 H(k,l) H(l,i) times (1 + 6e-8) for the roundings of its two FP64 products and of
+// its conversion to float, so none would pass the reference's test P4 > eps.
 template <typename Fn>
-__device__ inline void unit_entries(const PIMC_LDS float* H, int n, const double* G, double eps, int i, int j, Fn f) {
+__device__ inline void unit_entries(const PIMC_LDS float* H, const PIMC_LDS float* Mx, int n, const Gammas& G,
+                                    const UnitPos& p, Fn f) {
 #pragma clang fp contract(off)
+  const int i = p.i, j = p.j;
   if (j < 0) {
-    f(1, (uint32_t)i | 0xffffff00u, (float)G[0]);
+    f((uint32_t)i | 0xffffff00u, (float)G.g0);
     return;
   }
   const uint32_t r01 = (uint32_t)i | ((uint32_t)j << 8);
   const double hprod = (double)H[i * n + j];
-  const float P2 = (float)(G[1] * hprod * (double)H[j * n + i]);
-  if ((double)P2 > eps) f(2, r01 | 0xffff0000u, P2);
-  if (!(hprod * G[2] * G[3] > eps)) return;
-  for (int k = i + 1; k < n; k++) {
-    if (k == j) continue;
-    const double hprod2 = hprod * (double)H[j * n + k];
-    const float P3 = (float)(G[2] * hprod2 * (double)H[k * n + i]);
-    const uint32_t r012 = r01 | ((uint32_t)k << 16);
-    if ((double)P3 > eps) f(3, r012 | 0xff000000u, P3);
-    if (!(hprod2 * G[3] > eps)) continue;
-    for (int l = i + 1; l < n; l++) {
-      if (l == j || l == k) continue;
-      const double hprod3 = hprod2 * (double)H[k * n + l];
-      const float P4 = (float)(G[3] * hprod3 * (double)H[l * n + i]);
-      if ((double)P4 > eps) f(4, r012 | ((uint32_t)l << 24), P4);
-    }
+  if (p.k < 0) {
+    const float P2 = (float)(G.g1 * hprod * (double)H[j * n + i]);
+    if ((double)P2 > G.eps) f(r01 | 0xffff0000u, P2);
+    return;
+  }
+  if (!(hprod * G.g2 * G.g3 > G.eps)) return;
+  const int k = p.k;
+  const double hprod2 = hprod * (double)H[j * n + k];
+  const float P3 = (float)(G.g2 * hprod2 * (double)H[k * n + i]);
+  const uint32_t r012 = r01 | ((uint32_t)k << 16);
+  if ((double)P3 > G.eps) f(r012 | 0xff000000u, P3);
+  if (!(hprod2 * G.g3 > G.eps)) return;
+  if (G.g3 * hprod2 * (double)Mx[k * n + i] * (1.0 + 1e-6) <= G.eps) return;
+  for (int l = i + 1; l < n; l++) {
+    if (l == j || l == k) continue;
+    const double hprod3 = hprod2 * (double)H[k * n + l];
+    const float P4 = (float)(G.g3 * hprod3 * (double)H[l * n + i]);
+    if ((double)P4 > G.eps) f(r012 | ((uint32_t)l << 24), P4);
   }
 }
 
-// The block builds the cycle table of H into S: returns the entry count (every thread), the
-// entries written when the count is within S.cap.  Called by every thread.
-__device__ inline int cycle_table_block(const PIMC_LDS float* H, int n, const double* G, double eps,
-                                        const CycleScratch& S, PIMC_LDS int* word) {
+// Exclusive prefix sum of one int per thread over the block (whole waves); *total gets the
+// sum.  Called by every thread.
+__device__ inline int block_exclusive_scan(int v, PIMC_LDS int* wsum, int* total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) wsum[wave] = x;
+  __syncthreads();
+  int base = 0, tot = 0;
+  for (int w = 0; w < nw; w++) {
+    const int t = wsum[w];
+    base += w < wave ? t : 0;
+    tot += t;
+  }
+  __syncthreads();
+  *total = tot;
+  return base + x - v;
+}
+
+// A built table: its entry count and where its entries are (P == nullptr when it exceeds both
+// the LDS area and the global region: the caller walks the loops instead).
+struct CycleTable {
+  float* P;
+  float* C;
+  uint32_t* rep;
+  int E;
+};
+
+// The block builds the cycle table of H: each thread takes a contiguous run of units, counts
+// their kept entries, a block scan gives each run its offset, and the runs write their entries
+// there, into LDS (lcap entries at lP, lC, lrep) or the global region S.  Called by every thread.
+__device__ inline CycleTable cycle_table_block(const PIMC_LDS float* H, PIMC_LDS float* Mx, int n, const Gammas& G,
+                                               float* lP, float* lC, uint32_t* lrep, int lcap, const CycleScratch& S,
+                                               PIMC_LDS int* wsum) {
+  cycle_bounds(H, Mx, n);
   const int U = cycle_units(n);
-  for (int u = threadIdx.x; u < U; u += blockDim.x) {
-    int i, j, c = 0;
-    unit_ij(u, n, i, j);
-    unit_entries(H, n, G, eps, i, j, [&](int, uint32_t, float) { c++; });
-    S.unit[u] = c;
+  const int per = (U + (int)blockDim.x - 1) / (int)blockDim.x;
+  const int u0 = min(U, (int)threadIdx.x * per), u1 = min(U, u0 + per);
+  int c = 0;
+  if (u0 < u1) {
+    UnitPos p = unit_at(u0, n);
+    for (int u = u0; u < u1; u++, unit_next(p, n)) unit_entries(H, Mx, n, G, p, [&](uint32_t, float) { c++; });
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int off = 0;
-    for (int u = 0; u < U; u++) {
-      const int c = S.unit[u];
-      S.unit[u] = off;
-      off += c;
-    }
-    *word = off;
+  CycleTable t;
+  int e = block_exclusive_scan(c, wsum, &t.E);
+  if (t.E <= lcap) {
+    t.P = lP;
+    t.C = lC;
+    t.rep = lrep;
+  } else if (t.E <= S.cap) {
+    t.P = S.P;
+    t.C = S.C;
+    t.rep = S.rep;
+  } else {
+    t.P = nullptr;
+    t.C = nullptr;
+    t.rep = nullptr;
   }
-  __syncthreads();
-  const int E = *word;
-  if (E <= S.cap)
-    for (int u = threadIdx.x; u < U; u += blockDim.x) {
-      int i, j;
-      unit_ij(u, n, i, j);
-      int e = S.unit[u];
-      unit_entries(H, n, G, eps, i, j, [&](int, uint32_t r, float P) {
-        S.P[e] = P;
-        S.rep[e] = r;
+  if (t.P && u0 < u1) {
+    UnitPos p = unit_at(u0, n);
+    for (int u = u0; u < u1; u++, unit_next(p, n))
+      unit_entries(H, Mx, n, G, p, [&](uint32_t r, float P) {
+        t.P[e] = P;
+        t.rep[e] = r;
         e++;
       });
-    }
+  }
   __syncthreads();
-  return E;
+  return t;
 }
 
-// Norm of a stored table: the float running sum of its weights in order (one thread).
-__device__ inline float stored_norm(const CycleScratch& S, int E) {
-  float last = 0.0f;
-  for (int e = 0; e < E; e++) last = S.P[e] + last;
-  return last;
-}
-// The drawn entry of a stored table: the first whose running sum exceeds target, else the
-// last (one thread).
-__device__ inline CycleHit stored_find(const CycleScratch& S, int E, double target) {
+// The float running sums of a stored table in order, C[e] = P[e] + C[e-1]; returns Norm, the
+// last.  One thread.
+__device__ inline float table_sums(const CycleTable& t) {
   float last = 0.0f;
   int e = 0;
-  for (; e < E; e++) {
-    last = S.P[e] + last;
-    if ((double)last > target) break;
+  for (; e + 4 <= t.E; e += 4) {
+    const float p0 = t.P[e], p1 = t.P[e + 1], p2 = t.P[e + 2], p3 = t.P[e + 3];
+    last = p0 + last;
+    t.C[e] = last;
+    last = p1 + last;
+    t.C[e + 1] = last;
+    last = p2 + last;
+    t.C[e + 2] = last;
+    last = p3 + last;
+    t.C[e + 3] = last;
   }
-  if (e == E) e = E - 1;
+  for (; e < t.E; e++) {
+    last = t.P[e] + last;
+    t.C[e] = last;
+  }
+  return last;
+}
+// The drawn entry: the first whose running sum exceeds target, else the last.  The sums are
+// non-decreasing (non-negative weights), so a binary search finds it.  One thread.
+__device__ inline CycleHit table_find(const CycleTable& t, double target) {
+  int lo = 0, hi = t.E;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if ((double)t.C[mid] > target)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  const int e = lo < t.E ? lo : t.E - 1;
   CycleHit hit{};
-  const uint32_t r = S.rep[e];
+  const uint32_t r = t.rep[e];
   hit.len = 0;
 #pragma unroll
   for (int q = 0; q < kMaxActive; q++) {
@@ -290,7 +415,7 @@ __device__ inline CycleHit stored_find(const CycleScratch& S, int E, double targ
     hit.rep[q] = b == 0xffu ? 0 : (int)b;
     hit.len += b == 0xffu ? 0 : 1;
   }
-  hit.P = S.P[e];
+  hit.P = t.P[e];
   return hit;
 }
 
@@ -350,6 +475,7 @@ __device__ void bisection_replica_table(const LaunchBisection& a, int item, PIMC
   const TableLds lay = table_lds(a);
   PIMC_LDS unsigned char* tbl = smem;
   PIMC_LDS float* H = (PIMC_LDS float*)(smem + lay.h);
+  PIMC_LDS float* Mx = (PIMC_LDS float*)(smem + lay.mx);
   PIMC_LDS double* mold = (PIMC_LDS double*)(smem + lay.mold);   // [kMaxActive][W+1][3]
   PIMC_LDS double* mnew = (PIMC_LDS double*)(smem + lay.mnew);
   PIMC_LDS double* lnew = (PIMC_LDS double*)(smem + lay.lnew);   // [np][nb]
@@ -361,6 +487,10 @@ __device__ void bisection_replica_table(const LaunchBisection& a, int item, PIMC
   PIMC_LDS double* gauss = (PIMC_LDS double*)(smem + lay.gauss);
   PIMC_LDS double* logu_tab = (PIMC_LDS double*)(smem + lay.logu);
   PIMC_LDS int* cp = ctl + 2;                              // the cycle's particles (absolute)
+  PIMC_LDS int* wsum = ctl + 8;                            // scan scratch, one word per wave
+  float* lP = (float*)(PIMC_LDS float*)(smem + lay.cyc);    // LDS cycle-table storage
+  float* lC = lP + lay.cyc_cap;
+  uint32_t* lrep = (uint32_t*)(lC + lay.cyc_cap);
   PIMC_LDS double* dctl = (PIMC_LDS double*)(ctl + 16);   // forwT, P, prevActionChange
   const int rowlen = 3 * (W + 1);
   const CycleScratch CS = cycle_scratch(a, blockIdx.x);
@@ -372,6 +502,7 @@ __device__ void bisection_replica_table(const LaunchBisection& a, int item, PIMC
   const double beta = R.tau * (double)W;
   const double fourLambdaBetaInv = 1.0 / (4.0 * lambda * beta);
   const double logEps = log(Pm.epsilon);
+  const Gammas Gm{Pm.gamma[0], Pm.gamma[1], Pm.gamma[2], Pm.gamma[3], Pm.epsilon};
   int join = __builtin_amdgcn_readfirstlane(*(const PIMC_GLOBAL int*)R.join);
   int nonid = 0;
   for (int p = tid; p < R.N; p += blockDim.x) nonid |= (R.perm0[p] != p) | (R.perm1[p] != p);
@@ -395,19 +526,19 @@ __device__ void bisection_replica_table(const LaunchBisection& a, int item, PIMC
     }
     __syncthreads();
     // first Attempt of the permute stage: the cycle table, the drawn cycle
-    const int E = cycle_table_block(H, n, Pm.gamma, Pm.epsilon, CS, ctl + 8);
+    const CycleTable ft = cycle_table_block(H, Mx, n, Gm, lP, lC, lrep, lay.cyc_cap, CS, wsum);
     if (tid == 0) {
-      int count = E;
+      int count = ft.E;
       CycleHit hit{};
       const double xi = rng_uniform(a.seed, R.stream, move, kTagPermute, 0);
       double Norm;
-      if (E <= CS.cap) {
-        Norm = stored_norm(CS, E);
-        hit = stored_find(CS, E, xi * Norm);
-      } else {   // beyond the scratch: walk the loops twice
-        Norm = cycle_walk(H, n, Pm.gamma, Pm.epsilon, -1.0, &count, &hit);
+      if (ft.P) {
+        Norm = table_sums(ft);
+        hit = table_find(ft, xi * Norm);
+      } else {   // beyond the storage: walk the loops twice
+        Norm = cycle_walk(H, Mx, n, Gm, -1.0, &count, &hit);
         int count2 = 0;
-        cycle_walk(H, n, Pm.gamma, Pm.epsilon, xi * Norm, &count2, &hit);
+        cycle_walk(H, Mx, n, Gm, xi * Norm, &count2, &hit);
       }
       ctl[1] = hit.len;
       for (int q = 0; q < kMaxActive; q++) ctl[2 + q] = hit.rep[q] + first;
@@ -415,7 +546,7 @@ __device__ void bisection_replica_table(const LaunchBisection& a, int item, PIMC
       const double forwT = hit.P * (1.0 / Norm);
       dctl[0] = forwT;
       dctl[1] = (double)hit.P;
-      dctl[2] = -log(hit.P / Pm.gamma[hit.len - 1]);   // prevActionChange
+      dctl[2] = -log(hit.P / Gm.of(hit.len));   // prevActionChange
     }
     {
       // the move's Gaussian pairs (np particles) and Metropolis draws
@@ -597,20 +728,21 @@ __device__ void bisection_replica_table(const LaunchBisection& a, int item, PIMC
         }
         __syncthreads();
       }
-      const int Er = np != 1 ? cycle_table_block(H, n, Pm.gamma, Pm.epsilon, CS, ctl + 8) : 0;
+      CycleTable rt{nullptr, nullptr, nullptr, 0};
+      if (np != 1) rt = cycle_table_block(H, Mx, n, Gm, lP, lC, lrep, lay.cyc_cap, CS, wsum);
       if (tid == 0) {
         const double P = dctl[1];
         const double forwT = dctl[0];
         if (np != 1) {
           int count = 0;
           CycleHit unused{};
-          const double revNorm = Er <= CS.cap ? (double)stored_norm(CS, Er)
-                                              : (double)cycle_walk(H, n, Pm.gamma, Pm.epsilon, -1.0, &count, &unused);
-          revT = (Pm.gamma[np - 1] * Pm.gamma[np - 1]) / (P) * (1.0 / revNorm);
+          const double revNorm = rt.P ? (double)table_sums(rt)
+                                      : (double)cycle_walk(H, Mx, n, Gm, -1.0, &count, &unused);
+          revT = (Gm.of(np) * Gm.of(np)) / (P) * (1.0 / revNorm);
         } else {
           revT = forwT;
         }
-        const double actionChange = -log(P / Pm.gamma[np - 1]);
+        const double actionChange = -log(P / Gm.of(np));
         const double pi_ratio = exp(-actionChange);
         const double Tratio = forwT / revT;
         acceptProb = fmin(1.0, pi_ratio / Tratio);
@@ -682,7 +814,7 @@ __device__ void bisection_replica_table(const LaunchBisection& a, int item, PIMC
 }
 
 template <int NT, bool MIXED, int VAR>
-__global__ void __launch_bounds__(kBlock) k_bisection_table(const LaunchBisection a) {
+__global__ void __launch_bounds__(kBlock, kTableWaves) k_bisection_table(const LaunchBisection a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   for (int item = blockIdx.x; item < a.nrep; item += gridDim.x) {   // uniform per block
     bisection_replica_table<NT, MIXED, VAR>(a, item, (PIMC_LDS unsigned char*)smem_raw);
@@ -690,9 +822,35 @@ __global__ void __launch_bounds__(kBlock) k_bisection_table(const LaunchBisectio
   }
 }
 
-size_t bisection_table_scratch_bytes(int n_max, int cap, int grid) { return cycle_region_bytes(n_max, cap) * grid; }
+size_t bisection_table_scratch_bytes(int cap, int grid) { return cycle_region_bytes(cap) * grid; }
 
 size_t bisection_table_lds_bytes(const LaunchBisection& a) { return table_lds(a).total; }
+
+size_t bisection_table_lds_limit(size_t lds_per_cu) {
+  // 1 KB of each share is left for the kernel's static LDS (reduction words)
+  const size_t share = lds_per_cu / kTableWaves;
+  return std::min<size_t>(64 * 1024, (share > 1024 ? share - 1024 : 0) & ~(size_t)255);
+}
+
+template <bool MIXED, int VAR>
+static hipError_t table_occupancy_t(int& n, int nt, size_t lds) {
+  switch (nt) {
+    case 0: return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_bisection_table<0, MIXED, VAR>, kBlock, lds);
+    case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_bisection_table<1, MIXED, VAR>, kBlock, lds);
+    case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_bisection_table<2, MIXED, VAR>, kBlock, lds);
+    default: return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_bisection_table<3, MIXED, VAR>, kBlock, lds);
+  }
+}
+
+int bisection_table_blocks_per_cu(int num_terms, bool mixed, bool lean, size_t lds) {
+  int n = 0;
+  const int nt = num_terms > 3 ? 3 : num_terms;
+  const hipError_t e = lean ? (mixed ? table_occupancy_t<true, kLeanDavid>(n, nt, lds)
+                                     : table_occupancy_t<false, kLeanDavid>(n, nt, lds))
+                            : (mixed ? table_occupancy_t<true, kGeneric>(n, nt, lds)
+                                     : table_occupancy_t<false, kGeneric>(n, nt, lds));
+  return e == hipSuccess ? n : 0;
+}
 
 template <bool MIXED, int VAR>
 static hipError_t launch_table_t(int nt, const LaunchBisection& a, size_t lds, hipStream_t s) {

@@ -114,9 +114,9 @@ struct LaunchBisection {
   const struct PermuteDev* permute;
   size_t lds_htable_bytes;    // the HTable of the largest moved species (n^2 floats)
   void* permute_scratch;      // permute_grid regions of cycle-table scratch
-  int permute_nmax;           // the largest moved species' particle count
   int permute_cap;            // table entries a region holds
   int permute_grid;           // blocks of a TABLE launch (each takes replicas in turn)
+  size_t permute_lds_limit;   // LDS of a TABLE block; what the chain leaves holds cycle tables
 };
 
 // TablePermuteStageClass settings of one replica of a TABLE launch.
@@ -135,8 +135,12 @@ hipError_t launch_sweep_advance(uint32_t* first_move, int num_moves, hipStream_t
 // The chain with PermuteType TABLE (a.permute set): one block of 256 threads per replica.
 hipError_t launch_bisection_table(int num_terms, bool mixed, bool lean, const LaunchBisection& a, hipStream_t s);
 size_t bisection_table_lds_bytes(const LaunchBisection& a);
+// LDS a TABLE block may take so that the CU holds as many blocks as the kernel's registers allow.
+size_t bisection_table_lds_limit(size_t lds_per_cu);
+// Resident TABLE blocks per CU at `lds` bytes of LDS each (0 on error).
+int bisection_table_blocks_per_cu(int num_terms, bool mixed, bool lean, size_t lds);
 // Device bytes of the cycle-table scratch of a TABLE launch of `grid` blocks.
-size_t bisection_table_scratch_bytes(int n_max, int cap, int grid);
+size_t bisection_table_scratch_bytes(int cap, int grid);
 size_t bisection_lds_bytes_for(const LaunchBisection& a);
 // mode: 0 one block per replica, 1 persistent, 2 cluster.
 int bisection_blocks_per_cu(int num_terms, bool mixed, bool lean, int block, size_t lds, int mode);

@@ -277,15 +277,18 @@ def test_table_permutation_moves_refuse_unsupported_launches(gpu):
     gpu.bisection_run(ids, sysm.num_levels, sysm.seed, 0, 2, abi.FP64)
 
 
-def test_table_permutation_large_cycle_tables_match_the_oracle(oracle, gpu):
-    """32 helium atoms at the liquid density with 16-slice windows: the cycle tables hold about
-    2e5 entries, beyond a block's scratch, so the kernel walks the reference's loops with one
-    thread; the chains still equal the oracle's."""
-    sysm = workloads.lj_system("dense-bosons", 32, 32, 0.05, 4, 1234, num_replicas=2)
-    print("dense bosons, TABLE, 2 x 12 moves (cycle tables beyond the scratch)", flush=True)
-    io, ro, so = table_chains(oracle, sysm, 12)
-    ig, rg, sg = table_chains(gpu, sysm, 12)
-    assert ro["num_cycles"].min() > 65536
+@pytest.mark.parametrize("n,levels,moves,lo,hi", [(20, 3, 40, 5461, 65536), (32, 4, 12, 65536, 1 << 30)])
+def test_table_permutation_large_cycle_tables_match_the_oracle(oracle, gpu, n, levels, moves, lo, hi):
+    """Helium atoms at the liquid density: 20 atoms with 8-slice windows give cycle tables of
+    ~8e3 entries, beyond a block's 64 KB of LDS (5461 entries at most), which the block stores
+    in its global scratch; 32 atoms with 16-slice windows give ~2e5 entries, beyond that scratch
+    too, so the kernel walks the reference's loops with one thread.  The chains still equal the
+    oracle's."""
+    sysm = workloads.lj_system("dense-bosons", n, 32, 0.05, levels, 1234, num_replicas=2)
+    print(f"dense bosons, TABLE, 2 x {moves} moves, {n} atoms, level {levels}", flush=True)
+    io, ro, so = table_chains(oracle, sysm, moves)
+    ig, rg, sg = table_chains(gpu, sysm, moves)
+    assert lo < ro["num_cycles"].min() and ro["num_cycles"].max() <= hi
     assert np.array_equal(ro["num_cycles"], rg["num_cycles"])
     assert np.array_equal(ro["cycle"], rg["cycle"])
     compare_chains(ro, rg)
