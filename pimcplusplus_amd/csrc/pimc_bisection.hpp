@@ -41,7 +41,7 @@ __device__ inline double sr_link(const RepCore& R, const FitDev& F, const Table&
     {
 #pragma clang fp contract(off)
       const double n = -rint(raw[d] * R.boxinv[d]);
-      dn[d] = raw[d] + n * pbox[d];
+      dn[d] = MIXED ? fma(n, pbox[d], raw[d]) : raw[d] + n * pbox[d];   // FP64: DistDisp's rounding
     }
     // DistDisp's consistent image m = nearbyint((dA - raw)/L) equals the nearest
     // image n whenever |dA - dn| < L/2 (hsafe keeps a margin far above rounding)
@@ -60,7 +60,7 @@ __device__ inline double sr_link(const RepCore& R, const FitDev& F, const Table&
     for (int d = 0; d < 3; d++) {
 #pragma clang fp contract(off)
       const double mm = rint((dA[d] - raw[d]) * R.boxinv[d]);
-      db[d] = raw[d] + mm * pbox[d];
+      db[d] = MIXED ? fma(mm, pbox[d], raw[d]) : raw[d] + mm * pbox[d];
       e[d] = dA[d] - db[d];
     }
     const double b2 = db[0] * db[0] + db[1] * db[1] + db[2] * db[2];
@@ -75,8 +75,11 @@ __device__ inline double sr_link(const RepCore& R, const FitDev& F, const Table&
   if constexpr (BC) {
     U = MIXED ? pair_u_bc_mixed(F, Q.q, Q.level, Q.i32, q, s2) : pair_u_bc(F, Q.q, Q.level, q, s2);
   } else {
-    const double rmin = F.start;
-    U = (rA < rmin || rb < rmin) ? urmin : 0.5 * (uA + ub) + link_offdiag<NT, LIN>(F, T, q, z, s2);
+    // rA < rmin || rb < rmin, as two integer compares: the bit patterns of non-negative doubles
+    // (and NaN, above every finite one) order as the values do
+    const long long rmin_bits = __double_as_longlong(F.start);
+    const bool core = (__double_as_longlong(rA) < rmin_bits) | (__double_as_longlong(rb) < rmin_bits);
+    U = core ? urmin : 0.5 * (uA + ub) + link_offdiag<NT, LIN>(F, T, q, z, s2);
   }
   if (ul.t64) U -= 0.5 * (lA + lb);
 #pragma unroll
@@ -114,8 +117,8 @@ struct SkipSet {
 template <int NT, bool MIXED, bool BC, bool LIN, typename Table, typename Skip = SkipOne>
 __device__ inline void sr_partners(const RepCore& R, const FitDev& F, const Table& T, const BcView& Q,
                                    const UlongView ul, int j0, int j1, Skip skip, int slice1, int K, int step, int ls,
-                                   const PIMC_LDS double* mold, const PIMC_LDS double* mnew, Kahan& acc_old,
-                                   Kahan& acc_new) {
+                                   const PIMC_LDS double* mold, const PIMC_LDS double* mnew,
+                                   StageSums<MIXED>& acc) {
   const size_t row = (size_t)3 * R.N * step;     // distance between consecutive link slices
   const double pbox[3] = {R.isper[0] * R.box[0], R.isper[1] * R.box[1], R.isper[2] * R.box[2]};
   const double hsafe[3] = {0.4999999 * R.box[0], 0.4999999 * R.box[1], 0.4999999 * R.box[2]};
@@ -131,13 +134,7 @@ __device__ inline void sr_partners(const RepCore& R, const FitDev& F, const Tabl
     const int at = 3 * (k + 1) * step;
     const double uo = sr_link<NT, MIXED, BC, LIN>(R, F, T, Q, ul, pbox, hsafe, urmin, mold + at, p, dO, rO, uO, lO);
     const double un = sr_link<NT, MIXED, BC, LIN>(R, F, T, Q, ul, pbox, hsafe, urmin, mnew + at, p, dN, rN, uN, lN);
-    if (MIXED) {
-      acc_old.add(uo);
-      acc_new.add(un);
-    } else {
-      acc_old.s += uo;
-      acc_new.s += un;
-    }
+    acc.add(uo, un);
   };
   const int ppass = (int)blockDim.x / ls;             // partners per pass: a multiple of the wave size
   // this thread's link range; whole waves share one, so it is wave-uniform (scalar)
@@ -160,8 +157,13 @@ __device__ inline void sr_partners(const RepCore& R, const FitDev& F, const Tabl
     for (int d = 0; d < 3; d++) {
 #pragma clang fp contract(off)
       const double ao = p0[d] - mo0[d], an = p0[d] - mn0[d];
-      dO[d] = ao + (-rint(ao * R.boxinv[d])) * pbox[d];
-      dN[d] = an + (-rint(an * R.boxinv[d])) * pbox[d];
+      if constexpr (MIXED) {
+        dO[d] = fma(-rint(ao * R.boxinv[d]), pbox[d], ao);
+        dN[d] = fma(-rint(an * R.boxinv[d]), pbox[d], an);
+      } else {
+        dO[d] = ao + (-rint(ao * R.boxinv[d])) * pbox[d];
+        dN[d] = an + (-rint(an * R.boxinv[d])) * pbox[d];
+      }
     }
     const double o2 = dO[0] * dO[0] + dO[1] * dO[1] + dO[2] * dO[2];
     const double w2 = dN[0] * dN[0] + dN[1] * dN[1] + dN[2] * dN[2];
@@ -524,7 +526,7 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
       }
 
       // ---- Short-range pair action, OLD and NEW (ShortRangeClass.cc:145-171) ----
-      Kahan acc_old, acc_new;
+      StageSums<MIXED> acc;
       for (int s2 = 0; s2 < R.ns; s2++) {
         const FitDev F = load_uniform(&a.fits[S.fit[sp * R.ns + s2]]);
         const UlongView ul = ulong_for(R, F, sp * R.ns + s2, level);
@@ -534,7 +536,7 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
           if (F.kind == 1) {
             const BcView Q{qgrid_of(F), level, MIXED ? F.i32_lev[level] : 0};
             sr_partners<NT, MIXED, true, false>(R, F, LevelTable64<true>{}, Q, ul, j0, j1, SkipOne{ptcl}, slice1, K, step, a.link_split, mold,
-                                         mnew, acc_old, acc_new);
+                                         mnew, acc);
             continue;
           }
         }
@@ -547,10 +549,10 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
         }();
         __syncthreads();
         sr_partners<NT, MIXED, false, LIN>(R, F, T, BcView{}, UlongView{nullptr, nullptr}, j0, j1, SkipOne{ptcl}, slice1, K, step, a.link_split, mold, mnew,
-                                      acc_old, acc_new);
+                                      acc);
       }
-      double sr_old, sr_new;
-      block_sum2(acc_old.value(), acc_new.value(), scratch, sr_old, sr_new);
+      double sr_old, sr_new;   // sr_new: the NEW total (FP64) or the change (mixed) until the test
+      block_sum2(acc.a, acc.b, scratch, sr_old, sr_new);
 
       // ---- Long-range action at the lowest level (LongRangeClass.cc:273-371) ----
       // Only the moved species' rho_k changes: NEW = OLD + delta over the window, with
@@ -652,7 +654,7 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
         oldAction += kin_o;
         oldAction += sr_old;
         newAction += kin_n;
-        newAction += sr_new;
+        newAction += StageSums<MIXED>::new_total(sr_old, sr_new);
         double currActionChange = newAction - oldAction;
         if (with_lr) {
           // the k-independent terms are the same on both copies and cancel in the change;

@@ -624,7 +624,7 @@ __device__ void bisection_replica_table(const LaunchBisection& a, int item, PIMC
       }
       // ---- Short range, OLD and NEW: every moved particle against the unmoved ones, then the
       // pairs among the moved ones ----
-      Kahan acc_old, acc_new;
+      StageSums<MIXED> acc;
       double mp_old = 0.0, mp_new = 0.0;
       for (int s2 = 0; s2 < R.ns; s2++) {
         const FitDev F = load_uniform(&a.fits[S.fit[sp * R.ns + s2]]);
@@ -637,8 +637,7 @@ __device__ void bisection_replica_table(const LaunchBisection& a, int item, PIMC
             const BcView Q{qgrid_of(F), level, MIXED ? F.i32_lev[level] : 0};
             for (int pi = 0; pi < np; pi++)
               sr_partners<NT, MIXED, true, false>(R, F, LevelTable64<true>{}, Q, ul, j0, j1, cyc, slice1, K, step,
-                                                  a.link_split, mold + pi * rowlen, mnew + pi * rowlen, acc_old,
-                                                  acc_new);
+                                                  a.link_split, mold + pi * rowlen, mnew + pi * rowlen, acc);
             continue;
           }
         }
@@ -652,11 +651,10 @@ __device__ void bisection_replica_table(const LaunchBisection& a, int item, PIMC
         __syncthreads();
         for (int pi = 0; pi < np; pi++)
           sr_partners<NT, MIXED, false, LIN>(R, F, T, BcView{}, UlongView{nullptr, nullptr}, j0, j1, cyc, slice1, K,
-                                             step, a.link_split, mold + pi * rowlen, mnew + pi * rowlen, acc_old,
-                                             acc_new);
+                                             step, a.link_split, mold + pi * rowlen, mnew + pi * rowlen, acc);
       }
-      double sr_old, sr_new;
-      block_sum2(acc_old.value() + mp_old, acc_new.value() + mp_new, scratch, sr_old, sr_new);
+      double sr_old, sr_new;   // sr_new: the NEW total (FP64) or the change (mixed)
+      block_sum2(acc.a + mp_old, acc.b + StageSums<MIXED>::b_of(mp_old, mp_new), scratch, sr_old, sr_new);
 
       // ---- Metropolis test (StageClass.cc:69-72) ----
       if (tid == 0) {
@@ -675,7 +673,7 @@ __device__ void bisection_replica_table(const LaunchBisection& a, int item, PIMC
         oldAction += kin_o;
         oldAction += sr_old;
         newAction += kin_n;
-        newAction += sr_new;
+        newAction += StageSums<MIXED>::new_total(sr_old, sr_new);
         const double currActionChange = newAction - oldAction;
         const double logAcceptProb = logSampleRatio - currActionChange + prev;
         const double logu = logu_tab[level];

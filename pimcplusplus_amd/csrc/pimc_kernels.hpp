@@ -83,6 +83,30 @@ struct Kahan {
   __device__ inline double value() const { return s; }
 };
 
+// One thread's short-range sums of a stage.  The FP64 path sums the OLD and NEW link actions
+// apart, as the reference does (a = OLD, b = NEW).  The mixed path sums OLD plainly and the
+// change NEW - OLD of every link pair with Kahan compensation (a = OLD, b = the change): the
+// change is what the Metropolis test uses, and one compensated sum per pair costs half of two.
+template <bool MIXED>
+struct StageSums {
+  double a = 0.0, b = 0.0, c = 0.0;
+  __device__ inline void add(double uo, double un) {
+    a += uo;
+    if constexpr (MIXED) {
+      const double y = (un - uo) - c;
+      const double t = b + y;
+      c = (t - b) - y;
+      b = t;
+    } else {
+      b += un;
+    }
+  }
+  // NEW total from the block totals of a and b
+  __device__ static inline double new_total(double ta, double tb) { return MIXED ? ta + tb : tb; }
+  // b's contribution of a pair of terms added outside add()
+  __device__ static inline double b_of(double uo, double un) { return MIXED ? un - uo : un; }
+};
+
 // --------------------------------------------------------------------------
 // Table views: one tau level of a fit, in global memory or staged in LDS
 // --------------------------------------------------------------------------
