@@ -6,6 +6,10 @@ Two sources:
   does (Common/PairAction/DavidPAClass.cc:639-912): the potential block, then
   the rank-3 ``fit= n`` tables; the table whose fit order NMax is 2 becomes the
   spline data, with column 0 the potential and the last grid row zeroed.
+* `read_squarer_h5` reads the HDF5 form of a squarer file as
+  DavidPAClass::ReadDavidSquarerFileHDF5 does (DavidPAClass.cc:932-1224), and
+  `read_coulomb_bc_h5` a ``coulombBCfit`` file as ReadPAFit and
+  PAcoulombBCFitClass::Read do (PAFit.h:30-69, PAcoulombBCFit.cc:242-311).
 * `lj_david_table` builds the same layout for a Lennard-Jones pair.  PIMC++
   ships no Lennard-Jones pair action, so this is this package's own generator:
   the diagonal is the primitive end-point action u00(r) = tau V(r) and the
@@ -143,6 +147,74 @@ def read_dm(path: str) -> DavidTable:
     # fit order 2 -> NumOffDiagonalTerms 2 uses columns up to 2*3/2+2+1 = 6
     return DavidTable(grid_type=grid_type, grid_start=gstart, grid_end=gend, y=assemble(u),
                       num_terms=2, smallest_tau=tmin, dy=assemble(du))
+
+
+# --------------------------------------------------------------------------
+# HDF5 squarer files (PIMC++'s IO layer; see pimcplusplus_amd.h5)
+# --------------------------------------------------------------------------
+def read_squarer_h5(path: str) -> DavidTable:
+    """Reads the HDF5 form of a squarer file into the DavidPAClass table layout.
+
+    Follows DavidPAClass::ReadDavidSquarerFileHDF5 (DavidPAClass.cc:932-1224):
+    Squarer/NumFits; Potential/Data (column 0 of every level); sections Ukj<c> and
+    dUkjdBeta<c> for c = 0..NumFits, each with Rank (3), NumUkj, NumTau, a Grid section
+    (NumGridPoints, Type LINEAR or LOG, Start, End), Taus, NMax, Derv (1 for U, 2 for
+    dU/dbeta) and Data [grid][ukj][tau].  The NMax == 2 tables become the spline data
+    with the last grid row zeroed (:1045, :1170), as in the .dm reader.
+    """
+    import warnings
+
+    from . import h5
+    with h5.open_file(path) as f:
+        with f.group("Squarer") as sq:
+            num_fits = int(sq.read("NumFits"))
+        with f.group("Potential") as pg:
+            potential = np.atleast_1d(np.asarray(pg.read("Data"), dtype=np.float64))
+
+        def tables(prefix: str, expect_derv: int):
+            found = None
+            for counter in range(num_fits + 1):
+                with f.group(f"{prefix}{counter}") as g:
+                    if int(g.read("Rank")) != 3:
+                        raise ValueError(f"{path}: section {prefix}{counter} is not a rank-3 table")
+                    nukj, ntau = int(g.read("NumUkj")), int(g.read("NumTau"))
+                    with g.group("Grid") as gg:
+                        ngrid = int(gg.read("NumGridPoints"))
+                        gtype, gstart, gend = gg.read("Type"), float(gg.read("Start")), float(gg.read("End"))
+                    taus = np.atleast_1d(np.asarray(g.read("Taus"), dtype=np.float64))
+                    nmax, derv = int(g.read("NMax")), int(g.read("Derv"))
+                    data = np.asarray(g.read("Data"), dtype=np.float64)
+                if data.shape != (ngrid, nukj, ntau):
+                    raise ValueError(f"{path}: {prefix}{counter}/Data has shape {data.shape}, "
+                                     f"expected {(ngrid, nukj, ntau)}")
+                if gtype not in ("LINEAR", "LOG"):
+                    raise ValueError(f"{path}: unrecognized grid type {gtype!r} in {prefix}{counter}")
+                if derv != expect_derv:   # the reference reports this and goes on (:1030-1032, :1127-1129)
+                    warnings.warn(f"{path}: {prefix}{counter} has Derv {derv}, expected {expect_derv}")
+                num_tau_calc = int(np.floor(np.log(taus[-1] / taus[0]) / np.log(2.0) + 0.5 + 1.0))
+                if num_tau_calc != ntau:   # reported by the reference too (:1021-1026)
+                    warnings.warn(f"{path}: {prefix}{counter}: NumTau {ntau} against {num_tau_calc} from Taus")
+                if nmax == 2:
+                    found = (gtype, gstart, gend, ngrid, nukj, ntau, float(taus[0]), data)
+            if found is None:
+                raise ValueError(f"{path}: no NMax == 2 table among the {prefix} sections")
+            return found
+
+        gtype, gstart, gend, ngrid, nukj, ntau, tmin, u = tables("Ukj", 1)
+        _, _, _, ngrid_d, nukj_d, ntau_d, _, du = tables("dUkjdBeta", 2)
+    if potential.shape != (ngrid,) or (ngrid_d, nukj_d, ntau_d) != (ngrid, nukj, ntau):
+        raise ValueError(f"{path}: the potential, U and dU/dbeta tables disagree in size")
+
+    def assemble(tab):   # [grid][ukj][tau] -> [tau][grid][col], column 0 the potential
+        y = np.zeros((ntau, ngrid, nukj + 1))
+        y[:, :, 0] = potential[None, :]
+        y[:, :, 1:] = np.transpose(tab, (2, 0, 1))
+        y[:, ngrid - 1, :] = 0.0
+        return y
+
+    grid_type = GRID_LINEAR if gtype == "LINEAR" else GRID_LOG
+    return DavidTable(grid_type=grid_type, grid_start=gstart, grid_end=gend, y=assemble(u), num_terms=2,
+                      smallest_tau=tmin, dy=assemble(du))
 
 
 # --------------------------------------------------------------------------
@@ -368,3 +440,75 @@ def coulomb_chord_table(z1z2: float, tau: float, num_betas: int, q_grid=(GRID_LO
         umat[b] = tau * 2 ** b * z1z2 * g[None, :] / q[:, None]
     dumat = np.repeat((z1z2 * g[None, :] / q[:, None])[None], num_betas, axis=0)   # linear in tau
     return CoulombBCTable(q_grid, t_grid, umat, tau, z1z2, dumat)
+
+
+# --------------------------------------------------------------------------
+# HDF5 coulombBCfit files (PIMC++'s IO layer; see pimcplusplus_amd.h5)
+# --------------------------------------------------------------------------
+def _read_grid_h5(g, path: str) -> tuple[tuple, int]:
+    """((grid_type, start, end), num_points) of a grid section (ReadGrid, Common/Splines/Grid.h:654-680):
+    Linear reads Start, End, NumPoints (:107-112); Log reads r0, Spacing, NumPoints and puts its
+    points at r0 Spacing^i (:553-562), so its end is r0 Spacing^(n-1)."""
+    gtype = g.read("Type")
+    if gtype == "Linear":
+        n = int(g.read("NumPoints"))
+        return (GRID_LINEAR, float(g.read("Start")), float(g.read("End"))), n
+    if gtype == "Log":
+        n = int(g.read("NumPoints"))
+        r0, spacing = float(g.read("r0")), float(g.read("Spacing"))
+        return (GRID_LOG, r0, r0 * spacing ** (n - 1.0)), n
+    raise ValueError(f"{path}: grid type {gtype!r} is not supported (the tables take Linear and Log grids)")
+
+
+def read_coulomb_bc_h5(path: str, smallest_beta: float, num_betas: int) -> CoulombBCTable:
+    """Reads a ``coulombBCfit`` pair-action file as ReadPAFit and PAcoulombBCFitClass::Read do
+    (PAFit.h:30-69, PAcoulombBCFit.cc:242-311).
+
+    Fits/Type must be coulombBCfit; Particle1 and Particle2 carry Name, lambda, Charge and Ndim;
+    Potential is a Coulomb potential (Type, Z1Z2); Fits holds the qGrid and tGrid sections
+    and one Fit section per beta (beta, Umat [q][t], dUmat).  Level l takes the first Fit whose
+    beta is within 1e-12 (relative) of smallest_beta 2^l; a missing beta is an error, as in
+    the reference (:280-285).  A Log grid keeps its points to rounding: the tables recompute
+    the spacing from the end points.
+    """
+    from . import h5
+    with h5.open_file(path) as f:
+        with f.group("Fits") as fits:
+            ftype = fits.read("Type")
+        if ftype != "coulombBCfit":
+            raise ValueError(f"{path}: pair action fit type {ftype!r}, expected 'coulombBCfit'")
+        for name in ("Particle1", "Particle2"):
+            with f.group(name) as pg:
+                for var in ("Name", "lambda", "Charge", "Ndim"):
+                    pg.read(var)
+        with f.group("Potential") as pot:
+            ptype = pot.read("Type")
+            if ptype != "Coulomb":
+                raise ValueError(f"{path}: potential type {ptype!r}, expected 'Coulomb'")
+            z1z2 = float(pot.read("Z1Z2"))
+        with f.group("Fits") as fits:
+            with fits.group("qGrid") as g:
+                q_grid, nq = _read_grid_h5(g, path)
+            with fits.group("tGrid") as g:
+                t_grid, nt = _read_grid_h5(g, path)
+            sections = fits.sections("Fit")
+            try:
+                betas = [float(g.read("beta")) for _, g in sections]
+                umat = np.zeros((num_betas, nq, nt))
+                dumat = np.zeros((num_betas, nq, nt))
+                desired = smallest_beta
+                for b in range(num_betas):
+                    pick = next((i for i, beta in enumerate(betas) if abs(beta - desired) / desired < 1.0e-12), None)
+                    if pick is None:
+                        raise ValueError(f"{path}: no Fit with beta = {desired!r} (have {betas})")
+                    g = sections[pick][1]
+                    for out, var in ((umat, "Umat"), (dumat, "dUmat")):
+                        m = np.asarray(g.read(var), dtype=np.float64)
+                        if m.shape != (nq, nt):
+                            raise ValueError(f"{path}: Fit {var} has shape {m.shape}, expected {(nq, nt)}")
+                        out[b] = m
+                    desired *= 2.0
+            finally:
+                for _, g in sections:
+                    g.close()
+    return CoulombBCTable(q_grid, t_grid, umat, smallest_beta, z1z2, dumat)
