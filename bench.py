@@ -205,9 +205,8 @@ def report_other(args):
     nmean = float(np.mean([r.num_particles for r in system.replicas]))
     fixed = None
     setup = permute_setup(args.workload)
-    if setup:   # TABLE moves run with one block per replica, eager launches
-        fixed = (256, 1, False, 1)
-        out["config"]["driver"] = "PermuteType TABLE, one block per replica, eager launches"
+    if setup:   # TABLE moves: the tuner picks the link split; no sweep graphs
+        out["config"]["driver"] = "PermuteType TABLE, tuned link split, eager launches"
     if args.plan:
         b, ls, pers, cl = (int(x) for x in args.plan.split(","))
         fixed = (b, ls, bool(pers), cl)
@@ -217,7 +216,8 @@ def report_other(args):
         runs = tuple(r for r in runs if r[0] == args.only)
     for name, prec in runs:
         tot, wall, dev_ms, plan = run_gpu(system, prec, args.steps, args.warmup, args.moves_per_step,
-                                          tune=fixed is None, graph=fixed is None, plan=fixed, setup=setup)
+                                          tune=fixed is None, graph=fixed is None and setup is None, plan=fixed,
+                                          setup=setup)
         sr_bytes, lr_bytes = workload_bytes(tot, nmean)
         kms = dev_ms / args.steps
         out[name] = {"evals_per_s": tot.pair_evals / wall, "ms_per_step": 1e3 * wall / args.steps,

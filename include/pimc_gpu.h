@@ -260,12 +260,15 @@ int pimc_cycle_apply(pimc_ctx* ctx, int32_t replica, int32_t first_ptcl, int32_t
  * stages moving every particle of the cycle, and the final permutation test with the
  * reverse table (TablePermuteStageClass::Attempt, TablePermuteStage.cc:200-260,
  * CalcReverseProb, UpdateHTable).  gamma == NULL switches the replica back to no
- * permutations.  Replicas with the long-range action, cluster plans, num_levels above 5, a
- * moved species whose n x n float HTable and the staged spline level exceed a block's 64 KB
- * of LDS (about 100 particles), the autotuner and sweep graphs refuse TABLE moves
- * (PIMC_ERR_UNSUPPORTED); every replica of one call moves with TABLE or none does.  The GPU
- * walks the cycle table in the reference's order without storing it, so its size is not
- * bounded. */
+ * permutations.  TABLE moves run 256-thread blocks, each taking replicas in turn, whatever
+ * the plan's block size and persistence; of the plan they use the link split.  Replicas with
+ * the long-range action, cluster plans, num_levels above 5, a moved species whose n x n float
+ * HTable and the staged spline level exceed a block's 64 KB of LDS (about 100 particles), and
+ * sweep graphs refuse TABLE moves (PIMC_ERR_UNSUPPORTED); the autotuner only tries their
+ * link split.  Every replica of one
+ * call moves with TABLE or none does.  The GPU builds the cycle table in the reference's
+ * order in LDS or, when larger, in device scratch of 65,536 entries per block; a larger table
+ * is walked in that order without being stored, so its size is not bounded. */
 int pimc_replica_set_permute(pimc_ctx* ctx, int32_t replica, int32_t species, const double* gamma,
                              double epsilon);
 
@@ -487,8 +490,9 @@ int pimc_bisection_get_plan(pimc_ctx* ctx, pimc_bisection_plan* plan);
  * trial_moves moves [first_move, first_move + trial_moves) of the listed
  * replicas, restores their paths, permutations, rho_k and counters after each
  * trial, and makes the fastest plan the context's plan (also returned in *out,
- * which may be NULL).  Has no counterpart in the reference, whose CPU loops have
- * no launch shape. */
+ * which may be NULL).  TABLE permutation moves run one block of 256 threads per
+ * replica, so for them only the link split (1, 2, 4) is tried.  Has no
+ * counterpart in the reference, whose CPU loops have no launch shape. */
 int pimc_bisection_tune(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep,
                         const pimc_bisection_desc* desc, int64_t first_move, int32_t trial_moves,
                         int32_t precision, pimc_bisection_plan* out);

@@ -2222,10 +2222,15 @@ int pimc_bisection_tune(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, co
   for (int r = 0; r < nrep; r++) {
     Replica* R = get_rep(&ctx->c, replicas[r]);
     if (!R) return fail(PIMC_ERR_ARG, "bisection_tune: bad replica");
-    if (R->permute_on && R->permute_species == desc->species)
-      return fail(PIMC_ERR_UNSUPPORTED, "bisection_tune: TABLE permutation moves run with the one-block plan only");
   }
+  // the checks of a run (a TABLE list must be all TABLE, and so on), without moving anything
+  const pimc_bisection_plan caller = ctx->plan;
   ctx->plan = pimc_bisection_plan{256, 1, 0, 1, 0.0};
+  const int rc = pimc_bisection_run(ctx, replicas, nrep, desc, first_move, 0, precision, nullptr, nullptr);
+  if (rc) {
+    ctx->plan = caller;
+    return rc;
+  }
   if (out) *out = ctx->plan;
   return PIMC_OK;
 }

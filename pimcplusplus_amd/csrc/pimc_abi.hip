@@ -1688,6 +1688,35 @@ int persistent_grid(pimc_ctx* ctx, const BisectionPrep& P, const LaunchBisection
   return std::max(1, std::min(nrep, per_cu * prop.multiProcessorCount));
 }
 
+// The TABLE part of a launch: each list entry's cycle weights and cut-off; one block per
+// replica at a time, no work queue; LDS per block: the CU's LDS shared by the blocks the
+// kernel's registers allow; grid: the blocks resident at once; cycle-table scratch: one region
+// per block.
+int table_launch_setup(pimc_ctx* ctx, const BisectionPrep& P, int nrep, int precision, LaunchBisection& a) {
+  int rc;
+  std::vector<PermuteDev> pd(nrep);
+  for (int i = 0; i < nrep; i++) pd[i] = ctx->rep(P.ids[i])->permute;
+  if ((rc = ctx->d_permute.ensure((size_t)nrep * sizeof(PermuteDev)))) return rc;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));   // earlier launches may still read the old entries
+  HIP_TRY(hipMemcpy(ctx->d_permute.p, pd.data(), (size_t)nrep * sizeof(PermuteDev), hipMemcpyHostToDevice));
+  a.permute = ctx->d_permute.as<PermuteDev>();
+  a.lds_htable_bytes = P.htable_bytes;
+  a.queue = nullptr;
+  int cus = 0, lds_cu = 0;
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  HIP_TRY(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, ctx->device));
+  a.permute_lds_limit = bisection_table_lds_limit((size_t)lds_cu);
+  const int per_cu = bisection_table_blocks_per_cu(P.nt, precision == PIMC_MIXED, P.lean, bisection_table_lds_bytes(a));
+  a.permute_grid = std::max(1, std::min(nrep, std::max(1, per_cu) * std::max(1, cus)));
+  a.permute_cap = kCycleScratchEntries;
+  if ((rc = ctx->d_cycles.ensure(bisection_table_scratch_bytes(a.permute_cap, a.permute_grid)))) return rc;
+  a.permute_scratch = ctx->d_cycles.p;
+  if (bisection_table_lds_bytes(a) > 64 * 1024)
+    return fail(PIMC_ERR_UNSUPPORTED, "bisection: the moved species' HTable and the staged spline level exceed "
+                                      "the 64 KB of LDS of a TABLE permutation block (too many particles)");
+  return PIMC_OK;
+}
+
 int launch_timed(pimc_ctx* ctx, const BisectionPrep& P, const LaunchBisection& a, int precision, int nrep) {
   std::pair<hipEvent_t, hipEvent_t> ev;
   if (int rc_ev = timed_pair(ctx, ev)) return rc_ev;
@@ -1744,32 +1773,7 @@ int pimc_bisection_run(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, con
       make_launch(ctx, P, desc, nrep, (uint32_t)first_move, num_moves, ctx->d_repids.as<int>(), ctx->d_reppos.as<int>(),
                   ctx->d_stats.as<int64_t>(), records ? ctx->d_records.as<pimc_move_record>() : nullptr,
                   ctx->d_queue.as<int>(), nullptr, ctx->d_bar.as<int>(), ctx->d_part.as<double>());
-  if (P.table) {
-    // each list entry's cycle weights and cut-off; one block per replica, no work queue
-    std::vector<PermuteDev> pd(nrep);
-    for (int i = 0; i < nrep; i++) pd[i] = ctx->rep(P.ids[i])->permute;
-    if ((rc = ctx->d_permute.ensure((size_t)nrep * sizeof(PermuteDev)))) return rc;
-    HIP_TRY(hipStreamSynchronize(ctx->stream));   // earlier launches may still read the old entries
-    HIP_TRY(hipMemcpy(ctx->d_permute.p, pd.data(), (size_t)nrep * sizeof(PermuteDev), hipMemcpyHostToDevice));
-    a.permute = ctx->d_permute.as<PermuteDev>();
-    a.lds_htable_bytes = P.htable_bytes;
-    a.queue = nullptr;
-    // LDS per block: the CU's LDS shared by the blocks the kernel's registers allow; grid: the
-    // blocks resident at once; cycle-table scratch: one region per block
-    int cus = 0, lds_cu = 0;
-    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-    HIP_TRY(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, ctx->device));
-    a.permute_lds_limit = bisection_table_lds_limit((size_t)lds_cu);
-    const int per_cu = bisection_table_blocks_per_cu(P.nt, precision == PIMC_MIXED, P.lean, bisection_table_lds_bytes(a));
-    a.permute_grid = std::max(1, std::min(nrep, std::max(1, per_cu) * std::max(1, cus)));
-    a.permute_cap = kCycleScratchEntries;
-    if ((rc = ctx->d_cycles.ensure(bisection_table_scratch_bytes(a.permute_cap, a.permute_grid))))
-      return rc;
-    a.permute_scratch = ctx->d_cycles.p;
-    if (bisection_table_lds_bytes(a) > 64 * 1024)
-      return fail(PIMC_ERR_UNSUPPORTED, "bisection: the moved species' HTable and the staged spline level exceed "
-                                        "the 64 KB of LDS of a TABLE permutation block (too many particles)");
-  }
+  if (P.table && (rc = table_launch_setup(ctx, P, nrep, precision, a))) return rc;
   if ((rc = launch_timed(ctx, P, a, precision, nrep))) return rc;
   if (!records && !stats) return PIMC_OK;   // asynchronous: counters via pimc_bisection_stats_take
   if (records)
@@ -1829,7 +1833,6 @@ int pimc_bisection_tune(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, co
   int rc = prepare_bisection(ctx, replicas, nrep, desc, precision, P0);
   ctx->plan = caller_plan;
   if (rc) return rc;
-  if (P0.table) return fail(PIMC_ERR_UNSUPPORTED, "bisection_tune: TABLE permutation moves run with one block per replica");
   HIP_TRY(hipSetDevice(ctx->device));
   if ((rc = ctx->sync_descriptors())) return rc;
   if ((rc = ensure_acc(ctx))) return rc;
@@ -1869,13 +1872,18 @@ int pimc_bisection_tune(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, co
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
   // every (block, link split) with one block per replica, a persistent grid, and
-  // clusters of 2 .. kMaxCluster blocks per replica where all blocks fit at once
+  // clusters of 2 .. kMaxCluster blocks per replica where all blocks fit at once; TABLE moves
+  // run one 256-thread block per replica, so only their link split is tried
   std::vector<pimc_bisection_plan> plans;
-  for (int block : {256, 512})
-    for (int ls : {1, 2, 4}) {
-      for (int pers : {0, 1}) plans.push_back(pimc_bisection_plan{block, ls, pers, 1, 0.0});
-      for (int c = 2; c <= kMaxCluster; c *= 2) plans.push_back(pimc_bisection_plan{block, ls, 0, c, 0.0});
-    }
+  if (P0.table) {
+    for (int ls : {1, 2, 4}) plans.push_back(pimc_bisection_plan{256, ls, 0, 1, 0.0});
+  } else {
+    for (int block : {256, 512})
+      for (int ls : {1, 2, 4}) {
+        for (int pers : {0, 1}) plans.push_back(pimc_bisection_plan{block, ls, pers, 1, 0.0});
+        for (int c = 2; c <= kMaxCluster; c *= 2) plans.push_back(pimc_bisection_plan{block, ls, 0, c, 0.0});
+      }
+  }
   for (const pimc_bisection_plan& plan : plans) {
     ctx->plan = plan;
     BisectionPrep P;
@@ -1892,13 +1900,16 @@ int pimc_bisection_tune(pimc_ctx* ctx, const int32_t* replicas, int32_t nrep, co
       break;
     HIP_TRY(hipMemcpy(ids.p, P.ids.data(), (size_t)nrep * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(pos.p, P.order.data(), (size_t)nrep * 4, hipMemcpyHostToDevice));
-    const LaunchBisection a =
+    LaunchBisection a =
         make_launch(ctx, P, desc, nrep, (uint32_t)first_move, trial_moves, ids.as<int>(), pos.as<int>(),
                     st.as<int64_t>(), nullptr, q.as<int>(), nullptr, bar.as<int>(), part.as<double>());
+    if (P.table && (rc = table_launch_setup(ctx, P, nrep, precision, a))) break;
     if (plan.cluster > 1 && check_cluster_fits(ctx, P, a, precision, nrep) != PIMC_OK) continue;   // no trial
     const int grid = plan.persistent ? persistent_grid(ctx, P, a, precision, nrep) : nrep;
     HIP_TRY(hipEventRecord(e0, ctx->stream));
-    hipError_t e = launch_bisection(P.nt, precision == PIMC_MIXED, P.lean, a, plan.block_threads, grid, ctx->stream);
+    hipError_t e = P.table ? launch_bisection_table(P.nt, precision == PIMC_MIXED, P.lean, a, ctx->stream)
+                           : launch_bisection(P.nt, precision == PIMC_MIXED, P.lean, a, plan.block_threads, grid,
+                                              ctx->stream);
     if (e != hipSuccess) {
       rc = fail(PIMC_ERR_DEVICE, std::string("bisection_tune launch: ") + hipGetErrorString(e));
       break;

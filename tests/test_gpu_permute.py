@@ -262,11 +262,9 @@ def test_table_permutation_moves_refuse_unsupported_launches(gpu):
         gpu.bisection_run(ids, sysm.num_levels, sysm.seed, 0, 1, abi.FP64)
     assert e.value.code == abi.ERR_UNSUPPORTED
     gpu.set_permute(ids[1], 0, (1.0, 1.0, 1.0, 1.0), 1e-4)
-    for call in (lambda: gpu.bisection_tune(ids, sysm.num_levels, sysm.seed, 0, 2, abi.FP64),
-                 lambda: gpu.sweep_graph_create(ids, sysm.num_levels, sysm.seed, 0, 2, abi.FP64)):
-        with pytest.raises(abi.PimcError) as e:
-            call()
-        assert e.value.code == abi.ERR_UNSUPPORTED
+    with pytest.raises(abi.PimcError) as e:
+        gpu.sweep_graph_create(ids, sysm.num_levels, sysm.seed, 0, 2, abi.FP64)
+    assert e.value.code == abi.ERR_UNSUPPORTED
     gpu.set_plan(256, 1, False, 2)
     with pytest.raises(abi.PimcError) as e:
         gpu.bisection_run(ids, sysm.num_levels, sysm.seed, 0, 1, abi.FP64)
@@ -275,6 +273,34 @@ def test_table_permutation_moves_refuse_unsupported_launches(gpu):
     gpu.set_permute(ids[0], 0)                    # back to no permutations
     gpu.set_permute(ids[1], 0)
     gpu.bisection_run(ids, sysm.num_levels, sysm.seed, 0, 2, abi.FP64)
+
+
+def test_table_permutation_tuner_tries_link_splits_and_restores_the_chain(oracle, gpu):
+    """The tuner times TABLE moves with link splits 1, 2 and 4 (one 256-thread block per
+    replica), restores the replicas after every trial, and the chain that follows with the
+    chosen plan equals the oracle's (FP64: only the order of the partner sums changes)."""
+    sysm = workloads.dilute_boson_system(num_replicas=3)
+    print("dilute bosons, TABLE, tuned plan, 3 x 200 moves", flush=True)
+    ig = workloads.load(gpu, sysm)
+    for r in ig:
+        gpu.set_permute(r, 0, (1.0, 1.0, 1.0, 1.0), 1e-4)
+    before = [gpu.path_get(r, 0) for r in ig]
+    plan = gpu.bisection_tune(ig, sysm.num_levels, sysm.seed, 0, 8, abi.FP64)
+    assert (plan.block_threads, plan.persistent, plan.cluster) == (256, 0, 1) and plan.link_split in (1, 2, 4)
+    assert all(np.array_equal(b, gpu.path_get(r, 0)) for b, r in zip(before, ig))
+    rg, sg = gpu.bisection_run(ig, sysm.num_levels, sysm.seed, 0, 200, abi.FP64)
+    io, ro, so = table_chains(oracle, sysm, 200)
+    assert np.array_equal(ro["cycle"], rg["cycle"])
+    compare_chains(ro, rg)
+    for link_split in (2, 4):   # every split the tuner may pick gives the oracle's chain
+        gs = workloads.load(gpu, sysm)
+        for r in gs:
+            gpu.set_permute(r, 0, (1.0, 1.0, 1.0, 1.0), 1e-4)
+        gpu.set_plan(256, link_split, False, 1)
+        rs, _ = gpu.bisection_run(gs, sysm.num_levels, sysm.seed, 0, 200, abi.FP64)
+        assert np.array_equal(ro["cycle"], rs["cycle"])
+        compare_chains(ro, rs)
+    gpu.set_plan(256, 1, False, 1)
 
 
 @pytest.mark.parametrize("n,levels,moves,lo,hi", [(20, 3, 40, 5461, 65536), (32, 4, 12, 65536, 1 << 30)])

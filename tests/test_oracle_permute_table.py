@@ -196,7 +196,11 @@ def test_table_moves_refuse_what_the_gpu_refuses(oracle):
         oracle.bisection_run(ids, sysm.num_levels, sysm.seed, 0, 1)
     assert e.value.code == abi.ERR_UNSUPPORTED
     oracle.set_permute(ids[1], 0, (1.0, 1.0, 1.0, 1.0), 1e-4)
-    with pytest.raises(abi.PimcError):
+    plan = oracle.bisection_tune(ids, sysm.num_levels, sysm.seed, 0, 2)   # one block per replica
+    assert (plan.block_threads, plan.persistent, plan.cluster) == (256, 0, 1)
+    oracle.set_permute(ids[1], 0)
+    with pytest.raises(abi.PimcError):                  # the checks of a run
         oracle.bisection_tune(ids, sysm.num_levels, sysm.seed, 0, 2)
+    oracle.set_permute(ids[1], 0, (1.0, 1.0, 1.0, 1.0), 1e-4)
     with pytest.raises(abi.PimcError):
         oracle.set_permute(ids[1], 0, (1.0, 0.0, 1.0, 1.0), 1e-4)
