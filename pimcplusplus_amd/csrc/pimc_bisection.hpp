@@ -658,14 +658,17 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
         double currActionChange = newAction - oldAction;
         if (with_lr) {
           // the k-independent terms are the same on both copies and cancel in the change;
-          // the records carry the totals
+          // only the records carry the totals, so the per-slice constants (dependent
+          // loads on thread 0's serial path) are summed only when records are kept
           currActionChange += lr_delta;
-          double homo = 0.0, background = 0.0, k0 = 0.0;
-          for (int t = 0; t <= W; t += step)
-            lr_const_terms(R, S, level, (t == 0 || t == W) ? 0.5 : 1.0, homo, background, k0);
-          const double cst = R.use_background ? background : k0;
-          oldAction += (lr_kold + homo) + cst;
-          newAction += ((lr_kold + lr_delta) + homo) + cst;
+          if (a.records) {
+            double homo = 0.0, background = 0.0, k0 = 0.0;
+            for (int t = 0; t <= W; t += step)
+              lr_const_terms(R, S, level, (t == 0 || t == W) ? 0.5 : 1.0, homo, background, k0);
+            const double cst = R.use_background ? background : k0;
+            oldAction += (lr_kold + homo) + cst;
+            newAction += ((lr_kold + lr_delta) + homo) + cst;
+          }
           st_kevals += 2 * (int64_t)(W / step + 1) * R.nk * R.lr_terms;
         }
         const double logAcceptProb = logSampleRatio - currActionChange + prev;
