@@ -382,19 +382,35 @@ __device__ inline bool cluster_barrier(const LaunchBisection& a, int item, int n
   return true;
 }
 
-// Replaces v[0..3] (this member's {SR old, SR new, LR old, LR new}) by the sums over
-// the members, in member order.  Slots alternate between two buffers: a member
-// can only overwrite a buffer after every member has passed the next barrier,
-// that is, after every member has read it.  Thread 0 only.
+// Replaces v[0..3] (this member's {SR old, SR new, LR old, LR new}, valid in lane 0) by
+// the sums over the members, in member order.  Slots alternate between two buffers: a
+// member can only overwrite a buffer after every member has passed the next barrier, that
+// is, after every member has read it.  Called by the 64 lanes of wave 0: lane 0 stores and
+// waits at the barrier; then lane j loads value j of the 4 x cluster partial sums, so the
+// device-coherent loads are in flight together (one round trip instead of one per value),
+// and every lane adds them in member order.
 __device__ inline bool cluster_sum(const LaunchBisection& a, int item, int member, int nbar, double v[4]) {
+  const int lane = (int)threadIdx.x & 63;
   double* slot = a.part + ((size_t)item * 2 + (nbar & 1)) * a.cluster * 4;
+  int ok = 1;
+  if (lane == 0) {
 #pragma unroll
-  for (int i = 0; i < 4; i++) __hip_atomic_store(slot + member * 4 + i, v[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (!cluster_barrier(a, item, nbar + 1)) return false;
+    for (int i = 0; i < 4; i++)
+      __hip_atomic_store(slot + member * 4 + i, v[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ok = cluster_barrier(a, item, nbar + 1) ? 1 : 0;
+  }
+  if (__shfl(ok, 0) == 0) return false;
+  const int nv = 4 * a.cluster;   // at most 4 * kMaxCluster = 128 values: two per lane
+  const double x0 = lane < nv ? __hip_atomic_load(slot + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+  const double x1 =
+      lane + 64 < nv ? __hip_atomic_load(slot + 64 + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
   double t[4] = {0.0, 0.0, 0.0, 0.0};
   for (int m = 0; m < a.cluster; m++)
 #pragma unroll
-    for (int i = 0; i < 4; i++) t[i] += __hip_atomic_load(slot + m * 4 + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int i = 0; i < 4; i++) {
+      const int j = 4 * m + i;
+      t[i] += j < 64 ? __shfl(x0, j) : __shfl(x1, j - 64);
+    }
 #pragma unroll
   for (int i = 0; i < 4; i++) v[i] = t[i];
   return true;
@@ -627,7 +643,7 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
       // ---- Metropolis test (StageClass.cc:69-72) ----
       bool cl_ok = true;
       if constexpr (CL) {
-        if (tid == 0) {
+        if (tid < 64) {   // wave 0; the sums are valid in thread 0
           double v[4] = {sr_old, sr_new, lr_kold, lr_delta};
           cl_ok = cluster_sum(a, item, member, nbar, v);
           sr_old = v[0];
