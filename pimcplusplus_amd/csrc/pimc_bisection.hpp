@@ -568,7 +568,9 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
                                       acc);
       }
       double sr_old, sr_new;   // sr_new: the NEW total (FP64) or the change (mixed) until the test
-      block_sum2(acc.a, acc.b, scratch, sr_old, sr_new);
+      // scratch is written next by the long-range sum (after the phase-row barriers) or by the
+      // next stage (after the end-of-stage barrier): no trailing barrier
+      block_sum2(acc.a, acc.b, scratch, sr_old, sr_new, false);
 
       // ---- Long-range action at the lowest level (LongRangeClass.cc:273-371) ----
       // Only the moved species' rho_k changes: NEW = OLD + delta over the window, with
@@ -637,7 +639,7 @@ __device__ inline void bisection_replica(const LaunchBisection& a, int item, int
            }
           }
         }
-        block_sum2(part_o, part_d, scratch, lr_kold, lr_delta);
+        block_sum2(part_o, part_d, scratch, lr_kold, lr_delta, false);
       }
 
       // ---- Metropolis test (StageClass.cc:69-72) ----

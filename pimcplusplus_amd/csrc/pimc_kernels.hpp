@@ -51,8 +51,10 @@ __device__ inline double block_sum(double v, PIMC_LDS double* scratch) {
   return t;
 }
 
-// Block sums of two values in one pass (valid in thread 0), in a fixed order.
-__device__ inline void block_sum2(double a, double b, PIMC_LDS double* scratch, double& ta, double& tb) {
+// Block sums of two values in one pass (valid in thread 0), in a fixed order.  Without the
+// trailing barrier the caller must order the next write of scratch after thread 0's reads.
+__device__ inline void block_sum2(double a, double b, PIMC_LDS double* scratch, double& ta, double& tb,
+                                  bool trailing_barrier = true) {
   a = wave_sum(a);
   b = wave_sum(b);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
@@ -68,7 +70,7 @@ __device__ inline void block_sum2(double a, double b, PIMC_LDS double* scratch, 
       ta += scratch[w];
       tb += scratch[nw + w];
     }
-  __syncthreads();
+  if (trailing_barrier) __syncthreads();
 }
 
 // Kahan-compensated FP64 accumulation of the mixed path's terms.
