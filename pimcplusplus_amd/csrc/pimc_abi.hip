@@ -2195,7 +2195,7 @@ int pimc_estimator_pair_correlation(pimc_ctx* ctx, int32_t replica, int32_t mode
   const double delta = (end - start) / (double)(num_points - 1);
   hipError_t e = launch_pair_correlation(ctx->d_reps.as<ReplicaDev>(), replica, mode, S.first[s1], S.count[s1],
                                          S.first[s2], S.count[s2], same, start, 1.0 / delta, end, nbins,
-                                         ctx->d_img.as<unsigned long long>(), blocks, ctx->stream);
+                                         ctx->d_img.as<unsigned long long>(), nitems, blocks, ctx->stream);
   if (e != hipSuccess) return fail(PIMC_ERR_DEVICE, std::string("estimator_pair_correlation launch: ") + hipGetErrorString(e));
   std::vector<unsigned long long> h(nbins);
   HIP_TRY(hipMemcpyAsync(h.data(), ctx->d_img.p, (size_t)nbins * sizeof(unsigned long long), hipMemcpyDeviceToHost,

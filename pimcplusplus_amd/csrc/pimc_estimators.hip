@@ -319,42 +319,69 @@ __global__ void __launch_bounds__(kBlock) k_energy_long_range(const ReplicaDev* 
   }
 }
 
-// PairCorrelationClass::Accumulate into hist (64-bit counters).
+// PairCorrelationClass::Accumulate into hist (64-bit counters).  Each block counts its items
+// in a histogram of 32-bit counters in LDS (lds_hist: nbins <= kLdsBins, and fewer than 2^32
+// items per block) and adds it to hist once, so the global counters see one atomic per bin
+// and block instead of one per pair; otherwise every count goes to hist directly.  The items
+// (slice, pair) are walked with an incremental index, without a 64-bit division per item.
+constexpr int kLdsBins = 8192;
 __global__ void __launch_bounds__(kBlock) k_pair_correlation(const ReplicaDev* reps, int rep, int mode, int f1,
                                                              int n1, int f2, int n2, int same, double start,
-                                                             double deltainv, double end, int nbins,
+                                                             double deltainv, double end, int nbins, int lds_hist,
                                                              unsigned long long* hist) {
+  extern __shared__ unsigned int lh[];   // [nbins] when lds_hist
   const ReplicaDev* RD = reps + rep;
   const RepCore R = load_uniform(&RD->core);
   const int M = R.nslices - 1;
   const long long npairs = same ? (long long)n1 * (n1 - 1) / 2 : (long long)n1 * n2;
-  const long long nitems = npairs * M;
-  for (long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x; it < nitems;
-       it += (long long)gridDim.x * blockDim.x) {
-    const int slice = (int)(it / npairs);
-    const long long p = it % npairs;
-    int p1, p2;
-    if (same) {
-      int hi, lo;
-      pair_of(p, hi, lo);   // hi > lo
-      p1 = f1 + lo;
-      p2 = f1 + hi;
-    } else {
-      p1 = f1 + (int)(p / n2);
-      p2 = f2 + (int)(p % n2);
-    }
-    double a[3], b[3];
-    load_bead(R, mode, slice, p1, a);
-    load_bead(R, mode, slice, p2, b);
-    const double dist = slice_dist(R, a, b);
-    if (dist < end && dist >= start) {
-      int index;
-      {
-#pragma clang fp contract(off)
-        index = clamp_to_int(rint((dist - start) * deltainv - 0.5), nbins + 1);
+  if (lds_hist) {
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) lh[b] = 0u;
+    __syncthreads();
+  }
+  if (npairs > 0) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long sq = stride / npairs, sr = stride % npairs;
+    const long long it0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long slice = it0 / npairs, p = it0 % npairs;
+    for (; slice < M; slice += sq, p += sr) {
+      if (p >= npairs) {
+        p -= npairs;
+        slice++;
+        if (slice >= M) break;
       }
-      if (index >= 0 && index < nbins) atomicAdd(hist + index, 1ull);
+      int p1, p2;
+      if (same) {
+        int hi, lo;
+        pair_of(p, hi, lo);   // hi > lo
+        p1 = f1 + lo;
+        p2 = f1 + hi;
+      } else {
+        p1 = f1 + (int)(p / n2);
+        p2 = f2 + (int)(p % n2);
+      }
+      double a[3], b[3];
+      load_bead(R, mode, (int)slice, p1, a);
+      load_bead(R, mode, (int)slice, p2, b);
+      const double dist = slice_dist(R, a, b);
+      if (dist < end && dist >= start) {
+        int index;
+        {
+#pragma clang fp contract(off)
+          index = clamp_to_int(rint((dist - start) * deltainv - 0.5), nbins + 1);
+        }
+        if (index >= 0 && index < nbins) {
+          if (lds_hist)
+            atomicAdd(lh + index, 1u);
+          else
+            atomicAdd(hist + index, 1ull);
+        }
+      }
     }
+  }
+  if (lds_hist) {
+    __syncthreads();
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x)
+      if (lh[b]) atomicAdd(hist + b, (unsigned long long)lh[b]);
   }
 }
 
@@ -379,9 +406,12 @@ hipError_t launch_energy(int num_terms, const ReplicaDev* reps, const FitDev* fi
 
 hipError_t launch_pair_correlation(const ReplicaDev* reps, int rep, int mode, int f1, int n1, int f2, int n2,
                                    int same, double start, double deltainv, double end, int nbins,
-                                   unsigned long long* hist, int blocks, hipStream_t s) {
-  k_pair_correlation<<<blocks, kBlock, 0, s>>>(reps, rep, mode, f1, n1, f2, n2, same, start, deltainv, end, nbins,
-                                               hist);
+                                   unsigned long long* hist, long long nitems, int blocks, hipStream_t s) {
+  // a block counts at most ceil(nitems / threads) items per thread
+  const long long threads = (long long)blocks * kBlock;
+  const int lds_hist = nbins <= kLdsBins && (nitems + threads - 1) / threads * kBlock < (1LL << 32);
+  k_pair_correlation<<<blocks, kBlock, lds_hist ? (size_t)nbins * sizeof(unsigned int) : 0, s>>>(
+      reps, rep, mode, f1, n1, f2, n2, same, start, deltainv, end, nbins, lds_hist, hist);
   return hipGetLastError();
 }
 

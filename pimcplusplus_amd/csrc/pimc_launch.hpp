@@ -82,7 +82,7 @@ hipError_t launch_energy(int num_terms, const ReplicaDev* reps, const FitDev* fi
                          double* pair_partials, double* kin_partials, double* lr_partials, hipStream_t s);
 hipError_t launch_pair_correlation(const ReplicaDev* reps, int rep, int mode, int f1, int n1, int f2, int n2,
                                    int same, double start, double deltainv, double end, int nbins,
-                                   unsigned long long* hist, int blocks, hipStream_t s);
+                                   unsigned long long* hist, long long nitems, int blocks, hipStream_t s);
 
 // The GPU-resident bisection chain: one block per replica, num_moves moves each.
 struct LaunchBisection {

@@ -200,13 +200,12 @@ def load(lib, system: System):
             kv, _ = lib.kvecs(rid)
             for (sa, sb), key in spec.breakup.items():
                 tab = system.tables[key]
-                u, du, v = _breakup_tables(tab, spec.box, spec.k_cutoff, kv, system.num_levels)
-                lib.set_ulong(rid, sa, sb, 0.0, float(u.r_grid[-1]), quintic_data(u.r_grid, u.ulong_r),
-                              quintic_data(du.r_grid, du.ulong_r))
+                u, du, v, qu, qdu, qv = _breakup_tables(tab, spec.box, spec.k_cutoff, kv, system.num_levels)
+                lib.set_ulong(rid, sa, sb, 0.0, float(u.r_grid[-1]), qu, qdu)
                 lib.set_long_range(rid, sa, sb, u.ulong_k, u.ulong_r0, u.ushort_k0, u.ulong_k0,
                                    du.ulong_k, du.ulong_r0, du.ushort_k0, du.ulong_k0)
                 lib.set_vlong(rid, sa, sb, v.ulong_k[0], v.ulong_r0[0], v.ushort_k0[0], v.ulong_k0[0],
-                              0.0, float(v.r_grid[-1]), quintic_data(v.r_grid, v.ulong_r)[0])
+                              0.0, float(v.r_grid[-1]), qv[0])
             lib.rho_k_update_all(rid)
         if spec.ewald:
             kv, _ = lib.kvecs(rid)
@@ -266,7 +265,8 @@ _BREAKUP_CACHE: dict = {}
 
 def _breakup_tables(tab, box, kc: float, kvecs: np.ndarray, num_levels: int):
     """Optimized breakups of U, dU/dbeta and V of a Coulomb pair table for one box and k set
-    (LongRangeClass::OptimizedBreakup_U / _dU / _V), cached across replicas with the same box."""
+    (LongRangeClass::OptimizedBreakup_U / _dU / _V) and the quintic data of their real-space
+    parts, cached across replicas with the same box."""
     key = (id(tab), tuple(box), kc, num_levels, kvecs.shape[0])
     if key not in _BREAKUP_CACHE:
         q_end = float(tab.grid_points("q")[-1])
@@ -279,7 +279,8 @@ def _breakup_tables(tab, box, kc: float, kvecs: np.ndarray, num_levels: int):
         du = optimized_breakup([tab.udiag(l, derivative=True) for l in range(num_levels)], box, kc, kvecs,
                                tail_end=q_end, short_of=coulomb)
         v = optimized_breakup([coulomb], box, kc, kvecs, tail_coef=(z1z2, 0.0, 0.0))
-        _BREAKUP_CACHE[key] = (u, du, v)
+        _BREAKUP_CACHE[key] = (u, du, v, quintic_data(u.r_grid, u.ulong_r), quintic_data(du.r_grid, du.ulong_r),
+                               quintic_data(v.r_grid, v.ulong_r))
     return _BREAKUP_CACHE[key]
 
 

@@ -66,16 +66,29 @@ def test_energy_ocp_background(oracle, gpu):
     assert math.isfinite(e[0].v_short) and e[0].v_long != 0.0
 
 
-def test_pair_correlation_matches_oracle(oracle, gpu):
+# 101 points: every block counts in its LDS histogram; 12001 points (12000 bins): too many bins
+# for LDS, each count goes to the global histogram.
+@pytest.mark.parametrize("points", [101, 12001])
+def test_pair_correlation_matches_oracle(oracle, gpu, points):
     sysm = workloads.charged_lj_system(n_plus=12, n_minus=20, m=16, num_replicas=1)
     io, ig = both(oracle, gpu, sysm)
     oracle.bisection_run(io, sysm.num_levels, sysm.seed, 0, 40, species=1)
     gpu.bisection_run(ig, sysm.num_levels, sysm.seed, 0, 40, abi.FP64, species=1)
     L = sysm.replicas[0].box[0]
     for s1, s2 in ((0, 0), (1, 1), (0, 1)):
-        ho = oracle.pair_correlation(io[0], s1, s2, 0.0, L / 2, 101)
-        hg = gpu.pair_correlation(ig[0], s1, s2, 0.0, L / 2, 101)
+        ho = oracle.pair_correlation(io[0], s1, s2, 0.0, L / 2, points)
+        hg = gpu.pair_correlation(ig[0], s1, s2, 0.0, L / 2, points)
         assert np.array_equal(ho, hg) and ho.sum() > 0
+
+
+def test_pair_correlation_of_the_large_ocp_matches_oracle(oracle, gpu):
+    """N = 1024, M = 128: 67 M pair distances over the LDS histograms of 4096 blocks."""
+    sysm = workloads.ocp_system(num_replicas=1)
+    io, ig = both(oracle, gpu, sysm)
+    L = sysm.replicas[0].box[0]
+    ho = oracle.pair_correlation(io[0], 0, 0, 0.0, L / 2, 41)
+    hg = gpu.pair_correlation(ig[0], 0, 0, 0.0, L / 2, 41)
+    assert np.array_equal(ho, hg) and ho.sum() > 0
 
 
 def test_free_particle_energy_is_three_n_over_two_beta(gpu):
