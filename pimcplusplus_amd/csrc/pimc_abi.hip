@@ -294,6 +294,12 @@ struct pimc_ctx {
   DevBuf d_fits, d_reps;
   bool fits_dirty = true, reps_dirty = true;
   DevBuf d_reqs, d_partials, d_out, d_in0, d_in1, d_in2, d_idx, d_img, d_records, d_stats, d_repids;
+  // Pinned, coherent host memory for the requests and results of small action calls (up to
+  // kPinnedReqs requests): the kernels read the requests and write the results there
+  // directly, so a call is its launches and one synchronisation, with no copies.  Allocated
+  // on first use, freed with the context.
+  void* h_pin = nullptr;
+  void* d_pin = nullptr;
   // bisection counters accumulated per replica id ([acc_cap][5]) until pimc_bisection_stats_take
   DevBuf d_acc;
   int acc_cap = 0;
@@ -403,6 +409,74 @@ int check_req(pimc_ctx* ctx, const pimc_action_req& q) {
   return PIMC_OK;
 }
 
+// Requests of one call and where its results go: small calls use the context's pinned host
+// region (requests read, results written by the kernels in place), larger ones device buffers
+// with a copy each way.
+constexpr int kPinnedReqs = 64;
+constexpr size_t kPinnedOutOffset = 4096;
+static_assert(kPinnedReqs * sizeof(ReqDev) <= kPinnedOutOffset, "pinned request region");
+constexpr size_t kPinnedBytes = kPinnedOutOffset + kPinnedReqs * sizeof(double);
+struct CallIO {
+  const ReqDev* reqs = nullptr;   // device-visible
+  double* out = nullptr;          // device-visible
+  bool pinned = false;
+};
+
+int upload_reqs(pimc_ctx* ctx, const pimc_action_req* reqs, int nreq);
+
+int stage_call(pimc_ctx* ctx, const pimc_action_req* reqs, int nreq, CallIO& io) {
+  if (nreq > kPinnedReqs) {
+    int rc = upload_reqs(ctx, reqs, nreq);
+    if (!rc) rc = ctx->d_out.ensure((size_t)nreq * sizeof(double));
+    if (rc) return rc;
+    io.reqs = ctx->d_reqs.as<ReqDev>();
+    io.out = ctx->d_out.as<double>();
+    io.pinned = false;
+    return PIMC_OK;
+  }
+  for (int i = 0; i < nreq; i++)
+    if (int rc = check_req(ctx, reqs[i])) return rc;
+  if (!ctx->h_pin) {
+    HIP_TRY(hipHostMalloc(&ctx->h_pin, kPinnedBytes, hipHostMallocMapped | hipHostMallocCoherent));
+    if (hipHostGetDevicePointer(&ctx->d_pin, ctx->h_pin, 0) != hipSuccess) {
+      (void)hipHostFree(ctx->h_pin);
+      ctx->h_pin = nullptr;
+      return fail(PIMC_ERR_DEVICE, "action call: no device address for the pinned request region");
+    }
+  }
+  ReqDev* h = (ReqDev*)ctx->h_pin;
+  for (int i = 0; i < nreq; i++) {
+    ReqDev q{};
+    q.replica = reqs[i].replica;
+    q.mode = reqs[i].mode;
+    q.slice1 = reqs[i].slice1;
+    q.slice2 = reqs[i].slice2;
+    q.level = reqs[i].level;
+    q.nptcls = reqs[i].nptcls;
+    for (int k = 0; k < PIMC_MAX_ACTIVE; k++) q.ptcls[k] = k < reqs[i].nptcls ? reqs[i].ptcls[k] : -1;
+    h[i] = q;
+  }
+  io.reqs = (const ReqDev*)ctx->d_pin;
+  io.out = (double*)((unsigned char*)ctx->d_pin + kPinnedOutOffset);
+  io.pinned = true;
+  return PIMC_OK;
+}
+
+// After the call's launches: the results into out, and the one synchronisation.
+int finish_call(pimc_ctx* ctx, const CallIO& io, int nreq, double* out) {
+  if (!io.pinned)
+    HIP_TRY(hipMemcpyAsync(out, io.out, (size_t)nreq * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (io.pinned) memcpy(out, (const unsigned char*)ctx->h_pin + kPinnedOutOffset, (size_t)nreq * sizeof(double));
+  return PIMC_OK;
+}
+
+// Link ranges per partner tile of a short-range call (LaunchShortRange::nsplit): a call with
+// few requests splits every tile's links over up to 8 blocks, so that each thread walks a few
+// links rather than the whole window.  Both short-range entry points use it, so they give the
+// same sums bit for bit.
+int short_range_split(int nreq, int ntiles) { return nreq * ntiles >= 256 ? 1 : std::min(8, 256 / (nreq * ntiles)); }
+
 int upload_reqs(pimc_ctx* ctx, const pimc_action_req* reqs, int nreq) {
   std::vector<ReqDev>& h = ctx->h_reqs;
   h.assign(nreq, ReqDev{});
@@ -488,6 +562,7 @@ int pimc_ctx_destroy(pimc_ctx* ctx) {
       (void)hipEventDestroy(e.first);
       (void)hipEventDestroy(e.second);
     }
+  if (ctx->h_pin) (void)hipHostFree(ctx->h_pin);
   delete ctx;
   if (s) (void)hipStreamDestroy(s);
   return PIMC_OK;
@@ -1297,7 +1372,8 @@ int pimc_action_short_range(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t 
   HIP_TRY(hipSetDevice(ctx->device));
   int rc = ctx->sync_descriptors();
   if (rc) return rc;
-  rc = upload_reqs(ctx, reqs, nreq);
+  CallIO io;
+  rc = stage_call(ctx, reqs, nreq, io);
   if (rc) return rc;
   int maxN = 0, nt = -1;
   for (int i = 0; i < nreq; i++) {
@@ -1319,16 +1395,13 @@ int pimc_action_short_range(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t 
   }
   if (nt < 0) nt = 0;
   const int ntiles = (maxN + kBlock - 1) / kBlock;
-  if ((rc = ctx->d_partials.ensure((size_t)nreq * ntiles * sizeof(double))) ||
-      (rc = ctx->d_out.ensure((size_t)nreq * sizeof(double))))
-    return rc;
-  LaunchShortRange a{ctx->d_reps.as<ReplicaDev>(), ctx->d_fits.as<FitDev>(), ctx->d_reqs.as<ReqDev>(), nreq, ntiles,
-                     ctx->d_partials.as<double>(), ctx->d_out.as<double>()};
+  const int nsplit = short_range_split(nreq, ntiles);
+  if ((rc = ctx->d_partials.ensure((size_t)nreq * ntiles * nsplit * sizeof(double)))) return rc;
+  LaunchShortRange a{ctx->d_reps.as<ReplicaDev>(), ctx->d_fits.as<FitDev>(), io.reqs, nreq, ntiles,
+                     ctx->d_partials.as<double>(), io.out, nsplit};
   launch_short_range(nt, precision == PIMC_MIXED, a, ctx->stream);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(out, ctx->d_out.p, (size_t)nreq * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
-  return PIMC_OK;
+  return finish_call(ctx, io, nreq, out);
 }
 
 int pimc_action_short_range_noalloc(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t nreq, int32_t precision,
@@ -1375,11 +1448,12 @@ int pimc_action_short_range_noalloc(pimc_ctx* ctx, const pimc_action_req* reqs, 
     for (int k = 0; k < PIMC_MAX_ACTIVE; k++) h[i].ptcls[k] = k < reqs[i].nptcls ? reqs[i].ptcls[k] : -1;
   }
   if (nt < 0) nt = 0;
-  // scratch layout: requests [nreq], outputs [nreq], partial sums [nreq][ntiles]
+  // scratch layout: requests [nreq], outputs [nreq], partial sums [nreq][ntiles * nsplit]
   const int ntiles = (maxN + kBlock - 1) / kBlock;
+  const int nsplit = short_range_split(nreq, ntiles);
   const size_t req_bytes = ((size_t)nreq * sizeof(ReqDev) + 255) & ~(size_t)255;
   const size_t out_bytes = ((size_t)nreq * sizeof(double) + 255) & ~(size_t)255;
-  const size_t part_bytes = (size_t)nreq * ntiles * sizeof(double);
+  const size_t part_bytes = (size_t)nreq * ntiles * nsplit * sizeof(double);
   if (req_bytes + out_bytes + part_bytes > PIMC_NOALLOC_SCRATCH_BYTES)
     return fail(PIMC_ERR_ARG, "action_short_range_noalloc: requests x particle tiles exceed the scratch array");
   ReqDev* d_reqs = (ReqDev*)base;
@@ -1387,7 +1461,8 @@ int pimc_action_short_range_noalloc(pimc_ctx* ctx, const pimc_action_req* reqs, 
   double* d_part = (double*)(base + req_bytes + out_bytes);
   HIP_TRY(hipSetDevice(ctx->device));
   HIP_TRY(hipMemcpyAsync(d_reqs, h, (size_t)nreq * sizeof(ReqDev), hipMemcpyHostToDevice, ctx->stream));
-  LaunchShortRange a{ctx->d_reps.as<ReplicaDev>(), ctx->d_fits.as<FitDev>(), d_reqs, nreq, ntiles, d_part, d_out};
+  LaunchShortRange a{ctx->d_reps.as<ReplicaDev>(), ctx->d_fits.as<FitDev>(), d_reqs, nreq, ntiles, d_part, d_out,
+                     nsplit};
   launch_short_range(nt, precision == PIMC_MIXED, a, ctx->stream);   // every tile writes its partial sum
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(out, d_out, (size_t)nreq * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -1401,14 +1476,11 @@ int pimc_action_kinetic(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t nreq
   HIP_TRY(hipSetDevice(ctx->device));
   int rc = ctx->sync_descriptors();
   if (rc) return rc;
-  rc = upload_reqs(ctx, reqs, nreq);
-  if (rc) return rc;
-  if ((rc = ctx->d_out.ensure((size_t)nreq * sizeof(double)))) return rc;
-  launch_kinetic(ctx->d_reps.as<ReplicaDev>(), ctx->d_reqs.as<ReqDev>(), nreq, ctx->d_out.as<double>(), ctx->stream);
+  CallIO io;
+  if ((rc = stage_call(ctx, reqs, nreq, io))) return rc;
+  launch_kinetic(ctx->d_reps.as<ReplicaDev>(), io.reqs, nreq, io.out, ctx->stream);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(out, ctx->d_out.p, (size_t)nreq * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
-  return PIMC_OK;
+  return finish_call(ctx, io, nreq, out);
 }
 
 int pimc_action_long_range(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t nreq, int32_t precision,
@@ -1438,22 +1510,18 @@ int pimc_action_long_range(pimc_ctx* ctx, const pimc_action_req* reqs, int32_t n
   if (rc) return rc;
   for (int i = 0; i < nreq; i++)
     if ((rc = refresh_rho1(ctx, ctx->reps[reqs[i].replica].get()))) return rc;
-  rc = upload_reqs(ctx, reqs, nreq);
-  if (rc) return rc;
-  if ((rc = ctx->d_out.ensure((size_t)nreq * sizeof(double))) ||
-      (rc = ctx->d_partials.ensure((size_t)nreq * kblocks * sizeof(double))))
-    return rc;
+  CallIO io;
+  if ((rc = stage_call(ctx, reqs, nreq, io))) return rc;
+  if ((rc = ctx->d_partials.ensure((size_t)nreq * kblocks * sizeof(double)))) return rc;
   const std::vector<LrGroup> groups = long_range_groups(nreq, rids.data(), nss.data(), modes.data());
   std::pair<hipEvent_t, hipEvent_t> ev;   // device time, read by pimc_bisection_stats_take
   if (int rc_ev = timed_pair(ctx, ev)) return rc_ev;
   HIP_TRY(hipEventRecord(ev.first, ctx->stream));
-  launch_long_range(precision == PIMC_MIXED, ctx->d_reps.as<ReplicaDev>(), ctx->d_reqs.as<ReqDev>(), nreq,
-                    groups, kblocks, ctx->d_partials.as<double>(), ctx->d_out.as<double>(), ctx->stream);
+  launch_long_range(precision == PIMC_MIXED, ctx->d_reps.as<ReplicaDev>(), io.reqs, nreq, groups, kblocks,
+                    ctx->d_partials.as<double>(), io.out, ctx->stream);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(ev.second, ctx->stream));
-  HIP_TRY(hipMemcpyAsync(out, ctx->d_out.p, (size_t)nreq * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
-  return PIMC_OK;
+  return finish_call(ctx, io, nreq, out);
 }
 
 int pimc_dist_disp(pimc_ctx* ctx, int32_t replica, int32_t mode, int64_t n, const int32_t* idx, double* out,

@@ -46,15 +46,16 @@ void launch_fit_eval(int num_terms, bool mixed, const LaunchFitEval& a, hipStrea
 
 // --------------------------------------------------------------------------
 // ShortRangeClass::SingleAction for a batch of requests.
-// grid.x: partner tiles of kBlock particles, grid.y: requests.  Each thread owns
-// one partner and runs the moved-particle and link loops of the reference for
-// it; partial sums land in partials[req][tile] and a second kernel adds the
-// tiles of each request in tile order.
+// grid.x: partner tiles of kBlock particles times nsplit link ranges, grid.y: requests.
+// Each thread owns one partner and runs the moved-particle and link loops of the reference
+// for it over its block's range of the links (nsplit > 1 shortens the dependent walk of a
+// call with few requests); partial sums land in partials[req][tile * nsplit + range] and a
+// second kernel adds them in that order.
 // --------------------------------------------------------------------------
 template <int NT, bool MIXED>
 __global__ void __launch_bounds__(kBlock) k_short_range(const ReplicaDev* __restrict__ reps,
                                                         const FitDev* __restrict__ fits,
-                                                        const ReqDev* __restrict__ reqs, int ntiles,
+                                                        const ReqDev* __restrict__ reqs, int ntiles, int nsplit,
                                                         double* __restrict__ partials) {
   __shared__ double scratch_raw[kWaves];
   PIMC_LDS double* scratch = (PIMC_LDS double*)scratch_raw;
@@ -63,8 +64,12 @@ __global__ void __launch_bounds__(kBlock) k_short_range(const ReplicaDev* __rest
   const ReplicaDev* RD = reps + Q.replica;
   const RepCore R = load_uniform(&RD->core);
   const RepSpecies& S = RD->sp;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tile = blockIdx.x / nsplit, range = blockIdx.x % nsplit;
+  const int j = tile * blockDim.x + threadIdx.x;
   const int skip = 1 << Q.level;
+  const int nlinks = (Q.slice2 - Q.slice1 + skip - 1) / skip;
+  const int first = Q.slice1 + (int)((long long)range * nlinks / nsplit) * skip;
+  const int last = Q.slice1 + (int)((long long)(range + 1) * nlinks / nsplit) * skip;   // this range's links end here
   Kahan acc;
   if (j < R.N) {
     const int sp2 = [&] {
@@ -84,9 +89,9 @@ __global__ void __launch_bounds__(kBlock) k_short_range(const ReplicaDev* __rest
       const int lev = Q.level + F.taupos;
       const UlongView ul = ulong_for(R, F, sp1 * R.ns + sp2, Q.level);
       double ma[3], mb[3], pa[3], pb[3];
-      load_bead(R, Q.mode, Q.slice1, p1, ma);
-      load_bead(R, Q.mode, Q.slice1, j, pa);
-      for (int slice = Q.slice1; slice < Q.slice2; slice += skip) {
+      load_bead(R, Q.mode, first, p1, ma);
+      load_bead(R, Q.mode, first, j, pa);
+      for (int slice = first; slice < min(last, Q.slice2); slice += skip) {
         load_bead(R, Q.mode, slice + skip, p1, mb);
         load_bead(R, Q.mode, slice + skip, j, pb);
         double u;
@@ -111,7 +116,7 @@ __global__ void __launch_bounds__(kBlock) k_short_range(const ReplicaDev* __rest
     }
   }
   double t = block_sum(acc.value(), scratch);
-  if (threadIdx.x == 0) partials[(size_t)blockIdx.y * ntiles + blockIdx.x] = t;
+  if (threadIdx.x == 0) partials[(size_t)blockIdx.y * ntiles * nsplit + blockIdx.x] = t;
 }
 
 __global__ void k_sum_tiles(const double* __restrict__ partials, int ntiles, int nreq, double* __restrict__ out) {
@@ -124,29 +129,39 @@ __global__ void k_sum_tiles(const double* __restrict__ partials, int ntiles, int
 
 template <bool MIXED>
 static void launch_sr_t(int nt, const LaunchShortRange& a, hipStream_t s) {
-  dim3 grid(a.ntiles, a.nreq);
+  dim3 grid(a.ntiles * a.nsplit, a.nreq);
+#define PIMC_SR(NT_) k_short_range<NT_, MIXED><<<grid, kBlock, 0, s>>>(a.reps, a.fits, a.reqs, a.ntiles, a.nsplit, a.partials)
   switch (nt) {
-    case 0: k_short_range<0, MIXED><<<grid, kBlock, 0, s>>>(a.reps, a.fits, a.reqs, a.ntiles, a.partials); break;
-    case 1: k_short_range<1, MIXED><<<grid, kBlock, 0, s>>>(a.reps, a.fits, a.reqs, a.ntiles, a.partials); break;
-    case 2: k_short_range<2, MIXED><<<grid, kBlock, 0, s>>>(a.reps, a.fits, a.reqs, a.ntiles, a.partials); break;
-    default: k_short_range<3, MIXED><<<grid, kBlock, 0, s>>>(a.reps, a.fits, a.reqs, a.ntiles, a.partials); break;
+    case 0: PIMC_SR(0); break;
+    case 1: PIMC_SR(1); break;
+    case 2: PIMC_SR(2); break;
+    default: PIMC_SR(3); break;
   }
+#undef PIMC_SR
 }
+// With one block per request (one tile, one link range) the block sums are the results: one
+// launch.
 void launch_short_range(int num_terms, bool mixed, const LaunchShortRange& a, hipStream_t s) {
   if (a.nreq <= 0) return;
+  const int parts = a.ntiles * a.nsplit;
+  LaunchShortRange b = a;
+  if (parts == 1) b.partials = a.out;
   if (mixed)
-    launch_sr_t<true>(num_terms, a, s);
+    launch_sr_t<true>(num_terms, b, s);
   else
-    launch_sr_t<false>(num_terms, a, s);
-  k_sum_tiles<<<(a.nreq + 255) / 256, 256, 0, s>>>(a.partials, a.ntiles, a.nreq, a.out);
+    launch_sr_t<false>(num_terms, b, s);
+  if (parts > 1) k_sum_tiles<<<(a.nreq + 255) / 256, 256, 0, s>>>(a.partials, parts, a.nreq, a.out);
 }
 
 // --------------------------------------------------------------------------
-// KineticClass::SingleAction, one thread per request (a few links each)
+// KineticClass::SingleAction, one wave per request: lane l takes links l, l + 64, ..., and the
+// wave adds the lanes' sums (the links' loads are in flight together instead of one link after
+// another)
 // --------------------------------------------------------------------------
-__global__ void k_kinetic(const ReplicaDev* __restrict__ reps, const ReqDev* __restrict__ reqs, int nreq,
-                          double* __restrict__ out) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(64) k_kinetic(const ReplicaDev* __restrict__ reps, const ReqDev* __restrict__ reqs,
+                                                int nreq, double* __restrict__ out) {
+  const int r = blockIdx.x;
+  const int lane = threadIdx.x;
   if (r >= nreq) return;
   const ReqDev Q = reqs[r];
   const PIMC_GLOBAL int* qp = (const PIMC_GLOBAL int*)reqs[r].ptcls;   // indexed at run time: not from Q
@@ -162,7 +177,7 @@ __global__ void k_kinetic(const ReplicaDev* __restrict__ reps, const ReqDev* __r
     double lambda = S.lambda[sp];
     if (lambda != 0) {
       double FourLambdaTauInv = 1.0 / (4.0 * lambda * levelTau);
-      for (int slice = Q.slice1; slice < Q.slice2; slice += skip) {
+      for (int slice = Q.slice1 + lane * skip; slice < Q.slice2; slice += 64 * skip) {
         double a[3], b[3];
         load_bead(R, Q.mode, slice, ptcl, a);
         load_bead(R, Q.mode, slice + skip, ptcl, b);
@@ -170,11 +185,12 @@ __global__ void k_kinetic(const ReplicaDev* __restrict__ reps, const ReqDev* __r
       }
     }
   }
-  out[r] = TotalK;
+  TotalK = wave_sum(TotalK);
+  if (lane == 0) out[r] = TotalK;
 }
 void launch_kinetic(const ReplicaDev* reps, const ReqDev* reqs, int nreq, double* out, hipStream_t s) {
   if (nreq <= 0) return;
-  k_kinetic<<<(nreq + 255) / 256, 256, 0, s>>>(reps, reqs, nreq, out);
+  k_kinetic<<<nreq, 64, 0, s>>>(reps, reqs, nreq, out);
 }
 
 // --------------------------------------------------------------------------

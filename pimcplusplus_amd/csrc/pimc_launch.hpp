@@ -28,8 +28,9 @@ struct LaunchShortRange {
   const FitDev* fits;
   const ReqDev* reqs;
   int nreq, ntiles;
-  double* partials;  // [nreq][ntiles]
+  double* partials;  // [nreq][ntiles * nsplit]
   double* out;       // [nreq]
+  int nsplit = 1;    // blocks sharing a tile's partners, each with a contiguous range of the links
 };
 void launch_short_range(int num_terms, bool mixed, const LaunchShortRange& a, hipStream_t s);
 
