@@ -92,13 +92,19 @@ def test_short_range_and_kinetic_requests(oracle, gpu, nspecies):
     ro_reqs = [(ro,) + r[1:] for r in reqs_all]
     rg_reqs = [(rg,) + r[1:] for r in reqs_all]
     want = oracle.action_short_range(ro_reqs)
-    got = gpu.action_short_range(rg_reqs, FP64)
-    assert np.max(np.abs(got - want) / (1 + np.abs(want))) < 1e-12
-    mixed = gpu.action_short_range(rg_reqs, MIXED)
-    assert np.max(np.abs(mixed - want)) < 1e-5
     kw = oracle.action_kinetic(ro_reqs)
-    kg = gpu.action_kinetic(rg_reqs)
-    assert np.max(np.abs(kg - kw) / (1 + np.abs(kw))) < 1e-13
+    # 60 requests in one call (pinned host requests, links split over 4 blocks), the list twice
+    # (120 requests: device buffers, split over 2), and one request per call (split over 8)
+    calls = {"one call": lambda f: f(rg_reqs),
+             "list twice": lambda f: f(rg_reqs + rg_reqs)[:len(rg_reqs)],
+             "one per call": lambda f: np.concatenate([f([q]) for q in rg_reqs])}
+    for name, call in calls.items():
+        got = call(lambda q: gpu.action_short_range(q, FP64))
+        assert np.max(np.abs(got - want) / (1 + np.abs(want))) < 1e-12, name
+        mixed = call(lambda q: gpu.action_short_range(q, MIXED))
+        assert np.max(np.abs(mixed - want)) < 1e-5, name
+        kg = call(gpu.action_kinetic)
+        assert np.max(np.abs(kg - kw) / (1 + np.abs(kw))) < 1e-13, name
 
 
 def test_accept_reject_join_bit_exact(oracle, gpu):
