@@ -37,6 +37,10 @@ LDS_PEAK_GBS = 256 * 128 * 2.4      # 256 CUs x 128 B/clk x 2.4 GHz
 CALIB_LIB = os.path.join(ROOT, "pimcplusplus_amd", "_lib", "libpimc_calib.so")
 FLOP_PER_EVAL = 180.0      # FP64 flops of one pair·slice evaluation (DistDisp, q/z/s², splines, sum)
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_bisection_lj_n256_m64_l4_traffic.md")
+# Replicas of the headline workload: 4 blocks per CU x 256 CUs are resident at once, so 16384
+# replicas run as 16 rounds of blocks (4096 replicas, 4 rounds, spend ~10% of a launch in its
+# tail, where the last blocks finish one by one)
+HEADLINE_REPLICAS = 16384
 
 
 def traffic_profile(workload: str) -> str:
@@ -633,7 +637,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--replicas", type=int, default=4096)
+    ap.add_argument("--replicas", type=int, default=None,
+                    help="independent replicas (default: 16384 for the headline workload, 4096 for the "
+                         "others); the headline runs 16 blocks per resident block slot, so the end of a "
+                         "launch, where the last blocks finish one by one, is a small part of it")
     ap.add_argument("--moves-per-step", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -658,6 +665,8 @@ def main():
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
         return
+    if args.replicas is None:
+        args.replicas = HEADLINE_REPLICAS if args.workload == "lj256" else 4096
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(replicas would shard across GPUs with no communication; see the design notes)",
