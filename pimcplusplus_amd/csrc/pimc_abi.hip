@@ -300,6 +300,7 @@ struct pimc_ctx {
   // on first use, freed with the context.
   void* h_pin = nullptr;
   void* d_pin = nullptr;
+  bool pin_busy = false;   // a call staged its requests there and has not synchronised yet
   // bisection counters accumulated per replica id ([acc_cap][5]) until pimc_bisection_stats_take
   DevBuf d_acc;
   int acc_cap = 0;
@@ -444,6 +445,9 @@ int stage_call(pimc_ctx* ctx, const pimc_action_req* reqs, int nreq, CallIO& io)
       return fail(PIMC_ERR_DEVICE, "action call: no device address for the pinned request region");
     }
   }
+  // a call that returned an error after staging may have left kernels that still read the region
+  if (ctx->pin_busy) HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->pin_busy = true;
   ReqDev* h = (ReqDev*)ctx->h_pin;
   for (int i = 0; i < nreq; i++) {
     ReqDev q{};
@@ -467,7 +471,10 @@ int finish_call(pimc_ctx* ctx, const CallIO& io, int nreq, double* out) {
   if (!io.pinned)
     HIP_TRY(hipMemcpyAsync(out, io.out, (size_t)nreq * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
-  if (io.pinned) memcpy(out, (const unsigned char*)ctx->h_pin + kPinnedOutOffset, (size_t)nreq * sizeof(double));
+  if (io.pinned) {
+    memcpy(out, (const unsigned char*)ctx->h_pin + kPinnedOutOffset, (size_t)nreq * sizeof(double));
+    ctx->pin_busy = false;
+  }
   return PIMC_OK;
 }
 
