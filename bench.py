@@ -638,9 +638,11 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--replicas", type=int, default=None,
-                    help="independent replicas (default: 16384 for the headline workload, 4096 for the "
-                         "others); the headline runs 16 blocks per resident block slot, so the end of a "
-                         "launch, where the last blocks finish one by one, is a small part of it")
+                    help="independent replicas (default: 16384 for the LJ workloads lj256 and lj64: 16 "
+                         "rounds of the blocks the GPU holds at once, so the end of a launch, where the last "
+                         "blocks finish one by one, is a small part of it; 4096 for the others, whose "
+                         "replicas each upload their own long-range tables; uneven and ocp have their own "
+                         "counts)")
     ap.add_argument("--moves-per-step", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -666,7 +668,7 @@ def main():
         calibrate_copy(args.calibrate_copy)
         return
     if args.replicas is None:
-        args.replicas = HEADLINE_REPLICAS if args.workload == "lj256" else 4096
+        args.replicas = HEADLINE_REPLICAS if args.workload in ("lj256", "lj64") else 4096
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(replicas would shard across GPUs with no communication; see the design notes)",
