@@ -124,6 +124,16 @@ def load_system(lib, system):
     return ids
 
 
+PROGRESS = False   # --progress: a line on stderr at the end of each phase
+
+
+def progress(msg: str):
+    """A line on stderr when a phase of the run ends (--progress), so that long set-ups of many
+    replicas show that they advance; stdout keeps the one JSON line."""
+    if PROGRESS:
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def run_gpu(system, precision, steps, warmup, moves_per_step, tune=False, graph=False, paths_out=None, plan=None,
             setup=None):
     """W untimed steps, then K steps queued back to back on the context's stream;
@@ -133,12 +143,15 @@ def run_gpu(system, precision, steps, warmup, moves_per_step, tune=False, graph=
     that receives the paths of the first eight replicas after the run."""
     lib = native.open_context(0)
     ids = load_system(lib, system)
+    progress(f"{system.name}: {len(ids)} replicas loaded")
     if setup:
         setup(lib, ids)
     L, seed = system.num_levels, system.seed
     if plan is not None:
         lib.set_plan(*plan)
     plan = lib.bisection_tune(ids, L, seed, 0, moves_per_step, precision) if tune else lib.get_plan()
+    if tune:
+        progress(f"{system.name}: tuned plan {plan.block_threads}, {plan.link_split}, {plan.persistent}, {plan.cluster}")
     move = 0
     if graph:
         g = lib.sweep_graph_create(ids, L, seed, 0, moves_per_step, precision)
@@ -156,6 +169,7 @@ def run_gpu(system, precision, steps, warmup, moves_per_step, tune=False, graph=
         step()
     tot = lib.bisection_stats_take(ids)    # waits for the timed steps
     wall = time.perf_counter() - t0
+    progress(f"{system.name}: precision {precision}, {steps} timed steps in {wall:.2f} s")
     if graph:
         lib.sweep_graph_destroy(g)
     if paths_out is not None:
@@ -651,6 +665,7 @@ def main():
     ap.add_argument("--no-dserr", action="store_true", help="skip the mixed-vs-FP64 dS and energy comparisons")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run an elementwise copy of MIB MiB (read MIB, write MIB) for counter scaling")
+    ap.add_argument("--progress", action="store_true", help="a line on stderr at the end of each phase")
     ap.add_argument("--plan", default=None, metavar="BLOCK,SPLIT,PERSISTENT,CLUSTER",
                     help="non-headline workloads: this launch plan with eager launches instead of the tuner "
                          "and a captured sweep graph (profiling runs)")
@@ -664,6 +679,8 @@ def main():
                          "calls of the drop-in mode; longrange: batched LongRange SingleAction requests; bose: "
                          "dilute helium bosons with PermuteType TABLE moves)")
     args = ap.parse_args()
+    global PROGRESS
+    PROGRESS = args.progress
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
         return
